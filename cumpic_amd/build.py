@@ -1,0 +1,40 @@
+"""Compile the HIP library (and the Fortran host) in-tree for gfx950."""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB = os.path.join(PKG, "libcumc.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off: no fused multiply-add, every operation rounds as in the reference source.
+# -packed-fp32-ops off: the compiler must not form v_pk_{add,mul,fma}_f32.
+HIP_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Wno-unused-result", "-Wno-unused-value",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build_library(verbose: bool = False) -> str:
+    src = os.path.join(PKG, "csrc", "cumc.hip")
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, src]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout)
+    if verbose:
+        print(r.stdout)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def flang() -> str | None:
+    for c in ("/opt/rocm/lib/llvm/bin/flang", shutil.which("flang") or "", shutil.which("amdflang") or ""):
+        if c and os.path.exists(c):
+            return c
+    return None

@@ -1,0 +1,541 @@
+// cumc.hip -- C ABI (include/cumc.h) over the HIP kernels of kernels.hpp.
+//
+// One context = one subdomain on one GPU, one HIP stream.  The context owns every device
+// buffer; host arrays crossing the ABI are caller-owned and in the reference's Fortran layout.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cumc.h"
+#include "kernels.hpp"
+
+using namespace cumc;
+
+struct cumc_ctx {
+    cumc_config cfg;
+    Lay L;
+    Model M;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *pool = nullptr;  // all FP64 fields
+    size_t pool_bytes = 0;
+    void *fluxpool = nullptr;
+    Fields<double> F64;
+    Fields<float> F32;
+    double *tab[3] = {nullptr, nullptr, nullptr};
+    double *dt_dev = nullptr;
+    unsigned long long *red = nullptr;  // reduction slots
+    bool grid_set = false;
+    double last_ms = 0.0;
+    std::string err;
+};
+
+static const char *g_err_names[] = {"ok", "invalid argument", "HIP error", "non-finite time step",
+                                    "out of device memory", "call out of order"};
+
+#define HIPCHK(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                    \
+            return CUMC_EHIP;                                                                   \
+        }                                                                                       \
+    } while (0)
+
+static int fail(cumc_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+// setup.f90:121-230 parity factors
+static void setup_bfac(Model &M)
+{
+    for (int s = 0; s < 6; s++)
+        for (int i = 0; i < 8; i++) M.bfac[s][i] = 1;
+    const int *bf = M.bflag;
+    auto set = [&](int s, std::initializer_list<int> idx) {
+        for (int i : idx) M.bfac[s][i] = -1;
+    };
+    if (bf[0] == 2) set(0, {IVX, IBX});
+    else if (bf[0] == 3) set(0, {IVX, IBX, IVZ, IBZ});
+    if (bf[1] == 2) set(1, {IVX, IBX});
+    else if (bf[1] == 3) set(1, {IVX, IBX, IVZ, IBZ});
+    if (bf[2] == 2) set(2, {IVY, IBY});
+    else if (bf[2] == 3) set(2, {IVY, IBY, IVZ, IBZ});
+    else if (bf[2] == 4) set(2, {IVY, IBX, IBZ});
+    if (bf[3] == 2) set(3, {IVY, IBY});
+    else if (bf[3] == 3) set(3, {IVY, IBY, IVZ, IBZ});
+    else if (bf[3] == 4) set(3, {IVY, IBX, IBZ});
+    if (bf[4] == 2) set(4, {IVZ, IBZ});
+    else if (bf[4] == 4) set(4, {IVZ, IBX, IBY});
+    if (bf[5] == 2) set(5, {IVZ, IBZ});
+    else if (bf[5] == 4) set(5, {IVZ, IBX, IBY});
+}
+
+extern "C" const char *cumc_error_string(int code)
+{
+    if (code < 0 || code > 5) return "unknown error";
+    return g_err_names[code];
+}
+
+extern "C" const char *cumc_last_error(cumc_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
+{
+    if (!cfg || !out) return CUMC_EINVAL;
+    *out = nullptr;
+    const cumc_config &c = *cfg;
+    if (c.nx < 1 || c.ny < 1 || c.nz < 1) return CUMC_EINVAL;
+    if (c.coordinate != CUMC_CARTESIAN && c.coordinate != CUMC_CYLINDRICAL) return CUMC_EINVAL;
+    if (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM) return CUMC_EINVAL;
+    if (c.model != CUMC_MODEL_RIEMANN && c.model != CUMC_MODEL_TORUS_CYN) return CUMC_EINVAL;
+    if (c.precision != CUMC_FP64 && c.precision != CUMC_MIXED) return CUMC_EINVAL;
+    if (!(c.ggas > 1.0) || !(c.cfl > 0.0)) return CUMC_EINVAL;
+    const int n3[3] = {c.nx, c.ny, c.nz};
+    for (int s = 0; s < 6; s++) {
+        const int f = c.boundary_flag[s];
+        if (f < 0 || f > 4) return CUMC_EINVAL;
+        if (f >= 2 && n3[s / 2] < NG) return CUMC_EINVAL;  // parity copies need NGHOST interior cells
+    }
+    // setup.f90:147-151 and :206-210: equatorial symmetry on x / axial symmetry on z stop the run
+    if (c.boundary_flag[0] == 4 || c.boundary_flag[1] == 4) return CUMC_EINVAL;
+    if (c.boundary_flag[4] == 3 || c.boundary_flag[5] == 3) return CUMC_EINVAL;
+
+    cumc_ctx *ctx = new cumc_ctx();
+    ctx->cfg = c;
+    Lay &L = ctx->L;
+    L.nx = c.nx;
+    L.ny = c.ny;
+    L.nz = c.nz;
+    L.px = ((c.nx + 2 * NG) + 7) / 8 * 8;
+    L.py = L.px * (c.ny + 2 * NG);
+    const long cells = L.py * (c.nz + 2 * NG);
+    L.ps = (cells + 63) / 64 * 64;
+    Model &M = ctx->M;
+    M.coordinate = c.coordinate;
+    M.recon = c.recon;
+    M.model = c.model;
+    M.geom_repeat = c.geom_source_repeat;
+    for (int s = 0; s < 6; s++) M.bflag[s] = c.boundary_flag[s];
+    setup_bfac(M);
+    M.ggas = c.ggas;
+    M.cfl = c.cfl;
+    M.rho_floor = c.rho_floor;
+    M.eps_floor = c.eps_floor;
+    M.r_sh = c.r_sh;
+
+    // FP64 state: P 8 + EPS 1 + BC 3 + U0 8 + U1 8 = 28 planes ; fluxes 21 + EMF 3 planes in R
+    const size_t nplanes = 28;
+    ctx->pool_bytes = nplanes * (size_t)L.ps * sizeof(double);
+    const size_t rsz = c.precision == CUMC_MIXED ? sizeof(float) : sizeof(double);
+    const size_t fbytes = 24 * (size_t)L.ps * rsz;
+    hipError_t e = hipMalloc(&ctx->pool, ctx->pool_bytes);
+    if (e != hipSuccess) { delete ctx; return CUMC_ENOMEM; }
+    e = hipMalloc(&ctx->fluxpool, fbytes);
+    if (e != hipSuccess) { hipFree(ctx->pool); delete ctx; return CUMC_ENOMEM; }
+    if (hipMalloc(&ctx->dt_dev, 64) != hipSuccess || hipMalloc(&ctx->red, 64) != hipSuccess) {
+        hipFree(ctx->pool); hipFree(ctx->fluxpool); delete ctx; return CUMC_ENOMEM;
+    }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        hipFree(ctx->pool); hipFree(ctx->fluxpool); delete ctx; return CUMC_EHIP;
+    }
+    // zero everything once: ghost EMFs at index -1 (mhd.f90:142-148) and initial bcell stay zero
+    hipMemset(ctx->pool, 0, ctx->pool_bytes);
+    hipMemset(ctx->fluxpool, 0, fbytes);
+    hipMemset(ctx->dt_dev, 0, 64);
+    double *p = ctx->pool;
+    auto take = [&](int n) { double *r = p; p += (size_t)n * L.ps; return r; };
+    Fields<double> &F = ctx->F64;
+    F.P = take(8);
+    F.EPS = take(1);
+    F.BC = take(3);
+    F.U0 = take(8);
+    F.U1 = take(8);
+    Fields<float> &G = ctx->F32;
+    G.P = F.P; G.EPS = F.EPS; G.BC = F.BC; G.U0 = F.U0; G.U1 = F.U1;
+    for (int d = 0; d < 3; d++) {
+        F.F[d] = (double *)ctx->fluxpool + (size_t)d * 7 * L.ps;
+        G.F[d] = (float *)ctx->fluxpool + (size_t)d * 7 * L.ps;
+    }
+    F.E = (double *)ctx->fluxpool + (size_t)21 * L.ps;
+    G.E = (float *)ctx->fluxpool + (size_t)21 * L.ps;
+    if (hipDeviceSynchronize() != hipSuccess) { cumc_destroy(ctx); return CUMC_EHIP; }
+    *out = ctx;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_destroy(cumc_ctx *ctx)
+{
+    if (!ctx) return CUMC_EINVAL;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    hipFree(ctx->pool);
+    hipFree(ctx->fluxpool);
+    hipFree(ctx->dt_dev);
+    hipFree(ctx->red);
+    for (int a = 0; a < 3; a++) hipFree(ctx->tab[a]);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CUMC_OK;
+}
+
+// Host-side geometry tables of one axis (coords.f90:49-97 widths/centres, reconstruct.f90
+// coefficients), FP64 with the reference's expressions so that hoisting changes no bit.
+static std::vector<double> build_tables(const double *f_in, int n)
+{
+    const int len = n + 7;
+    std::vector<double> t((size_t)NTAB * len, 0.0);
+    auto F = [&](int j) { return f_in[j + 3]; };
+    auto D = [&](int j) { return F(j) - F(j - 1); };
+    auto T = [&](int row, int j) -> double & { return t[(size_t)row * len + j + 3]; };
+    for (int j = -3; j <= n + 3; j++) T(TB_F, j) = F(j);
+    for (int j = -2; j <= n + 3; j++) {
+        T(TB_D, j) = D(j);
+        T(TB_C, j) = 0.5 * (F(j) + F(j - 1));
+    }
+    for (int j = -1; j <= n + 2; j++) {
+        double b0, b1;
+        ppm_slope_coef(D(j - 1), D(j), D(j + 1), &b0, &b1);
+        T(TB_B0C, j) = b0;
+        T(TB_B1C, j) = b1;
+        const double dc = D(j);
+        T(TB_CF, j) = (dc + D(j + 1)) / dc;
+        T(TB_CB, j) = (dc + D(j - 1)) / dc;
+    }
+    for (int j = 0; j <= n + 1; j++) {
+        PPMCoef k = ppm_coef(D(j - 2), D(j - 1), D(j), D(j + 1), D(j + 2));
+        T(TB_A0M, j) = k.a0m; T(TB_A1M, j) = k.a1m; T(TB_A2M, j) = k.a2m; T(TB_A3M, j) = k.a3m;
+        T(TB_A0P, j) = k.a0p; T(TB_A1P, j) = k.a1p; T(TB_A2P, j) = k.a2p; T(TB_A3P, j) = k.a3p;
+    }
+    return t;
+}
+
+extern "C" int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, const double *zf)
+{
+    if (!ctx || !xf || !yf || !zf) return CUMC_EINVAL;
+    const double *fs[3] = {xf, yf, zf};
+    const int ns[3] = {ctx->L.nx, ctx->L.ny, ctx->L.nz};
+    for (int a = 0; a < 3; a++) {
+        for (int j = 1; j < ns[a] + 7; j++)
+            if (!(fs[a][j] > fs[a][j - 1])) return fail(ctx, CUMC_EINVAL, "face coordinates must increase");
+        std::vector<double> t = build_tables(fs[a], ns[a]);
+        if (!ctx->tab[a]) HIPCHK(ctx, hipMalloc(&ctx->tab[a], t.size() * sizeof(double)));
+        HIPCHK(ctx, hipMemcpy(ctx->tab[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (ctx->cfg.coordinate == CUMC_CYLINDRICAL && !(xf[3] > 0.0))
+        return fail(ctx, CUMC_EINVAL, "cylindrical grid needs R > 0 on the inner face");
+    ctx->grid_set = true;
+    return CUMC_OK;
+}
+
+static Tabs tabs_of(const cumc_ctx *ctx)
+{
+    Tabs tb;
+    tb.t[0] = ctx->tab[0]; tb.t[1] = ctx->tab[1]; tb.t[2] = ctx->tab[2];
+    tb.len[0] = ctx->L.nx + 7; tb.len[1] = ctx->L.ny + 7; tb.len[2] = ctx->L.nz + 7;
+    return tb;
+}
+
+static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+static int conv_in(cumc_ctx *ctx, const double *host, int nv, double *dst)
+{
+    const Lay &L = ctx->L;
+    const long ncell = (long)(L.nx + 6) * (L.ny + 6) * (L.nz + 6);
+    double *tmp = nullptr;
+    HIPCHK(ctx, hipMalloc(&tmp, (size_t)ncell * nv * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, host, (size_t)ncell * nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(ncell, 256)), dim3(256), 0, ctx->stream, L, tmp, nv, dst);
+        e = hipStreamSynchronize(ctx->stream);
+    }
+    hipFree(tmp);
+    HIPCHK(ctx, e);
+    return CUMC_OK;
+}
+
+static int conv_out(cumc_ctx *ctx, const double *src, int nv, double *host)
+{
+    const Lay &L = ctx->L;
+    const long ncell = (long)(L.nx + 6) * (L.ny + 6) * (L.nz + 6);
+    double *tmp = nullptr;
+    HIPCHK(ctx, hipMalloc(&tmp, (size_t)ncell * nv * sizeof(double)));
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(ncell, 256)), dim3(256), 0, ctx->stream, L, src, nv, tmp);
+    hipError_t e = hipMemcpyAsync(host, tmp, (size_t)ncell * nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(tmp);
+    HIPCHK(ctx, e);
+    return CUMC_OK;
+}
+
+extern "C" int cumc_upload(cumc_ctx *ctx, const double *prim, const double *eps)
+{
+    if (!ctx || !prim || !eps) return CUMC_EINVAL;
+    int r = conv_in(ctx, prim, 8, ctx->F64.P);
+    if (r) return r;
+    return conv_in(ctx, eps, 1, ctx->F64.EPS);
+}
+
+extern "C" int cumc_download(cumc_ctx *ctx, double *prim, double *eps, double *bcell)
+{
+    if (!ctx) return CUMC_EINVAL;
+    int r;
+    if (prim && (r = conv_out(ctx, ctx->F64.P, 8, prim))) return r;
+    if (eps && (r = conv_out(ctx, ctx->F64.EPS, 1, eps))) return r;
+    if (bcell && (r = conv_out(ctx, ctx->F64.BC, 3, bcell))) return r;
+    return CUMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------------------------
+static void launch_ghost(cumc_ctx *ctx)
+{
+    const Lay &L = ctx->L;
+    hipStream_t s = ctx->stream;
+    const long nx6 = L.nx + 6, ny6 = L.ny + 6;
+    const long n0 = (long)(L.ny + 1) * (L.nz + 1) * 6;
+    const long n1 = nx6 * (L.nz + 1) * 6;
+    const long n2 = nx6 * ny6 * 6;
+    hipLaunchKernelGGL(k_ghost<0>, dim3(nblk(n0, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
+    hipLaunchKernelGGL(k_ghost<1>, dim3(nblk(n1, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
+    hipLaunchKernelGGL(k_ghost<2>, dim3(nblk(n2, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
+    if (ctx->M.model == CUMC_MODEL_TORUS_CYN) {
+        const long nyz = (long)(L.ny + 6) * (L.nz + 6) * 3, nxy = nx6 * ny6 * 3;
+        hipLaunchKernelGGL(k_custom_boundary_x, dim3(nblk(nyz, 256)), dim3(256), 0, s, L, ctx->F64);
+        hipLaunchKernelGGL(k_custom_boundary_z, dim3(nblk(nxy, 256)), dim3(256), 0, s, L, ctx->F64);
+    }
+}
+
+template <typename R>
+static void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
+{
+    const Lay &L = ctx->L;
+    hipStream_t s = ctx->stream;
+    Tabs tb = tabs_of(ctx);
+    const Model &M = ctx->M;
+    const bool ppm = M.recon == CUMC_PPMC;
+    if (!only_dir || only_dir == 1) {
+        const long waves = ((long)(L.ny + 2) * L.px + XS_FACES - 1) / XS_FACES;
+        dim3 grid(nblk(waves * 64, 256), L.nz + 2);
+        if (ppm) hipLaunchKernelGGL((k_flux_x<R, 4>), grid, dim3(256), 0, s, L, F, tb, M);
+        else hipLaunchKernelGGL((k_flux_x<R, 1>), grid, dim3(256), 0, s, L, F, tb, M);
+    }
+    const int seg = 48;
+    if (!only_dir || only_dir == 2) {
+        const int nseg = (L.ny + 1 + seg - 1) / seg;
+        const int sl = (L.ny + 1 + nseg - 1) / nseg;
+        dim3 grid(nblk((long)L.px * (L.nz + 2), 128), nseg);
+        if (ppm) hipLaunchKernelGGL((k_flux_walk<R, 2, 4>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+        else hipLaunchKernelGGL((k_flux_walk<R, 2, 1>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+    }
+    if (!only_dir || only_dir == 3) {
+        const int nseg = (L.nz + 1 + seg - 1) / seg;
+        const int sl = (L.nz + 1 + nseg - 1) / nseg;
+        dim3 grid(nblk((long)L.px * (L.ny + 2), 128), nseg);
+        if (ppm) hipLaunchKernelGGL((k_flux_walk<R, 3, 4>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+        else hipLaunchKernelGGL((k_flux_walk<R, 3, 1>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+    }
+}
+
+template <typename R, int STAGE>
+static void launch_update(cumc_ctx *ctx, const Fields<R> &F)
+{
+    const Lay &L = ctx->L;
+    dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
+    Tabs tb = tabs_of(ctx);
+    if (ctx->M.coordinate == CUMC_CYLINDRICAL)
+        hipLaunchKernelGGL((k_update<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev);
+    else
+        hipLaunchKernelGGL((k_update<R, STAGE, 1>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev);
+}
+
+template <typename R>
+static void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
+{
+    const Lay &L = ctx->L;
+    launch_fluxes<R>(ctx, F);
+    dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
+    hipLaunchKernelGGL((k_ecorn<R>), grid, dim3(256), 0, ctx->stream, L, F);
+    if (st == 1) launch_update<R, 1>(ctx, F);
+    else if (st == 2) launch_update<R, 2>(ctx, F);
+    else launch_update<R, 3>(ctx, F);
+    hipLaunchKernelGGL((k_c2p<0>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, ctx->F64.U1);
+    launch_ghost(ctx);
+}
+
+static void swap_u(cumc_ctx *ctx)
+{
+    std::swap(ctx->F64.U0, ctx->F64.U1);
+    ctx->F32.U0 = ctx->F64.U0;
+    ctx->F32.U1 = ctx->F64.U1;
+}
+
+// RUNGEKUTTA body on the stream, dt already in dt_dev
+static void enqueue_step(cumc_ctx *ctx)
+{
+    for (int st = 1; st <= 3; st++) {
+        if (ctx->cfg.precision == CUMC_MIXED) stage<float>(ctx, ctx->F32, st);
+        else stage<double>(ctx, ctx->F64, st);
+    }
+    swap_u(ctx);
+}
+
+extern "C" int cumc_initial_update(cumc_ctx *ctx)
+{
+    if (!ctx) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "cumc_set_grid must precede cumc_initial_update");
+    const Lay &L = ctx->L;
+    dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
+    hipLaunchKernelGGL((k_c2p<1>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, ctx->F64.U0);
+    launch_ghost(ctx);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CUMC_OK;
+}
+
+extern "C" int cumc_boundary(cumc_ctx *ctx)
+{
+    if (!ctx) return CUMC_EINVAL;
+    launch_ghost(ctx);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CUMC_OK;
+}
+
+static void enqueue_finddt(cumc_ctx *ctx)
+{
+    const Lay &L = ctx->L;
+    const double init = 1.0e10;
+    unsigned long long bits;
+    std::memcpy(&bits, &init, 8);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (unsigned long long *)ctx->dt_dev, bits, 1);
+    dim3 grid(nblk((long)L.px * L.ny, 256), L.nz);
+    Tabs tb = tabs_of(ctx);
+    if (ctx->M.coordinate == CUMC_CYLINDRICAL)
+        hipLaunchKernelGGL((k_finddt<2>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M,
+                           (unsigned long long *)ctx->dt_dev);
+    else
+        hipLaunchKernelGGL((k_finddt<1>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M,
+                           (unsigned long long *)ctx->dt_dev);
+}
+
+extern "C" int cumc_finddt(cumc_ctx *ctx, double *dt)
+{
+    if (!ctx || !dt) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    enqueue_finddt(ctx);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(dt, ctx->dt_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!std::isfinite(*dt) || !(*dt > 0.0)) return fail(ctx, CUMC_ENONFINITE, "finddt produced a non-finite step");
+    return CUMC_OK;
+}
+
+extern "C" int cumc_rungekutta(cumc_ctx *ctx, double dt)
+{
+    if (!ctx) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    if (!std::isfinite(dt) || !(dt > 0.0)) return fail(ctx, CUMC_ENONFINITE, "time step must be finite and positive");
+    HIPCHK(ctx, hipMemcpyAsync(ctx->dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    enqueue_step(ctx);
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    ctx->last_ms = ms;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_run(cumc_ctx *ctx, int max_steps, double total_time, double *time, int *steps_done, double *dts)
+{
+    if (!ctx || !time) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    double global_time = *time;
+    int n = 0;
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    while (global_time < total_time && n < max_steps) {
+        double dt;
+        int r = cumc_finddt(ctx, &dt);
+        if (r) { *time = global_time; if (steps_done) *steps_done = n; return r; }
+        if (global_time + dt > total_time) {
+            dt = total_time - global_time;
+            HIPCHK(ctx, hipMemcpyAsync(ctx->dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        }
+        enqueue_step(ctx);
+        HIPCHK(ctx, hipGetLastError());
+        if (dts) dts[n] = dt;
+        n++;
+        global_time = global_time + dt;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    ctx->last_ms = ms;
+    *time = global_time;
+    if (steps_done) *steps_done = n;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_find_divb(cumc_ctx *ctx, double *max_signed, double *max_abs)
+{
+    if (!ctx || !max_signed || !max_abs) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    const Lay &L = ctx->L;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->red, 0ull, 2);
+    dim3 grid(nblk((long)L.px * L.ny, 256), L.nz);
+    Tabs tb = tabs_of(ctx);
+    if (ctx->M.coordinate == CUMC_CYLINDRICAL)
+        hipLaunchKernelGGL((k_divb<2>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->red);
+    else
+        hipLaunchKernelGGL((k_divb<1>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->red);
+    double r[2];
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(r, ctx->red, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *max_signed = r[0];
+    *max_abs = r[1];
+    return CUMC_OK;
+}
+
+extern "C" int cumc_sweep_flux(cumc_ctx *ctx, int dir, double *out)
+{
+    if (!ctx || !out || dir < 1 || dir > 3) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    const Lay &L = ctx->L;
+    const long ncell = (long)(L.nx + 6) * (L.ny + 6) * (L.nz + 6);
+    double *tmp = nullptr;
+    HIPCHK(ctx, hipMalloc(&tmp, (size_t)ncell * 7 * sizeof(double)));
+    if (ctx->cfg.precision == CUMC_MIXED) {
+        launch_fluxes<float>(ctx, ctx->F32, dir);
+        hipLaunchKernelGGL((k_flux_to_aos<float>), dim3(nblk(ncell, 256)), dim3(256), 0, ctx->stream, L,
+                           ctx->F32.F[dir - 1], tmp);
+    } else {
+        launch_fluxes<double>(ctx, ctx->F64, dir);
+        hipLaunchKernelGGL((k_flux_to_aos<double>), dim3(nblk(ncell, 256)), dim3(256), 0, ctx->stream, L,
+                           ctx->F64.F[dir - 1], tmp);
+    }
+    hipError_t e = hipMemcpyAsync(out, tmp, (size_t)ncell * 7 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(tmp);
+    HIPCHK(ctx, e);
+    return CUMC_OK;
+}
+
+extern "C" int cumc_last_step_ms(cumc_ctx *ctx, double *ms)
+{
+    if (!ctx || !ms) return CUMC_EINVAL;
+    *ms = ctx->last_ms;
+    return CUMC_OK;
+}

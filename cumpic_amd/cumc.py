@@ -1,0 +1,164 @@
+"""ctypes binding of libcumc.so, with a host-side mirror of the reference driver's calls.
+
+The methods carry the reference names: ``initial_update`` (solver/initial.f90:83),
+``finddt`` (solver/rungekutta.f90:161), ``rungekutta`` (solver/rungekutta.f90:12),
+``find_divb`` (solver/mhd.f90:15) and ``run`` for the time loop of solver/main.f90:127-163.
+Arrays are the reference's ghost-inclusive Fortran arrays (C shape (nz+6, ny+6, nx+6, nv)).
+There is no CPU fallback: without the compiled library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .config import Config, CConfig, to_cconfig
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcumc.so")
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+_CTX = ctypes.c_void_p
+
+
+class CumcError(RuntimeError):
+    pass
+
+
+class CConfigP(ctypes.Structure):
+    _fields_ = CConfig._fields_ + [("precision", ctypes.c_int)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CumcError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "cumc_create": [ctypes.POINTER(CConfigP), ctypes.POINTER(_CTX)],
+            "cumc_destroy": [_CTX],
+            "cumc_set_grid": [_CTX, _D, _D, _D],
+            "cumc_upload": [_CTX, _D, _D],
+            "cumc_download": [_CTX, _D, _D, _D],
+            "cumc_initial_update": [_CTX],
+            "cumc_finddt": [_CTX, _D],
+            "cumc_rungekutta": [_CTX, ctypes.c_double],
+            "cumc_run": [_CTX, ctypes.c_int, ctypes.c_double, _D, ctypes.POINTER(ctypes.c_int), _D],
+            "cumc_find_divb": [_CTX, _D, _D],
+            "cumc_boundary": [_CTX],
+            "cumc_sweep_flux": [_CTX, ctypes.c_int, _D],
+            "cumc_last_step_ms": [_CTX, _D],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.cumc_error_string.restype = ctypes.c_char_p
+        L.cumc_error_string.argtypes = [ctypes.c_int]
+        L.cumc_last_error.restype = ctypes.c_char_p
+        L.cumc_last_error.argtypes = [_CTX]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous float64"
+    return a.ctypes.data_as(_D)
+
+
+class Solver:
+    """One subdomain of the CUMC3D step resident on the current GPU."""
+
+    def __init__(self, cfg: Config, xf, yf, zf):
+        self.cfg = cfg
+        c = CConfigP()
+        base = to_cconfig(cfg)
+        for name, _ in CConfig._fields_:
+            setattr(c, name, getattr(base, name))
+        c.precision = int(cfg.precision)
+        h = _CTX()
+        r = lib().cumc_create(ctypes.byref(c), ctypes.byref(h))
+        if r:
+            raise CumcError(f"cumc_create: {lib().cumc_error_string(r).decode()}")
+        self.h = h
+        self._check(lib().cumc_set_grid(self.h, *(_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (xf, yf, zf))),
+                    "cumc_set_grid")
+
+    @classmethod
+    def from_problem(cls, prob, precision: int | None = None):
+        cfg = prob.cfg
+        if precision is not None:
+            import dataclasses
+            cfg = dataclasses.replace(cfg, precision=precision)
+        s = cls(cfg, prob.xf, prob.yf, prob.zf)
+        s.upload(prob.prim, prob.eps)
+        return s
+
+    def _check(self, r, what):
+        if r:
+            msg = lib().cumc_last_error(self.h).decode() if self.h else ""
+            raise CumcError(f"{what}: {lib().cumc_error_string(r).decode()} {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cumc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, prim, eps):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        eps = np.ascontiguousarray(eps, dtype=np.float64)
+        if prim.shape != self.cfg.shape + (8,) or eps.shape != self.cfg.shape:
+            raise ValueError("prim/eps shapes do not match the configuration")
+        self._check(lib().cumc_upload(self.h, _p(prim), _p(eps)), "cumc_upload")
+
+    def download(self):
+        prim = np.empty(self.cfg.shape + (8,))
+        eps = np.empty(self.cfg.shape)
+        bcell = np.empty(self.cfg.shape + (3,))
+        self._check(lib().cumc_download(self.h, _p(prim), _p(eps), _p(bcell)), "cumc_download")
+        return {"prim": prim, "eps": eps, "bcell": bcell}
+
+    def initial_update(self):
+        self._check(lib().cumc_initial_update(self.h), "cumc_initial_update")
+
+    def boundary(self):
+        self._check(lib().cumc_boundary(self.h), "cumc_boundary")
+
+    def finddt(self) -> float:
+        d = ctypes.c_double()
+        self._check(lib().cumc_finddt(self.h, ctypes.byref(d)), "cumc_finddt")
+        return d.value
+
+    def rungekutta(self, dt: float):
+        self._check(lib().cumc_rungekutta(self.h, float(dt)), "cumc_rungekutta")
+
+    def run(self, max_steps: int, total_time: float = 1.0e300, t0: float = 0.0):
+        t = ctypes.c_double(t0)
+        n = ctypes.c_int(0)
+        dts = np.zeros(max(1, max_steps))
+        self._check(lib().cumc_run(self.h, int(max_steps), float(total_time), ctypes.byref(t), ctypes.byref(n),
+                                   _p(dts)), "cumc_run")
+        return dts[: n.value].copy(), t.value
+
+    def find_divb(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().cumc_find_divb(self.h, ctypes.byref(a), ctypes.byref(b)), "cumc_find_divb")
+        return a.value, b.value
+
+    def sweep_flux(self, direction: int):
+        out = np.empty(self.cfg.shape + (7,))
+        self._check(lib().cumc_sweep_flux(self.h, int(direction), _p(out)), "cumc_sweep_flux")
+        return out
+
+    def last_ms(self) -> float:
+        d = ctypes.c_double()
+        self._check(lib().cumc_last_step_ms(self.h, ctypes.byref(d)), "cumc_last_step_ms")
+        return d.value

@@ -1,0 +1,296 @@
+"""Seeded initial conditions for the benchmark and parity workloads.
+
+Each builder returns a :class:`Problem`: the run configuration, the face
+coordinates xF/yF/zF(-3:n+3) and the primitive state in the reference's
+ghost-inclusive array layout (``prim`` C-shape (nz+6, ny+6, nx+6, 8), which is
+the Fortran ``prim(1:8,-2:nx+3,-2:ny+3,-2:nz+3)``; ``eps`` (nz+6, ny+6, nx+6)).
+Face-centred B lives in prim[..., 5:8] on the reference's staggering
+(prim(ibx,j,k,l) is B_x on the face x_F(j)), as GET_MODEL fills it
+(model/riemann/src/model.f90:4-25, model/torus_cyn/src/model.f90:149-158).
+
+Seeds are ``numpy.random.default_rng(1000 + workload_index)``.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .config import (CARTESIAN, CYLINDRICAL, IBX, IBY, IBZ, IRHO, ITAU, IVX, IVY, IVZ, MODEL_RIEMANN,
+                     MODEL_TORUS_CYN, NGHOST, PPMC, Config, log_faces, periodic_angle_faces, uniform_faces)
+
+G = NGHOST
+
+
+def ix(j: int) -> int:
+    """Array index of cell (or face) j in a ghost-inclusive axis that starts at cell 1-NGHOST."""
+    return j + G - 1
+
+
+@dataclasses.dataclass
+class Problem:
+    cfg: Config
+    xf: np.ndarray
+    yf: np.ndarray
+    zf: np.ndarray
+    prim: np.ndarray
+    eps: np.ndarray
+    steps: int = 0
+    total_time: float = 1.0e300
+    name: str = ""
+
+
+def _alloc(cfg: Config):
+    prim = np.zeros(cfg.shape + (8,), dtype=np.float64)
+    eps = np.zeros(cfg.shape, dtype=np.float64)
+    return prim, eps
+
+
+def riemann_1d(n: int = 800, seed: int = 1000, gamma: float = 2.0, cfl: float = 0.8,
+               t_end: float = 0.1) -> Problem:
+    """1D MHD Riemann problem on [0,1] with seeded left/right states, outflow on all sides.
+
+    rho, p ~ U[0.1,1]; v ~ U[-0.5,0.5]^3; (By, Bz) ~ U[-1,1] with the sign flipped across
+    x = 0.5; uniform Bx ~ U[0.5,1].  ny = nz = 1 with unit widths so dx sets the CFL step.
+    Draw order: rho_L, rho_R, p_L, p_R, v_L(3), v_R(3), (By, Bz), Bx.
+    """
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.1, 1.0, 2)
+    p = rng.uniform(0.1, 1.0, 2)
+    vl = rng.uniform(-0.5, 0.5, 3)
+    vr = rng.uniform(-0.5, 0.5, 3)
+    byz = rng.uniform(-1.0, 1.0, 2)
+    bx = rng.uniform(0.5, 1.0)
+    cfg = Config(nx=n, ny=1, nz=1, coordinate=CARTESIAN, boundary_flag=(1, 1, 1, 1, 1, 1), recon=PPMC,
+                 model=MODEL_RIEMANN, ggas=gamma, cfl=cfl)
+    xf = uniform_faces(n, 0.0, 1.0)
+    yf = uniform_faces(1, 0.0, 1.0)
+    zf = uniform_faces(1, 0.0, 1.0)
+    prim, eps = _alloc(cfg)
+    xc = 0.5 * (xf[1:] + xf[:-1])  # centres of cells j = -2..n+3 (array index j+2)
+    left = xc < 0.5
+    # GET_MODEL fills [0,n]^3 (riemann/src/model.f90:15-21); ghosts are set by initial_update
+    sl = (slice(ix(0), ix(1) + 1), slice(ix(0), ix(1) + 1))
+    for side, mask in ((0, left), (1, ~left)):
+        cols = np.nonzero(mask)[0]
+        cols = cols[(cols >= ix(0)) & (cols <= ix(n))]
+        v = vl if side == 0 else vr
+        sgn = 1.0 if side == 0 else -1.0
+        for c in cols:
+            prim[sl + (c, IRHO)] = rho[side]
+            prim[sl + (c, IVX)] = v[0]
+            prim[sl + (c, IVY)] = v[1]
+            prim[sl + (c, IVZ)] = v[2]
+            prim[sl + (c, ITAU)] = p[side]
+            prim[sl + (c, IBX)] = bx
+            prim[sl + (c, IBY)] = sgn * byz[0]
+            prim[sl + (c, IBZ)] = sgn * byz[1]
+            eps[sl + (c,)] = p[side] / rho[side] / (gamma - 1.0)
+    return Problem(cfg, xf, yf, zf, prim, eps, steps=10 ** 9, total_time=t_end, name="riemann1d")
+
+
+def _spectral_field(shape, kmax: float, slope: float, rng, kz_zero: bool = False) -> np.ndarray:
+    """Real random field with Gaussian random-phase modes of amplitude |k|^slope on integer
+    wavevectors 1 <= |k| <= kmax (zero elsewhere), normalised to unit rms.  shape = (nz, ny, nx);
+    kz_zero keeps only k_z = 0 modes."""
+    nz, ny, nx = shape
+    km = int(math.floor(kmax))
+    nkz = nz // 2 + 1
+    coeff = np.zeros((ny, nx, nkz), dtype=np.complex128)
+    for iz in range(0, min(km, nkz - 1) + 1):
+        if kz_zero and iz != 0:
+            continue
+        for iy in range(-km, km + 1):
+            for ixx in range(-km, km + 1):
+                k2 = ixx * ixx + iy * iy + iz * iz
+                if k2 < 1 or k2 > kmax * kmax:
+                    continue
+                amp = math.sqrt(k2) ** slope
+                coeff[iy % ny, ixx % nx, iz] = amp * (rng.standard_normal() + 1j * rng.standard_normal())
+    f = np.fft.irfftn(coeff, s=(ny, nx, nz), axes=(0, 1, 2))  # [k, j, l]
+    f = np.ascontiguousarray(np.transpose(f, (2, 0, 1)))  # [l, k, j]
+    rms = math.sqrt(float(np.mean(f * f)))
+    return f / rms if rms > 0 else f
+
+
+def _curl_faces(ax, ay, az, dx, dy, dz):
+    """Face B from edge-centred A on a periodic staggered grid (arrays indexed [l, k, j], entry j
+    holding the right face of cell j).
+
+    A_x sits at (xc, yF(k), zF(l)), A_y at (xF(j), yc, zF(l)), A_z at (xF(j), yF(k), zc); B_x at
+    the face xF(j), B_y at yF(k), B_z at zF(l).  The area-weighted divergence of the result is
+    zero to round-off."""
+    def dm(a, axis):
+        return a - np.roll(a, 1, axis=axis)
+    bx = dm(az, 1) / dy - dm(ay, 0) / dz
+    by = dm(ax, 0) / dz - dm(az, 2) / dx
+    bz = dm(ay, 2) / dx - dm(ax, 1) / dy
+    return bx, by, bz
+
+
+def _fill_periodic_ghosts(arr, n3):
+    """Copy the interior [1,n] periodically into index 0 and the 3-deep ghost shells."""
+    for axis, n in enumerate(n3):
+        for g in range(1, G + 1):
+            lo = [slice(None)] * arr.ndim
+            src = [slice(None)] * arr.ndim
+            lo[axis], src[axis] = ix(1 - g), ix(n + 1 - g)
+            arr[tuple(lo)] = arr[tuple(src)]
+            lo[axis], src[axis] = ix(n + g), ix(g)
+            arr[tuple(lo)] = arr[tuple(src)]
+
+
+def spectral_box(n: int = 256, seed: int = 1001, gamma: float = 5.0 / 3.0, cfl: float = 0.7,
+                 v_rms: float = 0.1, db_rms: float = 0.1, ndim: int = 3, ny: int | None = None,
+                 steps: int = 200) -> Problem:
+    """Periodic box on [0,1]^d: rho = 1, mean B_z = 1, p = beta*B^2/2 = 0.5 (beta = 1).
+
+    v and the vector potential A carry Gaussian random-phase modes with |k|^-2 amplitude over
+    1 <= |k| <= 8.  v is scaled to rms |v| = v_rms; A is scaled so that rms |dB| = db_rms
+    (a literal rms(A) = 0.1 would give rms |dB| of order 2).  B = B_z z-hat + discrete curl A
+    on the reference's face staggering, so the area-weighted div B is zero to round-off.
+    ndim = 2 restricts the spectrum to k_z = 0 with nz = 1.
+    """
+    rng = np.random.default_rng(seed)
+    nx = n
+    ny = n if ny is None else ny
+    nz = n if ndim == 3 else 1
+    cfg = Config(nx=nx, ny=ny, nz=nz, coordinate=CARTESIAN, boundary_flag=(0, 0, 0, 0, 0, 0), recon=PPMC,
+                 model=MODEL_RIEMANN, ggas=gamma, cfl=cfl)
+    xf = uniform_faces(nx, 0.0, 1.0)
+    yf = uniform_faces(ny, 0.0, 1.0)
+    zf = uniform_faces(nz, 0.0, 1.0)
+    shp = (nz, ny, nx)
+    kz0 = ndim == 2
+    v = [_spectral_field(shp, 8.0, -2.0, rng, kz_zero=kz0) for _ in range(3)]
+    vr = math.sqrt(float(np.mean(v[0] ** 2 + v[1] ** 2 + v[2] ** 2)))
+    v = [c * (v_rms / vr) for c in v]
+    a = [_spectral_field(shp, 8.0, -2.0, rng, kz_zero=kz0) for _ in range(3)]
+    dx, dy, dz = 1.0 / nx, 1.0 / ny, 1.0 / nz
+    bx, by, bz = _curl_faces(a[0], a[1], a[2], dx, dy, dz)
+    br = math.sqrt(float(np.mean(bx ** 2 + by ** 2 + bz ** 2)))
+    s = db_rms / br
+    a = [c * s for c in a]
+    bx, by, bz = _curl_faces(a[0], a[1], a[2], dx, dy, dz)
+    bz = bz + 1.0
+    del a
+    prim, eps = _alloc(cfg)
+    p0 = 0.5
+    inner = (slice(ix(1), ix(nz) + 1), slice(ix(1), ix(ny) + 1), slice(ix(1), ix(nx) + 1))
+    prim[inner + (IRHO,)] = 1.0
+    prim[inner + (IVX,)] = v[0]
+    prim[inner + (IVY,)] = v[1]
+    prim[inner + (IVZ,)] = v[2]
+    prim[inner + (ITAU,)] = p0
+    prim[inner + (IBX,)] = bx
+    prim[inner + (IBY,)] = by
+    prim[inner + (IBZ,)] = bz
+    eps[inner] = p0 / 1.0 / (gamma - 1.0)
+    del v, bx, by, bz
+    _fill_periodic_ghosts(prim, shp)
+    _fill_periodic_ghosts(eps, shp)
+    return Problem(cfg, xf, yf, zf, prim, eps, steps=steps, name=f"spectral{ndim}d_{n}")
+
+
+def torus_cyn(nr: int = 256, nphi: int = 128, nzz: int = 256, seed: int = 1003, gamma: float = 5.0 / 3.0,
+              cfl: float = 0.7, beta: float = 100.0, rho_cut: float = 0.2, dp_frac: float = 0.01,
+              r0: float = 1.0, r1: float = 40.0, zext: float = 20.0, steps: int = 50) -> Problem:
+    """Constant-angular-momentum hydrostatic torus in a Paczynski-Wiita potential on a
+    cylindrical (R, phi, z) grid with log-spaced R faces xF(j) = r0 (r1/r0)^(j/nr).
+
+    Torus constants follow the torus_cyn plugin (model/torus_cyn/src/defs.f90:14-46: q = 2,
+    r_sh = 1, s_in = 3, s_max = 4.7, rho_max = 1, floor rho = 1e-4 rho_max) and its density
+    solution (model/torus_cyn/src/model.f90:71-121); eps_floor follows model.f90:246-247.
+    A_phi = max(rho - rho_cut*rho_max, 0) at cell centres is averaged to (R,z) corners and
+    curled onto faces (model/torus_cyn/src/model.f90:135-158); B is then scaled so that
+    max(p)/max(B_cell^2/2) = beta.  A seeded U[-1,1] relative pressure perturbation of size
+    dp_frac is applied where rho exceeds the floor.  Boundary flags (1,1,0,0,1,1).
+    """
+    rng = np.random.default_rng(seed)
+    q, r_sh, s_in, s_max, rho_max, rho_fac = 2.0, 1.0, 3.0, 4.7, 1.0, 1.0e-4
+    xf = log_faces(nr, r0, r1)
+    yf = periodic_angle_faces(nphi)
+    zf = uniform_faces(nzz, -zext, zext)
+
+    def phi_pw(r, z):
+        return -1.0 / (np.sqrt(r * r + z * z) - r_sh)
+
+    lkep2 = s_max ** (2.0 * q - 1.0) / (s_max - r_sh) ** 2
+    c_const = phi_pw(s_in, 0.0) - lkep2 * s_in ** (2.0 - 2.0 * q) / (2.0 - 2.0 * q)
+    h_max = -phi_pw(s_max, 0.0) + lkep2 * s_max ** (2.0 - 2.0 * q) / (2.0 - 2.0 * q) + c_const
+    k_poly = h_max * (gamma - 1.0) / gamma / rho_max ** (gamma - 1.0)
+    rho_a = rho_fac * rho_max
+    rho_floor = rho_a
+    eps_floor = k_poly * rho_floor ** (gamma - 1.0) / (gamma - 1.0)
+
+    cfg = Config(nx=nr, ny=nphi, nz=nzz, coordinate=CYLINDRICAL, boundary_flag=(1, 1, 0, 0, 1, 1), recon=PPMC,
+                 model=MODEL_TORUS_CYN, ggas=gamma, cfl=cfl, rho_floor=rho_floor, eps_floor=eps_floor, r_sh=r_sh)
+    prim, eps = _alloc(cfg)
+    xc = 0.5 * (xf[1:] + xf[:-1])  # cell j at index ix(j)
+    zc = 0.5 * (zf[1:] + zf[:-1])
+    dxa = np.diff(xf)
+    dza = np.diff(zf)
+    R, Z = np.meshgrid(xc, zc, indexing="xy")  # [l, j] over all cells
+    h = -phi_pw(R, Z) + lkep2 * R ** (2.0 - 2.0 * q) / (2.0 - 2.0 * q) + c_const
+    h = np.maximum(h, 0.0)
+    rho_loc = (h * (gamma - 1.0) / gamma / k_poly) ** (1.0 / (gamma - 1.0))
+    inside = (rho_loc > rho_a) & (R >= s_in)
+    rho2 = np.where(inside, rho_loc, rho_a)
+    vphi2 = np.where(inside, np.sqrt(lkep2) * R ** (-q) * R, 0.0)
+    p2 = k_poly * rho2 ** gamma
+
+    # magnetic field: corners A(j,l) at (xF(j), zF(l)) from the four surrounding cell centres
+    aphi = np.maximum(rho2 - rho_cut * rho_max, 0.0) / rho_max
+    ac = 0.25 * (aphi[:-1, :-1] + aphi[:-1, 1:] + aphi[1:, :-1] + aphi[1:, 1:])  # ac[ix(l), ix(j)]
+    J = np.arange(0, nr + 1)
+    L = np.arange(0, nzz + 1)
+    LL, JJ = np.meshgrid(L, J, indexing="ij")
+    A = lambda j, l: ac[ix(l), ix(j)]  # noqa: E731
+    xF = lambda j: xf[j + G]  # noqa: E731
+    bx_face = np.zeros((nzz + 2 * G, nr + 2 * G))
+    bz_face = np.zeros((nzz + 2 * G, nr + 2 * G))
+    bx_face[ix(LL), ix(JJ)] = -(A(JJ, LL) - A(JJ, LL - 1)) / dza[ix(LL)]
+    bz_face[ix(LL), ix(JJ)] = (xF(JJ) * A(JJ, LL) - xF(JJ - 1) * A(JJ - 1, LL)) / (xc[ix(JJ)] * dxa[ix(JJ)])
+    il = slice(ix(1), ix(nzz) + 1)
+    ij = slice(ix(1), ix(nr) + 1)
+    ijm = slice(ix(0), ix(nr - 1) + 1)
+    ilm = slice(ix(0), ix(nzz - 1) + 1)
+    bcx = 0.5 * (bx_face[il, ij] + bx_face[il, ijm])
+    bcz = 0.5 * (bz_face[il, ij] + bz_face[ilm, ij])
+    pmag_max = float(np.max(0.5 * (bcx ** 2 + bcz ** 2)))
+    pgas_max = float(np.max(p2[il, ij]))
+    norm = math.sqrt(pgas_max / pmag_max / beta) if pmag_max > 0 else 0.0
+
+    pert = rng.uniform(-1.0, 1.0, size=(nzz, nphi, nr))
+    rho3 = np.broadcast_to(rho2[il, None, ij], (nzz, nphi, nr))
+    p3 = p2[il, None, ij] * (1.0 + dp_frac * pert * (rho3 > rho_floor))
+    inner = (il, slice(ix(1), ix(nphi) + 1), ij)
+    prim[inner + (IRHO,)] = rho3
+    prim[inner + (IVY,)] = np.broadcast_to(vphi2[il, None, ij], (nzz, nphi, nr))
+    prim[inner + (ITAU,)] = p3
+    eps[inner] = p3 / rho3 / (gamma - 1.0)
+    prim[:, :, :, IBX] = (bx_face * norm)[:, None, :]
+    prim[:, :, :, IBZ] = (bz_face * norm)[:, None, :]
+    return Problem(cfg, xf, yf, zf, prim, eps, steps=steps, name="torus_cyn")
+
+
+def small_periodic(n: int = 16, seed: int = 1005, ndim: int = 3) -> Problem:
+    """Small smooth random periodic state for parity tests (same construction as spectral_box)."""
+    return spectral_box(n=n, seed=seed, ndim=ndim, v_rms=0.3, db_rms=0.3, steps=4)
+
+
+def randomize(prob: Problem, seed: int, amp: float = 0.2) -> Problem:
+    """Add seeded cell-to-cell noise to rho, p and v on [0,n]^3 so that limiter and Riemann
+    branches are exercised; eps follows from the EOS."""
+    rng = np.random.default_rng(seed)
+    p = prob.prim
+    c = prob.cfg
+    inner = (slice(ix(0), ix(c.nz) + 1), slice(ix(0), ix(c.ny) + 1), slice(ix(0), ix(c.nx) + 1))
+    shp = p[inner + (0,)].shape
+    p[inner + (IRHO,)] *= 1.0 + amp * rng.uniform(-1, 1, shp)
+    p[inner + (ITAU,)] *= 1.0 + amp * rng.uniform(-1, 1, shp)
+    for v in (IVX, IVY, IVZ):
+        p[inner + (v,)] += amp * rng.uniform(-1, 1, shp)
+    prob.eps[inner] = p[inner + (ITAU,)] / p[inner + (IRHO,)] / (c.ggas - 1.0)
+    return prob

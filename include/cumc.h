@@ -1,0 +1,104 @@
+/*
+ * cumc.h -- C ABI of the MI355X CUMC3D MHD time step (libcumc.so).
+ *
+ * The reference drives its step through argument-free Fortran subroutines that
+ * operate on MODULE DEFINITION globals.  Each entry point below replaces one of
+ * them; the comment names the reference routine (file:line).  All arrays crossing
+ * this boundary are caller-owned host arrays in the reference's Fortran layout:
+ *
+ *   prim (1:8, -2:nx+3, -2:ny+3, -2:nz+3)   rho, vx, vy, vz, p, Bx_face, By_face, Bz_face
+ *   eps  (-2:nx+3, -2:ny+3, -2:nz+3)
+ *   bcell(1:3, -2:nx+3, -2:ny+3, -2:nz+3)
+ *   xf(-3:nx+3), yf(-3:ny+3), zf(-3:nz+3)   face coordinates
+ *
+ * (first index fastest).  Device storage is private.  Every call returns 0 or a
+ * CUMC_E* code and is synchronous on return; one HIP stream per context.
+ */
+#ifndef CUMC_H
+#define CUMC_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CUMC_OK = 0,
+    CUMC_EINVAL = 1,      /* bad configuration or argument */
+    CUMC_EHIP = 2,        /* HIP runtime error */
+    CUMC_ENONFINITE = 3,  /* non-finite or non-positive time step */
+    CUMC_ENOMEM = 4,      /* device allocation failed */
+    CUMC_ESTATE = 5       /* call out of order (e.g. no grid set) */
+};
+
+enum { CUMC_CARTESIAN = 1, CUMC_CYLINDRICAL = 2 };      /* solver/definition.f90:65-66 */
+enum { CUMC_TVDMM = 1, CUMC_PPMC = 4 };                  /* solver/definition.f90:158,167 */
+enum { CUMC_MODEL_RIEMANN = 0, CUMC_MODEL_TORUS_CYN = 1 };
+enum { CUMC_FP64 = 0, CUMC_MIXED = 1 };
+
+/* Runtime form of the compile-time PARAMETERs of solver/definition.f90:29-185 and the
+ * model plugin's defs.f90 (ggas, rho_floor, eps_floor, r_sh). */
+typedef struct cumc_config {
+    int nx, ny, nz;               /* interior cells of this subdomain (definition.f90:110-112) */
+    int coordinate;               /* CUMC_CARTESIAN | CUMC_CYLINDRICAL (definition.f90:72) */
+    int boundary_flag[6];         /* x-in, x-out, y-in, y-out, z-in, z-out (definition.f90:87) */
+    int recon;                    /* CUMC_PPMC | CUMC_TVDMM (definition.f90:173) */
+    int model;                    /* which model plugin's in-loop hooks to run */
+    int geom_source_repeat;       /* 8 reproduces spatial.f90:86-97; 1 adds the source once */
+    double ggas;                  /* adiabatic index */
+    double cfl;                   /* definition.f90:116 */
+    double rho_floor, eps_floor;  /* torus_cyn floors (model.f90:246-247) */
+    double r_sh;                  /* torus_cyn Schwarzschild radius (defs.f90:23) */
+    int precision;                /* CUMC_FP64 | CUMC_MIXED */
+} cumc_config;
+
+typedef struct cumc_ctx cumc_ctx;
+
+/* BUILD_HYDRO (solver/setup.f90:249-312) + SETUP_EQNS bfac tables (setup.f90:121-230) +
+ * device residency of POPULATE_DEVICE (solver/openacc.f90:28-59). */
+int cumc_create(const cumc_config *cfg, cumc_ctx **out);
+int cumc_destroy(cumc_ctx *ctx);
+
+/* GETGRID / CUSTOM_GRID result (solver/coords.f90:11-41): face coordinates incl. ghosts. */
+int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, const double *zf);
+
+/* POPULATE_DEVICE copy-in of prim and eps (solver/openacc.f90:35-50). */
+int cumc_upload(cumc_ctx *ctx, const double *prim, const double *eps);
+
+/* !$ACC UPDATE HOST(prim, eps) of print_hydroprofile (solver/dataio.f90:24); bcell may be NULL. */
+int cumc_download(cumc_ctx *ctx, double *prim, double *eps, double *bcell);
+
+/* initial_update (solver/initial.f90:83-140): CUSTOM_CHECKRHO, FROMRVETOU, BOUNDARY, UPDATE(0). */
+int cumc_initial_update(cumc_ctx *ctx);
+
+/* finddt (solver/rungekutta.f90:161-265): CFL time step of the current state. */
+int cumc_finddt(cumc_ctx *ctx, double *dt);
+
+/* RUNGEKUTTA (solver/rungekutta.f90:12-148): one SSP-RK3 step of size dt. */
+int cumc_rungekutta(cumc_ctx *ctx, double dt);
+
+/* The time loop of solver/main.f90:127-163 (finddt, clamp to total_time, RUNGEKUTTA) with a
+ * step cap; *time is read and advanced; dts (may be NULL) receives max_steps entries. */
+int cumc_run(cumc_ctx *ctx, int max_steps, double total_time, double *time, int *steps_done, double *dts);
+
+/* find_divb (solver/mhd.f90:15-66): reference signed maximum and the maximum |div B|. */
+int cumc_find_divb(cumc_ctx *ctx, double *max_signed, double *max_abs);
+
+/* BOUNDARY (solver/boundary.f90:5-13) followed by UPDATE's EOS on the ghost shell. */
+int cumc_boundary(cumc_ctx *ctx);
+
+/* Face fluxes of one sweep for the current state (GET_FLUXES reconstruction + RIEMANN,
+ * solver/spatial.f90:396-433): out has 7 ghost-inclusive cell arrays (rho, mx, my, mz, E and the
+ * two transverse B fluxes in x,y,z order), entry (j,k,l) holding the face at the high side of
+ * cell j (dir 1), k (dir 2) or l (dir 3). */
+int cumc_sweep_flux(cumc_ctx *ctx, int dir, double *out);
+
+/* Time of the last cumc_rungekutta / cumc_run measured with HIP events on the context stream,
+ * and the per-kernel share of it (names and average microseconds), for profiling. */
+int cumc_last_step_ms(cumc_ctx *ctx, double *ms);
+
+const char *cumc_error_string(int code);
+const char *cumc_last_error(cumc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
