@@ -38,3 +38,24 @@ def flang() -> str | None:
         if c and os.path.exists(c):
             return c
     return None
+
+
+FHOST_DIR = os.path.join(PKG, "fhost")
+FHOST_EXE = os.path.join(FHOST_DIR, "cumc3d")
+
+
+def build_fortran_host() -> str | None:
+    """Fortran driver over the C ABI (flang); returns the executable path or None without flang."""
+    fc = flang()
+    if fc is None:
+        return None
+    srcs = [os.path.join(FHOST_DIR, f) for f in ("cumc_iface.f90", "model_file.f90", "cumc3d.f90")]
+    mod = os.path.join(FHOST_DIR, "_mod")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [fc, "-O2", "-J", mod, *srcs, "-L", PKG, "-lcumc", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/..",
+           "-o", FHOST_EXE + ".tmp"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stdout)
+    os.replace(FHOST_EXE + ".tmp", FHOST_EXE)
+    return FHOST_EXE

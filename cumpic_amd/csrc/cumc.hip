@@ -33,7 +33,46 @@ struct cumc_ctx {
     bool grid_set = false;
     double last_ms = 0.0;
     std::string err;
+    // per-kernel-class HIP event timing (cumc_profile / cumc_kernel_times)
+    bool prof = false;
+    std::vector<hipEvent_t> evpool;
+    std::vector<int> evcls;
+    size_t evn = 0;
+    double kc_ms[16] = {0};
+    long kc_count[16] = {0};
 };
+
+enum { KC_FLUX_X, KC_FLUX_Y, KC_FLUX_Z, KC_ECORN, KC_UPDATE, KC_C2P, KC_GHOST, KC_FINDDT, KC_N };
+static const char *kc_names[KC_N] = {"flux_x", "flux_y", "flux_z", "ecorn", "update", "c2p", "ghost", "finddt"};
+
+// record the start (cls >= 0) or end (cls < 0) of a kernel class on the context stream
+static void mark(cumc_ctx *ctx, int cls)
+{
+    if (!ctx->prof) return;
+    if (ctx->evn == ctx->evpool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        ctx->evpool.push_back(e);
+        ctx->evcls.push_back(0);
+    }
+    ctx->evcls[ctx->evn] = cls;
+    hipEventRecord(ctx->evpool[ctx->evn], ctx->stream);
+    ctx->evn++;
+}
+
+// fold recorded event pairs into the per-class totals (stream must be idle)
+static void collect(cumc_ctx *ctx)
+{
+    for (size_t i = 0; i + 1 < ctx->evn; i += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->evpool[i], ctx->evpool[i + 1]) == hipSuccess) {
+            const int c = ctx->evcls[i];
+            ctx->kc_ms[c] += ms;
+            ctx->kc_count[c] += 1;
+        }
+    }
+    ctx->evn = 0;
+}
 
 static const char *g_err_names[] = {"ok", "invalid argument", "HIP error", "non-finite time step",
                                     "out of device memory", "call out of order"};
@@ -180,6 +219,7 @@ extern "C" int cumc_destroy(cumc_ctx *ctx)
     hipFree(ctx->dt_dev);
     hipFree(ctx->red);
     for (int a = 0; a < 3; a++) hipFree(ctx->tab[a]);
+    for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -324,25 +364,31 @@ static void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
     const Model &M = ctx->M;
     const bool ppm = M.recon == CUMC_PPMC;
     if (!only_dir || only_dir == 1) {
+        mark(ctx, KC_FLUX_X);
         const long waves = ((long)(L.ny + 2) * L.px + XS_FACES - 1) / XS_FACES;
         dim3 grid(nblk(waves * 64, 256), L.nz + 2);
         if (ppm) hipLaunchKernelGGL((k_flux_x<R, 4>), grid, dim3(256), 0, s, L, F, tb, M);
         else hipLaunchKernelGGL((k_flux_x<R, 1>), grid, dim3(256), 0, s, L, F, tb, M);
+        mark(ctx, -1);
     }
     const int seg = 48;
     if (!only_dir || only_dir == 2) {
+        mark(ctx, KC_FLUX_Y);
         const int nseg = (L.ny + 1 + seg - 1) / seg;
         const int sl = (L.ny + 1 + nseg - 1) / nseg;
         dim3 grid(nblk((long)L.px * (L.nz + 2), 128), nseg);
         if (ppm) hipLaunchKernelGGL((k_flux_walk<R, 2, 4>), grid, dim3(128), 0, s, L, F, tb, M, sl);
         else hipLaunchKernelGGL((k_flux_walk<R, 2, 1>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+        mark(ctx, -1);
     }
     if (!only_dir || only_dir == 3) {
+        mark(ctx, KC_FLUX_Z);
         const int nseg = (L.nz + 1 + seg - 1) / seg;
         const int sl = (L.nz + 1 + nseg - 1) / nseg;
         dim3 grid(nblk((long)L.px * (L.ny + 2), 128), nseg);
         if (ppm) hipLaunchKernelGGL((k_flux_walk<R, 3, 4>), grid, dim3(128), 0, s, L, F, tb, M, sl);
         else hipLaunchKernelGGL((k_flux_walk<R, 3, 1>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+        mark(ctx, -1);
     }
 }
 
@@ -364,12 +410,20 @@ static void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
     const Lay &L = ctx->L;
     launch_fluxes<R>(ctx, F);
     dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
+    mark(ctx, KC_ECORN);
     hipLaunchKernelGGL((k_ecorn<R>), grid, dim3(256), 0, ctx->stream, L, F);
+    mark(ctx, -1);
+    mark(ctx, KC_UPDATE);
     if (st == 1) launch_update<R, 1>(ctx, F);
     else if (st == 2) launch_update<R, 2>(ctx, F);
     else launch_update<R, 3>(ctx, F);
+    mark(ctx, -1);
+    mark(ctx, KC_C2P);
     hipLaunchKernelGGL((k_c2p<0>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, ctx->F64.U1);
+    mark(ctx, -1);
+    mark(ctx, KC_GHOST);
     launch_ghost(ctx);
+    mark(ctx, -1);
 }
 
 static void swap_u(cumc_ctx *ctx)
@@ -432,10 +486,13 @@ extern "C" int cumc_finddt(cumc_ctx *ctx, double *dt)
 {
     if (!ctx || !dt) return CUMC_EINVAL;
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    mark(ctx, KC_FINDDT);
     enqueue_finddt(ctx);
+    mark(ctx, -1);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(dt, ctx->dt_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    collect(ctx);
     if (!std::isfinite(*dt) || !(*dt > 0.0)) return fail(ctx, CUMC_ENONFINITE, "finddt produced a non-finite step");
     return CUMC_OK;
 }
@@ -538,4 +595,30 @@ extern "C" int cumc_last_step_ms(cumc_ctx *ctx, double *ms)
     if (!ctx || !ms) return CUMC_EINVAL;
     *ms = ctx->last_ms;
     return CUMC_OK;
+}
+
+extern "C" int cumc_profile(cumc_ctx *ctx, int on)
+{
+    if (!ctx) return CUMC_EINVAL;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->evn = 0;
+    for (int c = 0; c < KC_N; c++) {
+        ctx->kc_ms[c] = 0.0;
+        ctx->kc_count[c] = 0;
+    }
+    ctx->prof = on != 0;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_kernel_times(cumc_ctx *ctx, int n, double *total_ms, long *launches, const char **names)
+{
+    if (!ctx || n < 0) return CUMC_EINVAL;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    collect(ctx);
+    for (int c = 0; c < n && c < KC_N; c++) {
+        if (total_ms) total_ms[c] = ctx->kc_ms[c];
+        if (launches) launches[c] = ctx->kc_count[c];
+        if (names) names[c] = kc_names[c];
+    }
+    return KC_N;
 }

@@ -50,6 +50,9 @@ def lib():
             "cumc_boundary": [_CTX],
             "cumc_sweep_flux": [_CTX, ctypes.c_int, _D],
             "cumc_last_step_ms": [_CTX, _D],
+            "cumc_profile": [_CTX, ctypes.c_int],
+            "cumc_kernel_times": [_CTX, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_long),
+                                  ctypes.POINTER(ctypes.c_char_p)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -157,6 +160,20 @@ class Solver:
         out = np.empty(self.cfg.shape + (7,))
         self._check(lib().cumc_sweep_flux(self.h, int(direction), _p(out)), "cumc_sweep_flux")
         return out
+
+    def profile(self, on: bool = True):
+        self._check(lib().cumc_profile(self.h, 1 if on else 0), "cumc_profile")
+
+    def kernel_times(self):
+        """{class: (total_ms, launches)} of the HIP-event timing since profile(True)."""
+        n = 16
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_long * n)()
+        names = (ctypes.c_char_p * n)()
+        k = lib().cumc_kernel_times(self.h, n, ms, cnt, names)
+        if k < 0:
+            raise CumcError("cumc_kernel_times failed")
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
 
     def last_ms(self) -> float:
         d = ctypes.c_double()

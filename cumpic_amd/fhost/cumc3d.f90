@@ -1,0 +1,84 @@
+!-----------------------------------------------------------------------------------------------
+! cumc3d.f90 -- Fortran host driver of the MI355X CUMC3D step.
+!
+! Same shape as the reference driver (solver/main.f90): set up the model, initial_update, then
+! DO WHILE (global_time < total_time): finddt, clamp dt to the end time, Rungekutta, advance,
+! print "n_step, dt, global_time"; finally report the divergence of B.  The hot path runs in
+! libcumc.so through the ISO_C_BINDING interface of cumc_iface.f90.
+!
+! Usage: cumc3d <run file> <output file>
+! Output (stream): int32 n_step ; float64 global_time, divb_signed, divb_abs ;
+!                  prim(8,nx+6,ny+6,nz+6) ; eps(nx+6,ny+6,nz+6) ; dt(n_step)
+!-----------------------------------------------------------------------------------------------
+PROGRAM CUMC3D
+USE ISO_C_BINDING
+USE CUMC_IFACE
+USE MODEL_FILE
+IMPLICIT NONE
+
+TYPE(cumc_config) :: cfg
+TYPE(C_PTR) :: ctx
+REAL(C_DOUBLE), ALLOCATABLE :: xf(:), yf(:), zf(:), prim(:,:,:,:), eps(:,:,:), bcell(:,:,:,:)
+REAL(C_DOUBLE), ALLOCATABLE :: dts(:)
+REAL(C_DOUBLE) :: dt, global_time, total_time, maxs, maxa
+INTEGER :: max_steps, n_step, u
+INTEGER(C_INT) :: ierr
+CHARACTER(LEN=1024) :: infile, outfile
+
+CALL GET_COMMAND_ARGUMENT(1, infile)
+CALL GET_COMMAND_ARGUMENT(2, outfile)
+IF (LEN_TRIM(infile) == 0 .OR. LEN_TRIM(outfile) == 0) THEN
+  WRITE(*,*) 'usage: cumc3d <run file> <output file>'
+  STOP 2
+END IF
+
+! setup: model, grid, device residency (main.f90:57-99)
+CALL GET_MODEL(TRIM(infile), cfg, xf, yf, zf, prim, eps, max_steps, total_time)
+ctx = C_NULL_PTR
+ierr = cumc_create(cfg, ctx)
+CALL CUMC_CHECK(ierr, 'cumc_create', ctx)
+CALL CUMC_CHECK(cumc_set_grid(ctx, xf, yf, zf), 'cumc_set_grid', ctx)
+CALL CUMC_CHECK(cumc_upload(ctx, prim, eps), 'cumc_upload', ctx)
+CALL CUMC_CHECK(cumc_initial_update(ctx), 'cumc_initial_update', ctx)
+ALLOCATE(dts(MAX(max_steps, 1)))
+
+WRITE(*,*) 'We now start the time evolution'
+WRITE(*,*) '------------------------------------'
+WRITE(*,*) 'iteration, timestep, simulation time'
+WRITE(*,*) '------------------------------------'
+
+! time loop (main.f90:127-163) with a step cap
+global_time = 0.0D0
+n_step = 0
+DO WHILE (global_time < total_time .AND. n_step < max_steps)
+  CALL CUMC_CHECK(cumc_finddt(ctx, dt), 'cumc_finddt', ctx)
+  IF (global_time + dt > total_time) THEN
+    dt = total_time - global_time
+  END IF
+  CALL CUMC_CHECK(cumc_rungekutta(ctx, dt), 'cumc_rungekutta', ctx)
+  n_step = n_step + 1
+  global_time = global_time + dt
+  dts(n_step) = dt
+  WRITE(*,*) n_step, dt, global_time
+END DO
+
+ALLOCATE(bcell(1:3, -2:cfg%nx+3, -2:cfg%ny+3, -2:cfg%nz+3))
+CALL CUMC_CHECK(cumc_download(ctx, prim, eps, bcell), 'cumc_download', ctx)
+CALL CUMC_CHECK(cumc_find_divb(ctx, maxs, maxa), 'cumc_find_divb', ctx)
+WRITE(*,*)
+WRITE(*,*) 'Maximum divergence B', maxs, ' max |div B|', maxa
+WRITE(*,*)
+
+OPEN(NEWUNIT=u, FILE=TRIM(outfile), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', ACTION='WRITE')
+WRITE(u) INT(n_step, C_INT)
+WRITE(u) global_time, maxs, maxa
+WRITE(u) prim
+WRITE(u) eps
+IF (n_step > 0) WRITE(u) dts(1:n_step)
+CLOSE(u)
+
+ierr = cumc_destroy(ctx)
+WRITE(*,*) '----------------'
+WRITE(*,*) '-Simulation End-'
+WRITE(*,*) '----------------'
+END PROGRAM CUMC3D

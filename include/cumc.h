@@ -91,9 +91,16 @@ int cumc_boundary(cumc_ctx *ctx);
  * cell j (dir 1), k (dir 2) or l (dir 3). */
 int cumc_sweep_flux(cumc_ctx *ctx, int dir, double *out);
 
-/* Time of the last cumc_rungekutta / cumc_run measured with HIP events on the context stream,
- * and the per-kernel share of it (names and average microseconds), for profiling. */
+/* Wall time of the last cumc_rungekutta / cumc_run, measured with HIP events on the context
+ * stream. */
 int cumc_last_step_ms(cumc_ctx *ctx, double *ms);
+
+/* Per-kernel-class timing with HIP events recorded on the context stream around every launch
+ * group (no reference counterpart; the reference has no timers).  cumc_profile(ctx, 1) clears
+ * and enables it; cumc_kernel_times fills up to n entries of total milliseconds, launch counts
+ * and class names, and returns the number of classes. */
+int cumc_profile(cumc_ctx *ctx, int on);
+int cumc_kernel_times(cumc_ctx *ctx, int n, double *total_ms, long *launches, const char **names);
 
 const char *cumc_error_string(int code);
 const char *cumc_last_error(cumc_ctx *ctx);

@@ -1,0 +1,59 @@
+"""CPU-side checks of the boundary: libcumc.so loads and exports every entry point that
+include/cumc.h declares, the ctypes config mirrors the C struct, and the Fortran host links."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from cumpic_amd import build
+from cumpic_amd.config import CConfig
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"\b(cumc_[a-z_0-9]+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(build.LIB):
+        build.build_library()
+    lib = ctypes.CDLL(build.LIB)
+    names = declared_symbols()
+    assert "cumc_rungekutta" in names and "cumc_finddt" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_config_struct_layout_matches_header():
+    from cumpic_amd.cumc import CConfigP
+    # 3+1+6+3 ints (52 B) + pad + 5 doubles + 1 int, padded to 8
+    assert ctypes.sizeof(CConfigP) == 104
+    assert CConfigP.ggas.offset == 56 and CConfigP.precision.offset == 96
+    assert ctypes.sizeof(CConfig) == 96
+
+
+def test_error_strings_without_gpu():
+    lib = ctypes.CDLL(build.LIB)
+    lib.cumc_error_string.restype = ctypes.c_char_p
+    assert lib.cumc_error_string(0) == b"ok"
+    assert lib.cumc_error_string(3) == b"non-finite time step"
+
+
+def test_fortran_host_links():
+    if build.flang() is None:
+        pytest.skip("no Fortran compiler in this image")
+    exe = build.build_fortran_host()
+    assert os.path.exists(exe)
+    out = subprocess.run(["nm", "-D", "--undefined-only", exe], stdout=subprocess.PIPE, text=True).stdout
+    for sym in ("cumc_create", "cumc_finddt", "cumc_rungekutta", "cumc_initial_update", "cumc_download"):
+        assert sym in out
