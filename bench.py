@@ -29,8 +29,9 @@ BYTES_PER_ZONE_CYCLE = 512.0  # 64 FP64 state words per RK3 step (read U0 x3, st
 KERNEL_BYTES = {
     "flux_x": 15 * 8, "flux_y": 15 * 8, "flux_z": 15 * 8,  # 7 stencil vars + normal B in, 7 fluxes out
     "ecorn": 15 * 8,                                        # 6 B-fluxes + v + cell B in, 3 EMFs out
-    "update": 42 * 8,                                       # 15 fluxes + 3 EMFs + U0 + U1 in, U1 out
-    "c2p": 25 * 8,                                          # U1 in; prim, eps, bcell, U1 hydro out
+    # 15 fluxes + 3 EMFs + U0 (+ Uin in stages 2, 3) in; prim 8, eps, bcell 3, Uout 8 out:
+    # (46 + 54 + 54) / 3 words averaged over the three stage launches
+    "tail": (46 + 54 + 54) / 3 * 8,
 }
 
 
@@ -45,8 +46,7 @@ def kernel_units(name, n):
         "flux_y": (nx + 2) * (ny + 1) * (nz + 2),
         "flux_z": (nx + 2) * (ny + 2) * (nz + 1),
         "ecorn": (nx + 1) * (ny + 1) * (nz + 1),
-        "update": (nx + 1) * (ny + 1) * (nz + 1),
-        "c2p": (nx + 1) * (ny + 1) * (nz + 1),
+        "tail": (nx + 1) * (ny + 1) * (nz + 1),
     }.get(name)
 
 

@@ -28,8 +28,10 @@ struct cumc_ctx {
     Fields<double> F64;
     Fields<float> F32;
     double *tab[3] = {nullptr, nullptr, nullptr};
-    double *dt_dev = nullptr;
+    double *dt_dev = nullptr;           // [0] dt of the current step, [1] dt found by stage 3
     unsigned long long *red = nullptr;  // reduction slots
+    double *Ub = nullptr;               // third conserved buffer (stage 2 output)
+    bool dt_valid = false;              // dt_dev[1] holds finddt of the current state
     bool grid_set = false;
     double last_ms = 0.0;
     std::string err;
@@ -42,8 +44,8 @@ struct cumc_ctx {
     long kc_count[16] = {0};
 };
 
-enum { KC_FLUX_X, KC_FLUX_Y, KC_FLUX_Z, KC_ECORN, KC_UPDATE, KC_C2P, KC_GHOST, KC_FINDDT, KC_N };
-static const char *kc_names[KC_N] = {"flux_x", "flux_y", "flux_z", "ecorn", "update", "c2p", "ghost", "finddt"};
+enum { KC_FLUX_X, KC_FLUX_Y, KC_FLUX_Z, KC_ECORN, KC_TAIL, KC_GHOST, KC_FINDDT, KC_N };
+static const char *kc_names[KC_N] = {"flux_x", "flux_y", "flux_z", "ecorn", "tail", "ghost", "finddt"};
 
 // record the start (cls >= 0) or end (cls < 0) of a kernel class on the context stream
 static void mark(cumc_ctx *ctx, int cls)
@@ -169,8 +171,8 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     M.eps_floor = c.eps_floor;
     M.r_sh = c.r_sh;
 
-    // FP64 state: P 8 + EPS 1 + BC 3 + U0 8 + U1 8 = 28 planes ; fluxes 21 + EMF 3 planes in R
-    const size_t nplanes = 28;
+    // FP64 state: P 8 + EPS 1 + BC 3 + U0 8 + U1 8 + Ub 8 = 36 planes ; fluxes 21 + EMF 3 planes in R
+    const size_t nplanes = 36;
     ctx->pool_bytes = nplanes * (size_t)L.ps * sizeof(double);
     const size_t rsz = c.precision == CUMC_MIXED ? sizeof(float) : sizeof(double);
     const size_t fbytes = 24 * (size_t)L.ps * rsz;
@@ -197,6 +199,7 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     F.BC = take(3);
     F.U0 = take(8);
     F.U1 = take(8);
+    ctx->Ub = take(8);
     Fields<float> &G = ctx->F32;
     G.P = F.P; G.EPS = F.EPS; G.BC = F.BC; G.U0 = F.U0; G.U1 = F.U1;
     for (int d = 0; d < 3; d++) {
@@ -319,6 +322,7 @@ static int conv_out(cumc_ctx *ctx, const double *src, int nv, double *host)
 extern "C" int cumc_upload(cumc_ctx *ctx, const double *prim, const double *eps)
 {
     if (!ctx || !prim || !eps) return CUMC_EINVAL;
+    ctx->dt_valid = false;
     int r = conv_in(ctx, prim, 8, ctx->F64.P);
     if (r) return r;
     return conv_in(ctx, eps, 1, ctx->F64.EPS);
@@ -393,33 +397,43 @@ static void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
 }
 
 template <typename R, int STAGE>
-static void launch_update(cumc_ctx *ctx, const Fields<R> &F)
+static void launch_tail(cumc_ctx *ctx, const Fields<R> &F, const double *Uin, double *Uout)
 {
     const Lay &L = ctx->L;
-    dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
+    dim3 grid(nblk((long)L.px * (L.ny + 1), 256), (L.nz + ZCHUNK) / ZCHUNK);
     Tabs tb = tabs_of(ctx);
+    unsigned long long *dtn = (unsigned long long *)(ctx->dt_dev + 1);
     if (ctx->M.coordinate == CUMC_CYLINDRICAL)
-        hipLaunchKernelGGL((k_update<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev);
+        hipLaunchKernelGGL((k_tail<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
+                           Uin, Uout, dtn);
     else
-        hipLaunchKernelGGL((k_update<R, STAGE, 1>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev);
+        hipLaunchKernelGGL((k_tail<R, STAGE, 1>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
+                           Uin, Uout, dtn);
 }
 
+// one RK stage: fluxes of the three sweeps, corner EMFs, stage tail, ghost fill.
+// Conserved buffers: stage 1 U0 -> U1, stage 2 U1 -> Ub, stage 3 Ub -> U1.
 template <typename R>
 static void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
 {
     const Lay &L = ctx->L;
     launch_fluxes<R>(ctx, F);
-    dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
     mark(ctx, KC_ECORN);
-    hipLaunchKernelGGL((k_ecorn<R>), grid, dim3(256), 0, ctx->stream, L, F);
+    dim3 g2(nblk((long)L.px * (L.ny + 1), 256), (L.nz + ZCHUNK) / ZCHUNK);
+    hipLaunchKernelGGL((k_ecorn<R>), g2, dim3(256), 0, ctx->stream, L, F);
     mark(ctx, -1);
-    mark(ctx, KC_UPDATE);
-    if (st == 1) launch_update<R, 1>(ctx, F);
-    else if (st == 2) launch_update<R, 2>(ctx, F);
-    else launch_update<R, 3>(ctx, F);
-    mark(ctx, -1);
-    mark(ctx, KC_C2P);
-    hipLaunchKernelGGL((k_c2p<0>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, ctx->F64.U1);
+    mark(ctx, KC_TAIL);
+    if (st == 1) {
+        launch_tail<R, 1>(ctx, F, ctx->F64.U0, ctx->F64.U1);
+    } else if (st == 2) {
+        launch_tail<R, 2>(ctx, F, ctx->F64.U1, ctx->Ub);
+    } else {
+        const double init = 1.0e10;
+        unsigned long long bits;
+        std::memcpy(&bits, &init, 8);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (unsigned long long *)(ctx->dt_dev + 1), bits, 1);
+        launch_tail<R, 3>(ctx, F, ctx->Ub, ctx->F64.U1);
+    }
     mark(ctx, -1);
     mark(ctx, KC_GHOST);
     launch_ghost(ctx);
@@ -433,7 +447,8 @@ static void swap_u(cumc_ctx *ctx)
     ctx->F32.U1 = ctx->F64.U1;
 }
 
-// RUNGEKUTTA body on the stream, dt already in dt_dev
+// RUNGEKUTTA body on the stream, dt already in dt_dev[0]; leaves finddt of the new state in
+// dt_dev[1]
 static void enqueue_step(cumc_ctx *ctx)
 {
     for (int st = 1; st <= 3; st++) {
@@ -441,6 +456,7 @@ static void enqueue_step(cumc_ctx *ctx)
         else stage<double>(ctx, ctx->F64, st);
     }
     swap_u(ctx);
+    ctx->dt_valid = true;
 }
 
 extern "C" int cumc_initial_update(cumc_ctx *ctx)
@@ -449,8 +465,9 @@ extern "C" int cumc_initial_update(cumc_ctx *ctx)
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "cumc_set_grid must precede cumc_initial_update");
     const Lay &L = ctx->L;
     dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
-    hipLaunchKernelGGL((k_c2p<1>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, ctx->F64.U0);
+    hipLaunchKernelGGL(k_init, grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, ctx->F64.U0);
     launch_ghost(ctx);
+    ctx->dt_valid = false;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return CUMC_OK;
@@ -460,6 +477,7 @@ extern "C" int cumc_boundary(cumc_ctx *ctx)
 {
     if (!ctx) return CUMC_EINVAL;
     launch_ghost(ctx);
+    ctx->dt_valid = false;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return CUMC_OK;
@@ -471,30 +489,41 @@ static void enqueue_finddt(cumc_ctx *ctx)
     const double init = 1.0e10;
     unsigned long long bits;
     std::memcpy(&bits, &init, 8);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (unsigned long long *)ctx->dt_dev, bits, 1);
-    dim3 grid(nblk((long)L.px * L.ny, 256), L.nz);
+    unsigned long long *out = (unsigned long long *)(ctx->dt_dev + 1);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, out, bits, 1);
+    dim3 grid(nblk((long)L.px * L.ny, 256));
     Tabs tb = tabs_of(ctx);
     if (ctx->M.coordinate == CUMC_CYLINDRICAL)
-        hipLaunchKernelGGL((k_finddt<2>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M,
-                           (unsigned long long *)ctx->dt_dev);
+        hipLaunchKernelGGL((k_finddt<2>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M, out);
     else
-        hipLaunchKernelGGL((k_finddt<1>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M,
-                           (unsigned long long *)ctx->dt_dev);
+        hipLaunchKernelGGL((k_finddt<1>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M, out);
+    ctx->dt_valid = true;
+}
+
+// dt of the current state into dt_dev[0] (and *dt): the value the last stage 3 left in
+// dt_dev[1] when the state has not changed since, else a finddt launch
+static int finddt_to_current(cumc_ctx *ctx, double *dt)
+{
+    if (!ctx->dt_valid) {
+        mark(ctx, KC_FINDDT);
+        enqueue_finddt(ctx);
+        mark(ctx, -1);
+    }
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(ctx->dt_dev, ctx->dt_dev + 1, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(dt, ctx->dt_dev + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    collect(ctx);
+    if (!std::isfinite(*dt) || !(*dt > 0.0) || *dt >= 1.0e10)
+        return fail(ctx, CUMC_ENONFINITE, "finddt produced a non-finite step (state has NaN/Inf)");
+    return CUMC_OK;
 }
 
 extern "C" int cumc_finddt(cumc_ctx *ctx, double *dt)
 {
     if (!ctx || !dt) return CUMC_EINVAL;
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
-    mark(ctx, KC_FINDDT);
-    enqueue_finddt(ctx);
-    mark(ctx, -1);
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipMemcpyAsync(dt, ctx->dt_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    collect(ctx);
-    if (!std::isfinite(*dt) || !(*dt > 0.0)) return fail(ctx, CUMC_ENONFINITE, "finddt produced a non-finite step");
-    return CUMC_OK;
+    return finddt_to_current(ctx, dt);
 }
 
 extern "C" int cumc_rungekutta(cumc_ctx *ctx, double dt)
@@ -503,6 +532,7 @@ extern "C" int cumc_rungekutta(cumc_ctx *ctx, double dt)
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
     if (!std::isfinite(dt) || !(dt > 0.0)) return fail(ctx, CUMC_ENONFINITE, "time step must be finite and positive");
     HIPCHK(ctx, hipMemcpyAsync(ctx->dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     enqueue_step(ctx);
     HIPCHK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -523,11 +553,12 @@ extern "C" int cumc_run(cumc_ctx *ctx, int max_steps, double total_time, double 
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     while (global_time < total_time && n < max_steps) {
         double dt;
-        int r = cumc_finddt(ctx, &dt);
+        int r = finddt_to_current(ctx, &dt);
         if (r) { *time = global_time; if (steps_done) *steps_done = n; return r; }
         if (global_time + dt > total_time) {
             dt = total_time - global_time;
             HIPCHK(ctx, hipMemcpyAsync(ctx->dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         }
         enqueue_step(ctx);
         HIPCHK(ctx, hipGetLastError());
@@ -551,7 +582,7 @@ extern "C" int cumc_find_divb(cumc_ctx *ctx, double *max_signed, double *max_abs
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
     const Lay &L = ctx->L;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->red, 0ull, 2);
-    dim3 grid(nblk((long)L.px * L.ny, 256), L.nz);
+    dim3 grid(nblk((long)L.px * L.ny, 256));
     Tabs tb = tabs_of(ctx);
     if (ctx->M.coordinate == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((k_divb<2>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->red);

@@ -244,57 +244,83 @@ __global__ __launch_bounds__(128) void k_flux_walk(Lay L, Fields<R> F, Tabs tb, 
 // ------------------------------------------------------------------------------------------
 // Corner EMFs on [0,n]^3: face-flux averages accumulated in sweep order (spatial.f90:455-467,
 // :533-546, :611-624) then the upwind-CT combination with cell EMFs (mhd.f90:95-125).
+// 2.5D: a thread owns one (j,k) column and marches l through a chunk of ZCHUNK planes; the
+// plane-(l+1) values it loads are carried into the next iteration, so HBM sees each plane once.
 // ------------------------------------------------------------------------------------------
+constexpr int ZCHUNK = 32;
+
 template <typename R>
-__device__ inline R ecell_x(const Lay &L, const Fields<R> &F, long g)
+__device__ inline void ecells(const Lay &L, const Fields<R> &F, long g, R &ex, R &ey, R &ez)
 {
-    return R(F.P[g + 3 * L.ps]) * R(F.BC[g + L.ps]) - R(F.P[g + 2 * L.ps]) * R(F.BC[g + 2 * L.ps]);
-}
-template <typename R>
-__device__ inline R ecell_y(const Lay &L, const Fields<R> &F, long g)
-{
-    return R(F.P[g + 1 * L.ps]) * R(F.BC[g + 2 * L.ps]) - R(F.P[g + 3 * L.ps]) * R(F.BC[g]);
-}
-template <typename R>
-__device__ inline R ecell_z(const Lay &L, const Fields<R> &F, long g)
-{
-    return R(F.P[g + 2 * L.ps]) * R(F.BC[g]) - R(F.P[g + 1 * L.ps]) * R(F.BC[g + L.ps]);
+    const long ps = L.ps;
+    const R vx = R(F.P[g + ps]), vy = R(F.P[g + 2 * ps]), vz = R(F.P[g + 3 * ps]);
+    const R bx = R(F.BC[g]), by = R(F.BC[g + ps]), bz = R(F.BC[g + 2 * ps]);
+    ex = vz * by - vy * bz;
+    ey = vx * bz - vz * bx;
+    ez = vy * bx - vx * by;
 }
 
 template <typename R>
 __global__ __launch_bounds__(256) void k_ecorn(Lay L, Fields<R> F)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
     const int k = (int)(t / L.px);
     const int j = (int)(t - (long)k * L.px);
-    if (k > L.ny || j > L.nx) return;
-    const long g = L.id(j, k, l);
+    const int l0 = blockIdx.y * ZCHUNK;
+    if (k > L.ny || j > L.nx || l0 > L.nz) return;
+    const int l1 = (l0 + ZCHUNK - 1 < L.nz) ? l0 + ZCHUNK - 1 : L.nz;
     const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
     const R *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2];
     const R q = R(0.25);
-    // x-sweep contributions (Fx plane 5 = F(By), 6 = F(Bz))
-    R ey = R(0) + q * (Fx[g + 6 * ps] + Fx[g + sl + 6 * ps]);
-    R ez = R(0) + q * (-Fx[g + 5 * ps] - Fx[g + sk + 5 * ps]);
-    // y-sweep (Fy plane 5 = F(Bx), 6 = F(Bz))
-    R ex = R(0) + q * (-Fy[g + 6 * ps] - Fy[g + sl + 6 * ps]);
-    ez = ez + q * (Fy[g + 5 * ps] + Fy[g + sj + 5 * ps]);
-    // z-sweep (Fz plane 5 = F(Bx), 6 = F(By))
-    ex = ex + q * (Fz[g + 6 * ps] + Fz[g + sk + 6 * ps]);
-    ey = ey + q * (-Fz[g + 5 * ps] - Fz[g + sj + 5 * ps]);
-    // flux_ct corner EMF
-    ex = ex * R(2.0) - q * (ecell_x(L, F, g) + ecell_x(L, F, g + sl) + ecell_x(L, F, g + sk) + ecell_x(L, F, g + sk + sl));
-    ey = ey * R(2.0) - q * (ecell_y(L, F, g) + ecell_y(L, F, g + sl) + ecell_y(L, F, g + sj) + ecell_y(L, F, g + sj + sl));
-    ez = ez * R(2.0) - q * (ecell_z(L, F, g) + ecell_z(L, F, g + sk) + ecell_z(L, F, g + sj) + ecell_z(L, F, g + sj + sk));
-    F.E[g] = ex;
-    F.E[g + ps] = ey;
-    F.E[g + 2 * ps] = ez;
+    long g = L.id(j, k, l0);
+    // cell EMFs of plane l at (j,k), (j,k+1), (j+1,k), (j+1,k+1) and the plane-l B fluxes
+    // that the next plane needs again
+    R x00, y00, z00, x01, y01, z01, x10, y10, z10, x11, y11, z11;
+    ecells(L, F, g, x00, y00, z00);
+    ecells(L, F, g + sk, x01, y01, z01);
+    ecells(L, F, g + sj, x10, y10, z10);
+    ecells(L, F, g + sj + sk, x11, y11, z11);
+    R fx6 = Fx[g + 6 * ps], fy6 = Fy[g + 6 * ps];
+    for (int l = l0; l <= l1; l++, g += sl) {
+        const long gn = g + sl;
+        R X00, Y00, Z00, X01, Y01, Z01, X10, Y10, Z10, X11, Y11, Z11;
+        ecells(L, F, gn, X00, Y00, Z00);
+        ecells(L, F, gn + sk, X01, Y01, Z01);
+        ecells(L, F, gn + sj, X10, Y10, Z10);
+        ecells(L, F, gn + sj + sk, X11, Y11, Z11);
+        const R fx6n = Fx[gn + 6 * ps], fy6n = Fy[gn + 6 * ps];
+        // x sweep: ey += (F_x(Bz)(l) + F_x(Bz)(l+1))/4, ez += (-F_x(By)(k) - F_x(By)(k+1))/4
+        R ey = R(0) + q * (fx6 + fx6n);
+        R ez = R(0) + q * (-Fx[g + 5 * ps] - Fx[g + sk + 5 * ps]);
+        // y sweep: ex += (-F_y(Bz)(l) - F_y(Bz)(l+1))/4, ez += (F_y(Bx)(j) + F_y(Bx)(j+1))/4
+        R ex = R(0) + q * (-fy6 - fy6n);
+        ez = ez + q * (Fy[g + 5 * ps] + Fy[g + sj + 5 * ps]);
+        // z sweep: ex += (F_z(By)(k) + F_z(By)(k+1))/4, ey += (-F_z(Bx)(j) - F_z(Bx)(j+1))/4
+        ex = ex + q * (Fz[g + 6 * ps] + Fz[g + sk + 6 * ps]);
+        ey = ey + q * (-Fz[g + 5 * ps] - Fz[g + sj + 5 * ps]);
+        // flux_ct
+        ex = ex * R(2.0) - q * (x00 + X00 + x01 + X01);
+        ey = ey * R(2.0) - q * (y00 + Y00 + y10 + Y10);
+        ez = ez * R(2.0) - q * (z00 + z01 + z10 + z11);
+        F.E[g] = ex;
+        F.E[g + ps] = ey;
+        F.E[g + 2 * ps] = ez;
+        x00 = X00; y00 = Y00; z00 = Z00; x01 = X01; y01 = Y01; z01 = Z01;
+        x10 = X10; y10 = Y10; z10 = Z10; x11 = X11; y11 = Y11; z11 = Z11;
+        fx6 = fx6n;
+        fy6 = fy6n;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
-// RK stage update on [0,n]^3: flux divergence + sources (spatial.f90:436-452, :365-375,
-// coords.f90:105-137, torus_cyn custom.f90:121-166), face-B right-hand side
-// (mhd.f90:130-153) and the SSP-RK3 combination (rungekutta.f90:41-51, :75-85, :109-119).
+// Stage tail on [0,n]^3, one pass: flux divergence + sources (spatial.f90:436-452, :365-375,
+// coords.f90:105-137, torus_cyn custom.f90:121-166), face-B right-hand side (mhd.f90:130-153),
+// the SSP-RK3 combination (rungekutta.f90:41-51, :75-85, :109-119), FROMUTORVE
+// (fromrvetou.f90:75-127, utop.f90:185-220), CUSTOM_CHECKRHO, FROMRVETOU (ptou.f90:33-66),
+// UPDATE's EOS (update.f90:12-45) and, in stage 3, the next finddt (rungekutta.f90:205-252).
+// A thread marches one (j,k) column over a chunk of planes.  Cell-centred B needs the new
+// face B of the lower neighbours; those are recomputed from the same inputs (bit-identical to
+// their owners), which is why the stage reads Uin and writes a different buffer Uout.
 // ------------------------------------------------------------------------------------------
 constexpr double RK20 = 3.0 / 4.0, RK21 = 1.0 / 4.0, RK22 = 1.0 / 4.0;
 constexpr double RK30 = 1.0 / 3.0, RK31 = 2.0 / 3.0, RK32 = 2.0 / 3.0;
@@ -307,109 +333,240 @@ __device__ inline double rk(double u0, double u1, double dt, double lrk)
     return RK30 * u0 + RK31 * u1 + RK32 * dt * lrk;
 }
 
+// right-hand sides of the three face fields of cell (j,k,l) (mhd.f90:130-153, GEOM_CT of
+// coords.f90:338-386 evaluated at the cell)
+template <int COORD>
+__device__ inline double rhs_bx(const Tabs &tb, int j, int k, int l, double ez, double ez_km, double ey, double ey_lm)
+{
+    const double cez = COORD == 2 ? tb.at(1, TB_D, k) * tb.at(0, TB_F, j) : tb.at(1, TB_D, k);
+    return -((ez * 1.0 - ez_km * 1.0) / cez - (ey * 1.0 - ey_lm * 1.0) / tb.at(2, TB_D, l));
+}
+template <int COORD>
+__device__ inline double rhs_by(const Tabs &tb, int j, int l, double ex, double ex_lm, double ez, double ez_jm)
+{
+    return -((ex * 1.0 - ex_lm * 1.0) / tb.at(2, TB_D, l) - (ez * 1.0 - ez_jm * 1.0) / tb.at(0, TB_D, j));
+}
+template <int COORD>
+__device__ inline double rhs_bz(const Tabs &tb, int j, int k, double ey, double ey_jm, double ex, double ex_km)
+{
+    if (COORD == 2) {
+        const double xl = tb.at(0, TB_C, j);
+        return -((ey * tb.at(0, TB_F, j) - ey_jm * tb.at(0, TB_F, j - 1)) / (xl * tb.at(0, TB_D, j)) -
+                 (ex * 1.0 - ex_km * 1.0) / (xl * tb.at(1, TB_D, k)));
+    }
+    return -((ey * 1.0 - ey_jm * 1.0) / tb.at(0, TB_D, j) - (ex * 1.0 - ex_km * 1.0) / tb.at(1, TB_D, k));
+}
+
+// CFL time step of one cell (rungekutta.f90:213-247; Q8: no discriminant clamp)
+template <int COORD>
+__device__ inline double cell_dt(const Tabs &tb, const Model &M, int j, int k, int l, double rho, double vx, double vy,
+                                 double vz, double p, double bx, double by, double bz)
+{
+    const double cs = sqrt(M.ggas * p / rho);
+    const double a2 = cs * cs;
+    const double a4 = a2 * a2;
+    const double b2x = (bx * bx / rho), b2y = (by * by / rho), b2z = (bz * bz / rho);
+    const double b2 = b2x + b2y + b2z;
+    const double b4 = b2 * b2;
+    const double cfx = sqrt(0.5 * (a2 + b2 + sqrt((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2x)));
+    const double cfy = sqrt(0.5 * (a2 + b2 + sqrt((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2y)));
+    const double cfz = sqrt(0.5 * (a2 + b2 + sqrt((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2z)));
+    const double l1 = fabs(vx) + cfx, l2 = fabs(vy) + cfy, l3 = fabs(vz) + cfz;
+    const double lambda = fmax2<double>(fmax2<double>(l1, l2), l3);
+    double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
+    if (COORD == 2) dyl = tb.at(0, TB_C, j) * dyl;
+    const double dt_out = fmin2<double>(fmin2<double>(dxl, dyl), dzl) * M.cfl / lambda;
+    return fmin2<double>(dt_out, 1.0e10);
+}
+
+__device__ inline double block_min(double v)
+{
+    __shared__ double red[16];
+    for (int o = 32; o > 0; o >>= 1) {
+        double w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] < m ? red[w] : m;
+    return m;
+}
+
 template <typename R, int STAGE, int COORD>
-__global__ __launch_bounds__(256) void k_update(Lay L, Fields<R> F, Tabs tb, Model M, const double *dtp)
+__global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model M, const double *dtp,
+                                              const double *__restrict__ Uin, double *__restrict__ Uout,
+                                              unsigned long long *dtnext)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
     const int k = (int)(t / L.px);
     const int j = (int)(t - (long)k * L.px);
-    if (k > L.ny || j > L.nx) return;
-    const long g = L.id(j, k, l);
-    const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
-    const double dt = *dtp;
-    const bool interior = j >= 1 && k >= 1 && l >= 1;
-
-    double lrk[8];
-    if (interior) {
+    const int l0 = blockIdx.y * ZCHUNK;
+    const bool col = k <= L.ny && j <= L.nx && l0 <= L.nz;
+    double dtmin = 1.0e10;
+    if (col) {
+        const int l1 = (l0 + ZCHUNK - 1 < L.nz) ? l0 + ZCHUNK - 1 : L.nz;
+        const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
+        const double dt = *dtp;
         const R *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2];
-        double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
+        const R *E = F.E;
+        const double *U0 = F.U0;
+        const double gm1 = M.ggas - 1.0;
+        // flux-divergence geometry of the column (coords.f90:255-282)
+        const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k);
+        double gxp = 1.0, gxm = 1.0, gxc = dxl, gyc = dyl, xloc = 0.0;
         if (COORD == 2) {
-            const double xloc = tb.at(0, TB_C, j);
-            const double rho = F.P[g], vx = F.P[g + ps], vy = F.P[g + 2 * ps], p = F.P[g + 4 * ps];
-            const double bx = F.BC[g], by = F.BC[g + ps], bz = F.BC[g + 2 * ps];
-            const double bsq = bx * bx + by * by + bz * bz;
-            const double tx = (p + rho * vy * vy + 0.5 * bsq - by * by) / (xloc);
-            const double ty = (rho * vx * vy - bx * by) / (xloc);
-            for (int r = 0; r < M.geom_repeat; r++) {
-                sc[IVX] = sc[IVX] + tx;
-                sc[IVY] = sc[IVY] - ty;
-            }
-        }
-        if (M.model == 1) {
-            const double xloc = tb.at(0, TB_C, j), zloc = tb.at(2, TB_C, l);
-            const double rho = F.P[g], vx = F.P[g + ps], vz = F.P[g + 3 * ps];
-            const double diff = rho - M.rho_floor;
-            const double factor = fmax2<double>(sign1<double>(diff), 0.0);
-            const double radius = sqrt(xloc * xloc + zloc * zloc);
-            const double dphidr = xloc / radius / ((radius - M.r_sh) * (radius - M.r_sh));
-            const double dphidz = zloc / radius / ((radius - M.r_sh) * (radius - M.r_sh));
-            sc[IVX] = sc[IVX] + (-factor * rho * dphidr);
-            sc[IVZ] = sc[IVZ] + (-factor * rho * dphidz);
-            sc[ITAU] = sc[ITAU] + (-factor * rho * (vx * dphidr + vz * dphidz));
-        }
-        double gxp = 1.0, gxm = 1.0, gxc = dxl, gyc = dyl;
-        if (COORD == 2) {
-            const double xloc = tb.at(0, TB_C, j);
+            xloc = tb.at(0, TB_C, j);
             gxp = tb.at(0, TB_F, j);
             gxm = tb.at(0, TB_F, j - 1);
             gxc = xloc * dxl;
             gyc = xloc * dyl;
         }
+        // carries from plane l0-1
+        long g = L.id(j, k, l0);
+        long gm = g - sl;
+        double exL = E[gm], eyL = E[gm + ps], eyjL = E[gm - sj + ps], exkL = E[gm - sk];
+        double bzL = 0.0;
+        if (l0 >= 1)
+            bzL = rk<STAGE>(U0[gm + 7 * ps], STAGE == 1 ? 0.0 : Uin[gm + 7 * ps], dt,
+                            rhs_bz<COORD>(tb, j, k, eyL, eyjL, exL, exkL));
+        double fzL[5];
 #pragma unroll
-        for (int i = 0; i < 5; i++) {
-            double a = 0.0;
-            if (COORD == 2)
-                a = a - ((double)Fx[g + i * ps] * gxp - (double)Fx[g - sj + i * ps] * gxm) / gxc;
-            else
-                a = a - ((double)Fx[g + i * ps] * 1.0 - (double)Fx[g - sj + i * ps] * 1.0) / gxc;
-            a = a - ((double)Fy[g + i * ps] * 1.0 - (double)Fy[g - sk + i * ps] * 1.0) / gyc;
-            a = a - ((double)Fz[g + i * ps] * 1.0 - (double)Fz[g - sl + i * ps] * 1.0) / dzl;
-            lrk[i] = a + sc[i];
+        for (int i = 0; i < 5; i++) fzL[i] = (double)Fz[gm + i * ps];
+        for (int l = l0; l <= l1; l++, g += sl) {
+            const bool interior = j >= 1 && k >= 1 && l >= 1;
+            // corner EMFs around the cell's faces
+            const double ex0 = E[g], ey0 = E[g + ps], ez0 = E[g + 2 * ps];
+            const double exk = E[g - sk], ezk = E[g - sk + 2 * ps];
+            const double eyj = E[g - sj + ps], ezj = E[g - sj + 2 * ps];
+            const double ezjk = E[g - sj - sk + 2 * ps];
+            // new face fields: own three, and the x / y faces of the lower neighbours
+            const double bxn = rk<STAGE>(U0[g + 5 * ps], STAGE == 1 ? 0.0 : Uin[g + 5 * ps], dt,
+                                         rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, eyL));
+            const double byn = rk<STAGE>(U0[g + 6 * ps], STAGE == 1 ? 0.0 : Uin[g + 6 * ps], dt,
+                                         rhs_by<COORD>(tb, j, l, ex0, exL, ez0, ezj));
+            const double bzn = rk<STAGE>(U0[g + 7 * ps], STAGE == 1 ? 0.0 : Uin[g + 7 * ps], dt,
+                                         rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
+            // hydro right-hand side on interior cells
+            double lrk[5];
+            double fz[5];
+#pragma unroll
+            for (int i = 0; i < 5; i++) fz[i] = (double)Fz[g + i * ps];
+            if (interior) {
+                double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                if (COORD == 2 || M.model == 1) {
+                    const double rho = F.P[g], vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
+                    const double p = F.P[g + 4 * ps];
+                    const double bx = F.BC[g], by = F.BC[g + ps], bz = F.BC[g + 2 * ps];
+                    if (COORD == 2) {
+                        const double bsq = bx * bx + by * by + bz * bz;
+                        const double tx = (p + rho * vy * vy + 0.5 * bsq - by * by) / (xloc);
+                        const double ty = (rho * vx * vy - bx * by) / (xloc);
+                        for (int r = 0; r < M.geom_repeat; r++) {
+                            sc[IVX] = sc[IVX] + tx;
+                            sc[IVY] = sc[IVY] - ty;
+                        }
+                    }
+                    if (M.model == 1) {
+                        const double xl = tb.at(0, TB_C, j), zl = tb.at(2, TB_C, l);
+                        const double factor = fmax2<double>(sign1<double>(rho - M.rho_floor), 0.0);
+                        const double radius = sqrt(xl * xl + zl * zl);
+                        const double dphidr = xl / radius / ((radius - M.r_sh) * (radius - M.r_sh));
+                        const double dphidz = zl / radius / ((radius - M.r_sh) * (radius - M.r_sh));
+                        sc[IVX] = sc[IVX] + (-factor * rho * dphidr);
+                        sc[IVZ] = sc[IVZ] + (-factor * rho * dphidz);
+                        sc[ITAU] = sc[ITAU] + (-factor * rho * (vx * dphidr + vz * dphidz));
+                    }
+                }
+                const double dzl = tb.at(2, TB_D, l);
+#pragma unroll
+                for (int i = 0; i < 5; i++) {
+                    double a = 0.0;
+                    a = a - ((double)Fx[g + i * ps] * gxp - (double)Fx[g - sj + i * ps] * gxm) / gxc;
+                    a = a - ((double)Fy[g + i * ps] * 1.0 - (double)Fy[g - sk + i * ps] * 1.0) / gyc;
+                    a = a - (fz[i] * 1.0 - fzL[i] * 1.0) / dzl;
+                    lrk[i] = a + sc[i];
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 5; i++) lrk[i] = 0.0;
+            }
+            double u[5];
+#pragma unroll
+            for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(U0[g + i * ps], STAGE == 1 ? 0.0 : Uin[g + i * ps], dt, lrk[i]);
+            // cell-centred field from the new face fields (interior) or the ghost value
+            double bcx, bcy, bcz;
+            if (interior) {
+                const double bxm = rk<STAGE>(U0[g - sj + 5 * ps], STAGE == 1 ? 0.0 : Uin[g - sj + 5 * ps], dt,
+                                             rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
+                const double bym = rk<STAGE>(U0[g - sk + 6 * ps], STAGE == 1 ? 0.0 : Uin[g - sk + 6 * ps], dt,
+                                             rhs_by<COORD>(tb, j, l, exk, exkL, ezk, ezjk));
+                bcx = 0.5 * (bxn + bxm);
+                bcy = 0.5 * (byn + bym);
+                bcz = 0.5 * (bzn + bzL);
+                F.BC[g] = bcx;
+                F.BC[g + ps] = bcy;
+                F.BC[g + 2 * ps] = bcz;
+            } else {
+                bcx = F.BC[g];
+                bcy = F.BC[g + ps];
+                bcz = F.BC[g + 2 * ps];
+            }
+            // U_to_P, floors, P_to_U, EOS
+            const double rho0 = u[0];
+            double vx = u[1] / rho0, vy = u[2] / rho0, vz = u[3] / rho0;
+            const double bsq = bcx * bcx + bcy * bcy + bcz * bcz;
+            double vsq = vx * vx + vy * vy + vz * vz;
+            double eps = (u[4] - 0.5 * bsq) / rho0 - 0.5 * vsq;
+            double rho = rho0;
+            if (interior) {
+                if (M.model == 1) {
+                    if (rho < M.rho_floor) rho = M.rho_floor;
+                    if (eps < M.eps_floor) eps = M.eps_floor;
+                } else {
+                    eps = fmax2<double>(eps, 0.0);
+                }
+            }
+            vsq = vx * vx + vy * vy + vz * vz;
+            const double p = rho * eps * gm1;
+            Uout[g] = rho;
+            Uout[g + ps] = vx * rho;
+            Uout[g + 2 * ps] = vy * rho;
+            Uout[g + 3 * ps] = vz * rho;
+            Uout[g + 4 * ps] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
+            Uout[g + 5 * ps] = bxn;
+            Uout[g + 6 * ps] = byn;
+            Uout[g + 7 * ps] = bzn;
+            F.P[g] = rho;
+            F.P[g + ps] = vx;
+            F.P[g + 2 * ps] = vy;
+            F.P[g + 3 * ps] = vz;
+            F.P[g + 4 * ps] = p;
+            F.P[g + 5 * ps] = bxn;
+            F.P[g + 6 * ps] = byn;
+            F.P[g + 7 * ps] = bzn;
+            F.EPS[g] = eps;
+            if (STAGE == 3 && interior)
+                dtmin = fmin2<double>(cell_dt<COORD>(tb, M, j, k, l, rho, vx, vy, vz, p, bcx, bcy, bcz), dtmin);
+            // carries
+            exL = ex0;
+            eyL = ey0;
+            eyjL = eyj;
+            exkL = exk;
+            bzL = bzn;
+#pragma unroll
+            for (int i = 0; i < 5; i++) fzL[i] = fz[i];
         }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 5; i++) lrk[i] = 0.0;
     }
-    {
-        // GEOM_CT factors at cell (j,k,l) (coords.f90:338-386)
-        const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
-        double bx_ez_c = dyl, bx_ey_c = dzl, by_ex_c = dzl, by_ez_c = dxl, bz_ey_c = dxl, bz_ex_c = dyl;
-        double bz_ey_m = 1.0, bz_ey_p = 1.0;
-        if (COORD == 2) {
-            const double xloc = tb.at(0, TB_C, j);
-            bx_ez_c = dyl * tb.at(0, TB_F, j);
-            bz_ey_m = tb.at(0, TB_F, j - 1);
-            bz_ey_c = xloc * dxl;
-            bz_ey_p = tb.at(0, TB_F, j);
-            bz_ex_c = xloc * dyl;
-        }
-        const R *E = F.E;
-        const double ex = E[g], ey = E[g + ps], ez = E[g + 2 * ps];
-        const double ez_km = E[g - sk + 2 * ps], ey_lm = E[g - sl + ps];
-        const double ex_lm = E[g - sl], ez_jm = E[g - sj + 2 * ps];
-        const double ey_jm = E[g - sj + ps], ex_km = E[g - sk];
-        lrk[IBX] = -((ez * 1.0 - ez_km * 1.0) / bx_ez_c - (ey * 1.0 - ey_lm * 1.0) / bx_ey_c);
-        lrk[IBY] = -((ex * 1.0 - ex_lm * 1.0) / by_ex_c - (ez * 1.0 - ez_jm * 1.0) / by_ez_c);
-        lrk[IBZ] = -((ey * bz_ey_p - ey_jm * bz_ey_m) / bz_ey_c - (ex * 1.0 - ex_km * 1.0) / bz_ex_c);
-    }
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const double u0 = F.U0[g + i * ps];
-        const double u1 = STAGE == 1 ? 0.0 : F.U1[g + i * ps];
-        F.U1[g + i * ps] = rk<STAGE>(u0, u1, dt, lrk[i]);
+    if (STAGE == 3) {
+        const double m = block_min(dtmin);
+        if (threadIdx.x == 0) atomicMin(dtnext, (unsigned long long)__double_as_longlong(m));
     }
 }
 
-// ------------------------------------------------------------------------------------------
-// Stage tail on [0,n]^3: FROMUTORVE (fromrvetou.f90:75-127, utop.f90:185-220), CUSTOM_CHECKRHO,
-// FROMRVETOU (fromrvetou.f90:21-70, ptou.f90:33-66) and UPDATE's EOS (update.f90:12-45).
-// With INIT the conserved state is built from the uploaded primitives instead
-// (initial.f90:103-126).
-// ------------------------------------------------------------------------------------------
-template <int INIT>
-__global__ __launch_bounds__(256) void k_c2p(Lay L, Fields<double> F, Model M, double *Uout)
+// initial_update's FROMRVETOU + CUSTOM_CHECKRHO + EOS on [0,n]^3 (initial.f90:103-126)
+__global__ __launch_bounds__(256) void k_init(Lay L, Fields<double> F, Model M, double *Uout)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
@@ -419,12 +576,12 @@ __global__ __launch_bounds__(256) void k_c2p(Lay L, Fields<double> F, Model M, d
     const long g = L.id(j, k, l);
     const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
     const bool interior = j >= 1 && k >= 1 && l >= 1;
-    const double *Bsrc = INIT ? F.P + 5 * ps : Uout + 5 * ps;  // face B: prim (initial) or cons
+    const double *B = F.P + 5 * ps;
     double bx, by, bz;
     if (interior) {
-        bx = 0.5 * (Bsrc[g] + Bsrc[g - sj]);
-        by = 0.5 * (Bsrc[g + ps] + Bsrc[g - sk + ps]);
-        bz = 0.5 * (Bsrc[g + 2 * ps] + Bsrc[g - sl + 2 * ps]);
+        bx = 0.5 * (B[g] + B[g - sj]);
+        by = 0.5 * (B[g + ps] + B[g - sk + ps]);
+        bz = 0.5 * (B[g + 2 * ps] + B[g - sl + 2 * ps]);
         F.BC[g] = bx;
         F.BC[g + ps] = by;
         F.BC[g + 2 * ps] = bz;
@@ -433,27 +590,9 @@ __global__ __launch_bounds__(256) void k_c2p(Lay L, Fields<double> F, Model M, d
         by = F.BC[g + ps];
         bz = F.BC[g + 2 * ps];
     }
-    double rho, vx, vy, vz, eps;
-    if (INIT) {
-        rho = F.P[g];
-        vx = F.P[g + ps];
-        vy = F.P[g + 2 * ps];
-        vz = F.P[g + 3 * ps];
-        eps = F.EPS[g];
-    } else {
-        const double *U = Uout;
-        const double ur = U[g];
-        rho = ur;
-        vx = U[g + ps] / ur;
-        vy = U[g + 2 * ps] / ur;
-        vz = U[g + 3 * ps] / ur;
-        const double bsq = bx * bx + by * by + bz * bz;
-        const double vsq = vx * vx + vy * vy + vz * vz;
-        eps = (U[g + 4 * ps] - 0.5 * bsq) / ur - 0.5 * vsq;
-        F.P[g + 5 * ps] = U[g + 5 * ps];
-        F.P[g + 6 * ps] = U[g + 6 * ps];
-        F.P[g + 7 * ps] = U[g + 7 * ps];
-    }
+    double rho = F.P[g];
+    const double vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
+    double eps = F.EPS[g];
     if (interior) {
         if (M.model == 1) {
             if (rho < M.rho_floor) rho = M.rho_floor;
@@ -462,7 +601,6 @@ __global__ __launch_bounds__(256) void k_c2p(Lay L, Fields<double> F, Model M, d
             eps = fmax2<double>(eps, 0.0);
         }
     }
-    // P_to_U with the cell-centred field
     const double vsq = vx * vx + vy * vy + vz * vz;
     const double bsq = bx * bx + by * by + bz * bz;
     Uout[g] = rho;
@@ -470,15 +608,10 @@ __global__ __launch_bounds__(256) void k_c2p(Lay L, Fields<double> F, Model M, d
     Uout[g + 2 * ps] = vy * rho;
     Uout[g + 3 * ps] = vz * rho;
     Uout[g + 4 * ps] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
-    if (INIT) {
-        Uout[g + 5 * ps] = F.P[g + 5 * ps];
-        Uout[g + 6 * ps] = F.P[g + 6 * ps];
-        Uout[g + 7 * ps] = F.P[g + 7 * ps];
-    }
+    Uout[g + 5 * ps] = F.P[g + 5 * ps];
+    Uout[g + 6 * ps] = F.P[g + 6 * ps];
+    Uout[g + 7 * ps] = F.P[g + 7 * ps];
     F.P[g] = rho;
-    F.P[g + ps] = vx;
-    F.P[g + 2 * ps] = vy;
-    F.P[g + 3 * ps] = vz;
     F.P[g + 4 * ps] = rho * eps * (M.ggas - 1.0);
     F.EPS[g] = eps;
 }
@@ -622,45 +755,26 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n)
     if (i < n) p[i] = v;
 }
 
-// finddt (rungekutta.f90:205-252); positive doubles order like their bit patterns
+// finddt (rungekutta.f90:205-252) on the current state; a thread marches one (j,k) column so a
+// launch issues one atomic per block.  Positive doubles order like their bit patterns.
 template <int COORD>
 __global__ __launch_bounds__(256) void k_finddt(Lay L, Fields<double> F, Tabs tb, Model M, unsigned long long *out)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y + 1;
     const int k = (int)(t / L.px) + 1;
     const int j = (int)(t - (long)(k - 1) * L.px) + 1;
     double dt = 1.0e10;
     if (k <= L.ny && j <= L.nx) {
-        const long g = L.id(j, k, l), ps = L.ps;
-        const double g_ = M.ggas;
-        const double rho = F.P[g];
-        const double cs = sqrt(g_ * F.P[g + 4 * ps] / rho);
-        const double a2 = cs * cs;
-        const double a4 = a2 * a2;
-        const double bx = F.BC[g], by = F.BC[g + ps], bz = F.BC[g + 2 * ps];
-        const double b2x = (bx * bx / rho), b2y = (by * by / rho), b2z = (bz * bz / rho);
-        const double b2 = b2x + b2y + b2z;
-        const double b4 = b2 * b2;
-        const double cfx = sqrt(0.5 * (a2 + b2 + sqrt((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2x)));
-        const double cfy = sqrt(0.5 * (a2 + b2 + sqrt((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2y)));
-        const double cfz = sqrt(0.5 * (a2 + b2 + sqrt((a4 + 2.0 * a2 * b2 + b4) - 4.0 * a2 * b2z)));
-        const double l1 = fabs(F.P[g + ps]) + cfx, l2 = fabs(F.P[g + 2 * ps]) + cfy, l3 = fabs(F.P[g + 3 * ps]) + cfz;
-        const double lambda = fmax2<double>(fmax2<double>(l1, l2), l3);
-        double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
-        if (COORD == 2) dyl = tb.at(0, TB_C, j) * dyl;
-        const double dt_out = fmin2<double>(fmin2<double>(dxl, dyl), dzl) * M.cfl / lambda;
-        dt = fmin2<double>(dt_out, dt);
+        const long ps = L.ps;
+        for (int l = 1; l <= L.nz; l++) {
+            const long g = L.id(j, k, l);
+            dt = fmin2<double>(cell_dt<COORD>(tb, M, j, k, l, F.P[g], F.P[g + ps], F.P[g + 2 * ps], F.P[g + 3 * ps],
+                                              F.P[g + 4 * ps], F.BC[g], F.BC[g + ps], F.BC[g + 2 * ps]),
+                               dt);
+        }
     }
-    __shared__ double red[4];
-    dt = wave_min(dt);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dt;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] < m ? red[w] : m;
-        atomicMin(out, (unsigned long long)__double_as_longlong(m));
-    }
+    const double m = block_min(dt);
+    if (threadIdx.x == 0) atomicMin(out, (unsigned long long)__double_as_longlong(m));
 }
 
 // find_divb (mhd.f90:15-66) with GEOM_AREA (coords.f90:173-223)
@@ -668,32 +782,33 @@ template <int COORD>
 __global__ __launch_bounds__(256) void k_divb(Lay L, Fields<double> F, Tabs tb, unsigned long long *out)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y + 1;
     const int k = (int)(t / L.px) + 1;
     const int j = (int)(t - (long)(k - 1) * L.px) + 1;
     double ms = 0.0, ma = 0.0;
     if (k <= L.ny && j <= L.nx) {
-        const long g = L.id(j, k, l), ps = L.ps;
-        const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
-        double axp, axm, ayp, aym, azp, azm;
-        if (COORD == 2) {
-            const double xp = tb.at(0, TB_F, j), xm = tb.at(0, TB_F, j - 1);
-            axp = xp * dyl * dzl;
-            axm = xm * dyl * dzl;
-            ayp = dxl * dzl;
-            aym = ayp;
-            azp = 0.5 * (xp * xp - xm * xm) * dyl;
-            azm = azp;
-        } else {
-            axp = dyl * dzl; axm = axp;
-            ayp = dxl * dzl; aym = ayp;
-            azp = dxl * dyl; azm = azp;
+        for (int l = 1; l <= L.nz; l++) {
+            const long g = L.id(j, k, l), ps = L.ps;
+            const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
+            double axp, axm, ayp, aym, azp, azm;
+            if (COORD == 2) {
+                const double xp = tb.at(0, TB_F, j), xm = tb.at(0, TB_F, j - 1);
+                axp = xp * dyl * dzl;
+                axm = xm * dyl * dzl;
+                ayp = dxl * dzl;
+                aym = ayp;
+                azp = 0.5 * (xp * xp - xm * xm) * dyl;
+                azm = azp;
+            } else {
+                axp = dyl * dzl; axm = axp;
+                ayp = dxl * dzl; aym = ayp;
+                azp = dxl * dyl; azm = azp;
+            }
+            const double *B = F.P + 5 * ps;
+            const double d = (axp * B[g] - axm * B[g - 1]) + (ayp * B[g + ps] - aym * B[g - L.px + ps]) +
+                             (azp * B[g + 2 * ps] - azm * B[g - L.py + 2 * ps]);
+            ms = fmax2<double>(ms, d);
+            ma = fmax2<double>(ma, fabs(d));
         }
-        const double *B = F.P + 5 * ps;
-        const double d = (axp * B[g] - axm * B[g - 1]) + (ayp * B[g + ps] - aym * B[g - L.px + ps]) +
-                         (azp * B[g + 2 * ps] - azm * B[g - L.py + 2 * ps]);
-        ms = fmax2<double>(0.0, d);
-        ma = fabs(d);
     }
     __shared__ double red[8];
     ms = wave_max(ms);
