@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="zones per axis of the periodic box")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and full-size parity check")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -102,7 +104,12 @@ def main():
 
     from cumpic_amd import ics
     from cumpic_amd.config import FP64, MIXED
-    from cumpic_amd.cumc import Solver
+    from cumpic_amd.cumc import Solver, copy_probe
+
+    copy_gbps = None
+    if args.copy_probe_mib:
+        copy_gbps = copy_probe(args.copy_probe_mib << 20, reps=3)
+        log(f"[bench] copy probe {args.copy_probe_mib} MiB: {copy_gbps:.1f} GB/s")
 
     n = args.n
     log(f"[bench] building {n}^3 spectral initial condition")
@@ -196,6 +203,7 @@ def main():
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items()},
         "divb_max_abs": dabs,
         "state_finite": finite,
+        "copy_probe_GBps": copy_gbps,
         "parity": parity,
         "dt_first_last": [float(dts[0]), float(dts[-1])] if len(dts) else None,
     }

@@ -288,6 +288,8 @@ static Tabs tabs_of(const cumc_ctx *ctx)
 }
 
 static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+// 1D launch size for a gx x gy tile grid remapped by xcd_tile (a multiple of the XCD count)
+static inline unsigned xcd_blocks(int gx, int gy) { return (unsigned)(((long)gx * gy + NXCD - 1) / NXCD * NXCD); }
 
 static int conv_in(cumc_ctx *ctx, const double *host, int nv, double *dst)
 {
@@ -400,15 +402,16 @@ template <typename R, int STAGE>
 static void launch_tail(cumc_ctx *ctx, const Fields<R> &F, const double *Uin, double *Uout)
 {
     const Lay &L = ctx->L;
-    dim3 grid(nblk((long)L.px * (L.ny + 1), 256), (L.nz + ZCHUNK) / ZCHUNK);
+    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + ZCHUNK) / ZCHUNK;
+    dim3 grid(xcd_blocks(gx, gy));
     Tabs tb = tabs_of(ctx);
     unsigned long long *dtn = (unsigned long long *)(ctx->dt_dev + 1);
     if (ctx->M.coordinate == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((k_tail<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn);
+                           Uin, Uout, dtn, gx, gy);
     else
         hipLaunchKernelGGL((k_tail<R, STAGE, 1>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn);
+                           Uin, Uout, dtn, gx, gy);
 }
 
 // one RK stage: fluxes of the three sweeps, corner EMFs, stage tail, ghost fill.
@@ -419,8 +422,8 @@ static void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
     const Lay &L = ctx->L;
     launch_fluxes<R>(ctx, F);
     mark(ctx, KC_ECORN);
-    dim3 g2(nblk((long)L.px * (L.ny + 1), 256), (L.nz + ZCHUNK) / ZCHUNK);
-    hipLaunchKernelGGL((k_ecorn<R>), g2, dim3(256), 0, ctx->stream, L, F);
+    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + ZCHUNK) / ZCHUNK;
+    hipLaunchKernelGGL((k_ecorn<R>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, F, gx, gy);
     mark(ctx, -1);
     mark(ctx, KC_TAIL);
     if (st == 1) {
@@ -652,4 +655,41 @@ extern "C" int cumc_kernel_times(cumc_ctx *ctx, int n, double *total_ms, long *l
         if (names) names[c] = kc_names[c];
     }
     return KC_N;
+}
+
+// Plain device copy of `bytes` bytes, run `reps` times on a private stream: the known-size
+// reference against which HBM traffic counters (FETCH_SIZE / WRITE_SIZE) are calibrated.
+__global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+extern "C" int cumc_copy_probe(long bytes, int reps, double *gbps)
+{
+    if (bytes < 1024 || reps < 1) return CUMC_EINVAL;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    if (hipMalloc(&a, bytes) != hipSuccess) return CUMC_ENOMEM;
+    if (hipMalloc(&b, bytes) != hipSuccess) { hipFree(a); return CUMC_ENOMEM; }
+    hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipMemsetAsync(a, 0, bytes, s);
+    const long n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
+    hipEventRecord(e0, s);
+    for (int r = 0; r < reps; r++)
+        hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
+    hipEventRecord(e1, s);
+    hipError_t e = hipStreamSynchronize(s);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (gbps) *gbps = 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(s);
+    hipFree(a);
+    hipFree(b);
+    return e == hipSuccess ? CUMC_OK : CUMC_EHIP;
 }

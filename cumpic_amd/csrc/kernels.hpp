@@ -248,6 +248,20 @@ __global__ __launch_bounds__(128) void k_flux_walk(Lay L, Fields<R> F, Tabs tb, 
 // plane-(l+1) values it loads are carried into the next iteration, so HBM sees each plane once.
 // ------------------------------------------------------------------------------------------
 constexpr int ZCHUNK = 32;
+constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+
+// Map the 1D launch index onto a logical (bx, by) tile so that logically consecutive tiles
+// (adjacent rows k, which share their k-1 / k+1 halos) run on the same XCD and meet in its
+// L2.  The launch has ceil(gx*gy/8)*8 blocks; returns false for the padding blocks.
+__device__ inline bool xcd_tile(int gx, int gy, int &bx, int &by)
+{
+    const unsigned per = gridDim.x / NXCD;
+    const unsigned logical = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+    if (logical >= (unsigned)(gx * gy)) return false;
+    bx = (int)(logical % (unsigned)gx);
+    by = (int)(logical / (unsigned)gx);
+    return true;
+}
 
 template <typename R>
 __device__ inline void ecells(const Lay &L, const Fields<R> &F, long g, R &ex, R &ey, R &ez)
@@ -261,12 +275,14 @@ __device__ inline void ecells(const Lay &L, const Fields<R> &F, long g, R &ex, R
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_ecorn(Lay L, Fields<R> F)
+__global__ __launch_bounds__(256) void k_ecorn(Lay L, Fields<R> F, int gx, int gy)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int bx, by;
+    if (!xcd_tile(gx, gy, bx, by)) return;
+    const long t = (long)bx * blockDim.x + threadIdx.x;
     const int k = (int)(t / L.px);
     const int j = (int)(t - (long)k * L.px);
-    const int l0 = blockIdx.y * ZCHUNK;
+    const int l0 = by * ZCHUNK;
     if (k > L.ny || j > L.nx || l0 > L.nz) return;
     const int l1 = (l0 + ZCHUNK - 1 < L.nz) ? l0 + ZCHUNK - 1 : L.nz;
     const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
@@ -397,12 +413,14 @@ __device__ inline double block_min(double v)
 template <typename R, int STAGE, int COORD>
 __global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model M, const double *dtp,
                                               const double *__restrict__ Uin, double *__restrict__ Uout,
-                                              unsigned long long *dtnext)
+                                              unsigned long long *dtnext, int gx, int gy)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int bx, by;
+    if (!xcd_tile(gx, gy, bx, by)) return;
+    const long t = (long)bx * blockDim.x + threadIdx.x;
     const int k = (int)(t / L.px);
     const int j = (int)(t - (long)k * L.px);
-    const int l0 = blockIdx.y * ZCHUNK;
+    const int l0 = by * ZCHUNK;
     const bool col = k <= L.ny && j <= L.nx && l0 <= L.nz;
     double dtmin = 1.0e10;
     if (col) {
@@ -653,8 +671,17 @@ __global__ __launch_bounds__(256) void k_ghost(Lay L, Fields<double> F, Model M)
     else { a0 = -2; na = L.nx + 6; b0 = -2; nb = L.ny + 6; }                 // j, k full
     const long per = (long)na * nb;
     if (t >= per * 6) return;
-    const int gs = (int)(t / per);  // 0..5 = (g-1)*2 + side
-    const long r = t - (long)gs * per;
+    int gs;  // 0..5 = (g-1)*2 + side
+    long r;
+    if (AX == 0) {
+        // x pass: the six ghost cells of one row are adjacent in memory, so let them be
+        // neighbouring threads
+        gs = (int)(t % 6);
+        r = t / 6;
+    } else {
+        gs = (int)(t / per);
+        r = t - (long)gs * per;
+    }
     const int g = gs / 2 + 1, side = gs & 1;
     const int a = (int)(r % na) + a0, b = (int)(r / na) + b0;
     const int n = AX == 0 ? L.nx : (AX == 1 ? L.ny : L.nz);

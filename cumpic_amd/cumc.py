@@ -58,6 +58,8 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = ctypes.c_int
+        L.cumc_copy_probe.argtypes = [ctypes.c_long, ctypes.c_int, _D]
+        L.cumc_copy_probe.restype = ctypes.c_int
         L.cumc_error_string.restype = ctypes.c_char_p
         L.cumc_error_string.argtypes = [ctypes.c_int]
         L.cumc_last_error.restype = ctypes.c_char_p
@@ -69,6 +71,15 @@ def lib():
 def _p(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous float64"
     return a.ctypes.data_as(_D)
+
+
+def copy_probe(nbytes: int, reps: int = 5) -> float:
+    """Bandwidth (GB/s, read + write) of a plain device copy of nbytes, run reps times."""
+    g = ctypes.c_double()
+    r = lib().cumc_copy_probe(int(nbytes), int(reps), ctypes.byref(g))
+    if r:
+        raise CumcError(f"cumc_copy_probe: {lib().cumc_error_string(r).decode()}")
+    return g.value
 
 
 class Solver:

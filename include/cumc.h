@@ -102,6 +102,11 @@ int cumc_last_step_ms(cumc_ctx *ctx, double *ms);
 int cumc_profile(cumc_ctx *ctx, int on);
 int cumc_kernel_times(cumc_ctx *ctx, int n, double *total_ms, long *launches, const char **names);
 
+/* Diagnostic, no reference counterpart: a plain device-to-device copy of `bytes` bytes run `reps`
+ * times; reports the copy bandwidth (read + write bytes per second, in GB/s).  It is the
+ * known-size kernel against which HBM traffic counters are calibrated. */
+int cumc_copy_probe(long bytes, int reps, double *gbps);
+
 const char *cumc_error_string(int code);
 const char *cumc_last_error(cumc_ctx *ctx);
 
