@@ -1,19 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X CUMC3D MHD step on one GPU.
 
-Workload (default): the 3D periodic spectral MHD box of 256^3 zones, gamma = 5/3, beta = 1,
-mean B_z = 1, PPM + HLL + upwind CT + SSP-RK3 in FP64, seeded synthetic initial condition
-(cumpic_amd.ics.spectral_box, seed 1001).  One step = finddt + one RUNGEKUTTA (three RK stages).
+Default workload: the 3D periodic spectral MHD box of 256^3 zones, gamma = 5/3, beta = 1,
+mean B_z = 1, PPM + HLL + upwind CT + SSP-RK3, seeded synthetic initial condition
+(cumpic_amd.ics.spectral_box, seed 1001).  One step = finddt + one RUNGEKUTTA (three stages).
 
-Prints ONE JSON line with the zone-cycles/s of the timed steps, the roofline of the dominant
-kernel (HIP events on the library stream during the timed region) and the CPU restatement's
-rate on the host cores on a bounded sample of the same workload.
+`value` is the FP64 rate.  The same run also times the mixed form (FP32 reconstruction, HLL and
+EMFs; FP64 state) on the same workload and reports its rate, its speed-up and its error against
+FP64 on the same dt sequence.  Prints ONE JSON line; progress goes to stderr.
+
+  --workload torus      cylindrical torus 256x128x256, 50 steps (torus_cyn hooks)
+  --n 512 --steps 100   the 512^3 box of the mixed-precision workload
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -24,8 +26,8 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
-BYTES_PER_ZONE_CYCLE = 512.0  # 64 FP64 state words per RK3 step (read U0 x3, stage states, writes)
-# algorithmic bytes per unit of each kernel class: loads + stores of its compulsory FP64 words
+BYTES_PER_ZONE_CYCLE = 512.0  # 64 FP64 state words per RK3 step
+# algorithmic bytes per unit of each kernel class (FP64 form): compulsory loads + stores
 KERNEL_BYTES = {
     "flux_x": 15 * 8, "flux_y": 15 * 8, "flux_z": 15 * 8,  # 7 stencil vars + normal B in, 7 fluxes out
     "ecorn": 15 * 8,                                        # 6 B-fluxes + v + cell B in, 3 EMFs out
@@ -39,8 +41,8 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_units(name, n):
-    nx = ny = nz = n
+def kernel_units(name, cfg):
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
     return {
         "flux_x": (nx + 1) * (ny + 2) * (nz + 2),
         "flux_y": (nx + 2) * (ny + 1) * (nz + 2),
@@ -50,16 +52,16 @@ def kernel_units(name, n):
     }.get(name)
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if present."""
+def read_traffic(kernel, tag):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel:
+        if len(parts) >= 3 and parts[0] == tag and parts[1] == kernel:
             try:
-                return float(parts[1])
+                return float(parts[2])
             except ValueError:
                 return None
     return None
@@ -76,24 +78,59 @@ def cpu_baseline(prob, steps=1):
     wall = time.perf_counter() - t0
     state = o.get_state()
     o.close()
+    c = prob.cfg
     return {
-        "value": prob.cfg.zones * steps / wall,
+        "value": c.zones * steps / wall,
         "unit": "zone-cycles/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{prob.cfg.nx}x{prob.cfg.ny}x{prob.cfg.nz} spectral box, {steps} RK3 step(s) after initial_update, "
+        "sample": f"{c.nx}x{c.ny}x{c.nz} {prob.name}, {steps} RK3 step(s) after initial_update, "
                   f"{wall:.1f} s wall on {cores} OpenMP threads",
     }, state
+
+
+def timed_run(Solver, prob, prec, steps, warmup):
+    s = Solver.from_problem(prob, precision=prec)
+    s.initial_update()
+    if warmup:
+        s.run(warmup)
+    s.profile(True)
+    t0 = time.perf_counter()
+    dts, _ = s.run(steps)  # synchronous on return (hipStreamSynchronize)
+    wall = time.perf_counter() - t0
+    kt = s.kernel_times()
+    s.profile(False)
+    return s, dts, wall, kt
+
+
+def roofline(kt, cfg, tag):
+    dom = max((k for k in kt if k in KERNEL_BYTES), key=lambda k: kt[k][0])
+    tot_ms, launches = kt[dom]
+    avg_s = tot_ms / launches * 1e-3
+    scale = 0.5 if (tag == "mixed" and dom.startswith("flux")) else 1.0  # FP32 flux stores
+    alg = KERNEL_BYTES[dom] * kernel_units(dom, cfg) * scale
+    ach = alg / avg_s
+    return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
+            "traffic": read_traffic(dom, tag), "kernel": dom, "avg_launch_ms": tot_ms / launches}
+
+
+def fields(state, cfg):
+    sl = (slice(3, 3 + cfg.nz), slice(3, 3 + cfg.ny), slice(3, 3 + cfg.nx))
+    p, b = state["prim"][sl], state["bcell"][sl]
+    return {"rho": p[..., 0], "p": p[..., 4], "|v|": np.linalg.norm(p[..., 1:4], axis=-1),
+            "|B|": np.linalg.norm(b, axis=-1)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["box", "torus"], default="box")
     ap.add_argument("--n", type=int, default=256, help="zones per axis of the periodic box")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and full-size parity check")
+    ap.add_argument("--no-error", action="store_true", help="skip the mixed-vs-FP64 error replay")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -111,102 +148,108 @@ def main():
         copy_gbps = copy_probe(args.copy_probe_mib << 20, reps=3)
         log(f"[bench] copy probe {args.copy_probe_mib} MiB: {copy_gbps:.1f} GB/s")
 
-    n = args.n
-    log(f"[bench] building {n}^3 spectral initial condition")
-    prob = ics.spectral_box(n=n, seed=1001, steps=args.steps)
-    prec = MIXED if args.precision == "mixed" else FP64
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    if args.workload == "torus":
+        steps = args.steps or 50
+        log("[bench] building 256x128x256 cylindrical torus")
+        prob = ics.torus_cyn(256, 128, 256, steps=steps)
+        workload = ("cylindrical torus 256x128x256 (log R in [1,40], phi periodic, z in [-20,20]), beta=100, "
+                    "PW gravity + floors + inflow clamp, PPM+HLL+upwind CT+SSP-RK3")
+    else:
+        n = args.n
+        steps = args.steps or (100 if n >= 512 else 200)
+        log(f"[bench] building {n}^3 spectral initial condition")
+        prob = ics.spectral_box(n=n, seed=1001, steps=steps)
+        workload = f"3D periodic spectral MHD box {n}^3, PPM+HLL+upwind CT+SSP-RK3"
+    cfg = prob.cfg
 
-    parity = None
-    cpu = None
-    if not args.no_cpu:
+    cpu, parity = None, None
+    if not args.no_cpu and "fp64" in forms:
         log("[bench] CPU restatement: 1 step on the host cores")
         cpu, ostate = cpu_baseline(prob, steps=1)
         log(f"[bench] cpu {cpu['value']:.3e} zone-cycles/s")
-        g = Solver.from_problem(prob, precision=prec)
+        g = Solver.from_problem(prob, precision=FP64)
         g.initial_update()
         g.run(1)
         gs = g.download()
         g.close()
-        if prec == FP64:
-            same = all(np.array_equal(gs[k], ostate[k], equal_nan=True) for k in ("prim", "eps", "bcell"))
-            parity = {"one_step_bitwise": bool(same)}
-        else:
-            a = gs["prim"][3:3 + n, 3:3 + n, 3:3 + n, 0]
-            b = ostate["prim"][3:3 + n, 3:3 + n, 3:3 + n, 0]
-            parity = {"one_step_rel_l1_rho": float(np.sum(np.abs(a - b)) / np.sum(np.abs(b)))}
+        same = all(np.array_equal(gs[k], ostate[k], equal_nan=True) for k in ("prim", "eps", "bcell"))
+        parity = {"one_step_bitwise": bool(same)}
         del ostate, gs
         log(f"[bench] full-size parity after one step: {parity}")
 
-    log("[bench] GPU: upload, initial_update, warmup")
-    s = Solver.from_problem(prob, precision=prec)
-    del prob
-    s.initial_update()
-    if args.warmup:
-        s.run(args.warmup)
-    s.profile(True)
-    log(f"[bench] timing {args.steps} steps")
-    t0 = time.perf_counter()
-    dts, _ = s.run(args.steps)  # synchronous on return (hipStreamSynchronize)
-    wall = time.perf_counter() - t0
-    kt = s.kernel_times()
-    s.profile(False)
-    dsig, dabs = s.find_divb()
-    final = s.download()["prim"][3:3 + n, 3:3 + n, 3:3 + n]
-    finite = bool(np.isfinite(final).all())
-    del final
-    zones = n ** 3
-    value = zones * args.steps / wall
+    results = {}
+    ref_state, ref_dts, warm_dts = None, None, None
+    for form in forms:
+        prec = FP64 if form == "fp64" else MIXED
+        log(f"[bench] {form}: upload, initial_update, {args.warmup} warm-up, {steps} timed steps")
+        s, dts, wall, kt = timed_run(Solver, prob, prec, steps, args.warmup)
+        dsig, dabs = s.find_divb()
+        st = s.download()
+        finite = bool(np.isfinite(st["prim"]).all())
+        r = {"value": cfg.zones * steps / wall, "ms_per_step": wall / steps * 1e3, "roofline": roofline(kt, cfg, form),
+             "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()}, "divb_max_abs": dabs,
+             "state_finite": finite}
+        s.close()
+        if form == "fp64":
+            ref_dts = dts
+            ref_state = fields(st, cfg)
+        results[form] = r
+        del st
+        log(f"[bench] {form}: {r['value']:.4e} zone-cycles/s ({r['ms_per_step']:.2f} ms/step)")
 
-    # dominant kernel of the timed region
-    dom = max((k for k in kt if k in KERNEL_BYTES), key=lambda k: kt[k][0])
-    tot_ms, launches = kt[dom]
-    avg_s = tot_ms / launches * 1e-3
-    alg_bytes = KERNEL_BYTES[dom] * kernel_units(dom, n)
-    achieved = alg_bytes / avg_s
-    roof = {
-        "bound": "hbm",
-        "achieved": achieved / 1e9,
-        "peak": HBM_PEAK / 1e9,
-        "unit": "GB/s",
-        "frac": achieved / HBM_PEAK,
-        "traffic": read_traffic(dom),
-        "kernel": dom,
-        "avg_launch_ms": tot_ms / launches,
-    }
-    step_alg = value * BYTES_PER_ZONE_CYCLE
+    if "mixed" in results and ref_state is not None and not args.no_error:
+        # error of the mixed form against FP64 on the FP64 run's dt sequence (warm-up + timed)
+        log("[bench] mixed vs FP64 error replay")
+        f64 = Solver.from_problem(prob, precision=FP64)
+        f64.initial_update()
+        warm_dts = f64.run(args.warmup)[0] if args.warmup else np.zeros(0)
+        f64.close()
+        s = Solver.from_problem(prob, precision=MIXED)
+        s.initial_update()
+        for dt in list(warm_dts) + list(ref_dts):
+            s.rungekutta(float(dt))
+        mf = fields(s.download(), cfg)
+        s.close()
+        errs = {k: float(np.sum(np.abs(mf[k] - ref_state[k])) / np.sum(np.abs(ref_state[k]))) for k in mf}
+        results["mixed"]["rel_l1_vs_fp64"] = errs
+        log(f"[bench] mixed rel L1 vs FP64: {errs}")
+
+    main_form = "fp64" if "fp64" in results else forms[0]
+    m = results[main_form]
+    step_alg = m["value"] * BYTES_PER_ZONE_CYCLE
     out = {
         "metric": "zone-cycles/s FP64 vs mixed, % HBM roofline, adjoint/forward step cost",
-        "value": value,
+        "value": m["value"],
         "unit": "zone-cycles/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == FP64 else "f64-state/f32-flux",
+        "dtype": "f64" if main_form == "fp64" else "f64-state/f32-flux",
         "data": "synthetic",
-        "config": {
-            "workload": f"3D periodic spectral MHD box {n}^3, PPM+HLL+upwind CT+SSP-RK3, "
-                        f"{'FP64' if prec == FP64 else 'mixed FP32 reconstruction/HLL/EMF, FP64 state'}",
-            "zones": zones,
-            "grid": [n, n, n],
-            "gamma": 5.0 / 3.0,
-            "cfl": 0.4,
-            "seed": 1001,
-        },
-        "roofline": roof,
+        "config": {"workload": workload, "zones": cfg.zones, "grid": [cfg.nx, cfg.ny, cfg.nz],
+                   "gamma": cfg.ggas, "cfl": cfg.cfl, "seed": 1001 if args.workload == "box" else 1003,
+                   "form_of_value": main_form},
+        "roofline": m["roofline"],
         "cpu_baseline": cpu,
         "step_roofline": {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE, "achieved_GBps": step_alg / 1e9,
                           "frac": step_alg / HBM_PEAK},
-        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kt.items()},
-        "divb_max_abs": dabs,
-        "state_finite": finite,
-        "copy_probe_GBps": copy_gbps,
+        "kernel_ms_per_step": m["kernel_ms_per_step"],
+        "divb_max_abs": m["divb_max_abs"],
+        "state_finite": m["state_finite"],
         "parity": parity,
-        "dt_first_last": [float(dts[0]), float(dts[-1])] if len(dts) else None,
+        "copy_probe_GBps": copy_gbps,
     }
+    if "mixed" in results and main_form != "mixed":
+        mx = results["mixed"]
+        out["mixed"] = {"value": mx["value"], "ms_per_step": mx["ms_per_step"],
+                        "speedup_vs_fp64": mx["value"] / m["value"], "rel_l1_vs_fp64": mx.get("rel_l1_vs_fp64"),
+                        "divb_max_abs": mx["divb_max_abs"], "state_finite": mx["state_finite"],
+                        "roofline": mx["roofline"], "kernel_ms_per_step": mx["kernel_ms_per_step"]}
     print(json.dumps(out), flush=True)
 
 

@@ -4,31 +4,47 @@ from __future__ import annotations
 import os
 import shutil
 import subprocess
+import tempfile
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libcumc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-# -ffp-contract=off: no fused multiply-add, every operation rounds as in the reference source.
-# -packed-fp32-ops off: the compiler must not form v_pk_{add,mul,fma}_f32.
-HIP_FLAGS = [
-    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",
-    "-ffp-contract=off", "-fno-fast-math",
+# Common flags.  -packed-fp32-ops off: the compiler must not form v_pk_{add,mul,fma}_f32.
+COMMON = [
+    "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
     "-Wno-unused-result", "-Wno-unused-value",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# FP64 drop-in path: no fused multiply-add, every operation rounds as in the reference source.
+FP64_FLAGS = ["-ffp-contract=off", "-fno-fast-math"]
+# Mixed form: the FP32 reconstruction / Riemann / EMF kernels may contract multiply-adds and use
+# the hardware FP32 reciprocal and square root.
+MIXED_FLAGS = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
+
+UNITS = [("cumc.hip", FP64_FLAGS), ("cumc_mixed.hip", MIXED_FLAGS)]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed:\n" + " ".join(cmd) + "\n" + r.stdout)
+    return r.stdout
 
 
 def build_library(verbose: bool = False) -> str:
-    src = os.path.join(PKG, "csrc", "cumc.hip")
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, src]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout)
-    if verbose:
-        print(r.stdout)
+    inc = ["-I", os.path.join(ROOT, "include")]
+    with tempfile.TemporaryDirectory() as td:
+        objs = []
+        for src, flags in UNITS:
+            obj = os.path.join(td, src + ".o")
+            out = _run([HIPCC, *COMMON, *flags, *inc, "-c", "-o", obj, os.path.join(PKG, "csrc", src)])
+            if verbose:
+                print(out)
+            objs.append(obj)
+        tmp = LIB + ".tmp"
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 
@@ -54,8 +70,6 @@ def build_fortran_host() -> str | None:
     os.makedirs(mod, exist_ok=True)
     cmd = [fc, "-O2", "-J", mod, *srcs, "-L", PKG, "-lcumc", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/..",
            "-o", FHOST_EXE + ".tmp"]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("flang failed:\n" + r.stdout)
+    _run(cmd)
     os.replace(FHOST_EXE + ".tmp", FHOST_EXE)
     return FHOST_EXE

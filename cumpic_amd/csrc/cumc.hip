@@ -11,88 +11,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/cumc.h"
-#include "kernels.hpp"
+#include "ctx.hpp"
 
-using namespace cumc;
+// float instantiations live in cumc_mixed.hip (built with FP32 contraction and fast division)
+void stage_mixed(cumc_ctx *ctx, int st);
+void fluxes_mixed(cumc_ctx *ctx, int dir, double *aos_tmp);
 
-struct cumc_ctx {
-    cumc_config cfg;
-    Lay L;
-    Model M;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double *pool = nullptr;  // all FP64 fields
-    size_t pool_bytes = 0;
-    void *fluxpool = nullptr;
-    Fields<double> F64;
-    Fields<float> F32;
-    double *tab[3] = {nullptr, nullptr, nullptr};
-    double *dt_dev = nullptr;           // [0] dt of the current step, [1] dt found by stage 3
-    unsigned long long *red = nullptr;  // reduction slots
-    double *Ub = nullptr;               // third conserved buffer (stage 2 output)
-    bool dt_valid = false;              // dt_dev[1] holds finddt of the current state
-    bool grid_set = false;
-    double last_ms = 0.0;
-    std::string err;
-    // per-kernel-class HIP event timing (cumc_profile / cumc_kernel_times)
-    bool prof = false;
-    std::vector<hipEvent_t> evpool;
-    std::vector<int> evcls;
-    size_t evn = 0;
-    double kc_ms[16] = {0};
-    long kc_count[16] = {0};
-};
-
-enum { KC_FLUX_X, KC_FLUX_Y, KC_FLUX_Z, KC_ECORN, KC_TAIL, KC_GHOST, KC_FINDDT, KC_N };
-static const char *kc_names[KC_N] = {"flux_x", "flux_y", "flux_z", "ecorn", "tail", "ghost", "finddt"};
-
-// record the start (cls >= 0) or end (cls < 0) of a kernel class on the context stream
-static void mark(cumc_ctx *ctx, int cls)
-{
-    if (!ctx->prof) return;
-    if (ctx->evn == ctx->evpool.size()) {
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
-        ctx->evpool.push_back(e);
-        ctx->evcls.push_back(0);
-    }
-    ctx->evcls[ctx->evn] = cls;
-    hipEventRecord(ctx->evpool[ctx->evn], ctx->stream);
-    ctx->evn++;
-}
-
-// fold recorded event pairs into the per-class totals (stream must be idle)
-static void collect(cumc_ctx *ctx)
-{
-    for (size_t i = 0; i + 1 < ctx->evn; i += 2) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ctx->evpool[i], ctx->evpool[i + 1]) == hipSuccess) {
-            const int c = ctx->evcls[i];
-            ctx->kc_ms[c] += ms;
-            ctx->kc_count[c] += 1;
-        }
-    }
-    ctx->evn = 0;
-}
-
-static const char *g_err_names[] = {"ok", "invalid argument", "HIP error", "non-finite time step",
-                                    "out of device memory", "call out of order"};
-
-#define HIPCHK(ctx, call)                                                                       \
-    do {                                                                                        \
-        hipError_t e_ = (call);                                                                 \
-        if (e_ != hipSuccess) {                                                                 \
-            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                    \
-            return CUMC_EHIP;                                                                   \
-        }                                                                                       \
-    } while (0)
-
-static int fail(cumc_ctx *ctx, int code, const std::string &msg)
-{
-    if (ctx) ctx->err = msg;
-    return code;
-}
 
 // setup.f90:121-230 parity factors
 static void setup_bfac(Model &M)
@@ -279,17 +203,6 @@ extern "C" int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, 
     return CUMC_OK;
 }
 
-static Tabs tabs_of(const cumc_ctx *ctx)
-{
-    Tabs tb;
-    tb.t[0] = ctx->tab[0]; tb.t[1] = ctx->tab[1]; tb.t[2] = ctx->tab[2];
-    tb.len[0] = ctx->L.nx + 7; tb.len[1] = ctx->L.ny + 7; tb.len[2] = ctx->L.nz + 7;
-    return tb;
-}
-
-static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
-// 1D launch size for a gx x gy tile grid remapped by xcd_tile (a multiple of the XCD count)
-static inline unsigned xcd_blocks(int gx, int gy) { return (unsigned)(((long)gx * gy + NXCD - 1) / NXCD * NXCD); }
 
 static int conv_in(cumc_ctx *ctx, const double *host, int nv, double *dst)
 {
@@ -343,7 +256,7 @@ extern "C" int cumc_download(cumc_ctx *ctx, double *prim, double *eps, double *b
 // ---------------------------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------------------------
-static void launch_ghost(cumc_ctx *ctx)
+void launch_ghost(cumc_ctx *ctx)
 {
     const Lay &L = ctx->L;
     hipStream_t s = ctx->stream;
@@ -361,87 +274,6 @@ static void launch_ghost(cumc_ctx *ctx)
     }
 }
 
-template <typename R>
-static void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
-{
-    const Lay &L = ctx->L;
-    hipStream_t s = ctx->stream;
-    Tabs tb = tabs_of(ctx);
-    const Model &M = ctx->M;
-    const bool ppm = M.recon == CUMC_PPMC;
-    if (!only_dir || only_dir == 1) {
-        mark(ctx, KC_FLUX_X);
-        const long waves = ((long)(L.ny + 2) * L.px + XS_FACES - 1) / XS_FACES;
-        dim3 grid(nblk(waves * 64, 256), L.nz + 2);
-        if (ppm) hipLaunchKernelGGL((k_flux_x<R, 4>), grid, dim3(256), 0, s, L, F, tb, M);
-        else hipLaunchKernelGGL((k_flux_x<R, 1>), grid, dim3(256), 0, s, L, F, tb, M);
-        mark(ctx, -1);
-    }
-    const int seg = 48;
-    if (!only_dir || only_dir == 2) {
-        mark(ctx, KC_FLUX_Y);
-        const int nseg = (L.ny + 1 + seg - 1) / seg;
-        const int sl = (L.ny + 1 + nseg - 1) / nseg;
-        dim3 grid(nblk((long)L.px * (L.nz + 2), 128), nseg);
-        if (ppm) hipLaunchKernelGGL((k_flux_walk<R, 2, 4>), grid, dim3(128), 0, s, L, F, tb, M, sl);
-        else hipLaunchKernelGGL((k_flux_walk<R, 2, 1>), grid, dim3(128), 0, s, L, F, tb, M, sl);
-        mark(ctx, -1);
-    }
-    if (!only_dir || only_dir == 3) {
-        mark(ctx, KC_FLUX_Z);
-        const int nseg = (L.nz + 1 + seg - 1) / seg;
-        const int sl = (L.nz + 1 + nseg - 1) / nseg;
-        dim3 grid(nblk((long)L.px * (L.ny + 2), 128), nseg);
-        if (ppm) hipLaunchKernelGGL((k_flux_walk<R, 3, 4>), grid, dim3(128), 0, s, L, F, tb, M, sl);
-        else hipLaunchKernelGGL((k_flux_walk<R, 3, 1>), grid, dim3(128), 0, s, L, F, tb, M, sl);
-        mark(ctx, -1);
-    }
-}
-
-template <typename R, int STAGE>
-static void launch_tail(cumc_ctx *ctx, const Fields<R> &F, const double *Uin, double *Uout)
-{
-    const Lay &L = ctx->L;
-    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + ZCHUNK) / ZCHUNK;
-    dim3 grid(xcd_blocks(gx, gy));
-    Tabs tb = tabs_of(ctx);
-    unsigned long long *dtn = (unsigned long long *)(ctx->dt_dev + 1);
-    if (ctx->M.coordinate == CUMC_CYLINDRICAL)
-        hipLaunchKernelGGL((k_tail<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy);
-    else
-        hipLaunchKernelGGL((k_tail<R, STAGE, 1>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy);
-}
-
-// one RK stage: fluxes of the three sweeps, corner EMFs, stage tail, ghost fill.
-// Conserved buffers: stage 1 U0 -> U1, stage 2 U1 -> Ub, stage 3 Ub -> U1.
-template <typename R>
-static void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
-{
-    const Lay &L = ctx->L;
-    launch_fluxes<R>(ctx, F);
-    mark(ctx, KC_ECORN);
-    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + ZCHUNK) / ZCHUNK;
-    hipLaunchKernelGGL((k_ecorn<R>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, F, gx, gy);
-    mark(ctx, -1);
-    mark(ctx, KC_TAIL);
-    if (st == 1) {
-        launch_tail<R, 1>(ctx, F, ctx->F64.U0, ctx->F64.U1);
-    } else if (st == 2) {
-        launch_tail<R, 2>(ctx, F, ctx->F64.U1, ctx->Ub);
-    } else {
-        const double init = 1.0e10;
-        unsigned long long bits;
-        std::memcpy(&bits, &init, 8);
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (unsigned long long *)(ctx->dt_dev + 1), bits, 1);
-        launch_tail<R, 3>(ctx, F, ctx->Ub, ctx->F64.U1);
-    }
-    mark(ctx, -1);
-    mark(ctx, KC_GHOST);
-    launch_ghost(ctx);
-    mark(ctx, -1);
-}
 
 static void swap_u(cumc_ctx *ctx)
 {
@@ -455,7 +287,7 @@ static void swap_u(cumc_ctx *ctx)
 static void enqueue_step(cumc_ctx *ctx)
 {
     for (int st = 1; st <= 3; st++) {
-        if (ctx->cfg.precision == CUMC_MIXED) stage<float>(ctx, ctx->F32, st);
+        if (ctx->cfg.precision == CUMC_MIXED) stage_mixed(ctx, st);
         else stage<double>(ctx, ctx->F64, st);
     }
     swap_u(ctx);
@@ -609,9 +441,7 @@ extern "C" int cumc_sweep_flux(cumc_ctx *ctx, int dir, double *out)
     double *tmp = nullptr;
     HIPCHK(ctx, hipMalloc(&tmp, (size_t)ncell * 7 * sizeof(double)));
     if (ctx->cfg.precision == CUMC_MIXED) {
-        launch_fluxes<float>(ctx, ctx->F32, dir);
-        hipLaunchKernelGGL((k_flux_to_aos<float>), dim3(nblk(ncell, 256)), dim3(256), 0, ctx->stream, L,
-                           ctx->F32.F[dir - 1], tmp);
+        fluxes_mixed(ctx, dir, tmp);
     } else {
         launch_fluxes<double>(ctx, ctx->F64, dir);
         hipLaunchKernelGGL((k_flux_to_aos<double>), dim3(nblk(ncell, 256)), dim3(256), 0, ctx->stream, L,

@@ -584,7 +584,7 @@ __global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model
 }
 
 // initial_update's FROMRVETOU + CUSTOM_CHECKRHO + EOS on [0,n]^3 (initial.f90:103-126)
-__global__ __launch_bounds__(256) void k_init(Lay L, Fields<double> F, Model M, double *Uout)
+static __global__ __launch_bounds__(256) void k_init(Lay L, Fields<double> F, Model M, double *Uout)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
@@ -726,7 +726,7 @@ __global__ __launch_bounds__(256) void k_ghost(Lay L, Fields<double> F, Model M)
 }
 
 // torus_cyn CUSTOM_BOUNDARY (custom.f90:47-81): no inflow through the R ghost zones ...
-__global__ __launch_bounds__(256) void k_custom_boundary_x(Lay L, Fields<double> F)
+static __global__ __launch_bounds__(256) void k_custom_boundary_x(Lay L, Fields<double> F)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nyz = (long)(L.ny + 6) * (L.nz + 6);
@@ -741,7 +741,7 @@ __global__ __launch_bounds__(256) void k_custom_boundary_x(Lay L, Fields<double>
 }
 
 // ... and through the z ghost zones
-__global__ __launch_bounds__(256) void k_custom_boundary_z(Lay L, Fields<double> F)
+static __global__ __launch_bounds__(256) void k_custom_boundary_z(Lay L, Fields<double> F)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ps = L.ps;
@@ -776,7 +776,7 @@ __device__ inline double wave_max(double v)
     return v;
 }
 
-__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n)
+static __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -860,7 +860,7 @@ __global__ __launch_bounds__(256) void k_divb(Lay L, Fields<double> F, Tabs tb, 
 // Layout conversion between the Fortran AoS arrays and the device planes
 // ------------------------------------------------------------------------------------------
 // aos[(cell)*nv + v] with cell = ((l+2)*(ny+6) + (k+2))*(nx+6) + (j+2)
-__global__ void k_aos_to_soa(Lay L, const double *aos, int nv, double *soa)
+static __global__ void k_aos_to_soa(Lay L, const double *aos, int nv, double *soa)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long sx = L.nx + 6, sy = L.ny + 6, sz = L.nz + 6;
@@ -870,7 +870,7 @@ __global__ void k_aos_to_soa(Lay L, const double *aos, int nv, double *soa)
     for (int v = 0; v < nv; v++) soa[g + v * L.ps] = aos[t * nv + v];
 }
 
-__global__ void k_soa_to_aos(Lay L, const double *soa, int nv, double *aos)
+static __global__ void k_soa_to_aos(Lay L, const double *soa, int nv, double *aos)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long sx = L.nx + 6, sy = L.ny + 6, sz = L.nz + 6;
