@@ -29,6 +29,8 @@ struct cumc_ctx {
     double *Ub = nullptr;               // third conserved buffer (stage 2 output)
     bool dt_valid = false;              // dt_dev[1] holds finddt of the current state
     bool grid_set = false;
+    int seg = 48;     // target faces per pencil segment of the y / z sweeps
+    int zchunk = 32;  // planes per column tile of the EMF and stage-tail kernels
     double last_ms = 0.0;
     std::string err;
     // per-kernel-class HIP event timing (cumc_profile / cumc_kernel_times)
@@ -102,6 +104,16 @@ static inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b);
 // 1D launch size for a gx x gy tile grid remapped by xcd_tile (a multiple of the XCD count)
 static inline unsigned xcd_blocks(int gx, int gy) { return (unsigned)(((long)gx * gy + NXCD - 1) / NXCD * NXCD); }
 
+// the step result (stage 3 output) becomes the next step's starting state
+static inline void swap_u(cumc_ctx *ctx)
+{
+    double *t = ctx->F64.U0;
+    ctx->F64.U0 = ctx->F64.U1;
+    ctx->F64.U1 = t;
+    ctx->F32.U0 = ctx->F64.U0;
+    ctx->F32.U1 = ctx->F64.U1;
+}
+
 // ghost passes + torus clamp (defined in cumc.hip)
 void launch_ghost(cumc_ctx *ctx);
 
@@ -121,7 +133,7 @@ void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
         else hipLaunchKernelGGL((k_flux_x<R, 1>), grid, dim3(256), 0, s, L, F, tb, M);
         mark(ctx, -1);
     }
-    const int seg = 48;
+    const int seg = ctx->seg;
     if (!only_dir || only_dir == 2) {
         mark(ctx, KC_FLUX_Y);
         const int nseg = (L.ny + 1 + seg - 1) / seg;
@@ -146,16 +158,17 @@ template <typename R, int STAGE>
 void launch_tail(cumc_ctx *ctx, const Fields<R> &F, const double *Uin, double *Uout)
 {
     const Lay &L = ctx->L;
-    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + ZCHUNK) / ZCHUNK;
+    const int zc = ctx->zchunk;
+    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + zc) / zc;
     dim3 grid(xcd_blocks(gx, gy));
     Tabs tb = tabs_of(ctx);
     unsigned long long *dtn = (unsigned long long *)(ctx->dt_dev + 1);
     if (ctx->M.coordinate == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((k_tail<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy);
+                           Uin, Uout, dtn, gx, gy, zc);
     else
         hipLaunchKernelGGL((k_tail<R, STAGE, 1>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy);
+                           Uin, Uout, dtn, gx, gy, zc);
 }
 
 // one RK stage: fluxes of the three sweeps, corner EMFs, stage tail, ghost fill.
@@ -166,8 +179,9 @@ void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
     const Lay &L = ctx->L;
     launch_fluxes<R>(ctx, F);
     mark(ctx, KC_ECORN);
-    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + ZCHUNK) / ZCHUNK;
-    hipLaunchKernelGGL((k_ecorn<R>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, F, gx, gy);
+    const int zc = ctx->zchunk;
+    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + zc) / zc;
+    hipLaunchKernelGGL((k_ecorn<R>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, F, gx, gy, zc);
     mark(ctx, -1);
     mark(ctx, KC_TAIL);
     if (st == 1) {

@@ -275,13 +275,6 @@ void launch_ghost(cumc_ctx *ctx)
 }
 
 
-static void swap_u(cumc_ctx *ctx)
-{
-    std::swap(ctx->F64.U0, ctx->F64.U1);
-    ctx->F32.U0 = ctx->F64.U0;
-    ctx->F32.U1 = ctx->F64.U1;
-}
-
 // RUNGEKUTTA body on the stream, dt already in dt_dev[0]; leaves finddt of the new state in
 // dt_dev[1]
 static void enqueue_step(cumc_ctx *ctx)
@@ -452,6 +445,101 @@ extern "C" int cumc_sweep_flux(cumc_ctx *ctx, int dir, double *out)
     hipFree(tmp);
     HIPCHK(ctx, e);
     return CUMC_OK;
+}
+
+extern "C" int cumc_get_tuning(cumc_ctx *ctx, cumc_tuning *out)
+{
+    if (!ctx || !out) return CUMC_EINVAL;
+    std::memset(out, 0, sizeof(*out));
+    out->seg = ctx->seg;
+    out->zchunk = ctx->zchunk;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_set_tuning(cumc_ctx *ctx, const cumc_tuning *t)
+{
+    if (!ctx || !t || t->seg < 1 || t->zchunk < 1) return CUMC_EINVAL;
+    ctx->seg = t->seg;
+    ctx->zchunk = t->zchunk;
+    return CUMC_OK;
+}
+
+// Tuner: one warm-up step, then one timed step per candidate with per-kernel-class HIP events:
+// the segment length is scored by the y + z sweeps, the plane chunk by the EMF + stage-tail
+// kernels.  The FP64 fields, the dt slots and the U0/U1 assignment are saved and restored, so
+// the state is as before; the per-class timing totals are cleared.
+extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
+{
+    if (!ctx) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    void *save = nullptr;
+    if (hipMalloc(&save, ctx->pool_bytes) != hipSuccess) return fail(ctx, CUMC_ENOMEM, "tuner state copy");
+    double dts[2];
+    double *u0 = ctx->F64.U0, *u1 = ctx->F64.U1;
+    const bool valid = ctx->dt_valid, prof = ctx->prof;
+    const int seg0 = ctx->seg, zc0 = ctx->zchunk;
+    hipMemcpyAsync(save, ctx->pool, ctx->pool_bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    hipMemcpyAsync(dts, ctx->dt_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    int rc = CUMC_OK;
+    double dt;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, CUMC_EHIP, "tuner state copy");
+    if (rc == CUMC_OK) rc = finddt_to_current(ctx, &dt);
+    auto timed_step = [&](double *cls) -> int {
+        ctx->prof = true;
+        ctx->evn = 0;
+        for (int c = 0; c < KC_N; c++) ctx->kc_ms[c] = 0.0;
+        hipMemcpyAsync(ctx->dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+        enqueue_step(ctx);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, CUMC_EHIP, "tuner step");
+        collect(ctx);
+        for (int c = 0; c < KC_N; c++) cls[c] = ctx->kc_ms[c];
+        return CUMC_OK;
+    };
+    double cls[KC_N];
+    int best_seg = seg0, best_zc = zc0;
+    if (rc == CUMC_OK) rc = timed_step(cls);  // warm-up
+    if (rc == CUMC_OK) {
+        const int segs[] = {16, 24, 32, 48, 64, 96, 128};
+        double bt = 1e300;
+        for (int sg : segs) {
+            ctx->seg = sg;
+            if ((rc = timed_step(cls))) break;
+            const double v = cls[KC_FLUX_Y] + cls[KC_FLUX_Z];
+            if (v < bt) { bt = v; best_seg = sg; }
+        }
+        ctx->seg = best_seg;
+    }
+    if (rc == CUMC_OK) {
+        const int zcs[] = {4, 8, 16, 32, 64};
+        double bt = 1e300;
+        for (int z : zcs) {
+            ctx->zchunk = z;
+            if ((rc = timed_step(cls))) break;
+            const double v = cls[KC_ECORN] + cls[KC_TAIL];
+            if (v < bt) { bt = v; best_zc = z; }
+        }
+    }
+    hipMemcpyAsync(ctx->pool, save, ctx->pool_bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    hipMemcpyAsync(ctx->dt_dev, dts, 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipFree(save);
+    ctx->F64.U0 = u0;
+    ctx->F64.U1 = u1;
+    ctx->F32.U0 = u0;
+    ctx->F32.U1 = u1;
+    ctx->dt_valid = valid;
+    ctx->prof = prof;
+    ctx->evn = 0;
+    for (int c = 0; c < KC_N; c++) {
+        ctx->kc_ms[c] = 0.0;
+        ctx->kc_count[c] = 0;
+    }
+    if (rc == CUMC_OK && e != hipSuccess) rc = fail(ctx, CUMC_EHIP, std::string("tuner restore: ") + hipGetErrorString(e));
+    ctx->seg = rc == CUMC_OK ? best_seg : seg0;
+    ctx->zchunk = rc == CUMC_OK ? best_zc : zc0;
+    if (rc == CUMC_OK && out) cumc_get_tuning(ctx, out);
+    return rc;
 }
 
 extern "C" int cumc_last_step_ms(cumc_ctx *ctx, double *ms)

@@ -1,0 +1,657 @@
+// cumc_batch.hip -- batched form: many independent subdomains of uneven sizes advanced by one
+// persistent launch per phase, optionally captured in a hipGraph.
+//
+// Each phase kernel walks a work list of (box, tile) items that covers every box: item i
+// belongs to the box b with pre[b] <= i < pre[b+1] (prefix sums of the per-box tile counts),
+// and a grid of G workgroups strides over the items (G is a tuning choice; G = number of
+// items gives one workgroup per tile).  The tile bodies are the ones the single-box kernels run
+// (kernels.hpp), compiled with the same FP64 flags, so a box advanced in a batch is bit-identical
+// to the same box advanced alone.
+//
+// Per-box time step: every box keeps [0] current and [1] next dt in its own device slots.  A
+// one-workgroup kernel at the start of each step moves next -> current (or takes the frozen
+// value) and appends it to a device history, so no host round trip sits inside a step and a
+// multi-step run can be replayed from one graph.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace {
+
+struct BoxDesc {
+    Lay L;
+    Fields<double> F;  // U0 / U1 as of the start of the run (steps alternate them by parity)
+    Tabs tb;
+    Model M;
+    double *dt;  // [0] current step, [1] next step (stage-3 reduction target)
+    double *Ub;
+    int fx_gx;             // x sweep: tiles per plane
+    int fy_gx, fy_sl;      // y sweep: tiles per segment row, faces per segment
+    int fz_gx, fz_sl;      // z sweep
+    int c_gx, zc;          // EMF / stage tail column tiles
+};
+
+enum { P_FX, P_FY, P_FZ, P_EC, P_TAIL, P_GX, P_GY, P_GZ, P_DT, P_N };
+
+// largest b in [0, nbox) with pre[b] <= it  (pre[0] = 0 <= it < pre[nbox])
+__device__ inline int box_of(const int *pre, int nbox, int it)
+{
+    int lo = 0, hi = nbox;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= it) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline Fields<double> fields_at(const BoxDesc &d, int par)
+{
+    Fields<double> F = d.F;
+    if (par) {
+        F.U0 = d.F.U1;
+        F.U1 = d.F.U0;
+    }
+    return F;
+}
+
+template <int RECON>
+__global__ __launch_bounds__(256) void kb_flux_x(const BoxDesc *__restrict__ D, const int *__restrict__ pre, int nbox)
+{
+    const int total = pre[nbox];
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int b = box_of(pre, nbox, it);
+        const int loc = it - pre[b];
+        const BoxDesc &d = D[b];
+        flux_x_tile<double, RECON>(d.L, d.F, d.tb, d.M, loc % d.fx_gx, loc / d.fx_gx);
+    }
+}
+
+template <int DIR, int RECON>
+__global__ __launch_bounds__(128) void kb_flux_walk(const BoxDesc *__restrict__ D, const int *__restrict__ pre,
+                                                    int nbox)
+{
+    const int total = pre[nbox];
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int b = box_of(pre, nbox, it);
+        const int loc = it - pre[b];
+        const BoxDesc &d = D[b];
+        const int gx = DIR == 2 ? d.fy_gx : d.fz_gx;
+        const int sl = DIR == 2 ? d.fy_sl : d.fz_sl;
+        flux_walk_tile<double, DIR, RECON>(d.L, d.F, d.tb, d.M, sl, loc % gx, loc / gx);
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_ecorn(const BoxDesc *__restrict__ D, const int *__restrict__ pre, int nbox)
+{
+    const int total = pre[nbox];
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int b = box_of(pre, nbox, it);
+        const int loc = it - pre[b];
+        const BoxDesc &d = D[b];
+        ecorn_tile<double>(d.L, d.F, loc % d.c_gx, loc / d.c_gx, d.zc);
+    }
+}
+
+// trip counts are block-uniform, so the stage-3 block reduction inside tail_tile is safe
+template <int STAGE, int COORD>
+__global__ __launch_bounds__(256) void kb_tail(const BoxDesc *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                               int par)
+{
+    const int total = pre[nbox];
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int b = box_of(pre, nbox, it);
+        const int loc = it - pre[b];
+        const BoxDesc &d = D[b];
+        const Fields<double> F = fields_at(d, par);
+        const double *Uin = STAGE == 1 ? F.U0 : (STAGE == 2 ? F.U1 : d.Ub);
+        double *Uout = STAGE == 2 ? d.Ub : F.U1;
+        tail_tile<double, STAGE, COORD>(d.L, F, d.tb, d.M, d.dt, Uin, Uout, (unsigned long long *)(d.dt + 1),
+                                        loc % d.c_gx, loc / d.c_gx, d.zc);
+    }
+}
+
+template <int AX>
+__global__ __launch_bounds__(256) void kb_ghost(const BoxDesc *__restrict__ D, const int *__restrict__ pre, int nbox)
+{
+    const int total = pre[nbox];
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int b = box_of(pre, nbox, it);
+        const long loc = it - pre[b];
+        const BoxDesc &d = D[b];
+        ghost_thread<AX>(d.L, d.F, d.M, loc * blockDim.x + threadIdx.x);
+    }
+}
+
+template <int COORD>
+__global__ __launch_bounds__(256) void kb_finddt(const BoxDesc *__restrict__ D, const int *__restrict__ pre, int nbox)
+{
+    const int total = pre[nbox];
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int b = box_of(pre, nbox, it);
+        const BoxDesc &d = D[b];
+        finddt_tile<COORD>(d.L, d.F, d.tb, d.M, (unsigned long long *)(d.dt + 1), it - pre[b]);
+    }
+}
+
+// next-dt slots to the reduction identity 1e10 (rungekutta.f90:205)
+static __global__ void kb_dt_reset(const BoxDesc *__restrict__ D, int nbox)
+{
+    for (int b = threadIdx.x; b < nbox; b += blockDim.x) D[b].dt[1] = 1.0e10;
+}
+
+// step start: current dt = next dt (or the frozen value), appended to the history at the
+// device step counter, which then advances.  One workgroup.
+static __global__ void kb_dt_begin(const BoxDesc *__restrict__ D, int nbox, double *hist, int *ctr, int frozen)
+{
+    const long s = *ctr;
+    for (int b = threadIdx.x; b < nbox; b += blockDim.x) {
+        double *dt = D[b].dt;
+        const double v = frozen ? hist[s * nbox + b] : dt[1];
+        dt[0] = v;
+        hist[s * nbox + b] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *ctr = (int)s + 1;
+}
+
+static __global__ void kb_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+}  // namespace
+
+struct cumc_batch {
+    std::vector<cumc_ctx *> box;
+    int nbox = 0;
+    int coord = 1, recon = 4;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    BoxDesc *d_desc = nullptr;
+    int *d_pre = nullptr;  // P_N arrays of nbox+1
+    int *d_ctr = nullptr;
+    double *d_hist = nullptr;
+    int hist_cap = 0;      // steps
+    std::vector<BoxDesc> h_desc;
+    std::vector<int> h_pre;
+    int total[P_N] = {0};
+    cumc_tuning tune{};
+    struct Graph {
+        int steps, frozen, par;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    double last_ms = 0.0;
+    std::string err;
+    // per-phase event timing used by the tuner
+    bool prof = false;
+    std::vector<hipEvent_t> evp;
+    std::vector<int> evc;
+    size_t evn = 0;
+    double ph_ms[P_N] = {0};
+};
+
+#define BCHK(b, call)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            (b)->err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+            return CUMC_EHIP;                                                                   \
+        }                                                                                       \
+    } while (0)
+
+static int bfail(cumc_batch *b, int code, const std::string &m)
+{
+    b->err = m;
+    return code;
+}
+
+static void bmark(cumc_batch *b, int ph)
+{
+    if (!b->prof) return;
+    if (b->evn == b->evp.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        b->evp.push_back(e);
+        b->evc.push_back(0);
+    }
+    b->evc[b->evn] = ph;
+    hipEventRecord(b->evp[b->evn], b->stream);
+    b->evn++;
+}
+
+static void bcollect(cumc_batch *b)
+{
+    for (size_t i = 0; i + 1 < b->evn; i += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, b->evp[i], b->evp[i + 1]) == hipSuccess) b->ph_ms[b->evc[i]] += ms;
+    }
+    b->evn = 0;
+}
+
+static void clear_graphs(cumc_batch *b)
+{
+    for (auto &g : b->graphs) hipGraphExecDestroy(g.exec);
+    b->graphs.clear();
+}
+
+// tile counts of one box under the current tuning
+static void box_tiles(const cumc_batch *b, const cumc_ctx *c, BoxDesc &d, int cnt[P_N])
+{
+    const Lay &L = c->L;
+    const int seg = b->tune.seg > 0 ? b->tune.seg : 48;
+    const int zc = b->tune.zchunk > 0 ? b->tune.zchunk : 32;
+    const long waves = ((long)(L.ny + 2) * L.px + XS_FACES - 1) / XS_FACES;
+    d.fx_gx = (int)nblk(waves * 64, 256);
+    cnt[P_FX] = d.fx_gx * (L.nz + 2);
+    const int nsy = (L.ny + 1 + seg - 1) / seg;
+    d.fy_sl = (L.ny + 1 + nsy - 1) / nsy;
+    d.fy_gx = (int)nblk((long)L.px * (L.nz + 2), 128);
+    cnt[P_FY] = d.fy_gx * nsy;
+    const int nsz = (L.nz + 1 + seg - 1) / seg;
+    d.fz_sl = (L.nz + 1 + nsz - 1) / nsz;
+    d.fz_gx = (int)nblk((long)L.px * (L.ny + 2), 128);
+    cnt[P_FZ] = d.fz_gx * nsz;
+    d.zc = zc;
+    d.c_gx = (int)nblk((long)L.px * (L.ny + 1), 256);
+    cnt[P_EC] = cnt[P_TAIL] = d.c_gx * ((L.nz + zc) / zc);
+    cnt[P_GX] = (int)nblk(ghost_threads<0>(L), 256);
+    cnt[P_GY] = (int)nblk(ghost_threads<1>(L), 256);
+    cnt[P_GZ] = (int)nblk(ghost_threads<2>(L), 256);
+    cnt[P_DT] = (int)nblk((long)L.px * L.ny, 256);
+}
+
+// descriptors and work lists from the boxes' current buffers and the tuning
+static int refresh(cumc_batch *b)
+{
+    const int nb = b->nbox;
+    b->h_desc.assign(nb, BoxDesc{});
+    b->h_pre.assign((size_t)P_N * (nb + 1), 0);
+    for (int i = 0; i < nb; i++) {
+        const cumc_ctx *c = b->box[i];
+        BoxDesc &d = b->h_desc[i];
+        d.L = c->L;
+        d.F = c->F64;
+        d.tb = tabs_of(c);
+        d.M = c->M;
+        d.dt = c->dt_dev;
+        d.Ub = c->Ub;
+        int cnt[P_N];
+        box_tiles(b, c, d, cnt);
+        for (int p = 0; p < P_N; p++) b->h_pre[(size_t)p * (nb + 1) + i + 1] = b->h_pre[(size_t)p * (nb + 1) + i] + cnt[p];
+    }
+    for (int p = 0; p < P_N; p++) {
+        const long t = b->h_pre[(size_t)p * (nb + 1) + nb];
+        if (t > 0x7fffffffL) return bfail(b, CUMC_EINVAL, "batch work list exceeds 2^31 tiles");
+        b->total[p] = (int)t;
+    }
+    BCHK(b, hipMemcpyAsync(b->d_desc, b->h_desc.data(), sizeof(BoxDesc) * nb, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(b->d_pre, b->h_pre.data(), sizeof(int) * b->h_pre.size(), hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return CUMC_OK;
+}
+
+static inline const int *pre_of(const cumc_batch *b, int p) { return b->d_pre + (size_t)p * (b->nbox + 1); }
+
+// persistent grid of a phase: the tuned workgroup count, never more than there are items
+static inline unsigned grid_of(const cumc_batch *b, int p)
+{
+    static const int tune_slot[P_N] = {CUMC_PH_FLUX_X, CUMC_PH_FLUX_Y, CUMC_PH_FLUX_Z, CUMC_PH_ECORN, CUMC_PH_TAIL,
+                                       CUMC_PH_GHOST,  CUMC_PH_GHOST,  CUMC_PH_GHOST,  CUMC_PH_FINDDT};
+    const int want = b->tune.blocks[tune_slot[p]];
+    const int t = b->total[p];
+    const int g = (want <= 0 || want > t) ? t : want;
+    return (unsigned)(g > 0 ? g : 1);
+}
+
+static void enqueue_finddt_b(cumc_batch *b)
+{
+    hipStream_t s = b->stream;
+    hipLaunchKernelGGL(kb_dt_reset, dim3(1), dim3(256), 0, s, b->d_desc, b->nbox);
+    bmark(b, P_DT);
+    if (b->coord == CUMC_CYLINDRICAL)
+        hipLaunchKernelGGL((kb_finddt<2>), dim3(grid_of(b, P_DT)), dim3(256), 0, s, b->d_desc, pre_of(b, P_DT), b->nbox);
+    else
+        hipLaunchKernelGGL((kb_finddt<1>), dim3(grid_of(b, P_DT)), dim3(256), 0, s, b->d_desc, pre_of(b, P_DT), b->nbox);
+    bmark(b, -1);
+}
+
+template <int STAGE>
+static void launch_tail_b(cumc_batch *b, int par)
+{
+    const unsigned g = grid_of(b, P_TAIL);
+    if (b->coord == CUMC_CYLINDRICAL)
+        hipLaunchKernelGGL((kb_tail<STAGE, 2>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
+    else
+        hipLaunchKernelGGL((kb_tail<STAGE, 1>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
+}
+
+// one RUNGEKUTTA step of every box; par = parity of the step within the run
+static void enqueue_step_b(cumc_batch *b, int par, int frozen)
+{
+    hipStream_t s = b->stream;
+    const BoxDesc *D = b->d_desc;
+    const int nb = b->nbox;
+    const bool ppm = b->recon == CUMC_PPMC;
+    hipLaunchKernelGGL(kb_dt_begin, dim3(1), dim3(256), 0, s, D, nb, b->d_hist, b->d_ctr, frozen);
+    for (int st = 1; st <= 3; st++) {
+        bmark(b, P_FX);
+        if (ppm) hipLaunchKernelGGL((kb_flux_x<4>), dim3(grid_of(b, P_FX)), dim3(256), 0, s, D, pre_of(b, P_FX), nb);
+        else hipLaunchKernelGGL((kb_flux_x<1>), dim3(grid_of(b, P_FX)), dim3(256), 0, s, D, pre_of(b, P_FX), nb);
+        bmark(b, -1);
+        bmark(b, P_FY);
+        if (ppm) hipLaunchKernelGGL((kb_flux_walk<2, 4>), dim3(grid_of(b, P_FY)), dim3(128), 0, s, D, pre_of(b, P_FY), nb);
+        else hipLaunchKernelGGL((kb_flux_walk<2, 1>), dim3(grid_of(b, P_FY)), dim3(128), 0, s, D, pre_of(b, P_FY), nb);
+        bmark(b, -1);
+        bmark(b, P_FZ);
+        if (ppm) hipLaunchKernelGGL((kb_flux_walk<3, 4>), dim3(grid_of(b, P_FZ)), dim3(128), 0, s, D, pre_of(b, P_FZ), nb);
+        else hipLaunchKernelGGL((kb_flux_walk<3, 1>), dim3(grid_of(b, P_FZ)), dim3(128), 0, s, D, pre_of(b, P_FZ), nb);
+        bmark(b, -1);
+        bmark(b, P_EC);
+        hipLaunchKernelGGL(kb_ecorn, dim3(grid_of(b, P_EC)), dim3(256), 0, s, D, pre_of(b, P_EC), nb);
+        bmark(b, -1);
+        if (st == 3) hipLaunchKernelGGL(kb_dt_reset, dim3(1), dim3(256), 0, s, D, nb);
+        bmark(b, P_TAIL);
+        if (st == 1) launch_tail_b<1>(b, par);
+        else if (st == 2) launch_tail_b<2>(b, par);
+        else launch_tail_b<3>(b, par);
+        bmark(b, -1);
+        bmark(b, P_GX);
+        hipLaunchKernelGGL(kb_ghost<0>, dim3(grid_of(b, P_GX)), dim3(256), 0, s, D, pre_of(b, P_GX), nb);
+        hipLaunchKernelGGL(kb_ghost<1>, dim3(grid_of(b, P_GY)), dim3(256), 0, s, D, pre_of(b, P_GY), nb);
+        hipLaunchKernelGGL(kb_ghost<2>, dim3(grid_of(b, P_GZ)), dim3(256), 0, s, D, pre_of(b, P_GZ), nb);
+        bmark(b, -1);
+    }
+}
+
+static int get_graph(cumc_batch *b, int steps, int frozen, int par, hipGraphExec_t *out)
+{
+    for (auto &g : b->graphs)
+        if (g.steps == steps && g.frozen == frozen && g.par == par) {
+            *out = g.exec;
+            return CUMC_OK;
+        }
+    hipGraph_t graph = nullptr;
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < steps; i++) enqueue_step_b(b, (par + i) & 1, frozen);
+    BCHK(b, hipStreamEndCapture(b->stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    BCHK(b, e);
+    b->graphs.push_back({steps, frozen, par, exec});
+    *out = exec;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_create(cumc_ctx *const *boxes, int nbox, cumc_batch **out)
+{
+    if (!boxes || !out || nbox < 1 || nbox > 65536) return CUMC_EINVAL;
+    *out = nullptr;
+    for (int i = 0; i < nbox; i++) {
+        const cumc_ctx *c = boxes[i];
+        if (!c || !c->grid_set) return CUMC_ESTATE;
+        if (c->cfg.precision != CUMC_FP64 || c->cfg.model != CUMC_MODEL_RIEMANN) return CUMC_EINVAL;
+        if (c->cfg.coordinate != boxes[0]->cfg.coordinate || c->cfg.recon != boxes[0]->cfg.recon) return CUMC_EINVAL;
+        for (int j = 0; j < i; j++)
+            if (boxes[j] == c) return CUMC_EINVAL;
+    }
+    cumc_batch *b = new cumc_batch();
+    b->box.assign(boxes, boxes + nbox);
+    b->nbox = nbox;
+    b->coord = boxes[0]->cfg.coordinate;
+    b->recon = boxes[0]->cfg.recon;
+    b->tune.seg = 48;
+    b->tune.zchunk = 32;
+    for (int p = 0; p < CUMC_PH_N; p++) b->tune.blocks[p] = 0;
+    auto bad = [&](int code) {
+        cumc_batch_destroy(b);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bad(CUMC_EHIP);
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return bad(CUMC_EHIP);
+    if (hipMalloc(&b->d_desc, sizeof(BoxDesc) * nbox) != hipSuccess) return bad(CUMC_ENOMEM);
+    if (hipMalloc(&b->d_pre, sizeof(int) * P_N * (nbox + 1)) != hipSuccess) return bad(CUMC_ENOMEM);
+    if (hipMalloc(&b->d_ctr, 64) != hipSuccess) return bad(CUMC_ENOMEM);
+    if (refresh(b)) return bad(CUMC_EHIP);
+    *out = b;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_destroy(cumc_batch *b)
+{
+    if (!b) return CUMC_EINVAL;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    clear_graphs(b);
+    hipFree(b->d_desc);
+    hipFree(b->d_pre);
+    hipFree(b->d_ctr);
+    hipFree(b->d_hist);
+    for (hipEvent_t e : b->evp) hipEventDestroy(e);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return CUMC_OK;
+}
+
+static int run_steps(cumc_batch *b, int nsteps, int graph_steps, int frozen)
+{
+    hipStream_t s = b->stream;
+    hipLaunchKernelGGL(kb_set_int, dim3(1), dim3(64), 0, s, b->d_ctr, 0);
+    if (!frozen) enqueue_finddt_b(b);
+    int done = 0;
+    if (graph_steps > 0) {
+        while (nsteps - done >= graph_steps) {
+            hipGraphExec_t g;
+            int r = get_graph(b, graph_steps, frozen, done & 1, &g);
+            if (r) return r;
+            BCHK(b, hipGraphLaunch(g, s));
+            done += graph_steps;
+        }
+    }
+    for (; done < nsteps; done++) enqueue_step_b(b, done & 1, frozen);
+    BCHK(b, hipGetLastError());
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_run(cumc_batch *b, int nsteps, int graph_steps, const double *dt_in, double *dt_out)
+{
+    if (!b || nsteps < 0 || graph_steps < 0) return CUMC_EINVAL;
+    if (nsteps == 0) return CUMC_OK;
+    const int nb = b->nbox;
+    if (dt_in)
+        for (long i = 0; i < (long)nsteps * nb; i++)
+            if (!std::isfinite(dt_in[i]) || !(dt_in[i] > 0.0)) return bfail(b, CUMC_ENONFINITE, "frozen dt must be finite and positive");
+    if (nsteps > b->hist_cap) {
+        clear_graphs(b);  // graphs hold the history pointer
+        hipFree(b->d_hist);
+        b->d_hist = nullptr;
+        b->hist_cap = 0;
+        if (hipMalloc(&b->d_hist, sizeof(double) * (size_t)nsteps * nb) != hipSuccess)
+            return bfail(b, CUMC_ENOMEM, "dt history allocation failed");
+        b->hist_cap = nsteps;
+    }
+    for (cumc_ctx *c : b->box) BCHK(b, hipStreamSynchronize(c->stream));
+    int r = refresh(b);
+    if (r) return r;
+    if (dt_in)
+        BCHK(b, hipMemcpyAsync(b->d_hist, dt_in, sizeof(double) * (size_t)nsteps * nb, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipEventRecord(b->ev0, b->stream));
+    r = run_steps(b, nsteps, graph_steps, dt_in != nullptr);
+    if (r) return r;
+    BCHK(b, hipEventRecord(b->ev1, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    b->last_ms = ms;
+    // the boxes' host-side view: the step result sits in U0 after an odd number of swaps
+    if (nsteps & 1)
+        for (cumc_ctx *c : b->box) swap_u(c);
+    for (cumc_ctx *c : b->box) c->dt_valid = true;
+    std::vector<double> h((size_t)nsteps * nb);
+    BCHK(b, hipMemcpy(h.data(), b->d_hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    if (dt_out) std::memcpy(dt_out, h.data(), sizeof(double) * h.size());
+    if (!dt_in)
+        for (size_t i = 0; i < h.size(); i++)
+            if (!std::isfinite(h[i]) || !(h[i] > 0.0) || h[i] >= 1.0e10)
+                return bfail(b, CUMC_ENONFINITE, "box " + std::to_string(i % nb) + " step " + std::to_string(i / nb) +
+                                                     ": finddt produced a non-finite step");
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_get_tuning(cumc_batch *b, cumc_tuning *out)
+{
+    if (!b || !out) return CUMC_EINVAL;
+    *out = b->tune;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_set_tuning(cumc_batch *b, const cumc_tuning *t)
+{
+    if (!b || !t) return CUMC_EINVAL;
+    if (t->seg < 1 || t->zchunk < 1) return CUMC_EINVAL;
+    for (int p = 0; p < CUMC_PH_N; p++)
+        if (t->blocks[p] < 0) return CUMC_EINVAL;
+    clear_graphs(b);
+    b->tune = *t;
+    return refresh(b);
+}
+
+extern "C" int cumc_batch_last_ms(cumc_batch *b, double *ms)
+{
+    if (!b || !ms) return CUMC_EINVAL;
+    *ms = b->last_ms;
+    return CUMC_OK;
+}
+
+extern "C" const char *cumc_batch_last_error(cumc_batch *b) { return b ? b->err.c_str() : "no batch"; }
+
+// ---------------------------------------------------------------------------------------------
+// Tuner: saves every box's FP64 fields and dt slots, times candidate shapes phase by phase with
+// HIP events on direct launches (two steps per candidate after a warm-up step), keeps the
+// fastest per phase and restores the saved state.
+// ---------------------------------------------------------------------------------------------
+static int time_candidate(cumc_batch *b, const cumc_tuning &t, double ph[P_N])
+{
+    b->tune = t;
+    int r = refresh(b);
+    if (r) return r;
+    hipLaunchKernelGGL(kb_set_int, dim3(1), dim3(64), 0, b->stream, b->d_ctr, 0);
+    enqueue_finddt_b(b);
+    enqueue_step_b(b, 0, 0);  // warm-up
+    b->prof = true;
+    b->evn = 0;
+    for (int p = 0; p < P_N; p++) b->ph_ms[p] = 0.0;
+    enqueue_step_b(b, 1, 0);
+    enqueue_step_b(b, 0, 0);
+    b->prof = false;
+    BCHK(b, hipGetLastError());
+    BCHK(b, hipStreamSynchronize(b->stream));
+    bcollect(b);
+    for (int p = 0; p < P_N; p++) ph[p] = b->ph_ms[p];
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_autotune(cumc_batch *b, cumc_tuning *out)
+{
+    if (!b) return CUMC_EINVAL;
+    const int nb = b->nbox;
+    for (cumc_ctx *c : b->box) BCHK(b, hipStreamSynchronize(c->stream));
+    if (b->hist_cap < 3) {
+        clear_graphs(b);
+        hipFree(b->d_hist);
+        b->d_hist = nullptr;
+        b->hist_cap = 0;
+        if (hipMalloc(&b->d_hist, sizeof(double) * 3 * nb) != hipSuccess) return bfail(b, CUMC_ENOMEM, "tuner history");
+        b->hist_cap = 3;
+    }
+    // save
+    std::vector<void *> save(nb, nullptr);
+    std::vector<double> dts(2 * (size_t)nb);
+    int rc = CUMC_OK;
+    for (int i = 0; i < nb && rc == CUMC_OK; i++) {
+        cumc_ctx *c = b->box[i];
+        if (hipMalloc(&save[i], c->pool_bytes) != hipSuccess) rc = bfail(b, CUMC_ENOMEM, "tuner state copy");
+        else if (hipMemcpyAsync(save[i], c->pool, c->pool_bytes, hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+                 hipMemcpyAsync(&dts[2 * i], c->dt_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, b->stream) != hipSuccess)
+            rc = bfail(b, CUMC_EHIP, "tuner state copy");
+    }
+    const cumc_tuning orig = b->tune;
+    std::vector<double *> u0(nb), u1(nb);
+    for (int i = 0; i < nb; i++) {
+        u0[i] = b->box[i]->F64.U0;
+        u1[i] = b->box[i]->F64.U1;
+    }
+    cumc_tuning best = orig;
+    if (rc == CUMC_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = bfail(b, CUMC_EHIP, "tuner sync");
+    // 1) segment length of the pencil sweeps (one workgroup per tile)
+    if (rc == CUMC_OK) {
+        const int segs[] = {16, 32, 64, 128, 256};
+        double bestt = 1e300;
+        for (int sg : segs) {
+            cumc_tuning t = orig;
+            t.seg = sg;
+            for (int p = 0; p < CUMC_PH_N; p++) t.blocks[p] = 0;
+            double ph[P_N];
+            if ((rc = time_candidate(b, t, ph))) break;
+            const double v = ph[P_FY] + ph[P_FZ];
+            if (v < bestt) {
+                bestt = v;
+                best.seg = sg;
+            }
+        }
+    }
+    // 2) persistent workgroups per phase
+    if (rc == CUMC_OK) {
+        const int cands[] = {0, 256, 512, 1024, 2048, 4096};
+        double bt[P_N];
+        for (int p = 0; p < P_N; p++) bt[p] = 1e300;
+        int pick[P_N] = {0};
+        for (int g : cands) {
+            cumc_tuning t = best;
+            for (int p = 0; p < CUMC_PH_N; p++) t.blocks[p] = g;
+            double ph[P_N];
+            if ((rc = time_candidate(b, t, ph))) break;
+            for (int p = 0; p < P_N; p++)
+                if (ph[p] < bt[p]) {
+                    bt[p] = ph[p];
+                    pick[p] = g;
+                }
+        }
+        best.blocks[CUMC_PH_FLUX_X] = pick[P_FX];
+        best.blocks[CUMC_PH_FLUX_Y] = pick[P_FY];
+        best.blocks[CUMC_PH_FLUX_Z] = pick[P_FZ];
+        best.blocks[CUMC_PH_ECORN] = pick[P_EC];
+        best.blocks[CUMC_PH_TAIL] = pick[P_TAIL];
+        best.blocks[CUMC_PH_GHOST] = pick[P_GX];
+        best.blocks[CUMC_PH_FINDDT] = pick[P_DT];
+    }
+    // restore every FP64 plane (so which of U0/U1 holds the state is as saved) and the dt slots
+    for (int i = 0; i < nb; i++) {
+        cumc_ctx *c = b->box[i];
+        if (save[i]) {
+            hipMemcpyAsync(c->pool, save[i], c->pool_bytes, hipMemcpyDeviceToDevice, b->stream);
+            hipMemcpyAsync(c->dt_dev, &dts[2 * i], 2 * sizeof(double), hipMemcpyHostToDevice, b->stream);
+        }
+        c->F64.U0 = u0[i];
+        c->F64.U1 = u1[i];
+        c->F32.U0 = u0[i];
+        c->F32.U1 = u1[i];
+    }
+    hipError_t e = hipStreamSynchronize(b->stream);
+    for (void *p : save) hipFree(p);
+    if (rc == CUMC_OK && e != hipSuccess) rc = bfail(b, CUMC_EHIP, std::string("tuner restore: ") + hipGetErrorString(e));
+    clear_graphs(b);
+    b->tune = rc == CUMC_OK ? best : orig;
+    int r2 = refresh(b);
+    if (rc == CUMC_OK) rc = r2;
+    if (rc == CUMC_OK && out) *out = b->tune;
+    return rc;
+}

@@ -76,13 +76,14 @@ __device__ inline void recon_lane(const Tabs &tb, int ax, int jc, R c, R &vm, R 
     }
 }
 
+// One 256-thread tile of the x sweep: tile bx of plane l.  Tiles per plane:
+// ceil(ceil((ny+2)*px / XS_FACES) * 64 / 256), planes l in [0, nz+1].
 template <typename R, int RECON>
-__global__ __launch_bounds__(256) void k_flux_x(Lay L, Fields<R> F, Tabs tb, Model M)
+__device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, int bx, int l)
 {
     const int lane = threadIdx.x & 63;
-    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long wave = ((long)bx * blockDim.x + threadIdx.x) >> 6;
     const long f = wave * XS_FACES + lane - 2;  // flattened (j+2) + k*px over rows k in [0, ny+1]
-    const int l = blockIdx.y;                   // 0 .. nz+1
     const long nload = (long)(L.ny + 4) * L.px;
     const bool inr = (f >= 0) && (f < nload);
     const long g = (long)(l + 2) * L.py + 2 * L.px + (inr ? f : 0);
@@ -121,23 +122,31 @@ __global__ __launch_bounds__(256) void k_flux_x(Lay L, Fields<R> F, Tabs tb, Mod
     }
 }
 
+template <typename R, int RECON>
+__global__ __launch_bounds__(256) void k_flux_x(Lay L, Fields<R> F, Tabs tb, Model M)
+{
+    flux_x_tile<R, RECON>(L, F, tb, M, blockIdx.x, blockIdx.y);
+}
+
 // ------------------------------------------------------------------------------------------
 // y and z sweeps: each thread walks one pencil along the sweep axis, lanes along x, keeping the
 // 4-cell value window and 3 limited slopes of every reconstructed variable in registers so that
 // each slope, each PPM call and each Riemann solve is computed exactly once.
 // (solver/spatial.f90:474-511 and :552-589)
 // ------------------------------------------------------------------------------------------
+// One 128-thread tile: bx covers px*(nt+2) pencils, segment by of seglen faces along DIR.
 template <typename R, int DIR, int RECON>
-__global__ __launch_bounds__(128) void k_flux_walk(Lay L, Fields<R> F, Tabs tb, Model M, int seglen)
+__device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, int seglen,
+                                      int bx, int by)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)bx * blockDim.x + threadIdx.x;
     const int nd = DIR == 2 ? L.ny : L.nz;   // cells along the sweep
     const int nt = DIR == 2 ? L.nz : L.ny;   // second transverse extent
     const int jj = (int)(t % L.px);
     const long tq = t / L.px;                // transverse index 0 .. nt+1
     const int j = jj - 2;
     if (tq > nt + 1 || j < 0 || j > L.nx + 1) return;
-    const int s0 = blockIdx.y * seglen;
+    const int s0 = by * seglen;
     if (s0 > nd) return;
     const int s1 = (s0 + seglen - 1 < nd) ? s0 + seglen - 1 : nd;  // last face of the segment
     const long stride = DIR == 2 ? L.px : L.py;
@@ -241,13 +250,19 @@ __global__ __launch_bounds__(128) void k_flux_walk(Lay L, Fields<R> F, Tabs tb, 
     }
 }
 
+template <typename R, int DIR, int RECON>
+__global__ __launch_bounds__(128) void k_flux_walk(Lay L, Fields<R> F, Tabs tb, Model M, int seglen)
+{
+    flux_walk_tile<R, DIR, RECON>(L, F, tb, M, seglen, blockIdx.x, blockIdx.y);
+}
+
 // ------------------------------------------------------------------------------------------
 // Corner EMFs on [0,n]^3: face-flux averages accumulated in sweep order (spatial.f90:455-467,
 // :533-546, :611-624) then the upwind-CT combination with cell EMFs (mhd.f90:95-125).
-// 2.5D: a thread owns one (j,k) column and marches l through a chunk of ZCHUNK planes; the
-// plane-(l+1) values it loads are carried into the next iteration, so HBM sees each plane once.
+// 2.5D: a thread owns one (j,k) column and marches l through a chunk of zc planes (a tuning
+// parameter); the plane-(l+1) values it loads are carried into the next iteration, so HBM sees
+// each plane once.
 // ------------------------------------------------------------------------------------------
-constexpr int ZCHUNK = 32;
 constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 // Map the 1D launch index onto a logical (bx, by) tile so that logically consecutive tiles
@@ -274,17 +289,17 @@ __device__ inline void ecells(const Lay &L, const Fields<R> &F, long g, R &ex, R
     ez = vy * bx - vx * by;
 }
 
+// Column tile (bx, by): 256 (j,k) columns of the flattened [0,nx]x[0,ny] plane (pitch px),
+// planes [by*zc, by*zc+zc-1] of [0,nz].
 template <typename R>
-__global__ __launch_bounds__(256) void k_ecorn(Lay L, Fields<R> F, int gx, int gy)
+__device__ inline void ecorn_tile(const Lay &L, const Fields<R> &F, int bx, int by, int zc)
 {
-    int bx, by;
-    if (!xcd_tile(gx, gy, bx, by)) return;
     const long t = (long)bx * blockDim.x + threadIdx.x;
     const int k = (int)(t / L.px);
     const int j = (int)(t - (long)k * L.px);
-    const int l0 = by * ZCHUNK;
+    const int l0 = by * zc;
     if (k > L.ny || j > L.nx || l0 > L.nz) return;
-    const int l1 = (l0 + ZCHUNK - 1 < L.nz) ? l0 + ZCHUNK - 1 : L.nz;
+    const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
     const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
     const R *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2];
     const R q = R(0.25);
@@ -326,6 +341,14 @@ __global__ __launch_bounds__(256) void k_ecorn(Lay L, Fields<R> F, int gx, int g
         fx6 = fx6n;
         fy6 = fy6n;
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_ecorn(Lay L, Fields<R> F, int gx, int gy, int zc)
+{
+    int bx, by;
+    if (!xcd_tile(gx, gy, bx, by)) return;
+    ecorn_tile<R>(L, F, bx, by, zc);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -410,21 +433,21 @@ __device__ inline double block_min(double v)
     return m;
 }
 
+// Column tile (bx, by) as in ecorn_tile.  Every thread of the block reaches the stage-3 block
+// reduction at the end, so callers may loop over tiles with block-uniform trip counts.
 template <typename R, int STAGE, int COORD>
-__global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model M, const double *dtp,
-                                              const double *__restrict__ Uin, double *__restrict__ Uout,
-                                              unsigned long long *dtnext, int gx, int gy)
+__device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, const double *dtp,
+                                 const double *__restrict__ Uin, double *__restrict__ Uout,
+                                 unsigned long long *dtnext, int bx, int by, int zc)
 {
-    int bx, by;
-    if (!xcd_tile(gx, gy, bx, by)) return;
     const long t = (long)bx * blockDim.x + threadIdx.x;
     const int k = (int)(t / L.px);
     const int j = (int)(t - (long)k * L.px);
-    const int l0 = by * ZCHUNK;
+    const int l0 = by * zc;
     const bool col = k <= L.ny && j <= L.nx && l0 <= L.nz;
     double dtmin = 1.0e10;
     if (col) {
-        const int l1 = (l0 + ZCHUNK - 1 < L.nz) ? l0 + ZCHUNK - 1 : L.nz;
+        const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
         const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
         const double dt = *dtp;
         const R *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2];
@@ -583,6 +606,16 @@ __global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model
     }
 }
 
+template <typename R, int STAGE, int COORD>
+__global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model M, const double *dtp,
+                                              const double *__restrict__ Uin, double *__restrict__ Uout,
+                                              unsigned long long *dtnext, int gx, int gy, int zc)
+{
+    int bx, by;
+    if (!xcd_tile(gx, gy, bx, by)) return;
+    tail_tile<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc);
+}
+
 // initial_update's FROMRVETOU + CUSTOM_CHECKRHO + EOS on [0,n]^3 (initial.f90:103-126)
 static __global__ __launch_bounds__(256) void k_init(Lay L, Fields<double> F, Model M, double *Uout)
 {
@@ -660,11 +693,19 @@ __device__ inline bool eps_y_outer(const int *bf, int *flag_eff)
     return false;
 }
 
+// number of ghost-fill threads of pass AX
 template <int AX>
-__global__ __launch_bounds__(256) void k_ghost(Lay L, Fields<double> F, Model M)
+__host__ __device__ inline long ghost_threads(const Lay &L)
+{
+    if (AX == 0) return (long)(L.ny + 1) * (L.nz + 1) * 6;
+    if (AX == 1) return (long)(L.nx + 6) * (L.nz + 1) * 6;
+    return (long)(L.nx + 6) * (L.ny + 6) * 6;
+}
+
+template <int AX>
+__device__ inline void ghost_thread(const Lay &L, const Fields<double> &F, const Model &M, long t)
 {
     // thread domain: (a, b) over the two other axes, g in 1..3, side in {0,1}
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int na, nb, a0, b0;  // ranges [a0, a0+na) x [b0, b0+nb)
     if (AX == 0) { a0 = 0; na = L.ny + 1; b0 = 0; nb = L.nz + 1; }           // k, l in [0,n]
     else if (AX == 1) { a0 = -2; na = L.nx + 6; b0 = 0; nb = L.nz + 1; }     // j full, l in [0,n]
@@ -723,6 +764,12 @@ __global__ __launch_bounds__(256) void k_ghost(Lay L, Fields<double> F, Model M)
         }
         F.P[dst + 4 * ps] = F.P[dst] * F.EPS[dst] * (M.ggas - 1.0);
     }
+}
+
+template <int AX>
+__global__ __launch_bounds__(256) void k_ghost(Lay L, Fields<double> F, Model M)
+{
+    ghost_thread<AX>(L, F, M, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // torus_cyn CUSTOM_BOUNDARY (custom.f90:47-81): no inflow through the R ghost zones ...
@@ -785,9 +832,10 @@ static __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, i
 // finddt (rungekutta.f90:205-252) on the current state; a thread marches one (j,k) column so a
 // launch issues one atomic per block.  Positive doubles order like their bit patterns.
 template <int COORD>
-__global__ __launch_bounds__(256) void k_finddt(Lay L, Fields<double> F, Tabs tb, Model M, unsigned long long *out)
+__device__ inline void finddt_tile(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M,
+                                   unsigned long long *out, int bx)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)bx * blockDim.x + threadIdx.x;
     const int k = (int)(t / L.px) + 1;
     const int j = (int)(t - (long)(k - 1) * L.px) + 1;
     double dt = 1.0e10;
@@ -802,6 +850,12 @@ __global__ __launch_bounds__(256) void k_finddt(Lay L, Fields<double> F, Tabs tb
     }
     const double m = block_min(dt);
     if (threadIdx.x == 0) atomicMin(out, (unsigned long long)__double_as_longlong(m));
+}
+
+template <int COORD>
+__global__ __launch_bounds__(256) void k_finddt(Lay L, Fields<double> F, Tabs tb, Model M, unsigned long long *out)
+{
+    finddt_tile<COORD>(L, F, tb, M, out, blockIdx.x);
 }
 
 // find_divb (mhd.f90:15-66) with GEOM_AREA (coords.f90:173-223)

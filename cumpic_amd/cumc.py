@@ -30,6 +30,27 @@ class CConfigP(ctypes.Structure):
     _fields_ = CConfig._fields_ + [("precision", ctypes.c_int)]
 
 
+PHASES = ("flux_x", "flux_y", "flux_z", "ecorn", "tail", "ghost", "finddt")
+
+
+class Tuning(ctypes.Structure):
+    """cumc_tuning: launch shapes (segment length, plane chunk, persistent workgroups per phase)."""
+    _fields_ = [("seg", ctypes.c_int), ("zchunk", ctypes.c_int), ("blocks", ctypes.c_int * len(PHASES))]
+
+    def as_dict(self):
+        return {"seg": self.seg, "zchunk": self.zchunk, "blocks": dict(zip(PHASES, list(self.blocks)))}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        t.seg = int(d["seg"])
+        t.zchunk = int(d["zchunk"])
+        b = d.get("blocks", {})
+        for i, name in enumerate(PHASES):
+            t.blocks[i] = int(b.get(name, 0))
+        return t
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -54,10 +75,25 @@ def lib():
             "cumc_kernel_times": [_CTX, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_long),
                                   ctypes.POINTER(ctypes.c_char_p)],
         }
+        T = ctypes.POINTER(Tuning)
+        sig.update({
+            "cumc_autotune": [_CTX, T],
+            "cumc_get_tuning": [_CTX, T],
+            "cumc_set_tuning": [_CTX, T],
+            "cumc_batch_create": [ctypes.POINTER(_CTX), ctypes.c_int, ctypes.POINTER(_CTX)],
+            "cumc_batch_destroy": [_CTX],
+            "cumc_batch_run": [_CTX, ctypes.c_int, ctypes.c_int, _D, _D],
+            "cumc_batch_autotune": [_CTX, T],
+            "cumc_batch_get_tuning": [_CTX, T],
+            "cumc_batch_set_tuning": [_CTX, T],
+            "cumc_batch_last_ms": [_CTX, _D],
+        })
         for name, args in sig.items():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = ctypes.c_int
+        L.cumc_batch_last_error.restype = ctypes.c_char_p
+        L.cumc_batch_last_error.argtypes = [_CTX]
         L.cumc_copy_probe.argtypes = [ctypes.c_long, ctypes.c_int, _D]
         L.cumc_copy_probe.restype = ctypes.c_int
         L.cumc_error_string.restype = ctypes.c_char_p
@@ -189,4 +225,81 @@ class Solver:
     def last_ms(self) -> float:
         d = ctypes.c_double()
         self._check(lib().cumc_last_step_ms(self.h, ctypes.byref(d)), "cumc_last_step_ms")
+        return d.value
+
+    def autotune(self) -> dict:
+        t = Tuning()
+        self._check(lib().cumc_autotune(self.h, ctypes.byref(t)), "cumc_autotune")
+        return t.as_dict()
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(lib().cumc_get_tuning(self.h, ctypes.byref(t)), "cumc_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, d: dict):
+        t = Tuning.from_dict(d)
+        self._check(lib().cumc_set_tuning(self.h, ctypes.byref(t)), "cumc_set_tuning")
+
+
+class Batch:
+    """Independent boxes advanced together: one persistent launch per phase over all boxes,
+    optionally replayed from a hipGraph (cumc_batch_*).  The boxes stay ordinary Solvers."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (_CTX * len(self.solvers))(*[s.h for s in self.solvers])
+        h = _CTX()
+        r = lib().cumc_batch_create(arr, len(self.solvers), ctypes.byref(h))
+        if r:
+            raise CumcError(f"cumc_batch_create: {lib().cumc_error_string(r).decode()}")
+        self.h = h
+
+    def _check(self, r, what):
+        if r:
+            msg = lib().cumc_batch_last_error(self.h).decode() if self.h else ""
+            raise CumcError(f"{what}: {lib().cumc_error_string(r).decode()} {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cumc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps: int, graph_steps: int = 0, dts=None):
+        """nsteps steps of every box; dts (nsteps, nbox) freezes the time steps.  Returns the
+        (nsteps, nbox) array of steps used."""
+        nb = len(self.solvers)
+        out = np.zeros((max(1, nsteps), nb))
+        din = None
+        if dts is not None:
+            din = np.ascontiguousarray(dts, dtype=np.float64)
+            if din.shape != (nsteps, nb):
+                raise ValueError(f"dts must have shape ({nsteps}, {nb})")
+        self._check(lib().cumc_batch_run(self.h, int(nsteps), int(graph_steps), _p(din) if din is not None else None,
+                                         _p(out)), "cumc_batch_run")
+        return out[:nsteps].copy()
+
+    def autotune(self) -> dict:
+        t = Tuning()
+        self._check(lib().cumc_batch_autotune(self.h, ctypes.byref(t)), "cumc_batch_autotune")
+        return t.as_dict()
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(lib().cumc_batch_get_tuning(self.h, ctypes.byref(t)), "cumc_batch_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, d: dict):
+        t = Tuning.from_dict(d)
+        self._check(lib().cumc_batch_set_tuning(self.h, ctypes.byref(t)), "cumc_batch_set_tuning")
+
+    def last_ms(self) -> float:
+        d = ctypes.c_double()
+        self._check(lib().cumc_batch_last_ms(self.h, ctypes.byref(d)), "cumc_batch_last_ms")
         return d.value

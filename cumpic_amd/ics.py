@@ -298,3 +298,22 @@ def randomize(prob: Problem, seed: int, amp: float = 0.2) -> Problem:
         p[inner + (v,)] += amp * rng.uniform(-1, 1, shp)
     prob.eps[inner] = p[inner + (ITAU,)] / p[inner + (IRHO,)] / (c.ggas - 1.0)
     return prob
+
+
+BATCH_SIZES = (64, 96, 128, 192, 256)
+
+
+def batch_boxes(nbox: int = 48, seed: int = 1004, sizes=BATCH_SIZES, steps: int = 40) -> list[Problem]:
+    """Independent 2D periodic boxes of uneven sizes for the batched form.
+
+    Box sizes n x n (nz = 1) are drawn from ``sizes`` by ``default_rng(seed)``; box i carries the
+    spectral_box initial condition restricted to k_z = 0 with its own seed ``100 * seed + i``.
+    """
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.asarray(sizes), size=nbox)
+    out = []
+    for i, n in enumerate(ns):
+        p = spectral_box(n=int(n), seed=100 * seed + i, ndim=2, steps=steps)
+        p.name = f"batch{i}_{int(n)}sq"
+        out.append(p)
+    return out

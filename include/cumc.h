@@ -110,6 +110,53 @@ int cumc_copy_probe(long bytes, int reps, double *gbps);
 const char *cumc_error_string(int code);
 const char *cumc_last_error(cumc_ctx *ctx);
 
+/* ------------------------------------------------------------------------------------------
+ * Launch-shape tuning.  No reference counterpart (OpenACC picks gangs/vectors itself); the
+ * shapes change no result bit.
+ * ------------------------------------------------------------------------------------------ */
+enum { CUMC_PH_FLUX_X = 0, CUMC_PH_FLUX_Y, CUMC_PH_FLUX_Z, CUMC_PH_ECORN, CUMC_PH_TAIL, CUMC_PH_GHOST,
+       CUMC_PH_FINDDT, CUMC_PH_N };
+typedef struct cumc_tuning {
+    int seg;                 /* target faces per pencil segment of the y and z sweeps */
+    int zchunk;              /* planes per column tile of the EMF and stage-tail kernels */
+    int blocks[CUMC_PH_N];   /* batch only: persistent workgroups per phase, 0 = one per tile */
+} cumc_tuning;
+
+/* Time candidate shapes on the context's current state (which is saved and restored) and keep
+ * the fastest; *out (may be NULL) receives the choice. */
+int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out);
+int cumc_get_tuning(cumc_ctx *ctx, cumc_tuning *out);
+int cumc_set_tuning(cumc_ctx *ctx, const cumc_tuning *t);
+
+/* ------------------------------------------------------------------------------------------
+ * Batched form: many independent subdomains of uneven sizes advanced together.  No reference
+ * counterpart: the reference runs one problem per process (solver/main.f90:127-163).  Every
+ * phase of the step is one persistent launch over a work list of (box, tile) items covering all
+ * boxes; each box keeps its own CFL time step on the device, and a run can be captured in a
+ * hipGraph and replayed.  Boxes must share coordinate, recon and FP64 precision and use the
+ * riemann model hooks; each box is an ordinary context (set_grid / upload / initial_update /
+ * download / find_divb stay per box) and must be idle while the batch runs.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct cumc_batch cumc_batch;
+
+int cumc_batch_create(cumc_ctx *const *boxes, int nbox, cumc_batch **out);
+int cumc_batch_destroy(cumc_batch *b);
+
+/* nsteps RUNGEKUTTA steps of every box.  dt_in (nsteps*nbox, step-major) freezes the time
+ * steps; NULL means each box's own finddt before every step (solver/main.f90:130, no
+ * total_time clamp).  dt_out (may be NULL) receives the steps used.  graph_steps > 0 captures
+ * that many steps in one hipGraph and replays it (the remainder runs as direct launches);
+ * 0 launches every kernel directly. */
+int cumc_batch_run(cumc_batch *b, int nsteps, int graph_steps, const double *dt_in, double *dt_out);
+
+/* Time candidate persistent grid sizes and segment lengths on the boxes' current states (saved
+ * and restored) and keep the fastest; *out (may be NULL) receives the choice. */
+int cumc_batch_autotune(cumc_batch *b, cumc_tuning *out);
+int cumc_batch_get_tuning(cumc_batch *b, cumc_tuning *out);
+int cumc_batch_set_tuning(cumc_batch *b, const cumc_tuning *t);
+int cumc_batch_last_ms(cumc_batch *b, double *ms);
+const char *cumc_batch_last_error(cumc_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,3 +57,22 @@ def test_fortran_host_links():
     out = subprocess.run(["nm", "-D", "--undefined-only", exe], stdout=subprocess.PIPE, text=True).stdout
     for sym in ("cumc_create", "cumc_finddt", "cumc_rungekutta", "cumc_initial_update", "cumc_download"):
         assert sym in out
+
+
+def test_tuning_struct_layout_matches_header():
+    from cumpic_amd.cumc import PHASES, Tuning
+    txt = open(os.path.join(ROOT, "include", "cumc.h")).read()
+    enum = re.search(r"enum \{ CUMC_PH_FLUX_X = 0,([^}]*)\}", txt).group(0)
+    assert enum.count("CUMC_PH_") - 1 == len(PHASES)  # CUMC_PH_N closes the list
+    assert ctypes.sizeof(Tuning) == 4 * (2 + len(PHASES))
+    t = Tuning.from_dict({"seg": 5, "zchunk": 6, "blocks": {"tail": 9}})
+    assert t.as_dict()["blocks"]["tail"] == 9 and t.as_dict()["seg"] == 5
+
+
+def test_batch_boxes_sizes_are_seeded():
+    from cumpic_amd import ics
+    a = [p.cfg.nx for p in ics.batch_boxes(nbox=6, sizes=(8, 12))]
+    b = [p.cfg.nx for p in ics.batch_boxes(nbox=6, sizes=(8, 12))]
+    assert a == b and set(a) <= {8, 12}
+    p = ics.batch_boxes(nbox=1, sizes=(8,))[0]
+    assert p.cfg.nz == 1 and p.cfg.ny == 8 and p.prim.shape == (7, 14, 14, 8)
