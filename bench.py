@@ -9,8 +9,14 @@ mean B_z = 1, PPM + HLL + upwind CT + SSP-RK3, seeded synthetic initial conditio
 EMFs; FP64 state) on the same workload and reports its rate, its speed-up and its error against
 FP64 on the same dt sequence.  Prints ONE JSON line; progress goes to stderr.
 
+It also runs the batched form: 48 independent 2D periodic boxes of sizes drawn by seed from
+{64,96,128,192,256}^2 (cumpic_amd.ics.batch_boxes, seed 1004), 40 FP64 steps, advanced one box
+at a time, by persistent work-list launches, and by a replayed hipGraph of 8 steps after the
+launch-shape tuner, with a bit-for-bit check of the batched result against the one-at-a-time one.
+
   --workload torus      cylindrical torus 256x128x256, 50 steps (torus_cyn hooks)
   --n 512 --steps 100   the 512^3 box of the mixed-precision workload
+  --no-batch            skip the batched workload
 """
 from __future__ import annotations
 
@@ -121,6 +127,141 @@ def fields(state, cfg):
             "|B|": np.linalg.norm(b, axis=-1)}
 
 
+def bench_batch(steps=40, graph_steps=8):
+    """Batched 2D boxes: one at a time vs work-list launches vs hipGraph replay."""
+    from cumpic_amd import ics
+    from cumpic_amd.cumc import Batch, Solver
+    log("[bench] batch: building 48 2D boxes")
+    probs = ics.batch_boxes(steps=steps)
+    zones = sum(p.cfg.zones for p in probs)
+
+    def fresh():
+        out = []
+        for p in probs:
+            s = Solver.from_problem(p)
+            s.initial_update()
+            out.append(s)
+        return out
+
+    seq = fresh()
+    seq[0].run(1)  # module load / first-launch cost out of the timed region
+    seq[0].close()
+    seq[0] = Solver.from_problem(probs[0])
+    seq[0].initial_update()
+    t0 = time.perf_counter()
+    seq_dts = [s.run(steps)[0] for s in seq]
+    t_seq = time.perf_counter() - t0
+    log(f"[bench] batch: one box at a time {t_seq * 1e3:.1f} ms")
+
+    boxes = fresh()
+    b = Batch(boxes)
+    t0 = time.perf_counter()
+    tune = b.autotune()
+    t_tune = time.perf_counter() - t0
+    log(f"[bench] batch: tuner {t_tune:.2f} s -> {tune}")
+    b.run(graph_steps, graph_steps=graph_steps)  # graph capture + warm-up
+    for s, p in zip(boxes, probs):
+        s.upload(p.prim, p.eps)
+        s.initial_update()
+    t0 = time.perf_counter()
+    dts = b.run(steps, graph_steps=graph_steps)
+    t_graph = time.perf_counter() - t0
+    same = bool(np.array_equal(dts, np.array(seq_dts).T))
+    for s, q in zip(boxes, seq):
+        a, c = s.download(), q.download()
+        same = same and all(np.array_equal(a[k], c[k]) for k in ("prim", "eps", "bcell"))
+    log(f"[bench] batch: graph {t_graph * 1e3:.1f} ms, bitwise vs one-at-a-time: {same}")
+    for s, p in zip(boxes, probs):
+        s.upload(p.prim, p.eps)
+        s.initial_update()
+    t0 = time.perf_counter()
+    b.run(steps, graph_steps=0)
+    t_direct = time.perf_counter() - t0
+    log(f"[bench] batch: direct launches {t_direct * 1e3:.1f} ms")
+    adj = bench_adjoint(probs, boxes, b, np.array(seq_dts).T, steps)
+    b.close()
+    for s in boxes + seq:
+        s.close()
+    zc = zones * steps
+    return {
+        "workload": f"48 2D periodic spectral boxes, sizes drawn by seed 1004 from {{64,96,128,192,256}}^2, "
+                    f"{zones} zones, {steps} FP64 steps, per-box CFL dt on the device",
+        "zone_cycles": zc,
+        "one_at_a_time": {"value": zc / t_seq, "ms": t_seq * 1e3},
+        "batched_direct": {"value": zc / t_direct, "ms": t_direct * 1e3},
+        "batched_graph": {"value": zc / t_graph, "ms": t_graph * 1e3, "graph_steps": graph_steps},
+        "speedup_graph_vs_one_at_a_time": t_seq / t_graph,
+        "tuning": tune,
+        "tuner_s": t_tune,
+        "bitwise_vs_one_at_a_time": same,
+        "unit": "zone-cycles/s",
+        "adjoint": adj,
+    }
+
+
+def _interior(cfg):
+    return (slice(3, 3 + cfg.nz), slice(3, 3 + cfg.ny), slice(3, 3 + cfg.nx))
+
+
+def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-4, 1e-5, 1e-6)):
+    """Reverse mode on the batched boxes: J_b = sum |B|^2/2 dV per box after `steps` frozen-dt
+    steps; per-box gradient with respect to the initial velocity from cumc_tape_backward
+    (checkpoints every `every` steps).  Cost: backward time / forward time of the same taped
+    run.  Check: central differences of sum_b J_b along ndir seeded random unit directions in the
+    concatenated velocity space, each J(+-) from one batched frozen-dt run."""
+    log(f"[bench] adjoint: {len(boxes)} boxes, {steps} steps, checkpoints every {every}")
+    grads, t_f, t_b = [], 0.0, 0.0
+    for i, (s, p) in enumerate(zip(boxes, probs)):
+        inner = _interior(p.cfg)
+        dv = (p.xf[4] - p.xf[3]) * (p.yf[4] - p.yf[3]) * (p.zf[4] - p.zf[3])
+        s.upload(p.prim, p.eps)
+        t0 = time.perf_counter()
+        s.tape_run(steps, every, dts[:, i])
+        t1 = time.perf_counter()
+        st = s.download()
+        seed = np.zeros(p.cfg.shape + (3,))
+        seed[inner] = st["bcell"][inner] * dv
+        t2 = time.perf_counter()
+        gp, _ = s.tape_backward(seed_bcell=seed)
+        t3 = time.perf_counter()
+        s.tape_free()
+        t_f += t1 - t0
+        t_b += t3 - t2
+        grads.append(gp[inner + (slice(1, 4),)].copy())
+    log(f"[bench] adjoint: forward (taped) {t_f * 1e3:.1f} ms, backward {t_b * 1e3:.1f} ms, ratio {t_b / t_f:.2f}")
+
+    def fields_after(vs):
+        for s, p, v in zip(boxes, probs, vs):
+            prim = p.prim.copy()
+            prim[_interior(p.cfg) + (slice(1, 4),)] = v
+            s.upload(prim, p.eps)
+            s.initial_update()
+        batch.run(steps, graph_steps=every, dts=dts)
+        return [s.download()["bcell"][_interior(p.cfg)] for s, p in zip(boxes, probs)]
+
+    v0 = [p.prim[_interior(p.cfg) + (slice(1, 4),)].copy() for p in probs]
+    dvs = [(p.xf[4] - p.xf[3]) * (p.yf[4] - p.yf[3]) * (p.zf[4] - p.zf[3]) for p in probs]
+    rng = np.random.default_rng(1004)
+    rel = {h: [] for h in hs}
+    for d in range(ndir):
+        dirs = [rng.normal(size=g.shape) for g in grads]
+        nrm = np.sqrt(sum(float(np.sum(x * x)) for x in dirs))
+        dirs = [x / nrm for x in dirs]
+        ad = sum(float(np.sum(g * x)) for g, x in zip(grads, dirs))
+        for h in hs:
+            bp = fields_after([v + h * x for v, x in zip(v0, dirs)])
+            bm = fields_after([v - h * x for v, x in zip(v0, dirs)])
+            dj = sum(0.5 * float(np.sum((a - c) * (a + c))) * w for a, c, w in zip(bp, bm, dvs))
+            fd = dj / (2 * h)
+            rel[h].append(abs(fd - ad) / max(abs(fd), abs(ad)))
+    summary = {f"{h:.0e}": {"median": float(np.median(v)), "max": float(np.max(v)),
+                            "within_1e-6": int(np.sum(np.array(v) <= 1e-6))} for h, v in rel.items()}
+    log(f"[bench] adjoint vs central differences (relative, {ndir} unit directions): {summary}")
+    return {"objective": "per box sum |B|^2/2 dV after the steps; gradient w.r.t. initial velocity",
+            "steps": steps, "checkpoint_every": every, "forward_taped_ms": t_f * 1e3, "backward_ms": t_b * 1e3,
+            "adjoint_over_forward": t_b / t_f, "fd_rel_error_by_h": summary, "directions": ndir}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +272,7 @@ def main():
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and full-size parity check")
     ap.add_argument("--no-error", action="store_true", help="skip the mixed-vs-FP64 error replay")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D-box workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -215,6 +357,10 @@ def main():
         results["mixed"]["rel_l1_vs_fp64"] = errs
         log(f"[bench] mixed rel L1 vs FP64: {errs}")
 
+    batch = None
+    if not args.no_batch and args.workload == "box":
+        batch = bench_batch()
+
     main_form = "fp64" if "fp64" in results else forms[0]
     m = results[main_form]
     step_alg = m["value"] * BYTES_PER_ZONE_CYCLE
@@ -250,6 +396,7 @@ def main():
                         "speedup_vs_fp64": mx["value"] / m["value"], "rel_l1_vs_fp64": mx.get("rel_l1_vs_fp64"),
                         "divb_max_abs": mx["divb_max_abs"], "state_finite": mx["state_finite"],
                         "roofline": mx["roofline"], "kernel_ms_per_step": mx["kernel_ms_per_step"]}
+    out["batch"] = batch
     print(json.dumps(out), flush=True)
 
 

@@ -23,7 +23,8 @@ FP64_FLAGS = ["-ffp-contract=off", "-fno-fast-math"]
 # the hardware FP32 reciprocal and square root.
 MIXED_FLAGS = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 
-UNITS = [("cumc.hip", FP64_FLAGS), ("cumc_mixed.hip", MIXED_FLAGS), ("cumc_batch.hip", FP64_FLAGS)]
+UNITS = [("cumc.hip", FP64_FLAGS), ("cumc_mixed.hip", MIXED_FLAGS), ("cumc_batch.hip", FP64_FLAGS),
+         ("cumc_adjoint.hip", FP64_FLAGS)]
 
 
 def _run(cmd):

@@ -29,6 +29,7 @@ struct cumc_ctx {
     double *Ub = nullptr;               // third conserved buffer (stage 2 output)
     bool dt_valid = false;              // dt_dev[1] holds finddt of the current state
     bool grid_set = false;
+    struct cumc_tape *tape = nullptr;  // reverse-mode record (cumc_adjoint.hip)
     int seg = 48;     // target faces per pencil segment of the y / z sweeps
     int zchunk = 32;  // planes per column tile of the EMF and stage-tail kernels
     double last_ms = 0.0;
@@ -113,6 +114,9 @@ static inline void swap_u(cumc_ctx *ctx)
     ctx->F32.U0 = ctx->F64.U0;
     ctx->F32.U1 = ctx->F64.U1;
 }
+
+// frees the reverse-mode record (defined in cumc_adjoint.hip)
+void tape_release(cumc_ctx *ctx);
 
 // ghost passes + torus clamp (defined in cumc.hip)
 void launch_ghost(cumc_ctx *ctx);

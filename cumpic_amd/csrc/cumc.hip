@@ -141,6 +141,7 @@ extern "C" int cumc_destroy(cumc_ctx *ctx)
 {
     if (!ctx) return CUMC_EINVAL;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    tape_release(ctx);
     hipFree(ctx->pool);
     hipFree(ctx->fluxpool);
     hipFree(ctx->dt_dev);
@@ -238,6 +239,9 @@ extern "C" int cumc_upload(cumc_ctx *ctx, const double *prim, const double *eps)
 {
     if (!ctx || !prim || !eps) return CUMC_EINVAL;
     ctx->dt_valid = false;
+    // bcell outside [1,n]^3 is read before the first BOUNDARY (as the reference's zero-initialised
+    // array): a re-upload starts from the same zeros as a fresh context
+    HIPCHK(ctx, hipMemsetAsync(ctx->F64.BC, 0, 3 * (size_t)ctx->L.ps * sizeof(double), ctx->stream));
     int r = conv_in(ctx, prim, 8, ctx->F64.P);
     if (r) return r;
     return conv_in(ctx, eps, 1, ctx->F64.EPS);

@@ -1,0 +1,905 @@
+// cumc_adjoint.hip -- reverse mode of initial_update + N RUNGEKUTTA steps (FP64, Cartesian,
+// riemann model hooks), with checkpoint-every-k recomputation.
+//
+// The forward map is the exact FP64 step of the other kernels with the time steps frozen (the
+// CFL reduction is a max over cells and is not differentiated).  Each kernel below reverses one
+// forward write phase: it reads the adjoints of what the phase wrote, adds their contributions
+// to the adjoints of what it read, and zeroes the adjoints of the overwritten values, so the
+// adjoint arrays always hold d(objective)/d(current contents) of the forward arrays.  Every
+// adjoint value has one writer per kernel (gathers, no atomics), so gradients are reproducible
+// bit for bit from run to run.
+//
+// Reverse of one stage (forward: sweeps -> corner EMFs -> tail -> ghost fill):
+//   ghost fill z, y, x (kr_ghost)      BOUNDARYP / BOUNDARY1D + UPDATE's EOS on the ghosts
+//   tail (kr_tail_a, kr_tail_b)        RK combination, FROMUTORVE, CHECKRHO, FROMRVETOU, EOS
+//   flux / EMF gathers (kr_gather)     flux divergence and the face-B right-hand sides
+//   corner EMFs (kr_ecorn)             flux_ct (mhd.f90:73-157)
+//   sweeps (kr_flux_hll, kr_flux_recon) RIEMANN (rsolver.f90) then PPM / TVD reconstruction
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "adjoint.hpp"
+#include "ctx.hpp"
+
+namespace {
+
+// adjoint arrays of one context, FP64 planes in the state layout
+struct AdjF {
+    double *P;    // 8
+    double *BC;   // 3
+    double *EPS;  // 1
+    double *F[3]; // 7 per sweep (xyz component order of the forward flux planes)
+    double *E;    // 3
+    double *T;    // 11 temporaries: 0-4 d/d(rhs hydro), 5-7 face-B, 8-10 half cell-B adjoints
+    double *H;    // 14: adjoints of the reconstructed left (0-6) and right (7-13) face states
+};
+constexpr int NADJ = 8 + 3 + 1 + 21 + 3 + 11 + 14;
+
+template <int STAGE> struct RKC;
+template <> struct RKC<1> { static constexpr double a0 = 1.0, a1 = 0.0, a2 = 1.0; };
+template <> struct RKC<2> { static constexpr double a0 = RK20, a1 = RK21, a2 = RK22; };
+template <> struct RKC<3> { static constexpr double a0 = RK30, a1 = RK31, a2 = RK32; };
+
+// ---------------------------------------------------------------------------------------------
+// ghost fill, one pass, reversed: a thread owns one line (the six ghost cells of the pass along
+// AX at fixed transverse (a, b)); sources are interior along AX, destinations ghosts, so lines
+// never share a cell.  rho_out / eps_out: forward values after the stage (at every destination
+// whose p adjoint is still live they are the values the pass wrote).
+// ---------------------------------------------------------------------------------------------
+template <int AX>
+__global__ __launch_bounds__(256) void kr_ghost(Lay L, AdjF A, Model M, const double *__restrict__ rho_out,
+                                                const double *__restrict__ eps_out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int na, nb, a0, b0;
+    if (AX == 0) { a0 = 0; na = L.ny + 1; b0 = 0; nb = L.nz + 1; }
+    else if (AX == 1) { a0 = -2; na = L.nx + 6; b0 = 0; nb = L.nz + 1; }
+    else { a0 = -2; na = L.nx + 6; b0 = -2; nb = L.ny + 6; }
+    if (t >= (long)na * nb) return;
+    const int a = (int)(t % na) + a0, b = (int)(t / na) + b0;
+    const int n = AX == 0 ? L.nx : (AX == 1 ? L.ny : L.nz);
+    const long ps = L.ps;
+    const double gm1 = M.ggas - 1.0;
+    auto cell = [&](int i) {
+        if (AX == 0) return L.id(i, a, b);
+        if (AX == 1) return L.id(a, i, b);
+        return L.id(a, b, i);
+    };
+    bool final_region;
+    if (AX == 0) final_region = a >= 1 && a <= L.ny && b >= 1 && b <= L.nz;
+    else if (AX == 1) final_region = b >= 1 && b <= L.nz;
+    else final_region = true;
+    const int tb1 = AX == 0 ? 6 : 5, tb2 = AX == 2 ? 6 : 7;
+    for (int side = 0; side < 2; side++) {
+        const int flag = M.bflag[2 * AX + side];
+        const int *fac = M.bfac[2 * AX + side];
+        const bool usef = flag >= 2;
+        for (int g = 1; g <= 3; g++) {
+            const long dst = cell(side == 0 ? 1 - g : n + g);
+            const long src = cell(ghost_src(flag, side, g, n));
+            if (final_region) {
+                // p(dst) = rho(dst) * eps(dst) * (gamma - 1)
+                const double lp = A.P[dst + 4 * ps];
+                A.P[dst + 4 * ps] = 0.0;
+                A.P[dst] += lp * eps_out[dst] * gm1;
+                A.EPS[dst] += lp * rho_out[dst] * gm1;
+                int ef = flag;
+                bool doit = true;
+                if (AX == 1 && side == 1) doit = eps_y_outer(M.bflag, &ef);
+                if (doit) {
+                    const long es = cell(ghost_src(ef, side, g, n));
+                    const double le = A.EPS[dst];
+                    A.EPS[dst] = 0.0;
+                    A.EPS[es] += le;
+                }
+            }
+            auto back = [&](double *X, long off, int fi) {
+                const double l = X[dst + off];
+                X[dst + off] = 0.0;
+                X[src + off] += usef ? (double)fac[fi] * l : l;
+            };
+            back(A.P, tb1 * ps, tb1);
+            back(A.P, tb2 * ps, tb2);
+            for (int c = 0; c < 3; c++) back(A.BC, c * ps, 5 + c);
+            for (int i = 0; i < 5; i++) back(A.P, i * ps, i);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// stage tail reversed, part a: per cell of [0,n]^3 the forward values are recomputed exactly as
+// k_tail does (so the floor branch replays), then the chain
+//   Uout, P, EPS, bcell  <-  (rho, v, eps, bcell)  <-  (u, bcell)  <-  (U0, Uin, rhs)
+// is reversed.  Hydro adjoints go to U0 / Uin directly; the face-B parts wait for part b
+// because a face field feeds the cell-centred field of two cells.
+// ---------------------------------------------------------------------------------------------
+template <int STAGE>
+__global__ __launch_bounds__(256) void kr_tail_a(Lay L, Fields<double> F, Tabs tb, Model M, double dt,
+                                                 const double *__restrict__ U0, const double *__restrict__ Uin,
+                                                 AdjF A, double *lU0, double *lUin, double *lUout)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
+    if (k > L.ny || j > L.nx) return;
+    const long g = L.id(j, k, l), sj = 1, sk = L.px, sl = L.py, ps = L.ps;
+    const bool interior = j >= 1 && k >= 1 && l >= 1;
+    const double *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2], *E = F.E;
+    auto uin = [&](long i) { return STAGE == 1 ? 0.0 : Uin[i]; };
+    const double gm1 = M.ggas - 1.0;
+    // forward recompute (k_tail, Cartesian)
+    const double ex0 = E[g], ey0 = E[g + ps], ez0 = E[g + 2 * ps];
+    const double exk = E[g - sk], ezk = E[g - sk + 2 * ps];
+    const double eyj = E[g - sj + ps], ezj = E[g - sj + 2 * ps];
+    const double ezjk = E[g - sj - sk + 2 * ps];
+    const double exL = E[g - sl], eyL = E[g - sl + ps], eyjL = E[g - sl - sj + ps], exkL = E[g - sl - sk];
+    double u[5], bc[3];
+    {
+        double lrk[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (interior) {
+            const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
+            for (int i = 0; i < 5; i++) {
+                double a = 0.0;
+                a = a - (Fx[g + i * ps] * 1.0 - Fx[g - sj + i * ps] * 1.0) / dxl;
+                a = a - (Fy[g + i * ps] * 1.0 - Fy[g - sk + i * ps] * 1.0) / dyl;
+                a = a - (Fz[g + i * ps] * 1.0 - Fz[g - sl + i * ps] * 1.0) / dzl;
+                lrk[i] = a + 0.0;
+            }
+        }
+        for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(U0[g + i * ps], uin(g + i * ps), dt, lrk[i]);
+        if (interior) {
+            const double bxn = rk<STAGE>(U0[g + 5 * ps], uin(g + 5 * ps), dt, rhs_bx<1>(tb, j, k, l, ez0, ezk, ey0, eyL));
+            const double byn = rk<STAGE>(U0[g + 6 * ps], uin(g + 6 * ps), dt, rhs_by<1>(tb, j, l, ex0, exL, ez0, ezj));
+            const double bzn = rk<STAGE>(U0[g + 7 * ps], uin(g + 7 * ps), dt, rhs_bz<1>(tb, j, k, ey0, eyj, ex0, exk));
+            const double bxm = rk<STAGE>(U0[g - sj + 5 * ps], uin(g - sj + 5 * ps), dt,
+                                         rhs_bx<1>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
+            const double bym = rk<STAGE>(U0[g - sk + 6 * ps], uin(g - sk + 6 * ps), dt,
+                                         rhs_by<1>(tb, j, l, exk, exkL, ezk, ezjk));
+            const double bzm = rk<STAGE>(U0[g - sl + 7 * ps], uin(g - sl + 7 * ps), dt,
+                                         rhs_bz<1>(tb, j, k, eyL, eyjL, exL, exkL));
+            bc[0] = 0.5 * (bxn + bxm);
+            bc[1] = 0.5 * (byn + bym);
+            bc[2] = 0.5 * (bzn + bzm);
+        } else {
+            bc[0] = F.BC[g];
+            bc[1] = F.BC[g + ps];
+            bc[2] = F.BC[g + 2 * ps];
+        }
+    }
+    const double rho = u[0];
+    const double vx = u[1] / rho, vy = u[2] / rho, vz = u[3] / rho;
+    const double bsq = bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2];
+    const double vsq = vx * vx + vy * vy + vz * vz;
+    const double eps0 = (u[4] - 0.5 * bsq) / rho - 0.5 * vsq;
+    const bool floored = interior && (0.0 > eps0);
+    const double eps = floored ? 0.0 : eps0;
+    // adjoints of the outputs
+    double lP[8], lUo[8], lBc[3];
+    for (int v = 0; v < 8; v++) {
+        lP[v] = A.P[g + v * ps];
+        lUo[v] = lUout[g + v * ps];
+        A.P[g + v * ps] = 0.0;
+        lUout[g + v * ps] = 0.0;
+    }
+    const double lE = A.EPS[g];
+    A.EPS[g] = 0.0;
+    for (int c = 0; c < 3; c++) {
+        lBc[c] = interior ? A.BC[g + c * ps] : 0.0;
+        if (interior) A.BC[g + c * ps] = 0.0;
+    }
+    // Uout = (rho, rho v, rho (eps + v^2/2) + B^2/2), P = (rho, v, rho eps (gamma-1)), EPS = eps
+    double lrho = lUo[0] + lP[0] + lUo[1] * vx + lUo[2] * vy + lUo[3] * vz + lUo[4] * (eps + 0.5 * vsq) +
+                  lP[4] * eps * gm1;
+    double lv[3] = {lUo[1] * rho + lP[1] + lUo[4] * rho * vx, lUo[2] * rho + lP[2] + lUo[4] * rho * vy,
+                    lUo[3] * rho + lP[3] + lUo[4] * rho * vz};
+    const double leps = lE + lUo[4] * rho + lP[4] * rho * gm1;
+    double lbc[3];
+    for (int c = 0; c < 3; c++) lbc[c] = lBc[c] + lUo[4] * bc[c];  // d(B^2/2)
+    const double leps0 = floored ? 0.0 : leps;
+    // eps0 = (u4 - B^2/2)/rho - v^2/2
+    double lu[5];
+    lu[4] = leps0 / rho;
+    for (int c = 0; c < 3; c++) lbc[c] -= leps0 * bc[c] / rho;
+    lrho -= leps0 * (u[4] - 0.5 * bsq) / (rho * rho);
+    lv[0] -= leps0 * vx;
+    lv[1] -= leps0 * vy;
+    lv[2] -= leps0 * vz;
+    // v = m / rho
+    lu[1] = lv[0] / rho;
+    lu[2] = lv[1] / rho;
+    lu[3] = lv[2] / rho;
+    lrho -= (lv[0] * vx + lv[1] * vy + lv[2] * vz) / rho;
+    lu[0] = lrho;
+    // u = a0 U0 + a1 Uin + a2 dt rhs
+    for (int i = 0; i < 5; i++) {
+        lU0[g + i * ps] += RKC<STAGE>::a0 * lu[i];
+        if (STAGE > 1) lUin[g + i * ps] += RKC<STAGE>::a1 * lu[i];
+        A.T[g + i * ps] = interior ? RKC<STAGE>::a2 * dt * lu[i] : 0.0;
+    }
+    for (int c = 0; c < 3; c++) {
+        const double half = interior ? 0.5 * lbc[c] : 0.0;
+        A.T[g + (5 + c) * ps] = lP[5 + c] + lUo[5 + c] + half;
+        A.T[g + (8 + c) * ps] = half;
+        if (!interior) A.BC[g + c * ps] += lbc[c];  // the ghost bcell the tail read
+    }
+}
+
+// part b: face-B adjoints (own cell's share + the upper neighbour's share of its cell field)
+template <int STAGE>
+__global__ __launch_bounds__(256) void kr_tail_b(Lay L, AdjF A, double dt, double *lU0, double *lUin)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
+    if (k > L.ny || j > L.nx) return;
+    const long g = L.id(j, k, l), ps = L.ps;
+    double lb[3];
+    lb[0] = A.T[g + 5 * ps] + ((j + 1 <= L.nx && k >= 1 && l >= 1) ? A.T[g + 1 + 8 * ps] : 0.0);
+    lb[1] = A.T[g + 6 * ps] + ((k + 1 <= L.ny && j >= 1 && l >= 1) ? A.T[g + L.px + 9 * ps] : 0.0);
+    lb[2] = A.T[g + 7 * ps] + ((l + 1 <= L.nz && j >= 1 && k >= 1) ? A.T[g + L.py + 10 * ps] : 0.0);
+    for (int c = 0; c < 3; c++) {
+        lU0[g + (5 + c) * ps] += RKC<STAGE>::a0 * lb[c];
+        if (STAGE > 1) lUin[g + (5 + c) * ps] += RKC<STAGE>::a1 * lb[c];
+        A.T[g + (5 + c) * ps] = RKC<STAGE>::a2 * dt * lb[c];  // d/d(rhs of the face field)
+    }
+}
+
+// flux and corner-EMF adjoints gathered from the per-cell right-hand-side adjoints, on [0,n+1]^3
+// (hydro flux components; the B-flux components are set by kr_ecorn)
+__global__ __launch_bounds__(256) void kr_gather(Lay L, AdjF A, Tabs tb)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
+    if (k > L.ny + 1 || j > L.nx + 1) return;
+    const long g = L.id(j, k, l), ps = L.ps;
+    auto inter = [&](int jj, int kk, int ll) {
+        return jj >= 1 && jj <= L.nx && kk >= 1 && kk <= L.ny && ll >= 1 && ll <= L.nz;
+    };
+    auto inbox = [&](int jj, int kk, int ll) { return jj <= L.nx && kk <= L.ny && ll <= L.nz; };
+    // flux divergence: lrk(c) = -(F(face c) - F(face c-1)) / d(c)
+    if (j <= L.nx)
+        for (int i = 0; i < 5; i++)
+            A.F[0][g + i * ps] = (inter(j, k, l) ? -A.T[g + i * ps] / tb.at(0, TB_D, j) : 0.0) +
+                                 (inter(j + 1, k, l) ? A.T[g + 1 + i * ps] / tb.at(0, TB_D, j + 1) : 0.0);
+    if (k <= L.ny)
+        for (int i = 0; i < 5; i++)
+            A.F[1][g + i * ps] = (inter(j, k, l) ? -A.T[g + i * ps] / tb.at(1, TB_D, k) : 0.0) +
+                                 (inter(j, k + 1, l) ? A.T[g + L.px + i * ps] / tb.at(1, TB_D, k + 1) : 0.0);
+    if (l <= L.nz)
+        for (int i = 0; i < 5; i++)
+            A.F[2][g + i * ps] = (inter(j, k, l) ? -A.T[g + i * ps] / tb.at(2, TB_D, l) : 0.0) +
+                                 (inter(j, k, l + 1) ? A.T[g + L.py + i * ps] / tb.at(2, TB_D, l + 1) : 0.0);
+    // corner EMFs from the face-field right-hand sides (mhd.f90:130-153, Cartesian)
+    if (inbox(j, k, l)) {
+        auto rb = [&](int c, int jj, int kk, int ll) {
+            return inbox(jj, kk, ll) ? A.T[L.id(jj, kk, ll) + (5 + c) * ps] : 0.0;
+        };
+        const double dx = tb.at(0, TB_D, j), dy = tb.at(1, TB_D, k), dz = tb.at(2, TB_D, l);
+        const double dx1 = tb.at(0, TB_D, j + 1), dy1 = tb.at(1, TB_D, k + 1), dz1 = tb.at(2, TB_D, l + 1);
+        const double lez = -rb(0, j, k, l) / dy + rb(0, j, k + 1, l) / dy1 + rb(1, j, k, l) / dx - rb(1, j + 1, k, l) / dx1;
+        const double ley = rb(0, j, k, l) / dz - rb(0, j, k, l + 1) / dz1 - rb(2, j, k, l) / dx + rb(2, j + 1, k, l) / dx1;
+        const double lex = -rb(1, j, k, l) / dz + rb(1, j, k, l + 1) / dz1 + rb(2, j, k, l) / dy - rb(2, j, k + 1, l) / dy1;
+        A.E[g] = lex;
+        A.E[g + ps] = ley;
+        A.E[g + 2 * ps] = lez;
+    }
+}
+
+// flux_ct reversed (forward in ecorn_tile): on [0,n+1]^3 a thread owns the B-flux components of
+// the three faces at its index and the cell EMF of its cell
+__global__ __launch_bounds__(256) void kr_ecorn(Lay L, Fields<double> F, AdjF A)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
+    if (k > L.ny + 1 || j > L.nx + 1) return;
+    const long g = L.id(j, k, l), ps = L.ps;
+    auto le = [&](int c, int jj, int kk, int ll) {
+        return (jj >= 0 && kk >= 0 && ll >= 0 && jj <= L.nx && kk <= L.ny && ll <= L.nz)
+                   ? A.E[L.id(jj, kk, ll) + c * ps] : 0.0;
+    };
+    if (j <= L.nx) {
+        A.F[0][g + 6 * ps] = 0.5 * (le(1, j, k, l) + le(1, j, k, l - 1));    // F_x(Bz) -> ey
+        A.F[0][g + 5 * ps] = -0.5 * (le(2, j, k, l) + le(2, j, k - 1, l));   // F_x(By) -> ez
+    }
+    if (k <= L.ny) {
+        A.F[1][g + 6 * ps] = -0.5 * (le(0, j, k, l) + le(0, j, k, l - 1));   // F_y(Bz) -> ex
+        A.F[1][g + 5 * ps] = 0.5 * (le(2, j, k, l) + le(2, j - 1, k, l));    // F_y(Bx) -> ez
+    }
+    if (l <= L.nz) {
+        A.F[2][g + 6 * ps] = 0.5 * (le(0, j, k, l) + le(0, j, k - 1, l));    // F_z(By) -> ex
+        A.F[2][g + 5 * ps] = -0.5 * (le(1, j, k, l) + le(1, j - 1, k, l));   // F_z(Bx) -> ey
+    }
+    // cell EMF -v x B at this cell, used by four edges per component with weight -1/4
+    const double lx = -0.25 * (le(0, j, k, l) + le(0, j, k, l - 1) + le(0, j, k - 1, l) + le(0, j, k - 1, l - 1));
+    const double ly = -0.25 * (le(1, j, k, l) + le(1, j, k, l - 1) + le(1, j - 1, k, l) + le(1, j - 1, k, l - 1));
+    const double lz = -0.25 * (le(2, j, k, l) + le(2, j, k - 1, l) + le(2, j - 1, k, l) + le(2, j - 1, k - 1, l));
+    const double vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
+    const double bx = F.BC[g], by = F.BC[g + ps], bz = F.BC[g + 2 * ps];
+    // ex = vz by - vy bz ; ey = vx bz - vz bx ; ez = vy bx - vx by
+    A.P[g + ps] += ly * bz - lz * by;
+    A.P[g + 2 * ps] += -lx * bz + lz * bx;
+    A.P[g + 3 * ps] += lx * by - ly * bx;
+    A.BC[g] += -ly * vz + lz * vy;
+    A.BC[g + ps] += lx * vz - lz * vx;
+    A.BC[g + 2 * ps] += -lx * vy + ly * vx;
+}
+
+// ---------------------------------------------------------------------------------------------
+// sweeps reversed.  Pencils along DIR cover the forward flux domain; cells along the pencil run
+// from -2 to n+3, faces from 0 to n (face f between cells f and f+1).
+// ---------------------------------------------------------------------------------------------
+template <int DIR>
+struct Pencil {
+    long base, stride;
+    int n;
+    __device__ bool init(const Lay &L, long t)
+    {
+        if (DIR == 1) {
+            const int k = (int)(t % (L.ny + 2)), l = (int)(t / (L.ny + 2));
+            if (l > L.nz + 1) return false;
+            base = L.id(-2, k, l); stride = 1; n = L.nx;
+        } else if (DIR == 2) {
+            const int j = (int)(t % (L.nx + 2)), l = (int)(t / (L.nx + 2));
+            if (l > L.nz + 1) return false;
+            base = L.id(j, -2, l); stride = L.px; n = L.ny;
+        } else {
+            const int j = (int)(t % (L.nx + 2)), k = (int)(t / (L.nx + 2));
+            if (k > L.ny + 1) return false;
+            base = L.id(j, k, -2); stride = L.py; n = L.nz;
+        }
+        return true;
+    }
+    __device__ long at(int c) const { return base + (long)(c + 2) * stride; }
+};
+
+template <int DIR>
+__host__ __device__ inline long npencils(const Lay &L)
+{
+    return DIR == 1 ? (long)(L.ny + 2) * (L.nz + 2) : (DIR == 2 ? (long)(L.nx + 2) * (L.nz + 2) : (long)(L.nx + 2) * (L.ny + 2));
+}
+
+// reconstructed variable v (0..6) of a sweep: P planes 0..4, then the two transverse cell fields
+template <int DIR>
+__device__ inline const double *qplane(const Fields<double> &F, long ps, int v)
+{
+    if (v < 5) return F.P + v * ps;
+    const int bA = DIR == 1 ? 1 : 0, bB = DIR == 3 ? 1 : 2;
+    return F.BC + (v == 5 ? bA : bB) * ps;
+}
+template <int DIR>
+__device__ inline double *qplane_adj(const AdjF &A, long ps, int v)
+{
+    if (v < 5) return A.P + v * ps;
+    const int bA = DIR == 1 ? 1 : 0, bB = DIR == 3 ? 1 : 2;
+    return A.BC + (v == 5 ? bA : bB) * ps;
+}
+
+// PPM / TVD edge values of cell c from w(c-2..c+2)
+template <int RECON>
+__device__ inline void recon_cell(const Tabs &tb, int ax, int c, const double *w, double &vm, double &vp)
+{
+    if (RECON == 4) {
+        const double dmm = ppm_slope<double>(tb.at(ax, TB_B0C, c - 1), tb.at(ax, TB_B1C, c - 1), w[0], w[1], w[2]);
+        const double dmc = ppm_slope<double>(tb.at(ax, TB_B0C, c), tb.at(ax, TB_B1C, c), w[1], w[2], w[3]);
+        const double dmp = ppm_slope<double>(tb.at(ax, TB_B0C, c + 1), tb.at(ax, TB_B1C, c + 1), w[2], w[3], w[4]);
+        ppm_edges<double>(tb.at(ax, TB_A0M, c), tb.at(ax, TB_A1M, c), tb.at(ax, TB_A2M, c), tb.at(ax, TB_A3M, c),
+                          tb.at(ax, TB_A0P, c), tb.at(ax, TB_A1P, c), tb.at(ax, TB_A2P, c), tb.at(ax, TB_A3P, c), w[1],
+                          w[2], w[3], dmm, dmc, dmp, vm, vp);
+    } else {
+        tvd_edges<double>(tb.at(ax, TB_CF, c), tb.at(ax, TB_CB, c), w[1], w[2], w[3], vm, vp);
+    }
+}
+
+// face states in the rotated frame of hll_face<DIR> from the reconstructed q order
+template <int DIR>
+__device__ inline FaceState<double> rot(const double *q, double bn)
+{
+    if (DIR == 1) return FaceState<double>{q[0], q[1], q[2], q[3], q[4], bn, q[5], q[6]};
+    if (DIR == 2) return FaceState<double>{q[0], q[2], q[3], q[1], q[4], bn, q[6], q[5]};
+    return FaceState<double>{q[0], q[3], q[1], q[2], q[4], bn, q[5], q[6]};
+}
+template <int DIR>
+__device__ inline void unrot(const FaceState<double> &s, double *q)
+{
+    if (DIR == 1) { q[0] = s.rho; q[1] = s.vn; q[2] = s.vt1; q[3] = s.vt2; q[4] = s.p; q[5] = s.bt1; q[6] = s.bt2; }
+    else if (DIR == 2) { q[0] = s.rho; q[2] = s.vn; q[3] = s.vt1; q[1] = s.vt2; q[4] = s.p; q[6] = s.bt1; q[5] = s.bt2; }
+    else { q[0] = s.rho; q[3] = s.vn; q[1] = s.vt1; q[2] = s.vt2; q[4] = s.p; q[5] = s.bt1; q[6] = s.bt2; }
+}
+// stored flux planes (xyz order) -> rotated hll_face output order
+template <int DIR>
+__device__ inline void flux_to_rot(const double *o, double *f)
+{
+    if (DIR == 1) { for (int i = 0; i < 7; i++) f[i] = o[i]; }
+    else if (DIR == 2) { f[0] = o[0]; f[3] = o[1]; f[1] = o[2]; f[2] = o[3]; f[4] = o[4]; f[6] = o[5]; f[5] = o[6]; }
+    else { f[0] = o[0]; f[2] = o[1]; f[3] = o[2]; f[1] = o[3]; f[4] = o[4]; f[5] = o[5]; f[6] = o[6]; }
+}
+
+// HLL reversed at every face of a pencil: writes the left / right state adjoints (q order) to
+// H and adds the normal-field adjoint to the face-B plane
+template <int DIR, int RECON>
+__global__ __launch_bounds__(64) void kr_flux_hll(Lay L, Fields<double> F, Tabs tb, Model M, AdjF A)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npencils<DIR>(L)) return;
+    Pencil<DIR> pc;
+    if (!pc.init(L, t)) return;
+    const long ps = L.ps;
+    const int ax = DIR - 1;
+    const double *bnp = F.P + (4 + DIR) * ps;
+    double w[6][7];  // cells f-2 .. f+3
+    for (int c = -2; c <= 2; c++)
+        for (int v = 0; v < 7; v++) w[c + 2][v] = qplane<DIR>(F, ps, v)[pc.at(c)];
+    for (int f = 0; f <= pc.n; f++) {
+        for (int v = 0; v < 7; v++) w[5][v] = qplane<DIR>(F, ps, v)[pc.at(f + 3)];
+        double ql[7], qr[7];
+        for (int v = 0; v < 7; v++) {
+            double s0[5] = {w[0][v], w[1][v], w[2][v], w[3][v], w[4][v]};
+            double s1[5] = {w[1][v], w[2][v], w[3][v], w[4][v], w[5][v]};
+            double vm, vp;
+            recon_cell<RECON>(tb, ax, f, s0, vm, vp);
+            ql[v] = vp;
+            recon_cell<RECON>(tb, ax, f + 1, s1, vm, vp);
+            qr[v] = vm;
+        }
+        const long gf = pc.at(f);
+        const double bn = bnp[gf];
+        double o[7], lf[7];
+        for (int i = 0; i < 7; i++) o[i] = A.F[ax][gf + i * ps];
+        flux_to_rot<DIR>(o, lf);
+        FaceState<double> lL{0, 0, 0, 0, 0, 0, 0, 0}, lR{0, 0, 0, 0, 0, 0, 0, 0};
+        hll_face_adj(M.ggas, rot<DIR>(ql, bn), rot<DIR>(qr, bn), lf, lL, lR);
+        double hl[7], hr[7];
+        unrot<DIR>(lL, hl);
+        unrot<DIR>(lR, hr);
+        for (int v = 0; v < 7; v++) {
+            A.H[gf + v * ps] = hl[v];
+            A.H[gf + (7 + v) * ps] = hr[v];
+        }
+        A.P[gf + (4 + DIR) * ps] += lL.bn + lR.bn;
+        for (int c = 0; c < 5; c++)
+            for (int v = 0; v < 7; v++) w[c][v] = w[c + 1][v];
+    }
+}
+
+// reconstruction reversed along a pencil for one variable (blockIdx.y): a sliding window of
+// values, slopes and adjoint accumulators; each cell's adjoint is emitted once complete
+template <int DIR, int RECON>
+__global__ __launch_bounds__(64) void kr_flux_recon(Lay L, Fields<double> F, Tabs tb, AdjF A)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npencils<DIR>(L)) return;
+    Pencil<DIR> pc;
+    if (!pc.init(L, t)) return;
+    const long ps = L.ps;
+    const int ax = DIR - 1, v = blockIdx.y, n = pc.n;
+    const double *W = qplane<DIR>(F, ps, v);
+    double *LW = qplane_adj<DIR>(A, ps, v);
+    const double *HL = A.H + v * ps, *HR = A.H + (7 + v) * ps;
+    auto lvm_of = [&](int c) { return c >= 1 ? HR[pc.at(c - 1)] : 0.0; };
+    auto lvp_of = [&](int c) { return c <= n ? HL[pc.at(c)] : 0.0; };
+    if (RECON == 4) {
+        double w[5], dm[3], aw[4] = {0, 0, 0, 0}, ad[3] = {0, 0, 0};
+        for (int i = 0; i < 5; i++) w[i] = W[pc.at(i - 2)];  // w(c-2 .. c+2), c = 0
+        auto slope = [&](int s, double a, double b, double c) {
+            return ppm_slope<double>(tb.at(ax, TB_B0C, s), tb.at(ax, TB_B1C, s), a, b, c);
+        };
+        dm[0] = slope(-1, w[0], w[1], w[2]);
+        dm[1] = slope(0, w[1], w[2], w[3]);
+        dm[2] = slope(1, w[2], w[3], w[4]);
+        for (int c = 0; c <= n + 1; c++) {
+            ppm_edges_adj(tb.at(ax, TB_A0M, c), tb.at(ax, TB_A1M, c), tb.at(ax, TB_A2M, c), tb.at(ax, TB_A3M, c),
+                          tb.at(ax, TB_A0P, c), tb.at(ax, TB_A1P, c), tb.at(ax, TB_A2P, c), tb.at(ax, TB_A3P, c), w[1],
+                          w[2], w[3], dm[0], dm[1], dm[2], lvm_of(c), lvp_of(c), aw[1], aw[2], aw[3], ad[0], ad[1],
+                          ad[2]);
+            // slope c-1 has all its users
+            ppm_slope_adj(tb.at(ax, TB_B0C, c - 1), tb.at(ax, TB_B1C, c - 1), w[0], w[1], w[2], ad[0], aw[0], aw[1],
+                          aw[2]);
+            LW[pc.at(c - 2)] += aw[0];
+            aw[0] = aw[1]; aw[1] = aw[2]; aw[2] = aw[3]; aw[3] = 0.0;
+            ad[0] = ad[1]; ad[1] = ad[2]; ad[2] = 0.0;
+            if (c <= n) {
+                w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
+                w[4] = W[pc.at(c + 3)];
+                dm[0] = dm[1]; dm[1] = dm[2];
+                dm[2] = slope(c + 2, w[2], w[3], w[4]);
+            } else {
+                w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
+            }
+        }
+        // now w[0..3] = w(n .. n+3); slopes n+1 and n+2 are complete
+        ppm_slope_adj(tb.at(ax, TB_B0C, n + 1), tb.at(ax, TB_B1C, n + 1), w[0], w[1], w[2], ad[0], aw[0], aw[1], aw[2]);
+        ppm_slope_adj(tb.at(ax, TB_B0C, n + 2), tb.at(ax, TB_B1C, n + 2), w[1], w[2], w[3], ad[1], aw[1], aw[2], aw[3]);
+        for (int i = 0; i < 4; i++) LW[pc.at(n + i)] += aw[i];
+    } else {
+        double w[3], aw[3] = {0, 0, 0};
+        for (int i = 0; i < 3; i++) w[i] = W[pc.at(i - 1)];  // w(c-1 .. c+1), c = 0
+        for (int c = 0; c <= n + 1; c++) {
+            tvd_edges_adj(tb.at(ax, TB_CF, c), tb.at(ax, TB_CB, c), w[0], w[1], w[2], lvm_of(c), lvp_of(c), aw[0], aw[1],
+                          aw[2]);
+            LW[pc.at(c - 1)] += aw[0];
+            aw[0] = aw[1]; aw[1] = aw[2]; aw[2] = 0.0;
+            w[0] = w[1]; w[1] = w[2];
+            if (c <= n) w[2] = W[pc.at(c + 2)];
+        }
+        LW[pc.at(n + 1)] += aw[0];
+        LW[pc.at(n + 2)] += aw[1];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// initial_update reversed (k_init: CHECKRHO, FROMRVETOU, EOS); the ghost fill that follows it is
+// reversed by kr_ghost first.  Forward inputs: the uploaded prim / eps (and the zero ghost bcell).
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kr_init_a(Lay L, Fields<double> F, Model M, AdjF A, double *lU)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
+    if (k > L.ny || j > L.nx) return;
+    const long g = L.id(j, k, l), sj = 1, sk = L.px, sl = L.py, ps = L.ps;
+    const bool interior = j >= 1 && k >= 1 && l >= 1;
+    const double gm1 = M.ggas - 1.0;
+    const double *B = F.P + 5 * ps;
+    double bc[3];
+    if (interior) {
+        bc[0] = 0.5 * (B[g] + B[g - sj]);
+        bc[1] = 0.5 * (B[g + ps] + B[g - sk + ps]);
+        bc[2] = 0.5 * (B[g + 2 * ps] + B[g - sl + 2 * ps]);
+    } else {
+        bc[0] = F.BC[g]; bc[1] = F.BC[g + ps]; bc[2] = F.BC[g + 2 * ps];
+    }
+    const double rho = F.P[g], vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
+    const double eps_in = F.EPS[g];
+    const bool floored = interior && (0.0 > eps_in);
+    const double eps = floored ? 0.0 : eps_in;
+    const double vsq = vx * vx + vy * vy + vz * vz;
+    double lUv[8];
+    for (int v = 0; v < 8; v++) {
+        lUv[v] = lU[g + v * ps];
+        lU[g + v * ps] = 0.0;
+    }
+    const double lP0 = A.P[g], lP4 = A.P[g + 4 * ps], lE = A.EPS[g];
+    const double lrho = lP0 + lUv[0] + lUv[1] * vx + lUv[2] * vy + lUv[3] * vz + lUv[4] * (eps + 0.5 * vsq) +
+                        lP4 * eps * gm1;
+    A.P[g] = lrho;
+    A.P[g + ps] += lUv[1] * rho + lUv[4] * rho * vx;
+    A.P[g + 2 * ps] += lUv[2] * rho + lUv[4] * rho * vy;
+    A.P[g + 3 * ps] += lUv[3] * rho + lUv[4] * rho * vz;
+    A.P[g + 4 * ps] = 0.0;  // p is recomputed from rho and eps
+    const double leps = lE + lUv[4] * rho + lP4 * rho * gm1;
+    A.EPS[g] = floored ? 0.0 : leps;
+    for (int c = 0; c < 3; c++) {
+        const double lbc = (interior ? A.BC[g + c * ps] : 0.0) + lUv[4] * bc[c];
+        A.T[g + (8 + c) * ps] = interior ? 0.5 * lbc : 0.0;
+        if (interior) A.BC[g + c * ps] = 0.0;
+        else A.BC[g + c * ps] += lbc;
+        A.T[g + (5 + c) * ps] = lUv[5 + c];
+    }
+}
+
+__global__ __launch_bounds__(256) void kr_init_b(Lay L, AdjF A)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
+    if (k > L.ny || j > L.nx) return;
+    const long g = L.id(j, k, l), ps = L.ps;
+    const bool in = j >= 1 && k >= 1 && l >= 1;
+    A.P[g + 5 * ps] += A.T[g + 5 * ps] + (in ? A.T[g + 8 * ps] : 0.0) +
+                       ((j + 1 <= L.nx && k >= 1 && l >= 1) ? A.T[g + 1 + 8 * ps] : 0.0);
+    A.P[g + 6 * ps] += A.T[g + 6 * ps] + (in ? A.T[g + 9 * ps] : 0.0) +
+                       ((k + 1 <= L.ny && j >= 1 && l >= 1) ? A.T[g + L.px + 9 * ps] : 0.0);
+    A.P[g + 7 * ps] += A.T[g + 7 * ps] + (in ? A.T[g + 10 * ps] : 0.0) +
+                       ((l + 1 <= L.nz && j >= 1 && k >= 1) ? A.T[g + L.py + 10 * ps] : 0.0);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------------
+struct cumc_tape {
+    int nsteps = 0, every = 1;
+    std::vector<double> dts;
+    double *mem = nullptr;     // one allocation
+    double *init = nullptr;    // uploaded prim (8) + eps (1)
+    double *ckpt = nullptr;    // nck states of SW planes: P 8, EPS 1, BC 3, U 8
+    int nck = 0;
+    double *seg = nullptr;     // every + 1 states
+    double *stA = nullptr;     // stage-1 output P/EPS/BC (12)
+    double *stB = nullptr;     // stage-2 output rho, eps (2)
+    double *adj = nullptr;     // NADJ planes
+    double *lu[3] = {nullptr, nullptr, nullptr};  // 8 planes each: U adjoints by role U0, U1, Ub
+    AdjF A;
+};
+static constexpr int SW = 20;  // planes of a stored state
+
+void tape_release(cumc_ctx *ctx)
+{
+    if (!ctx || !ctx->tape) return;
+    hipFree(ctx->tape->mem);
+    delete ctx->tape;
+    ctx->tape = nullptr;
+}
+
+static inline size_t plane_bytes(const cumc_ctx *ctx) { return (size_t)ctx->L.ps * sizeof(double); }
+
+// P, EPS, BC are the first 12 planes of the pool; U of the current step start is F64.U0
+static void save_state(cumc_ctx *ctx, double *dst)
+{
+    const size_t pb = plane_bytes(ctx);
+    hipMemcpyAsync(dst, ctx->pool, 12 * pb, hipMemcpyDeviceToDevice, ctx->stream);
+    hipMemcpyAsync(dst + 12 * ctx->L.ps, ctx->F64.U0, 8 * pb, hipMemcpyDeviceToDevice, ctx->stream);
+}
+static void load_state(cumc_ctx *ctx, const double *src, bool with_u = true)
+{
+    const size_t pb = plane_bytes(ctx);
+    hipMemcpyAsync(ctx->pool, src, 12 * pb, hipMemcpyDeviceToDevice, ctx->stream);
+    if (with_u) hipMemcpyAsync(ctx->F64.U0, src + 12 * ctx->L.ps, 8 * pb, hipMemcpyDeviceToDevice, ctx->stream);
+}
+
+static void set_dt(cumc_ctx *ctx, double dt)
+{
+    hipMemcpyAsync(ctx->dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    hipStreamSynchronize(ctx->stream);  // dt lives on the host stack
+}
+
+static void forward_step(cumc_ctx *ctx, double dt)
+{
+    set_dt(ctx, dt);
+    for (int st = 1; st <= 3; st++) stage<double>(ctx, ctx->F64, st);
+    swap_u(ctx);
+}
+
+static void recompute_flux_emf(cumc_ctx *ctx)
+{
+    const Lay &L = ctx->L;
+    launch_fluxes<double>(ctx, ctx->F64);
+    const int zc = ctx->zchunk;
+    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + zc) / zc;
+    hipLaunchKernelGGL((k_ecorn<double>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, ctx->F64, gx, gy, zc);
+}
+
+template <int DIR>
+static void reverse_sweep(cumc_ctx *ctx, const AdjF &A)
+{
+    const Lay &L = ctx->L;
+    Tabs tb = tabs_of(ctx);
+    const long np = npencils<DIR>(L);
+    if (ctx->M.recon == CUMC_PPMC) {
+        hipLaunchKernelGGL((kr_flux_hll<DIR, 4>), dim3(nblk(np, 64)), dim3(64), 0, ctx->stream, L, ctx->F64, tb, ctx->M, A);
+        hipLaunchKernelGGL((kr_flux_recon<DIR, 4>), dim3(nblk(np, 64), 7), dim3(64), 0, ctx->stream, L, ctx->F64, tb, A);
+    } else {
+        hipLaunchKernelGGL((kr_flux_hll<DIR, 1>), dim3(nblk(np, 64)), dim3(64), 0, ctx->stream, L, ctx->F64, tb, ctx->M, A);
+        hipLaunchKernelGGL((kr_flux_recon<DIR, 1>), dim3(nblk(np, 64), 7), dim3(64), 0, ctx->stream, L, ctx->F64, tb, A);
+    }
+}
+
+static void reverse_ghost(cumc_ctx *ctx, const AdjF &A, const double *rho_out, const double *eps_out)
+{
+    const Lay &L = ctx->L;
+    const long n0 = (long)(L.ny + 1) * (L.nz + 1), n1 = (long)(L.nx + 6) * (L.nz + 1), n2 = (long)(L.nx + 6) * (L.ny + 6);
+    hipStream_t s = ctx->stream;
+    hipLaunchKernelGGL(kr_ghost<2>, dim3(nblk(n2, 256)), dim3(256), 0, s, L, A, ctx->M, rho_out, eps_out);
+    hipLaunchKernelGGL(kr_ghost<1>, dim3(nblk(n1, 256)), dim3(256), 0, s, L, A, ctx->M, rho_out, eps_out);
+    hipLaunchKernelGGL(kr_ghost<0>, dim3(nblk(n0, 256)), dim3(256), 0, s, L, A, ctx->M, rho_out, eps_out);
+}
+
+// reverse one stage whose forward input (P, BC) is in the context and whose flux / EMF planes
+// have been recomputed from it
+template <int STAGE>
+static void reverse_stage(cumc_ctx *ctx, double dt, const double *rho_out, const double *eps_out, const double *Uin,
+                          double *lU0, double *lUin, double *lUout)
+{
+    cumc_tape *tp = ctx->tape;
+    const AdjF &A = tp->A;
+    const Lay &L = ctx->L;
+    hipStream_t s = ctx->stream;
+    Tabs tb = tabs_of(ctx);
+    reverse_ghost(ctx, A, rho_out, eps_out);
+    dim3 g0(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
+    hipLaunchKernelGGL((kr_tail_a<STAGE>), g0, dim3(256), 0, s, L, ctx->F64, tb, ctx->M, dt, ctx->F64.U0, Uin, A, lU0,
+                       lUin, lUout);
+    hipLaunchKernelGGL((kr_tail_b<STAGE>), g0, dim3(256), 0, s, L, A, dt, lU0, lUin);
+    dim3 g1(nblk((long)L.px * (L.ny + 2), 256), L.nz + 2);
+    hipLaunchKernelGGL(kr_gather, g1, dim3(256), 0, s, L, A, tb);
+    hipLaunchKernelGGL(kr_ecorn, g1, dim3(256), 0, s, L, ctx->F64, A);
+    reverse_sweep<3>(ctx, A);
+    reverse_sweep<2>(ctx, A);
+    reverse_sweep<1>(ctx, A);
+}
+
+// reverse step m: S_m in seg slot i, S_{m+1} in slot i+1; the adjoint of S_{m+1} in A / lu[U1]
+static void reverse_step(cumc_ctx *ctx, int i, double dt)
+{
+    cumc_tape *tp = ctx->tape;
+    const long ps = ctx->L.ps;
+    const size_t pb = plane_bytes(ctx);
+    const double *Sm = tp->seg + (size_t)i * SW * ps, *Sn = tp->seg + (size_t)(i + 1) * SW * ps;
+    double *&l0 = tp->lu[0], *&l1 = tp->lu[1], *&lb = tp->lu[2];
+    load_state(ctx, Sm);
+    set_dt(ctx, dt);
+    stage<double>(ctx, ctx->F64, 1);
+    hipMemcpyAsync(tp->stA, ctx->pool, 12 * pb, hipMemcpyDeviceToDevice, ctx->stream);
+    stage<double>(ctx, ctx->F64, 2);
+    hipMemcpyAsync(tp->stB, ctx->F64.P, pb, hipMemcpyDeviceToDevice, ctx->stream);
+    hipMemcpyAsync(tp->stB + ps, ctx->F64.EPS, pb, hipMemcpyDeviceToDevice, ctx->stream);
+    // stage 3: input (P2, BC2), Uin = Ub, output S_{m+1}
+    recompute_flux_emf(ctx);
+    reverse_stage<3>(ctx, dt, Sn, Sn + 8 * ps, ctx->Ub, l0, lb, l1);
+    // stage 2: input (P1, BC1), Uin = U1, output (rho2, eps2)
+    hipMemcpyAsync(ctx->pool, tp->stA, 12 * pb, hipMemcpyDeviceToDevice, ctx->stream);
+    recompute_flux_emf(ctx);
+    reverse_stage<2>(ctx, dt, tp->stB, tp->stB + ps, ctx->F64.U1, l0, l1, lb);
+    // stage 1: input S_m, output (rho1, eps1)
+    load_state(ctx, Sm, false);
+    recompute_flux_emf(ctx);
+    reverse_stage<1>(ctx, dt, tp->stA, tp->stA + 8 * ps, nullptr, l0, nullptr, l1);
+    // the adjoint of S_m's U becomes the output-role adjoint of step m-1
+    double *tmp = l0;
+    l0 = l1;
+    l1 = tmp;
+}
+
+static bool adjoint_supported(const cumc_ctx *ctx, std::string &why)
+{
+    const cumc_config &c = ctx->cfg;
+    if (c.precision != CUMC_FP64) { why = "reverse mode needs the FP64 form"; return false; }
+    if (c.coordinate != CUMC_CARTESIAN) { why = "reverse mode covers Cartesian grids"; return false; }
+    if (c.model != CUMC_MODEL_RIEMANN) { why = "reverse mode covers the riemann model hooks"; return false; }
+    if (c.boundary_flag[2] != 0 && c.boundary_flag[3] == 0) {
+        why = "y-outer periodic with y-inner non-periodic leaves ghost eps stale (BOUNDARY1D); not differentiated";
+        return false;
+    }
+    return true;
+}
+
+extern "C" int cumc_tape_run(cumc_ctx *ctx, int nsteps, int every, const double *dt_in, double *dt_out)
+{
+    if (!ctx || nsteps < 0 || every < 1) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    std::string why;
+    if (!adjoint_supported(ctx, why)) return fail(ctx, CUMC_EINVAL, why);
+    tape_release(ctx);
+    cumc_tape *tp = new cumc_tape();
+    ctx->tape = tp;
+    tp->nsteps = nsteps;
+    tp->every = every;
+    tp->nck = nsteps / every + 2;  // step 0, every, 2 every, ... and the final state
+    const size_t ps = ctx->L.ps;
+    const size_t planes = 9 + (size_t)SW * tp->nck + (size_t)SW * (every + 1) + 12 + 2 + NADJ + 24;
+    if (hipMalloc(&tp->mem, planes * ps * sizeof(double)) != hipSuccess) {
+        tape_release(ctx);
+        return fail(ctx, CUMC_ENOMEM, "tape allocation");
+    }
+    double *p = tp->mem;
+    auto take = [&](size_t n) { double *r = p; p += n * ps; return r; };
+    tp->init = take(9);
+    tp->ckpt = take((size_t)SW * tp->nck);
+    tp->seg = take((size_t)SW * (every + 1));
+    tp->stA = take(12);
+    tp->stB = take(2);
+    double *adj = take(NADJ);
+    for (int r = 0; r < 3; r++) tp->lu[r] = take(8);
+    AdjF &A = tp->A;
+    double *q = adj;
+    auto sub = [&](int n) { double *r = q; q += (size_t)n * ps; return r; };
+    A.P = sub(8); A.BC = sub(3); A.EPS = sub(1);
+    for (int d = 0; d < 3; d++) A.F[d] = sub(7);
+    A.E = sub(3); A.T = sub(11); A.H = sub(14);
+    const size_t pb = plane_bytes(ctx);
+    // the uploaded state, then initial_update and the steps
+    HIPCHK(ctx, hipMemcpyAsync(tp->init, ctx->F64.P, 8 * pb, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(tp->init + 8 * ps, ctx->F64.EPS, pb, hipMemcpyDeviceToDevice, ctx->stream));
+    int r = cumc_initial_update(ctx);
+    if (r) return r;
+    tp->dts.assign(nsteps, 0.0);
+    save_state(ctx, tp->ckpt);
+    for (int m = 0; m < nsteps; m++) {
+        double dt;
+        if (dt_in) {
+            dt = dt_in[m];
+            if (!std::isfinite(dt) || !(dt > 0.0)) return fail(ctx, CUMC_ENONFINITE, "frozen dt must be finite and positive");
+        } else if ((r = cumc_finddt(ctx, &dt))) {
+            return r;
+        }
+        tp->dts[m] = dt;
+        forward_step(ctx, dt);
+        if ((m + 1) % every == 0) save_state(ctx, tp->ckpt + (size_t)SW * ps * ((m + 1) / every));
+    }
+    save_state(ctx, tp->ckpt + (size_t)SW * ps * (tp->nck - 1));
+    ctx->dt_valid = false;
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (dt_out) std::memcpy(dt_out, tp->dts.data(), sizeof(double) * nsteps);
+    return CUMC_OK;
+}
+
+static int aos_into(cumc_ctx *ctx, const double *host, int nv, double *dst)
+{
+    const Lay &L = ctx->L;
+    const long ncell = (long)(L.nx + 6) * (L.ny + 6) * (L.nz + 6);
+    double *tmp = nullptr;
+    HIPCHK(ctx, hipMalloc(&tmp, (size_t)ncell * nv * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(tmp, host, (size_t)ncell * nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_aos_to_soa, dim3(nblk(ncell, 256)), dim3(256), 0, ctx->stream, L, tmp, nv, dst);
+        e = hipStreamSynchronize(ctx->stream);
+    }
+    hipFree(tmp);
+    HIPCHK(ctx, e);
+    return CUMC_OK;
+}
+
+static int soa_out(cumc_ctx *ctx, const double *src, int nv, double *host)
+{
+    const Lay &L = ctx->L;
+    const long ncell = (long)(L.nx + 6) * (L.ny + 6) * (L.nz + 6);
+    double *tmp = nullptr;
+    HIPCHK(ctx, hipMalloc(&tmp, (size_t)ncell * nv * sizeof(double)));
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(ncell, 256)), dim3(256), 0, ctx->stream, L, src, nv, tmp);
+    hipError_t e = hipMemcpyAsync(host, tmp, (size_t)ncell * nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(tmp);
+    HIPCHK(ctx, e);
+    return CUMC_OK;
+}
+
+extern "C" int cumc_tape_backward(cumc_ctx *ctx, const double *seed_prim, const double *seed_eps,
+                                  const double *seed_bcell, double *grad_prim, double *grad_eps)
+{
+    if (!ctx) return CUMC_EINVAL;
+    cumc_tape *tp = ctx->tape;
+    if (!tp) return fail(ctx, CUMC_ESTATE, "cumc_tape_run must precede cumc_tape_backward");
+    const Lay &L = ctx->L;
+    const size_t ps = L.ps, pb = plane_bytes(ctx);
+    AdjF &A = tp->A;
+    HIPCHK(ctx, hipMemsetAsync(A.P, 0, (size_t)NADJ * pb, ctx->stream));
+    for (int r = 0; r < 3; r++) HIPCHK(ctx, hipMemsetAsync(tp->lu[r], 0, 8 * pb, ctx->stream));
+    int r;
+    if (seed_prim && (r = aos_into(ctx, seed_prim, 8, A.P))) return r;
+    if (seed_eps && (r = aos_into(ctx, seed_eps, 1, A.EPS))) return r;
+    if (seed_bcell && (r = aos_into(ctx, seed_bcell, 3, A.BC))) return r;
+    const int N = tp->nsteps, k = tp->every;
+    for (int s0 = ((N - 1) / k) * k; N > 0 && s0 >= 0; s0 -= k) {
+        const int s1 = s0 + k < N ? s0 + k : N;
+        load_state(ctx, tp->ckpt + (size_t)SW * ps * (s0 / k));
+        save_state(ctx, tp->seg);
+        for (int m = s0; m < s1; m++) {
+            forward_step(ctx, tp->dts[m]);
+            save_state(ctx, tp->seg + (size_t)SW * ps * (m - s0 + 1));
+        }
+        for (int m = s1 - 1; m >= s0; m--) reverse_step(ctx, m - s0, tp->dts[m]);
+        HIPCHK(ctx, hipGetLastError());
+    }
+    // initial_update: its ghost fill, then k_init on the uploaded state (zero ghost bcell)
+    const double *S0 = tp->ckpt;
+    reverse_ghost(ctx, A, S0, S0 + 8 * ps);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->F64.P, tp->init, 8 * pb, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->F64.EPS, tp->init + 8 * ps, pb, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->F64.BC, 0, 3 * pb, ctx->stream));
+    dim3 g0(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
+    hipLaunchKernelGGL(kr_init_a, g0, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, A, tp->lu[1]);
+    hipLaunchKernelGGL(kr_init_b, g0, dim3(256), 0, ctx->stream, L, A);
+    HIPCHK(ctx, hipGetLastError());
+    if (grad_prim && (r = soa_out(ctx, A.P, 8, grad_prim))) return r;
+    if (grad_eps && (r = soa_out(ctx, A.EPS, 1, grad_eps))) return r;
+    // leave the context at the taped forward result
+    load_state(ctx, tp->ckpt + (size_t)SW * ps * (tp->nck - 1));
+    ctx->dt_valid = false;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CUMC_OK;
+}
+
+extern "C" int cumc_tape_free(cumc_ctx *ctx)
+{
+    if (!ctx) return CUMC_EINVAL;
+    tape_release(ctx);
+    return CUMC_OK;
+}

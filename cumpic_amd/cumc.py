@@ -87,6 +87,9 @@ def lib():
             "cumc_batch_get_tuning": [_CTX, T],
             "cumc_batch_set_tuning": [_CTX, T],
             "cumc_batch_last_ms": [_CTX, _D],
+            "cumc_tape_run": [_CTX, ctypes.c_int, ctypes.c_int, _D, _D],
+            "cumc_tape_backward": [_CTX, _D, _D, _D, _D, _D],
+            "cumc_tape_free": [_CTX],
         })
         for name, args in sig.items():
             f = getattr(L, name)
@@ -231,6 +234,44 @@ class Solver:
         t = Tuning()
         self._check(lib().cumc_autotune(self.h, ctypes.byref(t)), "cumc_autotune")
         return t.as_dict()
+
+    def tape_run(self, nsteps: int, every: int = 8, dts=None):
+        """initial_update + nsteps steps from the uploaded state, checkpointed every `every` steps
+        for tape_backward.  dts freezes the time steps.  Returns the steps used."""
+        out = np.zeros(max(1, nsteps))
+        din = None
+        if dts is not None:
+            din = np.ascontiguousarray(dts, dtype=np.float64)
+            if din.shape != (nsteps,):
+                raise ValueError(f"dts must have shape ({nsteps},)")
+        self._check(lib().cumc_tape_run(self.h, int(nsteps), int(every), _p(din) if din is not None else None,
+                                        _p(out)), "cumc_tape_run")
+        return out[:nsteps].copy()
+
+    def tape_backward(self, seed_prim=None, seed_eps=None, seed_bcell=None):
+        """Gradients of <seed, final state> with respect to the uploaded prim and eps."""
+        shp = self.cfg.shape
+
+        def arr(a, nv):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            want = shp + ((nv,) if nv > 1 else ())
+            if a.shape != want:
+                raise ValueError(f"seed shape {a.shape} != {want}")
+            return a
+
+        sp, se, sb = arr(seed_prim, 8), arr(seed_eps, 1), arr(seed_bcell, 3)
+        gp = np.zeros(shp + (8,))
+        ge = np.zeros(shp)
+        self._check(lib().cumc_tape_backward(self.h, _p(sp) if sp is not None else None,
+                                             _p(se) if se is not None else None,
+                                             _p(sb) if sb is not None else None, _p(gp), _p(ge)),
+                    "cumc_tape_backward")
+        return gp, ge
+
+    def tape_free(self):
+        self._check(lib().cumc_tape_free(self.h), "cumc_tape_free")
 
     def tuning(self) -> dict:
         t = Tuning()

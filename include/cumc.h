@@ -157,6 +157,27 @@ int cumc_batch_set_tuning(cumc_batch *b, const cumc_tuning *t);
 int cumc_batch_last_ms(cumc_batch *b, double *ms);
 const char *cumc_batch_last_error(cumc_batch *b);
 
+/* ------------------------------------------------------------------------------------------
+ * Reverse mode.  No reference counterpart: the reference has no derivatives.  The taped map is
+ * initial_update (solver/initial.f90:83-140) followed by nsteps RUNGEKUTTA steps
+ * (solver/rungekutta.f90:12-148) with the time steps frozen; limiter, Riemann-region and floor
+ * branches are replayed from the forward values.  FP64, Cartesian, riemann model hooks.
+ * ------------------------------------------------------------------------------------------ */
+
+/* From the state as uploaded (not yet initial_update'd): run initial_update and nsteps steps,
+ * storing the state every `every` steps on the device.  dt_in (nsteps) freezes the steps; NULL
+ * takes each step's finddt.  dt_out (may be NULL) receives the steps.  Leaves the context at the
+ * final state. */
+int cumc_tape_run(cumc_ctx *ctx, int nsteps, int every, const double *dt_in, double *dt_out);
+
+/* Adjoint of the taped map.  Seeds: adjoints of the final prim / eps / bcell (same host layouts
+ * as cumc_download; NULL = zero).  Outputs: gradients with respect to the uploaded prim / eps
+ * (cumc_upload layouts; NULL = not wanted).  Recomputes each `every`-step segment from its
+ * checkpoint; leaves the context at the taped final state. */
+int cumc_tape_backward(cumc_ctx *ctx, const double *seed_prim, const double *seed_eps, const double *seed_bcell,
+                       double *grad_prim, double *grad_eps);
+int cumc_tape_free(cumc_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

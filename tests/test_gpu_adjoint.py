@@ -1,0 +1,171 @@
+"""Reverse mode of initial_update + N RK3 steps against central finite differences.
+
+Objective: J = sum over interior cells of |bcell|^2/2 dV after N steps with the time steps
+frozen.  The gradient with respect to the uploaded velocity field from cumc_tape_backward must
+match (J(v + h d) - J(v - h d)) / 2h along seeded random directions d (entries uniform in
+[-1, 1]) to 1e-6 relative, with J evaluated by the plain forward path on the GPU and, for some
+directions, by the CPU restatement (whose forward is bit-identical, so its differences are the
+oracle's own).
+
+Step size: the limiters, HLL region choice and floors make the step piecewise smooth, and a
+central difference that straddles a switching surface is off by O(1) in that cell's
+contribution; the number of surfaces crossed grows with h.  The difference J(v+) - J(v-) is
+formed as sum (B+ - B-)(B+ + B-)/2 dV, which keeps the rounding floor near 1e-17, so h = 2e-8
+sits between the kink and rounding regimes for these sizes.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cumpic_amd import ics
+from cumpic_amd.config import PPMC, TVDMM
+from cumpic_amd.ics import ix
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _inner(cfg):
+    return (slice(ix(1), ix(cfg.nz) + 1), slice(ix(1), ix(cfg.ny) + 1), slice(ix(1), ix(cfg.nx) + 1))
+
+
+def _dv(p):
+    return (p.xf[4] - p.xf[3]) * (p.yf[4] - p.yf[3]) * (p.zf[4] - p.zf[3])
+
+
+def _B_gpu(s, prob, prim, dts):
+    s.upload(prim, prob.eps)
+    s.initial_update()
+    for dt in dts:
+        s.rungekutta(float(dt))
+    return s.download()["bcell"][_inner(prob.cfg)]
+
+
+def _B_oracle(prob, prim, dts):
+    o = Oracle.from_problem(prob)
+    o.set_state(prim, prob.eps)
+    o.initial_update()
+    for dt in dts:
+        o.rungekutta(float(dt))
+    b = o.get_state()["bcell"][_inner(prob.cfg)]
+    o.close()
+    return b
+
+
+def _dJ(bp, bm, prob):
+    """J(+) - J(-) for J = sum |B|^2/2 dV, formed from the field differences."""
+    return 0.5 * float(np.sum((bp - bm) * (bp + bm))) * _dv(prob)
+
+
+def _gradient(s, prob, nsteps, every, dts=None):
+    s.upload(prob.prim, prob.eps)
+    used = s.tape_run(nsteps, every, dts)
+    st = s.download()
+    seed = np.zeros(prob.cfg.shape + (3,))
+    inner = _inner(prob.cfg)
+    seed[inner] = st["bcell"][inner] * _dv(prob)
+    gp, ge = s.tape_backward(seed_bcell=seed)
+    return gp, ge, used
+
+
+def _fd_check(prob, nsteps, every, ndir, seed, h=2e-8, oracle_dirs=0):
+    from cumpic_amd.cumc import Solver
+    s = Solver.from_problem(prob)
+    gp, ge, dts = _gradient(s, prob, nsteps, every)
+    inner = _inner(prob.cfg)
+    gv = gp[inner + (slice(1, 4),)]
+    assert np.isfinite(gp).all() and np.abs(gv).max() > 0
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for d in range(ndir):
+        dirn = rng.uniform(-1.0, 1.0, size=gv.shape)
+        pp, pm = prob.prim.copy(), prob.prim.copy()
+        pp[inner + (slice(1, 4),)] += h * dirn
+        pm[inner + (slice(1, 4),)] -= h * dirn
+        fd = _dJ(_B_gpu(s, prob, pp, dts), _B_gpu(s, prob, pm, dts), prob) / (2 * h)
+        ad = float(np.sum(gv * dirn))
+        rel = abs(fd - ad) / max(abs(fd), abs(ad))
+        worst = max(worst, rel)
+        print(f"  dir {d}: adjoint {ad:.12e} fd {fd:.12e} rel {rel:.2e}", flush=True)
+        assert rel <= TOL, (d, ad, fd, rel)
+        if d < oracle_dirs:
+            fdo = _dJ(_B_oracle(prob, pp, dts), _B_oracle(prob, pm, dts), prob) / (2 * h)
+            assert fdo == fd  # the CPU restatement's forward is bit-identical
+    s.close()
+    return worst
+
+
+def test_adjoint_fd_2d_periodic_ppm():
+    print("[case] adjoint vs FD, 2D 16^2 periodic PPM, 6 steps, checkpoints every 4", flush=True)
+    prob = ics.randomize(ics.spectral_box(n=16, seed=3000, ndim=2, v_rms=0.3, db_rms=0.3), seed=1, amp=0.05)
+    w = _fd_check(prob, nsteps=6, every=4, ndir=8, seed=31, oracle_dirs=2)
+    print(f"  worst rel {w:.2e}")
+
+
+@pytest.mark.parametrize("recon", [PPMC, TVDMM])
+def test_adjoint_fd_3d(recon):
+    print(f"[case] adjoint vs FD, 3D 8^3 periodic recon={recon}, 4 steps, checkpoints every 3", flush=True)
+    prob = ics.randomize(ics.small_periodic(8, seed=3100), seed=2, amp=0.05)
+    prob.cfg.recon = recon
+    _fd_check(prob, nsteps=4, every=3, ndir=6, seed=32)
+
+
+def test_adjoint_fd_outflow_and_reflecting():
+    print("[case] adjoint vs FD, 3D 8^3 outflow x, periodic y, reflecting z", flush=True)
+    prob = ics.randomize(ics.small_periodic(8, seed=3200), seed=3, amp=0.05)
+    # boundary flags: x outflow, y periodic, z reflecting (parity copies with bfac)
+    prob.cfg.boundary_flag = (1, 1, 0, 0, 2, 2)
+    _fd_check(prob, nsteps=3, every=2, ndir=6, seed=33)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    from cumpic_amd.cumc import Solver
+    print("[case] checkpoint interval invariance", flush=True)
+    prob = ics.randomize(ics.spectral_box(n=12, seed=3300, ndim=2, v_rms=0.3, db_rms=0.3), seed=4, amp=0.05)
+    s = Solver.from_problem(prob)
+    g1, e1, d1 = _gradient(s, prob, 7, 1)
+    g7, e7, d7 = _gradient(s, prob, 7, 7)
+    g3, e3, d3 = _gradient(s, prob, 7, 3, dts=d1)
+    assert np.array_equal(d1, d7) and np.array_equal(d1, d3)
+    assert np.array_equal(g1, g7) and np.array_equal(g1, g3)
+    assert np.array_equal(e1, e3)
+
+
+def test_tape_forward_matches_run_and_restores_state():
+    from cumpic_amd.cumc import Solver
+    print("[case] tape forward == plain run", flush=True)
+    prob = ics.randomize(ics.small_periodic(8, seed=3400), seed=5, amp=0.05)
+    a = Solver.from_problem(prob)
+    b = Solver.from_problem(prob)
+    dts = a.tape_run(5, 2)
+    b.initial_update()
+    db, _ = b.run(5)
+    assert np.array_equal(dts, db)
+    sa, sb = a.download(), b.download()
+    for k in ("prim", "eps", "bcell"):
+        assert np.array_equal(sa[k], sb[k]), k
+    seed = np.ones(prob.cfg.shape + (3,))
+    a.tape_backward(seed_bcell=seed)
+    sa = a.download()
+    for k in ("prim", "eps", "bcell"):
+        assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_autograd_function_matches_tape():
+    import torch
+    from cumpic_amd.autograd import Rollout, initial_velocity, magnetic_energy
+    from cumpic_amd.cumc import Solver
+    print("[case] torch.autograd.Function", flush=True)
+    prob = ics.randomize(ics.spectral_box(n=12, seed=3500, ndim=2, v_rms=0.3, db_rms=0.3), seed=6, amp=0.05)
+    s = Solver.from_problem(prob)
+    gp, _, dts = _gradient(s, prob, 5, 2)
+    v0 = initial_velocity(prob).clone().requires_grad_(True)
+    _, bc = Rollout.apply(v0, s, prob, 5, 2, dts)
+    J = magnetic_energy(bc, prob)
+    J.backward()
+    want = gp[_inner(prob.cfg) + (slice(1, 4),)]
+    # same seed up to how torch rounds d(|B|^2/2 dV)/dB
+    np.testing.assert_allclose(v0.grad.numpy(), want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
