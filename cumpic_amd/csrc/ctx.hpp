@@ -118,6 +118,26 @@ static inline void swap_u(cumc_ctx *ctx)
 // frees the reverse-mode record (defined in cumc_adjoint.hip)
 void tape_release(cumc_ctx *ctx);
 
+// batch internals used by the batched reverse mode (defined in cumc_batch.hip)
+struct batch_tape;
+void batch_tape_release(cumc_batch *b);  // defined in cumc_adjoint.hip
+batch_tape *&batch_adj(cumc_batch *b);
+double &batch_last_ms(cumc_batch *b);
+int batch_nbox(const cumc_batch *b);
+cumc_ctx *batch_box(const cumc_batch *b, int i);
+hipStream_t batch_stream(const cumc_batch *b);
+std::string &batch_err(cumc_batch *b);
+int batch_refresh(cumc_batch *b);
+int batch_reserve_hist(cumc_batch *b, int nsteps);
+double *batch_hist(cumc_batch *b);
+void batch_set_counter(cumc_batch *b, int v);
+void batch_finddt(cumc_batch *b);
+void batch_step(cumc_batch *b, int par, int frozen);
+void batch_stage(cumc_batch *b, int st, int par);
+void batch_fluxes_ecorn(cumc_batch *b);
+void batch_set_dt(cumc_batch *b, int m);
+void batch_after_run(cumc_batch *b, int nsteps);
+
 // ghost passes + torus clamp (defined in cumc.hip)
 void launch_ghost(cumc_ctx *ctx);
 

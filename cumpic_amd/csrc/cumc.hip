@@ -82,6 +82,9 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     L.py = L.px * (c.ny + 2 * NG);
     const long cells = L.py * (c.nz + 2 * NG);
     L.ps = (cells + 63) / 64 * 64;
+    // kernels address a plane with 32-bit byte offsets: fewer than 2^29 cells per plane
+    // (about 800^3 interior cells; a larger grid is sharded into several subdomains)
+    if (L.ps >= (1L << 29)) return CUMC_EINVAL;
     Model &M = ctx->M;
     M.coordinate = c.coordinate;
     M.recon = c.recon;

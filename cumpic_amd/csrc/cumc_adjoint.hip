@@ -51,10 +51,9 @@ template <> struct RKC<3> { static constexpr double a0 = RK30, a1 = RK31, a2 = R
 // whose p adjoint is still live they are the values the pass wrote).
 // ---------------------------------------------------------------------------------------------
 template <int AX>
-__global__ __launch_bounds__(256) void kr_ghost(Lay L, AdjF A, Model M, const double *__restrict__ rho_out,
-                                                const double *__restrict__ eps_out)
+__device__ inline void kr_ghost_line(const Lay &L, const AdjF &A, const Model &M, const double *__restrict__ rho_out,
+                                     const double *__restrict__ eps_out, long t)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int na, nb, a0, b0;
     if (AX == 0) { a0 = 0; na = L.ny + 1; b0 = 0; nb = L.nz + 1; }
     else if (AX == 1) { a0 = -2; na = L.nx + 6; b0 = 0; nb = L.nz + 1; }
@@ -110,6 +109,13 @@ __global__ __launch_bounds__(256) void kr_ghost(Lay L, AdjF A, Model M, const do
     }
 }
 
+template <int AX>
+__global__ __launch_bounds__(256) void kr_ghost(Lay L, AdjF A, Model M, const double *__restrict__ rho_out,
+                                                const double *__restrict__ eps_out)
+{
+    kr_ghost_line<AX>(L, A, M, rho_out, eps_out, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------------------------
 // stage tail reversed, part a: per cell of [0,n]^3 the forward values are recomputed exactly as
 // k_tail does (so the floor branch replays), then the chain
@@ -118,12 +124,10 @@ __global__ __launch_bounds__(256) void kr_ghost(Lay L, AdjF A, Model M, const do
 // because a face field feeds the cell-centred field of two cells.
 // ---------------------------------------------------------------------------------------------
 template <int STAGE>
-__global__ __launch_bounds__(256) void kr_tail_a(Lay L, Fields<double> F, Tabs tb, Model M, double dt,
-                                                 const double *__restrict__ U0, const double *__restrict__ Uin,
-                                                 AdjF A, double *lU0, double *lUin, double *lUout)
+__device__ inline void kr_tail_a_cell(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M, double dt,
+                                      const double *__restrict__ U0, const double *__restrict__ Uin, const AdjF &A,
+                                      double *lU0, double *lUin, double *lUout, long t, int l)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
     const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
     if (k > L.ny || j > L.nx) return;
     const long g = L.id(j, k, l), sj = 1, sk = L.px, sl = L.py, ps = L.ps;
@@ -228,12 +232,19 @@ __global__ __launch_bounds__(256) void kr_tail_a(Lay L, Fields<double> F, Tabs t
     }
 }
 
+template <int STAGE>
+__global__ __launch_bounds__(256) void kr_tail_a(Lay L, Fields<double> F, Tabs tb, Model M, double dt,
+                                                 const double *__restrict__ U0, const double *__restrict__ Uin,
+                                                 AdjF A, double *lU0, double *lUin, double *lUout)
+{
+    kr_tail_a_cell<STAGE>(L, F, tb, M, dt, U0, Uin, A, lU0, lUin, lUout, (long)blockIdx.x * blockDim.x + threadIdx.x,
+                          blockIdx.y);
+}
+
 // part b: face-B adjoints (own cell's share + the upper neighbour's share of its cell field)
 template <int STAGE>
-__global__ __launch_bounds__(256) void kr_tail_b(Lay L, AdjF A, double dt, double *lU0, double *lUin)
+__device__ inline void kr_tail_b_cell(const Lay &L, const AdjF &A, double dt, double *lU0, double *lUin, long t, int l)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
     const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
     if (k > L.ny || j > L.nx) return;
     const long g = L.id(j, k, l), ps = L.ps;
@@ -248,12 +259,16 @@ __global__ __launch_bounds__(256) void kr_tail_b(Lay L, AdjF A, double dt, doubl
     }
 }
 
+template <int STAGE>
+__global__ __launch_bounds__(256) void kr_tail_b(Lay L, AdjF A, double dt, double *lU0, double *lUin)
+{
+    kr_tail_b_cell<STAGE>(L, A, dt, lU0, lUin, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
 // flux and corner-EMF adjoints gathered from the per-cell right-hand-side adjoints, on [0,n+1]^3
 // (hydro flux components; the B-flux components are set by kr_ecorn)
-__global__ __launch_bounds__(256) void kr_gather(Lay L, AdjF A, Tabs tb)
+__device__ inline void kr_gather_cell(const Lay &L, const AdjF &A, const Tabs &tb, long t, int l)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
     const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
     if (k > L.ny + 1 || j > L.nx + 1) return;
     const long g = L.id(j, k, l), ps = L.ps;
@@ -290,12 +305,15 @@ __global__ __launch_bounds__(256) void kr_gather(Lay L, AdjF A, Tabs tb)
     }
 }
 
+__global__ __launch_bounds__(256) void kr_gather(Lay L, AdjF A, Tabs tb)
+{
+    kr_gather_cell(L, A, tb, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
 // flux_ct reversed (forward in ecorn_tile): on [0,n+1]^3 a thread owns the B-flux components of
 // the three faces at its index and the cell EMF of its cell
-__global__ __launch_bounds__(256) void kr_ecorn(Lay L, Fields<double> F, AdjF A)
+__device__ inline void kr_ecorn_cell(const Lay &L, const Fields<double> &F, const AdjF &A, long t, int l)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
     const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
     if (k > L.ny + 1 || j > L.nx + 1) return;
     const long g = L.id(j, k, l), ps = L.ps;
@@ -328,6 +346,11 @@ __global__ __launch_bounds__(256) void kr_ecorn(Lay L, Fields<double> F, AdjF A)
     A.BC[g] += -ly * vz + lz * vy;
     A.BC[g + ps] += lx * vz - lz * vx;
     A.BC[g + 2 * ps] += -lx * vy + ly * vx;
+}
+
+__global__ __launch_bounds__(256) void kr_ecorn(Lay L, Fields<double> F, AdjF A)
+{
+    kr_ecorn_cell(L, F, A, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -423,9 +446,9 @@ __device__ inline void flux_to_rot(const double *o, double *f)
 // HLL reversed at every face of a pencil: writes the left / right state adjoints (q order) to
 // H and adds the normal-field adjoint to the face-B plane
 template <int DIR, int RECON>
-__global__ __launch_bounds__(64) void kr_flux_hll(Lay L, Fields<double> F, Tabs tb, Model M, AdjF A)
+__device__ inline void kr_flux_hll_pencil(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M,
+                                          const AdjF &A, long t)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npencils<DIR>(L)) return;
     Pencil<DIR> pc;
     if (!pc.init(L, t)) return;
@@ -467,17 +490,23 @@ __global__ __launch_bounds__(64) void kr_flux_hll(Lay L, Fields<double> F, Tabs 
     }
 }
 
+template <int DIR, int RECON>
+__global__ __launch_bounds__(64) void kr_flux_hll(Lay L, Fields<double> F, Tabs tb, Model M, AdjF A)
+{
+    kr_flux_hll_pencil<DIR, RECON>(L, F, tb, M, A, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // reconstruction reversed along a pencil for one variable (blockIdx.y): a sliding window of
 // values, slopes and adjoint accumulators; each cell's adjoint is emitted once complete
 template <int DIR, int RECON>
-__global__ __launch_bounds__(64) void kr_flux_recon(Lay L, Fields<double> F, Tabs tb, AdjF A)
+__device__ inline void kr_flux_recon_pencil(const Lay &L, const Fields<double> &F, const Tabs &tb, const AdjF &A,
+                                            long t, int v)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npencils<DIR>(L)) return;
     Pencil<DIR> pc;
     if (!pc.init(L, t)) return;
     const long ps = L.ps;
-    const int ax = DIR - 1, v = blockIdx.y, n = pc.n;
+    const int ax = DIR - 1, n = pc.n;
     const double *W = qplane<DIR>(F, ps, v);
     double *LW = qplane_adj<DIR>(A, ps, v);
     const double *HL = A.H + v * ps, *HR = A.H + (7 + v) * ps;
@@ -532,14 +561,19 @@ __global__ __launch_bounds__(64) void kr_flux_recon(Lay L, Fields<double> F, Tab
     }
 }
 
+template <int DIR, int RECON>
+__global__ __launch_bounds__(64) void kr_flux_recon(Lay L, Fields<double> F, Tabs tb, AdjF A)
+{
+    kr_flux_recon_pencil<DIR, RECON>(L, F, tb, A, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------------------------
 // initial_update reversed (k_init: CHECKRHO, FROMRVETOU, EOS); the ghost fill that follows it is
 // reversed by kr_ghost first.  Forward inputs: the uploaded prim / eps (and the zero ghost bcell).
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void kr_init_a(Lay L, Fields<double> F, Model M, AdjF A, double *lU)
+__device__ inline void kr_init_a_cell(const Lay &L, const Fields<double> &F, const Model &M, const AdjF &A, double *lU,
+                                      long t, int l)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
     const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
     if (k > L.ny || j > L.nx) return;
     const long g = L.id(j, k, l), sj = 1, sk = L.px, sl = L.py, ps = L.ps;
@@ -583,10 +617,13 @@ __global__ __launch_bounds__(256) void kr_init_a(Lay L, Fields<double> F, Model 
     }
 }
 
-__global__ __launch_bounds__(256) void kr_init_b(Lay L, AdjF A)
+__global__ __launch_bounds__(256) void kr_init_a(Lay L, Fields<double> F, Model M, AdjF A, double *lU)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
+    kr_init_a_cell(L, F, M, A, lU, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+__device__ inline void kr_init_b_cell(const Lay &L, const AdjF &A, long t, int l)
+{
     const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
     if (k > L.ny || j > L.nx) return;
     const long g = L.id(j, k, l), ps = L.ps;
@@ -597,6 +634,11 @@ __global__ __launch_bounds__(256) void kr_init_b(Lay L, AdjF A)
                        ((k + 1 <= L.ny && j >= 1 && l >= 1) ? A.T[g + L.px + 9 * ps] : 0.0);
     A.P[g + 7 * ps] += A.T[g + 7 * ps] + (in ? A.T[g + 10 * ps] : 0.0) +
                        ((l + 1 <= L.nz && j >= 1 && k >= 1) ? A.T[g + L.py + 10 * ps] : 0.0);
+}
+
+__global__ __launch_bounds__(256) void kr_init_b(Lay L, AdjF A)
+{
+    kr_init_b_cell(L, A, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 }  // namespace
@@ -747,25 +789,10 @@ static void reverse_step(cumc_ctx *ctx, int i, double dt)
     l1 = tmp;
 }
 
-static bool adjoint_supported(const cumc_ctx *ctx, std::string &why)
+// one allocation per context: uploaded state, checkpoints, segment states, stage saves,
+// adjoint arrays and the three conserved-state adjoints
+static int tape_alloc(cumc_ctx *ctx, int nsteps, int every)
 {
-    const cumc_config &c = ctx->cfg;
-    if (c.precision != CUMC_FP64) { why = "reverse mode needs the FP64 form"; return false; }
-    if (c.coordinate != CUMC_CARTESIAN) { why = "reverse mode covers Cartesian grids"; return false; }
-    if (c.model != CUMC_MODEL_RIEMANN) { why = "reverse mode covers the riemann model hooks"; return false; }
-    if (c.boundary_flag[2] != 0 && c.boundary_flag[3] == 0) {
-        why = "y-outer periodic with y-inner non-periodic leaves ghost eps stale (BOUNDARY1D); not differentiated";
-        return false;
-    }
-    return true;
-}
-
-extern "C" int cumc_tape_run(cumc_ctx *ctx, int nsteps, int every, const double *dt_in, double *dt_out)
-{
-    if (!ctx || nsteps < 0 || every < 1) return CUMC_EINVAL;
-    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
-    std::string why;
-    if (!adjoint_supported(ctx, why)) return fail(ctx, CUMC_EINVAL, why);
     tape_release(ctx);
     cumc_tape *tp = new cumc_tape();
     ctx->tape = tp;
@@ -785,20 +812,45 @@ extern "C" int cumc_tape_run(cumc_ctx *ctx, int nsteps, int every, const double 
     tp->seg = take((size_t)SW * (every + 1));
     tp->stA = take(12);
     tp->stB = take(2);
-    double *adj = take(NADJ);
+    tp->adj = take(NADJ);
     for (int r = 0; r < 3; r++) tp->lu[r] = take(8);
     AdjF &A = tp->A;
-    double *q = adj;
+    double *q = tp->adj;
     auto sub = [&](int n) { double *r = q; q += (size_t)n * ps; return r; };
     A.P = sub(8); A.BC = sub(3); A.EPS = sub(1);
     for (int d = 0; d < 3; d++) A.F[d] = sub(7);
     A.E = sub(3); A.T = sub(11); A.H = sub(14);
+    return CUMC_OK;
+}
+
+static bool adjoint_supported(const cumc_ctx *ctx, std::string &why)
+{
+    const cumc_config &c = ctx->cfg;
+    if (c.precision != CUMC_FP64) { why = "reverse mode needs the FP64 form"; return false; }
+    if (c.coordinate != CUMC_CARTESIAN) { why = "reverse mode covers Cartesian grids"; return false; }
+    if (c.model != CUMC_MODEL_RIEMANN) { why = "reverse mode covers the riemann model hooks"; return false; }
+    if (c.boundary_flag[2] != 0 && c.boundary_flag[3] == 0) {
+        why = "y-outer periodic with y-inner non-periodic leaves ghost eps stale (BOUNDARY1D); not differentiated";
+        return false;
+    }
+    return true;
+}
+
+extern "C" int cumc_tape_run(cumc_ctx *ctx, int nsteps, int every, const double *dt_in, double *dt_out)
+{
+    if (!ctx || nsteps < 0 || every < 1) return CUMC_EINVAL;
+    if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
+    std::string why;
+    if (!adjoint_supported(ctx, why)) return fail(ctx, CUMC_EINVAL, why);
+    int r = tape_alloc(ctx, nsteps, every);
+    if (r) return r;
+    cumc_tape *tp = ctx->tape;
+    const size_t ps = ctx->L.ps;
     const size_t pb = plane_bytes(ctx);
     // the uploaded state, then initial_update and the steps
     HIPCHK(ctx, hipMemcpyAsync(tp->init, ctx->F64.P, 8 * pb, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(tp->init + 8 * ps, ctx->F64.EPS, pb, hipMemcpyDeviceToDevice, ctx->stream));
-    int r = cumc_initial_update(ctx);
-    if (r) return r;
+    if ((r = cumc_initial_update(ctx))) return r;
     tp->dts.assign(nsteps, 0.0);
     save_state(ctx, tp->ckpt);
     for (int m = 0; m < nsteps; m++) {
@@ -901,5 +953,482 @@ extern "C" int cumc_tape_free(cumc_ctx *ctx)
 {
     if (!ctx) return CUMC_EINVAL;
     tape_release(ctx);
+    return CUMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched reverse mode: the boxes of a cumc_batch taped and reversed together.  Every reverse
+// phase is one launch over a work list of (box, tile) items of all boxes, with the same bodies
+// as the single-box kernels (so a batched gradient equals the box's own gradient bit for bit);
+// the forward recomputation uses the batch's stage launches with the recorded per-box steps.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct AdjBox {
+    Lay L;
+    Fields<double> F;  // forward fields; U0 / U1 as of the last descriptor upload
+    Tabs tb;
+    Model M;
+    double *pool, *Ub, *dt;
+    AdjF A;
+    double *lu[3];
+    double *seg, *stA, *stB, *ckpt, *init;
+    int gx_c, gx_e;  // 256-cell tiles per plane row set: [0,n]^2 and [0,n+1]^2 column domains
+    int gpen[3];     // 64-pencil tiles per sweep
+};
+
+enum { Q_TAIL, Q_GE, Q_GX, Q_GY, Q_GZ, Q_H1, Q_H2, Q_H3, Q_R1, Q_R2, Q_R3, Q_CP, Q_N };
+constexpr int CP_ELEMS = 1024;  // doubles per copy tile (256 threads x 4)
+
+__device__ inline int adj_box_of(const int *pre, int nbox, int it)
+{
+    int lo = 0, hi = nbox;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= it) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// forward values after a stage, selected per box: 0 segment state `slot`, 1 stage-2 save,
+// 2 stage-1 save, 3 the state after initial_update
+__device__ inline void out_fields(const AdjBox &d, int sel, int slot, const double *&rho, const double *&eps)
+{
+    const long ps = d.L.ps;
+    if (sel == 0) { rho = d.seg + (size_t)slot * SW * ps; eps = rho + 8 * ps; }
+    else if (sel == 1) { rho = d.stB; eps = d.stB + ps; }
+    else if (sel == 2) { rho = d.stA; eps = d.stA + 8 * ps; }
+    else { rho = d.ckpt; eps = d.ckpt + 8 * ps; }
+}
+
+#define FOR_ITEMS(Q)                                                                            \
+    const int *pq = pre + (size_t)(Q) * (nbox + 1);                                            \
+    const int total = pq[nbox];                                                                 \
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {                                    \
+        const int b = adj_box_of(pq, nbox, it);                                                 \
+        const int loc = it - pq[b];                                                             \
+        const AdjBox &d = D[b];
+
+template <int AX>
+__global__ __launch_bounds__(256) void kbr_ghost(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                                 int sel, int slot)
+{
+    FOR_ITEMS(Q_GX + AX)
+        const double *rho, *eps;
+        out_fields(d, sel, slot, rho, eps);
+        kr_ghost_line<AX>(d.L, d.A, d.M, rho, eps, (long)loc * 256 + threadIdx.x);
+    }
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void kbr_tail_a(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                                  int q)
+{
+    FOR_ITEMS(Q_TAIL)
+        double *l0 = d.lu[q], *l1 = d.lu[1 - q], *lb = d.lu[2];
+        const double *Uin = STAGE == 3 ? d.Ub : (STAGE == 2 ? d.F.U1 : nullptr);
+        double *lUin = STAGE == 3 ? lb : (STAGE == 2 ? l1 : nullptr);
+        double *lUout = STAGE == 2 ? lb : l1;
+        kr_tail_a_cell<STAGE>(d.L, d.F, d.tb, d.M, *d.dt, d.F.U0, Uin, d.A, l0, lUin, lUout,
+                              (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c);
+    }
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void kbr_tail_b(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                                  int q)
+{
+    FOR_ITEMS(Q_TAIL)
+        double *l0 = d.lu[q], *l1 = d.lu[1 - q], *lb = d.lu[2];
+        double *lUin = STAGE == 3 ? lb : (STAGE == 2 ? l1 : nullptr);
+        kr_tail_b_cell<STAGE>(d.L, d.A, *d.dt, l0, lUin, (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c);
+    }
+}
+
+__global__ __launch_bounds__(256) void kbr_gather(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox)
+{
+    FOR_ITEMS(Q_GE)
+        kr_gather_cell(d.L, d.A, d.tb, (long)(loc % d.gx_e) * 256 + threadIdx.x, loc / d.gx_e);
+    }
+}
+
+__global__ __launch_bounds__(256) void kbr_ecorn(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox)
+{
+    FOR_ITEMS(Q_GE)
+        kr_ecorn_cell(d.L, d.F, d.A, (long)(loc % d.gx_e) * 256 + threadIdx.x, loc / d.gx_e);
+    }
+}
+
+template <int DIR, int RECON>
+__global__ __launch_bounds__(64) void kbr_flux_hll(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox)
+{
+    FOR_ITEMS(Q_H1 + DIR - 1)
+        kr_flux_hll_pencil<DIR, RECON>(d.L, d.F, d.tb, d.M, d.A, (long)loc * 64 + threadIdx.x);
+    }
+}
+
+template <int DIR, int RECON>
+__global__ __launch_bounds__(64) void kbr_flux_recon(const AdjBox *__restrict__ D, const int *__restrict__ pre,
+                                                     int nbox)
+{
+    FOR_ITEMS(Q_R1 + DIR - 1)
+        const int gp = d.gpen[DIR - 1];
+        kr_flux_recon_pencil<DIR, RECON>(d.L, d.F, d.tb, d.A, (long)(loc % gp) * 64 + threadIdx.x, loc / gp);
+    }
+}
+
+__global__ __launch_bounds__(256) void kbr_init_a(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                                  int q)
+{
+    FOR_ITEMS(Q_TAIL)
+        kr_init_a_cell(d.L, d.F, d.M, d.A, d.lu[1 - q], (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c);
+    }
+}
+
+__global__ __launch_bounds__(256) void kbr_init_b(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox)
+{
+    FOR_ITEMS(Q_TAIL)
+        kr_init_b_cell(d.L, d.A, (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c);
+    }
+}
+
+// state copies between the context fields and the per-box tape (element e of 20 planes: the
+// first 12 are P, EPS, bcell of the pool, the last 8 the conserved state selected by par)
+enum { CP_SAVE_SEG, CP_LOAD_SEG, CP_LOAD12_SEG, CP_LOAD_CK, CP_SAVE_CK, CP_SAVE_A, CP_LOAD_A, CP_SAVE_B, CP_LOAD_INIT,
+       CP_SAVE_INIT };
+
+__global__ __launch_bounds__(256) void kbr_copy(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                                int mode, int slot, int par)
+{
+    FOR_ITEMS(Q_CP)
+        const long ps = d.L.ps;
+        double *U = par ? d.F.U1 : d.F.U0;
+        double *pool = d.pool;
+        for (int r = 0; r < CP_ELEMS / 256; r++) {
+            const long e = (long)loc * CP_ELEMS + r * 256 + threadIdx.x;
+            if (e >= SW * ps) break;
+            const bool lo = e < 12 * ps;
+            double *ctxp = lo ? pool + e : U + (e - 12 * ps);
+            switch (mode) {
+            case CP_SAVE_SEG: d.seg[(size_t)slot * SW * ps + e] = *ctxp; break;
+            case CP_LOAD_SEG: *ctxp = d.seg[(size_t)slot * SW * ps + e]; break;
+            case CP_LOAD12_SEG: if (lo) *ctxp = d.seg[(size_t)slot * SW * ps + e]; break;
+            case CP_LOAD_CK: *ctxp = d.ckpt[(size_t)slot * SW * ps + e]; break;
+            case CP_SAVE_CK: d.ckpt[(size_t)slot * SW * ps + e] = *ctxp; break;
+            case CP_SAVE_A: if (lo) d.stA[e] = *ctxp; break;
+            case CP_LOAD_A: if (lo) *ctxp = d.stA[e]; break;
+            case CP_SAVE_B:
+                if (e < ps) d.stB[e] = pool[e];
+                else if (e < 2 * ps) d.stB[e] = pool[8 * ps + (e - ps)];
+                break;
+            case CP_LOAD_INIT:
+                if (e < 9 * ps) pool[e] = d.init[e];
+                else if (lo) pool[e] = 0.0;
+                break;
+            case CP_SAVE_INIT: if (e < 9 * ps) d.init[e] = pool[e]; break;
+            }
+        }
+    }
+}
+
+#undef FOR_ITEMS
+
+}  // namespace
+
+struct batch_tape {
+    int nsteps = 0, every = 1, nck = 0, recon = 4;
+    std::vector<AdjBox> h;
+    AdjBox *d = nullptr;
+    int *pre = nullptr;
+    int total[Q_N] = {0};
+    std::vector<double> dts;  // nsteps x nbox
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+void batch_tape_release(cumc_batch *b)
+{
+    batch_tape *&bt = batch_adj(b);
+    if (!bt) return;
+    hipFree(bt->d);
+    hipFree(bt->pre);
+    if (bt->e0) hipEventDestroy(bt->e0);
+    if (bt->e1) hipEventDestroy(bt->e1);
+    delete bt;
+    bt = nullptr;
+}
+
+// descriptors from the boxes' current buffers
+static int bt_upload(cumc_batch *b, batch_tape *bt)
+{
+    const int nb = batch_nbox(b);
+    hipStream_t s = batch_stream(b);
+    std::vector<int> pre((size_t)Q_N * (nb + 1), 0);
+    for (int i = 0; i < nb; i++) {
+        cumc_ctx *c = batch_box(b, i);
+        cumc_tape *tp = c->tape;
+        AdjBox &a = bt->h[i];
+        const Lay &L = c->L;
+        a.L = L;
+        a.F = c->F64;
+        a.tb = tabs_of(c);
+        a.M = c->M;
+        a.pool = c->pool;
+        a.Ub = c->Ub;
+        a.dt = c->dt_dev;
+        a.A = tp->A;
+        for (int r = 0; r < 3; r++) a.lu[r] = tp->lu[r];
+        a.seg = tp->seg; a.stA = tp->stA; a.stB = tp->stB; a.ckpt = tp->ckpt; a.init = tp->init;
+        a.gx_c = (int)nblk(L.px * (L.ny + 1), 256);
+        a.gx_e = (int)nblk(L.px * (L.ny + 2), 256);
+        a.gpen[0] = (int)nblk(npencils<1>(L), 64);
+        a.gpen[1] = (int)nblk(npencils<2>(L), 64);
+        a.gpen[2] = (int)nblk(npencils<3>(L), 64);
+        long cnt[Q_N];
+        cnt[Q_TAIL] = (long)a.gx_c * (L.nz + 1);
+        cnt[Q_GE] = (long)a.gx_e * (L.nz + 2);
+        cnt[Q_GX] = nblk((long)(L.ny + 1) * (L.nz + 1), 256);
+        cnt[Q_GY] = nblk((long)(L.nx + 6) * (L.nz + 1), 256);
+        cnt[Q_GZ] = nblk((long)(L.nx + 6) * (L.ny + 6), 256);
+        for (int dd = 0; dd < 3; dd++) {
+            cnt[Q_H1 + dd] = a.gpen[dd];
+            cnt[Q_R1 + dd] = 7L * a.gpen[dd];
+        }
+        cnt[Q_CP] = (long)((SW * L.ps + CP_ELEMS - 1) / CP_ELEMS);
+        for (int q = 0; q < Q_N; q++) {
+            const long v = (long)pre[(size_t)q * (nb + 1) + i] + cnt[q];
+            if (v > 0x7fffffffL) return fail(c, CUMC_EINVAL, "batched reverse work list exceeds 2^31 tiles");
+            pre[(size_t)q * (nb + 1) + i + 1] = (int)v;
+        }
+    }
+    for (int q = 0; q < Q_N; q++) bt->total[q] = pre[(size_t)q * (nb + 1) + nb];
+    if (hipMemcpyAsync(bt->d, bt->h.data(), sizeof(AdjBox) * nb, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(bt->pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        batch_err(b) = "batched reverse descriptors upload failed";
+        return CUMC_EHIP;
+    }
+    return CUMC_OK;
+}
+
+static inline const int *bt_pre(const batch_tape *bt, int nb) { return bt->pre; }
+static inline dim3 bt_grid(const batch_tape *bt, int q) { return dim3(bt->total[q] > 0 ? bt->total[q] : 1); }
+
+static void bt_copy(cumc_batch *b, batch_tape *bt, int mode, int slot, int par)
+{
+    const int nb = batch_nbox(b);
+    hipLaunchKernelGGL(kbr_copy, bt_grid(bt, Q_CP), dim3(256), 0, batch_stream(b), bt->d, bt->pre, nb, mode, slot, par);
+}
+
+template <int DIR>
+static void bt_sweep(cumc_batch *b, batch_tape *bt)
+{
+    const int nb = batch_nbox(b);
+    hipStream_t s = batch_stream(b);
+    if (bt->recon == CUMC_PPMC) {
+        hipLaunchKernelGGL((kbr_flux_hll<DIR, 4>), bt_grid(bt, Q_H1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+        hipLaunchKernelGGL((kbr_flux_recon<DIR, 4>), bt_grid(bt, Q_R1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+    } else {
+        hipLaunchKernelGGL((kbr_flux_hll<DIR, 1>), bt_grid(bt, Q_H1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+        hipLaunchKernelGGL((kbr_flux_recon<DIR, 1>), bt_grid(bt, Q_R1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+    }
+}
+
+static void bt_ghost(cumc_batch *b, batch_tape *bt, int sel, int slot)
+{
+    const int nb = batch_nbox(b);
+    hipStream_t s = batch_stream(b);
+    hipLaunchKernelGGL(kbr_ghost<2>, bt_grid(bt, Q_GZ), dim3(256), 0, s, bt->d, bt->pre, nb, sel, slot);
+    hipLaunchKernelGGL(kbr_ghost<1>, bt_grid(bt, Q_GY), dim3(256), 0, s, bt->d, bt->pre, nb, sel, slot);
+    hipLaunchKernelGGL(kbr_ghost<0>, bt_grid(bt, Q_GX), dim3(256), 0, s, bt->d, bt->pre, nb, sel, slot);
+}
+
+template <int STAGE>
+static void bt_reverse_stage(cumc_batch *b, batch_tape *bt, int sel, int slot, int q)
+{
+    const int nb = batch_nbox(b);
+    hipStream_t s = batch_stream(b);
+    bt_ghost(b, bt, sel, slot);
+    hipLaunchKernelGGL((kbr_tail_a<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q);
+    hipLaunchKernelGGL((kbr_tail_b<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q);
+    hipLaunchKernelGGL(kbr_gather, bt_grid(bt, Q_GE), dim3(256), 0, s, bt->d, bt->pre, nb);
+    hipLaunchKernelGGL(kbr_ecorn, bt_grid(bt, Q_GE), dim3(256), 0, s, bt->d, bt->pre, nb);
+    bt_sweep<3>(b, bt);
+    bt_sweep<2>(b, bt);
+    bt_sweep<1>(b, bt);
+}
+
+#define BTCHK(b, call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            batch_err(b) = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+            return CUMC_EHIP;                                                                   \
+        }                                                                                       \
+    } while (0)
+
+extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const double *dt_in, double *dt_out)
+{
+    if (!b || nsteps < 0 || every < 1) return CUMC_EINVAL;
+    const int nb = batch_nbox(b);
+    for (int i = 0; i < nb; i++) {
+        std::string why;
+        if (!adjoint_supported(batch_box(b, i), why)) {
+            batch_err(b) = "box " + std::to_string(i) + ": " + why;
+            return CUMC_EINVAL;
+        }
+    }
+    if (dt_in)
+        for (long i = 0; i < (long)nsteps * nb; i++)
+            if (!std::isfinite(dt_in[i]) || !(dt_in[i] > 0.0)) {
+                batch_err(b) = "frozen dt must be finite and positive";
+                return CUMC_ENONFINITE;
+            }
+    for (int i = 0; i < nb; i++) BTCHK(b, hipStreamSynchronize(batch_box(b, i)->stream));
+    batch_tape_release(b);
+    for (int i = 0; i < nb; i++) {
+        int r = tape_alloc(batch_box(b, i), nsteps, every);
+        if (r) {
+            batch_err(b) = "box " + std::to_string(i) + ": " + batch_box(b, i)->err;
+            return r;
+        }
+    }
+    batch_tape *bt = new batch_tape();
+    batch_adj(b) = bt;
+    bt->nsteps = nsteps;
+    bt->every = every;
+    bt->nck = nsteps / every + 2;
+    bt->recon = batch_box(b, 0)->cfg.recon;
+    bt->h.assign(nb, AdjBox{});
+    BTCHK(b, hipMalloc(&bt->d, sizeof(AdjBox) * nb));
+    BTCHK(b, hipMalloc(&bt->pre, sizeof(int) * Q_N * (nb + 1)));
+    BTCHK(b, hipEventCreate(&bt->e0));
+    BTCHK(b, hipEventCreate(&bt->e1));
+    int r = batch_refresh(b);
+    if (r) return r;
+    if ((r = bt_upload(b, bt))) return r;
+    hipStream_t s = batch_stream(b);
+    bt_copy(b, bt, CP_SAVE_INIT, 0, 0);
+    BTCHK(b, hipStreamSynchronize(s));
+    for (int i = 0; i < nb; i++)
+        if ((r = cumc_initial_update(batch_box(b, i)))) {
+            batch_err(b) = "box " + std::to_string(i) + ": " + batch_box(b, i)->err;
+            return r;
+        }
+    bt_copy(b, bt, CP_SAVE_CK, 0, 0);
+    if ((r = batch_reserve_hist(b, nsteps > 0 ? nsteps : 1))) return r;
+    if (dt_in) BTCHK(b, hipMemcpyAsync(batch_hist(b), dt_in, sizeof(double) * (size_t)nsteps * nb, hipMemcpyHostToDevice, s));
+    batch_set_counter(b, 0);
+    if (!dt_in) batch_finddt(b);
+    for (int m = 0; m < nsteps; m++) {
+        batch_step(b, m & 1, dt_in != nullptr);
+        if ((m + 1) % every == 0) bt_copy(b, bt, CP_SAVE_CK, (m + 1) / every, (m + 1) & 1);
+    }
+    bt_copy(b, bt, CP_SAVE_CK, bt->nck - 1, nsteps & 1);
+    BTCHK(b, hipGetLastError());
+    BTCHK(b, hipStreamSynchronize(s));
+    bt->dts.assign((size_t)nsteps * nb, 0.0);
+    if (nsteps > 0)
+        BTCHK(b, hipMemcpy(bt->dts.data(), batch_hist(b), sizeof(double) * bt->dts.size(), hipMemcpyDeviceToHost));
+    batch_after_run(b, nsteps);
+    if ((r = batch_refresh(b)) || (r = bt_upload(b, bt))) return r;
+    if (dt_out && nsteps > 0) std::memcpy(dt_out, bt->dts.data(), sizeof(double) * bt->dts.size());
+    if (!dt_in)
+        for (size_t i = 0; i < bt->dts.size(); i++)
+            if (!std::isfinite(bt->dts[i]) || !(bt->dts[i] > 0.0) || bt->dts[i] >= 1.0e10) {
+                batch_err(b) = "box " + std::to_string(i % nb) + ": finddt produced a non-finite step";
+                return CUMC_ENONFINITE;
+            }
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed_bcell, double *const *grad_prim)
+{
+    if (!b) return CUMC_EINVAL;
+    batch_tape *bt = batch_adj(b);
+    if (!bt) {
+        batch_err(b) = "cumc_batch_tape_run must precede cumc_batch_tape_backward";
+        return CUMC_ESTATE;
+    }
+    const int nb = batch_nbox(b);
+    hipStream_t s = batch_stream(b);
+    int r = batch_refresh(b);
+    if (r || (r = bt_upload(b, bt))) return r;
+    for (int i = 0; i < nb; i++) {
+        cumc_ctx *c = batch_box(b, i);
+        cumc_tape *tp = c->tape;
+        const size_t pb = plane_bytes(c);
+        BTCHK(b, hipMemsetAsync(tp->adj, 0, (size_t)NADJ * pb, c->stream));
+        for (int k = 0; k < 3; k++) BTCHK(b, hipMemsetAsync(tp->lu[k], 0, 8 * pb, c->stream));
+        if (seed_bcell && seed_bcell[i] && (r = aos_into(c, seed_bcell[i], 3, tp->A.BC))) {
+            batch_err(b) = "box " + std::to_string(i) + ": " + c->err;
+            return r;
+        }
+        BTCHK(b, hipStreamSynchronize(c->stream));
+    }
+    const int N = bt->nsteps, k = bt->every;
+    if ((r = batch_reserve_hist(b, N > 0 ? N : 1))) return r;
+    if (N > 0)
+        BTCHK(b, hipMemcpyAsync(batch_hist(b), bt->dts.data(), sizeof(double) * bt->dts.size(), hipMemcpyHostToDevice, s));
+    BTCHK(b, hipEventRecord(bt->e0, s));
+    int q = 0;
+    for (int s0 = ((N - 1) / k) * k; N > 0 && s0 >= 0; s0 -= k) {
+        const int s1 = s0 + k < N ? s0 + k : N;
+        bt_copy(b, bt, CP_LOAD_CK, s0 / k, 0);
+        bt_copy(b, bt, CP_SAVE_SEG, 0, 0);
+        for (int m = s0; m < s1; m++) {
+            const int i = m - s0;
+            batch_set_dt(b, m);
+            for (int st = 1; st <= 3; st++) batch_stage(b, st, i & 1);
+            bt_copy(b, bt, CP_SAVE_SEG, i + 1, (i + 1) & 1);
+        }
+        for (int m = s1 - 1; m >= s0; m--) {
+            const int i = m - s0;
+            bt_copy(b, bt, CP_LOAD_SEG, i, 0);
+            batch_set_dt(b, m);
+            batch_stage(b, 1, 0);
+            bt_copy(b, bt, CP_SAVE_A, 0, 0);
+            batch_stage(b, 2, 0);
+            bt_copy(b, bt, CP_SAVE_B, 0, 0);
+            batch_fluxes_ecorn(b);
+            bt_reverse_stage<3>(b, bt, 0, i + 1, q);
+            bt_copy(b, bt, CP_LOAD_A, 0, 0);
+            batch_fluxes_ecorn(b);
+            bt_reverse_stage<2>(b, bt, 1, 0, q);
+            bt_copy(b, bt, CP_LOAD12_SEG, i, 0);
+            batch_fluxes_ecorn(b);
+            bt_reverse_stage<1>(b, bt, 2, 0, q);
+            q ^= 1;
+        }
+        BTCHK(b, hipGetLastError());
+    }
+    // initial_update: its ghost fill, then k_init on the uploaded state
+    bt_ghost(b, bt, 3, 0);
+    bt_copy(b, bt, CP_LOAD_INIT, 0, 0);
+    hipLaunchKernelGGL(kbr_init_a, bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q);
+    hipLaunchKernelGGL(kbr_init_b, bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb);
+    BTCHK(b, hipEventRecord(bt->e1, s));
+    BTCHK(b, hipGetLastError());
+    BTCHK(b, hipStreamSynchronize(s));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, bt->e0, bt->e1);
+    batch_last_ms(b) = ms;
+    for (int i = 0; i < nb; i++)
+        if (grad_prim && grad_prim[i] && (r = soa_out(batch_box(b, i), batch_box(b, i)->tape->A.P, 8, grad_prim[i]))) {
+            batch_err(b) = "box " + std::to_string(i) + ": " + batch_box(b, i)->err;
+            return r;
+        }
+    // leave every box at the taped final state
+    bt_copy(b, bt, CP_LOAD_CK, bt->nck - 1, 0);
+    BTCHK(b, hipStreamSynchronize(s));
+    for (int i = 0; i < nb; i++) batch_box(b, i)->dt_valid = false;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_tape_free(cumc_batch *b)
+{
+    if (!b) return CUMC_EINVAL;
+    batch_tape_release(b);
+    for (int i = 0; i < batch_nbox(b); i++) tape_release(batch_box(b, i));
     return CUMC_OK;
 }

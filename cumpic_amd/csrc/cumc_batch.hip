@@ -189,6 +189,7 @@ struct cumc_batch {
     std::vector<Graph> graphs;
     double last_ms = 0.0;
     std::string err;
+    batch_tape *adj = nullptr;  // batched reverse-mode record (cumc_adjoint.hip)
     // per-phase event timing used by the tuner
     bool prof = false;
     std::vector<hipEvent_t> evp;
@@ -332,42 +333,101 @@ static void launch_tail_b(cumc_batch *b, int par)
         hipLaunchKernelGGL((kb_tail<STAGE, 1>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
 }
 
-// one RUNGEKUTTA step of every box; par = parity of the step within the run
-static void enqueue_step_b(cumc_batch *b, int par, int frozen)
+// face fluxes of the three sweeps and the corner EMFs of every box (the first half of a stage)
+static void enqueue_fluxes_ecorn_b(cumc_batch *b)
 {
     hipStream_t s = b->stream;
     const BoxDesc *D = b->d_desc;
     const int nb = b->nbox;
     const bool ppm = b->recon == CUMC_PPMC;
-    hipLaunchKernelGGL(kb_dt_begin, dim3(1), dim3(256), 0, s, D, nb, b->d_hist, b->d_ctr, frozen);
-    for (int st = 1; st <= 3; st++) {
-        bmark(b, P_FX);
-        if (ppm) hipLaunchKernelGGL((kb_flux_x<4>), dim3(grid_of(b, P_FX)), dim3(256), 0, s, D, pre_of(b, P_FX), nb);
-        else hipLaunchKernelGGL((kb_flux_x<1>), dim3(grid_of(b, P_FX)), dim3(256), 0, s, D, pre_of(b, P_FX), nb);
-        bmark(b, -1);
-        bmark(b, P_FY);
-        if (ppm) hipLaunchKernelGGL((kb_flux_walk<2, 4>), dim3(grid_of(b, P_FY)), dim3(128), 0, s, D, pre_of(b, P_FY), nb);
-        else hipLaunchKernelGGL((kb_flux_walk<2, 1>), dim3(grid_of(b, P_FY)), dim3(128), 0, s, D, pre_of(b, P_FY), nb);
-        bmark(b, -1);
-        bmark(b, P_FZ);
-        if (ppm) hipLaunchKernelGGL((kb_flux_walk<3, 4>), dim3(grid_of(b, P_FZ)), dim3(128), 0, s, D, pre_of(b, P_FZ), nb);
-        else hipLaunchKernelGGL((kb_flux_walk<3, 1>), dim3(grid_of(b, P_FZ)), dim3(128), 0, s, D, pre_of(b, P_FZ), nb);
-        bmark(b, -1);
-        bmark(b, P_EC);
-        hipLaunchKernelGGL(kb_ecorn, dim3(grid_of(b, P_EC)), dim3(256), 0, s, D, pre_of(b, P_EC), nb);
-        bmark(b, -1);
-        if (st == 3) hipLaunchKernelGGL(kb_dt_reset, dim3(1), dim3(256), 0, s, D, nb);
-        bmark(b, P_TAIL);
-        if (st == 1) launch_tail_b<1>(b, par);
-        else if (st == 2) launch_tail_b<2>(b, par);
-        else launch_tail_b<3>(b, par);
-        bmark(b, -1);
-        bmark(b, P_GX);
-        hipLaunchKernelGGL(kb_ghost<0>, dim3(grid_of(b, P_GX)), dim3(256), 0, s, D, pre_of(b, P_GX), nb);
-        hipLaunchKernelGGL(kb_ghost<1>, dim3(grid_of(b, P_GY)), dim3(256), 0, s, D, pre_of(b, P_GY), nb);
-        hipLaunchKernelGGL(kb_ghost<2>, dim3(grid_of(b, P_GZ)), dim3(256), 0, s, D, pre_of(b, P_GZ), nb);
-        bmark(b, -1);
-    }
+    bmark(b, P_FX);
+    if (ppm) hipLaunchKernelGGL((kb_flux_x<4>), dim3(grid_of(b, P_FX)), dim3(256), 0, s, D, pre_of(b, P_FX), nb);
+    else hipLaunchKernelGGL((kb_flux_x<1>), dim3(grid_of(b, P_FX)), dim3(256), 0, s, D, pre_of(b, P_FX), nb);
+    bmark(b, -1);
+    bmark(b, P_FY);
+    if (ppm) hipLaunchKernelGGL((kb_flux_walk<2, 4>), dim3(grid_of(b, P_FY)), dim3(128), 0, s, D, pre_of(b, P_FY), nb);
+    else hipLaunchKernelGGL((kb_flux_walk<2, 1>), dim3(grid_of(b, P_FY)), dim3(128), 0, s, D, pre_of(b, P_FY), nb);
+    bmark(b, -1);
+    bmark(b, P_FZ);
+    if (ppm) hipLaunchKernelGGL((kb_flux_walk<3, 4>), dim3(grid_of(b, P_FZ)), dim3(128), 0, s, D, pre_of(b, P_FZ), nb);
+    else hipLaunchKernelGGL((kb_flux_walk<3, 1>), dim3(grid_of(b, P_FZ)), dim3(128), 0, s, D, pre_of(b, P_FZ), nb);
+    bmark(b, -1);
+    bmark(b, P_EC);
+    hipLaunchKernelGGL(kb_ecorn, dim3(grid_of(b, P_EC)), dim3(256), 0, s, D, pre_of(b, P_EC), nb);
+    bmark(b, -1);
+}
+
+// one RK stage of every box; par = parity of the step within the run
+static void enqueue_stage_b(cumc_batch *b, int st, int par)
+{
+    hipStream_t s = b->stream;
+    const BoxDesc *D = b->d_desc;
+    const int nb = b->nbox;
+    enqueue_fluxes_ecorn_b(b);
+    if (st == 3) hipLaunchKernelGGL(kb_dt_reset, dim3(1), dim3(256), 0, s, D, nb);
+    bmark(b, P_TAIL);
+    if (st == 1) launch_tail_b<1>(b, par);
+    else if (st == 2) launch_tail_b<2>(b, par);
+    else launch_tail_b<3>(b, par);
+    bmark(b, -1);
+    bmark(b, P_GX);
+    hipLaunchKernelGGL(kb_ghost<0>, dim3(grid_of(b, P_GX)), dim3(256), 0, s, D, pre_of(b, P_GX), nb);
+    hipLaunchKernelGGL(kb_ghost<1>, dim3(grid_of(b, P_GY)), dim3(256), 0, s, D, pre_of(b, P_GY), nb);
+    hipLaunchKernelGGL(kb_ghost<2>, dim3(grid_of(b, P_GZ)), dim3(256), 0, s, D, pre_of(b, P_GZ), nb);
+    bmark(b, -1);
+}
+
+// one RUNGEKUTTA step of every box; par = parity of the step within the run
+static void enqueue_step_b(cumc_batch *b, int par, int frozen)
+{
+    hipLaunchKernelGGL(kb_dt_begin, dim3(1), dim3(256), 0, b->stream, b->d_desc, b->nbox, b->d_hist, b->d_ctr, frozen);
+    for (int st = 1; st <= 3; st++) enqueue_stage_b(b, st, par);
+}
+
+// history capacity for nsteps steps (graphs hold the history pointer)
+static int reserve_hist(cumc_batch *b, int nsteps)
+{
+    if (nsteps <= b->hist_cap) return CUMC_OK;
+    clear_graphs(b);
+    hipFree(b->d_hist);
+    b->d_hist = nullptr;
+    b->hist_cap = 0;
+    if (hipMalloc(&b->d_hist, sizeof(double) * (size_t)nsteps * b->nbox) != hipSuccess)
+        return bfail(b, CUMC_ENOMEM, "dt history allocation failed");
+    b->hist_cap = nsteps;
+    return CUMC_OK;
+}
+
+// ---- helpers for the batched reverse mode (cumc_adjoint.hip) ----
+int batch_nbox(const cumc_batch *b) { return b->nbox; }
+batch_tape *&batch_adj(cumc_batch *b) { return b->adj; }
+double &batch_last_ms(cumc_batch *b) { return b->last_ms; }
+cumc_ctx *batch_box(const cumc_batch *b, int i) { return b->box[i]; }
+hipStream_t batch_stream(const cumc_batch *b) { return b->stream; }
+std::string &batch_err(cumc_batch *b) { return b->err; }
+int batch_refresh(cumc_batch *b) { return refresh(b); }
+int batch_reserve_hist(cumc_batch *b, int nsteps) { return reserve_hist(b, nsteps); }
+double *batch_hist(cumc_batch *b) { return b->d_hist; }
+void batch_set_counter(cumc_batch *b, int v)
+{
+    hipLaunchKernelGGL(kb_set_int, dim3(1), dim3(64), 0, b->stream, b->d_ctr, v);
+}
+void batch_finddt(cumc_batch *b) { enqueue_finddt_b(b); }
+void batch_step(cumc_batch *b, int par, int frozen) { enqueue_step_b(b, par, frozen); }
+void batch_stage(cumc_batch *b, int st, int par) { enqueue_stage_b(b, st, par); }
+void batch_fluxes_ecorn(cumc_batch *b) { enqueue_fluxes_ecorn_b(b); }
+// current dt of every box := history entry of step m (the counter is left at m + 1)
+void batch_set_dt(cumc_batch *b, int m)
+{
+    batch_set_counter(b, m);
+    hipLaunchKernelGGL(kb_dt_begin, dim3(1), dim3(256), 0, b->stream, b->d_desc, b->nbox, b->d_hist, b->d_ctr, 1);
+}
+// host-side view after nsteps steps of a run that started from the boxes' buffers
+void batch_after_run(cumc_batch *b, int nsteps)
+{
+    if (nsteps & 1)
+        for (cumc_ctx *c : b->box) swap_u(c);
+    for (cumc_ctx *c : b->box) c->dt_valid = true;
 }
 
 static int get_graph(cumc_batch *b, int steps, int frozen, int par, hipGraphExec_t *out)
@@ -428,6 +488,7 @@ extern "C" int cumc_batch_destroy(cumc_batch *b)
 {
     if (!b) return CUMC_EINVAL;
     if (b->stream) hipStreamSynchronize(b->stream);
+    batch_tape_release(b);
     clear_graphs(b);
     hipFree(b->d_desc);
     hipFree(b->d_pre);
@@ -469,17 +530,10 @@ extern "C" int cumc_batch_run(cumc_batch *b, int nsteps, int graph_steps, const 
     if (dt_in)
         for (long i = 0; i < (long)nsteps * nb; i++)
             if (!std::isfinite(dt_in[i]) || !(dt_in[i] > 0.0)) return bfail(b, CUMC_ENONFINITE, "frozen dt must be finite and positive");
-    if (nsteps > b->hist_cap) {
-        clear_graphs(b);  // graphs hold the history pointer
-        hipFree(b->d_hist);
-        b->d_hist = nullptr;
-        b->hist_cap = 0;
-        if (hipMalloc(&b->d_hist, sizeof(double) * (size_t)nsteps * nb) != hipSuccess)
-            return bfail(b, CUMC_ENOMEM, "dt history allocation failed");
-        b->hist_cap = nsteps;
-    }
+    int r = reserve_hist(b, nsteps);
+    if (r) return r;
     for (cumc_ctx *c : b->box) BCHK(b, hipStreamSynchronize(c->stream));
-    int r = refresh(b);
+    r = refresh(b);
     if (r) return r;
     if (dt_in)
         BCHK(b, hipMemcpyAsync(b->d_hist, dt_in, sizeof(double) * (size_t)nsteps * nb, hipMemcpyHostToDevice, b->stream));
@@ -491,10 +545,7 @@ extern "C" int cumc_batch_run(cumc_batch *b, int nsteps, int graph_steps, const 
     float ms = 0.f;
     hipEventElapsedTime(&ms, b->ev0, b->ev1);
     b->last_ms = ms;
-    // the boxes' host-side view: the step result sits in U0 after an odd number of swaps
-    if (nsteps & 1)
-        for (cumc_ctx *c : b->box) swap_u(c);
-    for (cumc_ctx *c : b->box) c->dt_valid = true;
+    batch_after_run(b, nsteps);
     std::vector<double> h((size_t)nsteps * nb);
     BCHK(b, hipMemcpy(h.data(), b->d_hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     if (dt_out) std::memcpy(dt_out, h.data(), sizeof(double) * h.size());
@@ -564,14 +615,7 @@ extern "C" int cumc_batch_autotune(cumc_batch *b, cumc_tuning *out)
     if (!b) return CUMC_EINVAL;
     const int nb = b->nbox;
     for (cumc_ctx *c : b->box) BCHK(b, hipStreamSynchronize(c->stream));
-    if (b->hist_cap < 3) {
-        clear_graphs(b);
-        hipFree(b->d_hist);
-        b->d_hist = nullptr;
-        b->hist_cap = 0;
-        if (hipMalloc(&b->d_hist, sizeof(double) * 3 * nb) != hipSuccess) return bfail(b, CUMC_ENOMEM, "tuner history");
-        b->hist_cap = 3;
-    }
+    if (int r = reserve_hist(b, 3)) return r;
     // save
     std::vector<void *> save(nb, nullptr);
     std::vector<double> dts(2 * (size_t)nb);

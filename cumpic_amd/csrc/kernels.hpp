@@ -24,11 +24,27 @@ struct Lay {
     __host__ __device__ long id(int j, int k, int l) const { return (long)(l + 2) * py + (long)(k + 2) * px + (j + 2); }
 };
 
+// Element i of a plane whose base is uniform across the wavefront, addressed as base + 32-bit
+// lane byte offset: the compiler can then use the scalar-base form of the global load/store
+// (one 32-bit offset register per lane shared by every plane) instead of a 64-bit address per
+// lane and per plane.  Needs i * sizeof(T) < 2^32: planes of fewer than 2^29 cells (checked
+// at context creation).
+template <typename T>
+__device__ inline T ldi(const T *base, unsigned i)
+{
+    return *(const T *)((const char *)base + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ inline void sti(T *base, unsigned i, T v)
+{
+    *(T *)((char *)base + i * (unsigned)sizeof(T)) = v;
+}
+
 // Per-axis geometry tables: tab[row * (n + 7) + j + 3] for j in [-3, n+3].
 struct Tabs {
     const double *t[3];
     int len[3];  // n + 7 per axis
-    __device__ double at(int ax, int row, int j) const { return t[ax][row * len[ax] + j + 3]; }
+    __device__ double at(int ax, int row, int j) const { return ldi(t[ax] + (long)row * len[ax], (unsigned)(j + 3)); }
 };
 
 struct Model {
@@ -93,15 +109,17 @@ __device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs 
     const bool active = lane >= 2 && lane <= 60 && inr && k <= L.ny + 1 && j >= 0 && j <= L.nx;
     const int jc = j < -2 ? -2 : (j > L.nx + 3 ? L.nx + 3 : j);
 
+    const unsigned gu = (unsigned)g;
+    const long ps = L.ps;
     R q[7];
-    q[0] = R(F.P[g]);
-    q[1] = R(F.P[g + L.ps]);
-    q[2] = R(F.P[g + 2 * L.ps]);
-    q[3] = R(F.P[g + 3 * L.ps]);
-    q[4] = R(F.P[g + 4 * L.ps]);
-    q[5] = R(F.BC[g + L.ps]);       // By cell
-    q[6] = R(F.BC[g + 2 * L.ps]);   // Bz cell
-    const R bn = R(F.P[g + 5 * L.ps]);  // Bx on face j
+    q[0] = R(ldi(F.P, gu));
+    q[1] = R(ldi(F.P + ps, gu));
+    q[2] = R(ldi(F.P + 2 * ps, gu));
+    q[3] = R(ldi(F.P + 3 * ps, gu));
+    q[4] = R(ldi(F.P + 4 * ps, gu));
+    q[5] = R(ldi(F.BC + ps, gu));       // By cell
+    q[6] = R(ldi(F.BC + 2 * ps, gu));   // Bz cell
+    const R bn = R(ldi(F.P + 5 * ps, gu));  // Bx on face j
 
     R vl[7], vr[7];
 #pragma unroll
@@ -118,7 +136,7 @@ __device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs 
     if (active) {
         R *o = F.F[0];
 #pragma unroll
-        for (int c = 0; c < 7; c++) o[g + c * L.ps] = fl[c];
+        for (int c = 0; c < 7; c++) sti(o + c * ps, gu, fl[c]);
     }
 }
 
@@ -149,23 +167,24 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
     const int s0 = by * seglen;
     if (s0 > nd) return;
     const int s1 = (s0 + seglen - 1 < nd) ? s0 + seglen - 1 : nd;  // last face of the segment
-    const long stride = DIR == 2 ? L.px : L.py;
-    const long base = DIR == 2 ? (long)(tq + 2) * L.py + 2 + j : (long)(tq + 2) * L.px + 2 + j;
-    auto at = [&](int c) { return base + (long)(c + 2) * stride; };
+    const unsigned stride = (unsigned)(DIR == 2 ? L.px : L.py);
+    const unsigned base = (unsigned)(DIR == 2 ? (long)(tq + 2) * L.py + 2 + j : (long)(tq + 2) * L.px + 2 + j);
+    auto at = [&](int c) { return base + (unsigned)(c + 2) * stride; };
+    const long ps = L.ps;
     const int ax = DIR - 1;
     const int bA = DIR == 2 ? 0 : 0;        // first transverse cell-B (x order): y-sweep Bx, z-sweep Bx
     const int bB = DIR == 2 ? 2 : 1;        //                                     y-sweep Bz, z-sweep By
     const int ibn = DIR == 2 ? 6 : 7;       // face normal B plane in P
 
     auto load7 = [&](int c, R *o) {
-        long g = at(c);
-        o[0] = R(F.P[g]);
-        o[1] = R(F.P[g + L.ps]);
-        o[2] = R(F.P[g + 2 * L.ps]);
-        o[3] = R(F.P[g + 3 * L.ps]);
-        o[4] = R(F.P[g + 4 * L.ps]);
-        o[5] = R(F.BC[g + bA * L.ps]);
-        o[6] = R(F.BC[g + bB * L.ps]);
+        const unsigned g = at(c);
+        o[0] = R(ldi(F.P, g));
+        o[1] = R(ldi(F.P + ps, g));
+        o[2] = R(ldi(F.P + 2 * ps, g));
+        o[3] = R(ldi(F.P + 3 * ps, g));
+        o[4] = R(ldi(F.P + 4 * ps, g));
+        o[5] = R(ldi(F.BC + bA * ps, g));
+        o[6] = R(ldi(F.BC + bB * ps, g));
     };
 
     // window: w0 = v(c-1), w1 = v(c), w2 = v(c+1), w3 = v(c+2) ; slopes d0 = dm(c-1), d1 = dm(c), d2 = dm(c+1)
@@ -209,8 +228,8 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
         }
         if (c > s0) {
             // face c-1: left state = vp(c-1), right state = vm(c)
-            const long gf = at(c - 1);
-            const R bn = R(F.P[gf + ibn * L.ps]);
+            const unsigned gf = at(c - 1);
+            const R bn = R(ldi(F.P + ibn * ps, gf));
             R fl[7];
             if (DIR == 2) {
                 // rotated frame n=y, t1=z, t2=x ; reconstructed cell B order (x, z)
@@ -218,26 +237,26 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
                 FaceState<R> sr{vm[0], vm[2], vm[3], vm[1], vm[4], bn, vm[6], vm[5]};
                 hll_face<2, R>(R(M.ggas), sl, sr, fl);
                 R *o = F.F[1];
-                o[gf] = fl[0];
-                o[gf + 1 * L.ps] = fl[3];
-                o[gf + 2 * L.ps] = fl[1];
-                o[gf + 3 * L.ps] = fl[2];
-                o[gf + 4 * L.ps] = fl[4];
-                o[gf + 5 * L.ps] = fl[6];  // F(Bx)
-                o[gf + 6 * L.ps] = fl[5];  // F(Bz)
+                sti(o, gf, fl[0]);
+                sti(o + 1 * ps, gf, fl[3]);
+                sti(o + 2 * ps, gf, fl[1]);
+                sti(o + 3 * ps, gf, fl[2]);
+                sti(o + 4 * ps, gf, fl[4]);
+                sti(o + 5 * ps, gf, fl[6]);  // F(Bx)
+                sti(o + 6 * ps, gf, fl[5]);  // F(Bz)
             } else {
                 // rotated frame n=z, t1=x, t2=y ; reconstructed cell B order (x, y)
                 FaceState<R> sl{vpp[0], vpp[3], vpp[1], vpp[2], vpp[4], bn, vpp[5], vpp[6]};
                 FaceState<R> sr{vm[0], vm[3], vm[1], vm[2], vm[4], bn, vm[5], vm[6]};
                 hll_face<3, R>(R(M.ggas), sl, sr, fl);
                 R *o = F.F[2];
-                o[gf] = fl[0];
-                o[gf + 1 * L.ps] = fl[2];
-                o[gf + 2 * L.ps] = fl[3];
-                o[gf + 3 * L.ps] = fl[1];
-                o[gf + 4 * L.ps] = fl[4];
-                o[gf + 5 * L.ps] = fl[5];  // F(Bx)
-                o[gf + 6 * L.ps] = fl[6];  // F(By)
+                sti(o, gf, fl[0]);
+                sti(o + 1 * ps, gf, fl[2]);
+                sti(o + 2 * ps, gf, fl[3]);
+                sti(o + 3 * ps, gf, fl[1]);
+                sti(o + 4 * ps, gf, fl[4]);
+                sti(o + 5 * ps, gf, fl[5]);  // F(Bx)
+                sti(o + 6 * ps, gf, fl[6]);  // F(By)
             }
         }
 #pragma unroll
@@ -279,11 +298,11 @@ __device__ inline bool xcd_tile(int gx, int gy, int &bx, int &by)
 }
 
 template <typename R>
-__device__ inline void ecells(const Lay &L, const Fields<R> &F, long g, R &ex, R &ey, R &ez)
+__device__ inline void ecells(const Lay &L, const Fields<R> &F, unsigned g, R &ex, R &ey, R &ez)
 {
     const long ps = L.ps;
-    const R vx = R(F.P[g + ps]), vy = R(F.P[g + 2 * ps]), vz = R(F.P[g + 3 * ps]);
-    const R bx = R(F.BC[g]), by = R(F.BC[g + ps]), bz = R(F.BC[g + 2 * ps]);
+    const R vx = R(ldi(F.P + ps, g)), vy = R(ldi(F.P + 2 * ps, g)), vz = R(ldi(F.P + 3 * ps, g));
+    const R bx = R(ldi(F.BC, g)), by = R(ldi(F.BC + ps, g)), bz = R(ldi(F.BC + 2 * ps, g));
     ex = vz * by - vy * bz;
     ey = vx * bz - vz * bx;
     ez = vy * bx - vx * by;
@@ -300,10 +319,12 @@ __device__ inline void ecorn_tile(const Lay &L, const Fields<R> &F, int bx, int 
     const int l0 = by * zc;
     if (k > L.ny || j > L.nx || l0 > L.nz) return;
     const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
-    const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
-    const R *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2];
+    const unsigned sj = 1, sk = (unsigned)L.px, sl = (unsigned)L.py;
+    const long ps = L.ps;
+    const R *Fx5 = F.F[0] + 5 * ps, *Fx6 = F.F[0] + 6 * ps, *Fy5 = F.F[1] + 5 * ps, *Fy6 = F.F[1] + 6 * ps;
+    const R *Fz5 = F.F[2] + 5 * ps, *Fz6 = F.F[2] + 6 * ps;
     const R q = R(0.25);
-    long g = L.id(j, k, l0);
+    unsigned g = (unsigned)L.id(j, k, l0);
     // cell EMFs of plane l at (j,k), (j,k+1), (j+1,k), (j+1,k+1) and the plane-l B fluxes
     // that the next plane needs again
     R x00, y00, z00, x01, y01, z01, x10, y10, z10, x11, y11, z11;
@@ -311,31 +332,31 @@ __device__ inline void ecorn_tile(const Lay &L, const Fields<R> &F, int bx, int 
     ecells(L, F, g + sk, x01, y01, z01);
     ecells(L, F, g + sj, x10, y10, z10);
     ecells(L, F, g + sj + sk, x11, y11, z11);
-    R fx6 = Fx[g + 6 * ps], fy6 = Fy[g + 6 * ps];
+    R fx6 = ldi(Fx6, g), fy6 = ldi(Fy6, g);
     for (int l = l0; l <= l1; l++, g += sl) {
-        const long gn = g + sl;
+        const unsigned gn = g + sl;
         R X00, Y00, Z00, X01, Y01, Z01, X10, Y10, Z10, X11, Y11, Z11;
         ecells(L, F, gn, X00, Y00, Z00);
         ecells(L, F, gn + sk, X01, Y01, Z01);
         ecells(L, F, gn + sj, X10, Y10, Z10);
         ecells(L, F, gn + sj + sk, X11, Y11, Z11);
-        const R fx6n = Fx[gn + 6 * ps], fy6n = Fy[gn + 6 * ps];
+        const R fx6n = ldi(Fx6, gn), fy6n = ldi(Fy6, gn);
         // x sweep: ey += (F_x(Bz)(l) + F_x(Bz)(l+1))/4, ez += (-F_x(By)(k) - F_x(By)(k+1))/4
         R ey = R(0) + q * (fx6 + fx6n);
-        R ez = R(0) + q * (-Fx[g + 5 * ps] - Fx[g + sk + 5 * ps]);
+        R ez = R(0) + q * (-ldi(Fx5, g) - ldi(Fx5, g + sk));
         // y sweep: ex += (-F_y(Bz)(l) - F_y(Bz)(l+1))/4, ez += (F_y(Bx)(j) + F_y(Bx)(j+1))/4
         R ex = R(0) + q * (-fy6 - fy6n);
-        ez = ez + q * (Fy[g + 5 * ps] + Fy[g + sj + 5 * ps]);
+        ez = ez + q * (ldi(Fy5, g) + ldi(Fy5, g + sj));
         // z sweep: ex += (F_z(By)(k) + F_z(By)(k+1))/4, ey += (-F_z(Bx)(j) - F_z(Bx)(j+1))/4
-        ex = ex + q * (Fz[g + 6 * ps] + Fz[g + sk + 6 * ps]);
-        ey = ey + q * (-Fz[g + 5 * ps] - Fz[g + sj + 5 * ps]);
+        ex = ex + q * (ldi(Fz6, g) + ldi(Fz6, g + sk));
+        ey = ey + q * (-ldi(Fz5, g) - ldi(Fz5, g + sj));
         // flux_ct
         ex = ex * R(2.0) - q * (x00 + X00 + x01 + X01);
         ey = ey * R(2.0) - q * (y00 + Y00 + y10 + Y10);
         ez = ez * R(2.0) - q * (z00 + z01 + z10 + z11);
-        F.E[g] = ex;
-        F.E[g + ps] = ey;
-        F.E[g + 2 * ps] = ez;
+        sti(F.E, g, ex);
+        sti(F.E + ps, g, ey);
+        sti(F.E + 2 * ps, g, ez);
         x00 = X00; y00 = Y00; z00 = Z00; x01 = X01; y01 = Y01; z01 = Z01;
         x10 = X10; y10 = Y10; z10 = Z10; x11 = X11; y11 = Y11; z11 = Z11;
         fx6 = fx6n;
@@ -448,11 +469,19 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
     double dtmin = 1.0e10;
     if (col) {
         const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
-        const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
+        const unsigned sj = 1, sk = (unsigned)L.px, sl = (unsigned)L.py;
+        const long ps = L.ps;
         const double dt = *dtp;
-        const R *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2];
-        const R *E = F.E;
         const double *U0 = F.U0;
+        // plane c of each field at cell offset i (uniform plane bases, 32-bit lane offsets)
+        auto fx = [&](int c, unsigned i) { return (double)ldi(F.F[0] + c * ps, i); };
+        auto fy = [&](int c, unsigned i) { return (double)ldi(F.F[1] + c * ps, i); };
+        auto fzz = [&](int c, unsigned i) { return (double)ldi(F.F[2] + c * ps, i); };
+        auto e = [&](int c, unsigned i) { return (double)ldi(F.E + c * ps, i); };
+        auto u0 = [&](int c, unsigned i) { return ldi(U0 + c * ps, i); };
+        auto ui = [&](int c, unsigned i) { return STAGE == 1 ? 0.0 : ldi(Uin + c * ps, i); };
+        auto pp = [&](int c, unsigned i) { return ldi(F.P + c * ps, i); };
+        auto bcl = [&](int c, unsigned i) { return ldi(F.BC + c * ps, i); };
         const double gm1 = M.ggas - 1.0;
         // flux-divergence geometry of the column (coords.f90:255-282)
         const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k);
@@ -465,41 +494,36 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             gyc = xloc * dyl;
         }
         // carries from plane l0-1
-        long g = L.id(j, k, l0);
-        long gm = g - sl;
-        double exL = E[gm], eyL = E[gm + ps], eyjL = E[gm - sj + ps], exkL = E[gm - sk];
+        unsigned g = (unsigned)L.id(j, k, l0);
+        const unsigned gm = g - sl;
+        double exL = e(0, gm), eyL = e(1, gm), eyjL = e(1, gm - sj), exkL = e(0, gm - sk);
         double bzL = 0.0;
-        if (l0 >= 1)
-            bzL = rk<STAGE>(U0[gm + 7 * ps], STAGE == 1 ? 0.0 : Uin[gm + 7 * ps], dt,
-                            rhs_bz<COORD>(tb, j, k, eyL, eyjL, exL, exkL));
+        if (l0 >= 1) bzL = rk<STAGE>(u0(7, gm), ui(7, gm), dt, rhs_bz<COORD>(tb, j, k, eyL, eyjL, exL, exkL));
         double fzL[5];
 #pragma unroll
-        for (int i = 0; i < 5; i++) fzL[i] = (double)Fz[gm + i * ps];
+        for (int i = 0; i < 5; i++) fzL[i] = fzz(i, gm);
         for (int l = l0; l <= l1; l++, g += sl) {
             const bool interior = j >= 1 && k >= 1 && l >= 1;
             // corner EMFs around the cell's faces
-            const double ex0 = E[g], ey0 = E[g + ps], ez0 = E[g + 2 * ps];
-            const double exk = E[g - sk], ezk = E[g - sk + 2 * ps];
-            const double eyj = E[g - sj + ps], ezj = E[g - sj + 2 * ps];
-            const double ezjk = E[g - sj - sk + 2 * ps];
+            const double ex0 = e(0, g), ey0 = e(1, g), ez0 = e(2, g);
+            const double exk = e(0, g - sk), ezk = e(2, g - sk);
+            const double eyj = e(1, g - sj), ezj = e(2, g - sj);
+            const double ezjk = e(2, g - sj - sk);
             // new face fields: own three, and the x / y faces of the lower neighbours
-            const double bxn = rk<STAGE>(U0[g + 5 * ps], STAGE == 1 ? 0.0 : Uin[g + 5 * ps], dt,
-                                         rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, eyL));
-            const double byn = rk<STAGE>(U0[g + 6 * ps], STAGE == 1 ? 0.0 : Uin[g + 6 * ps], dt,
-                                         rhs_by<COORD>(tb, j, l, ex0, exL, ez0, ezj));
-            const double bzn = rk<STAGE>(U0[g + 7 * ps], STAGE == 1 ? 0.0 : Uin[g + 7 * ps], dt,
-                                         rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
+            const double bxn = rk<STAGE>(u0(5, g), ui(5, g), dt, rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, eyL));
+            const double byn = rk<STAGE>(u0(6, g), ui(6, g), dt, rhs_by<COORD>(tb, j, l, ex0, exL, ez0, ezj));
+            const double bzn = rk<STAGE>(u0(7, g), ui(7, g), dt, rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
             // hydro right-hand side on interior cells
             double lrk[5];
             double fz[5];
 #pragma unroll
-            for (int i = 0; i < 5; i++) fz[i] = (double)Fz[g + i * ps];
+            for (int i = 0; i < 5; i++) fz[i] = fzz(i, g);
             if (interior) {
                 double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
                 if (COORD == 2 || M.model == 1) {
-                    const double rho = F.P[g], vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
-                    const double p = F.P[g + 4 * ps];
-                    const double bx = F.BC[g], by = F.BC[g + ps], bz = F.BC[g + 2 * ps];
+                    const double rho = pp(0, g), vx = pp(1, g), vy = pp(2, g), vz = pp(3, g);
+                    const double p = pp(4, g);
+                    const double bx = bcl(0, g), by = bcl(1, g), bz = bcl(2, g);
                     if (COORD == 2) {
                         const double bsq = bx * bx + by * by + bz * bz;
                         const double tx = (p + rho * vy * vy + 0.5 * bsq - by * by) / (xloc);
@@ -524,8 +548,8 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
 #pragma unroll
                 for (int i = 0; i < 5; i++) {
                     double a = 0.0;
-                    a = a - ((double)Fx[g + i * ps] * gxp - (double)Fx[g - sj + i * ps] * gxm) / gxc;
-                    a = a - ((double)Fy[g + i * ps] * 1.0 - (double)Fy[g - sk + i * ps] * 1.0) / gyc;
+                    a = a - (fx(i, g) * gxp - fx(i, g - sj) * gxm) / gxc;
+                    a = a - (fy(i, g) * 1.0 - fy(i, g - sk) * 1.0) / gyc;
                     a = a - (fz[i] * 1.0 - fzL[i] * 1.0) / dzl;
                     lrk[i] = a + sc[i];
                 }
@@ -535,24 +559,24 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             }
             double u[5];
 #pragma unroll
-            for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(U0[g + i * ps], STAGE == 1 ? 0.0 : Uin[g + i * ps], dt, lrk[i]);
+            for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(u0(i, g), ui(i, g), dt, lrk[i]);
             // cell-centred field from the new face fields (interior) or the ghost value
             double bcx, bcy, bcz;
             if (interior) {
-                const double bxm = rk<STAGE>(U0[g - sj + 5 * ps], STAGE == 1 ? 0.0 : Uin[g - sj + 5 * ps], dt,
+                const double bxm = rk<STAGE>(u0(5, g - sj), ui(5, g - sj), dt,
                                              rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
-                const double bym = rk<STAGE>(U0[g - sk + 6 * ps], STAGE == 1 ? 0.0 : Uin[g - sk + 6 * ps], dt,
+                const double bym = rk<STAGE>(u0(6, g - sk), ui(6, g - sk), dt,
                                              rhs_by<COORD>(tb, j, l, exk, exkL, ezk, ezjk));
                 bcx = 0.5 * (bxn + bxm);
                 bcy = 0.5 * (byn + bym);
                 bcz = 0.5 * (bzn + bzL);
-                F.BC[g] = bcx;
-                F.BC[g + ps] = bcy;
-                F.BC[g + 2 * ps] = bcz;
+                sti(F.BC, g, bcx);
+                sti(F.BC + ps, g, bcy);
+                sti(F.BC + 2 * ps, g, bcz);
             } else {
-                bcx = F.BC[g];
-                bcy = F.BC[g + ps];
-                bcz = F.BC[g + 2 * ps];
+                bcx = bcl(0, g);
+                bcy = bcl(1, g);
+                bcz = bcl(2, g);
             }
             // U_to_P, floors, P_to_U, EOS
             const double rho0 = u[0];
@@ -571,23 +595,23 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             }
             vsq = vx * vx + vy * vy + vz * vz;
             const double p = rho * eps * gm1;
-            Uout[g] = rho;
-            Uout[g + ps] = vx * rho;
-            Uout[g + 2 * ps] = vy * rho;
-            Uout[g + 3 * ps] = vz * rho;
-            Uout[g + 4 * ps] = rho * (eps + 0.5 * vsq) + 0.5 * bsq;
-            Uout[g + 5 * ps] = bxn;
-            Uout[g + 6 * ps] = byn;
-            Uout[g + 7 * ps] = bzn;
-            F.P[g] = rho;
-            F.P[g + ps] = vx;
-            F.P[g + 2 * ps] = vy;
-            F.P[g + 3 * ps] = vz;
-            F.P[g + 4 * ps] = p;
-            F.P[g + 5 * ps] = bxn;
-            F.P[g + 6 * ps] = byn;
-            F.P[g + 7 * ps] = bzn;
-            F.EPS[g] = eps;
+            sti(Uout, g, rho);
+            sti(Uout + ps, g, vx * rho);
+            sti(Uout + 2 * ps, g, vy * rho);
+            sti(Uout + 3 * ps, g, vz * rho);
+            sti(Uout + 4 * ps, g, rho * (eps + 0.5 * vsq) + 0.5 * bsq);
+            sti(Uout + 5 * ps, g, bxn);
+            sti(Uout + 6 * ps, g, byn);
+            sti(Uout + 7 * ps, g, bzn);
+            sti(F.P, g, rho);
+            sti(F.P + ps, g, vx);
+            sti(F.P + 2 * ps, g, vy);
+            sti(F.P + 3 * ps, g, vz);
+            sti(F.P + 4 * ps, g, p);
+            sti(F.P + 5 * ps, g, bxn);
+            sti(F.P + 6 * ps, g, byn);
+            sti(F.P + 7 * ps, g, bzn);
+            sti(F.EPS, g, eps);
             if (STAGE == 3 && interior)
                 dtmin = fmin2<double>(cell_dt<COORD>(tb, M, j, k, l, rho, vx, vy, vz, p, bcx, bcy, bcz), dtmin);
             // carries

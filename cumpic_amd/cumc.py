@@ -90,6 +90,9 @@ def lib():
             "cumc_tape_run": [_CTX, ctypes.c_int, ctypes.c_int, _D, _D],
             "cumc_tape_backward": [_CTX, _D, _D, _D, _D, _D],
             "cumc_tape_free": [_CTX],
+            "cumc_batch_tape_run": [_CTX, ctypes.c_int, ctypes.c_int, _D, _D],
+            "cumc_batch_tape_backward": [_CTX, ctypes.POINTER(_D), ctypes.POINTER(_D)],
+            "cumc_batch_tape_free": [_CTX],
         })
         for name, args in sig.items():
             f = getattr(L, name)
@@ -344,3 +347,33 @@ class Batch:
         d = ctypes.c_double()
         self._check(lib().cumc_batch_last_ms(self.h, ctypes.byref(d)), "cumc_batch_last_ms")
         return d.value
+
+    def tape_run(self, nsteps: int, every: int = 8, dts=None):
+        """initial_update + nsteps steps of every box from its uploaded state, checkpointed every
+        `every` steps.  dts (nsteps, nbox) freezes the steps.  Returns the (nsteps, nbox) steps."""
+        nb = len(self.solvers)
+        out = np.zeros((max(1, nsteps), nb))
+        din = None
+        if dts is not None:
+            din = np.ascontiguousarray(dts, dtype=np.float64)
+            if din.shape != (nsteps, nb):
+                raise ValueError(f"dts must have shape ({nsteps}, {nb})")
+        self._check(lib().cumc_batch_tape_run(self.h, int(nsteps), int(every), _p(din) if din is not None else None,
+                                              _p(out)), "cumc_batch_tape_run")
+        return out[:nsteps].copy()
+
+    def tape_backward(self, seeds_bcell):
+        """Per-box gradients (uploaded prim layout) of sum_b <seed_b, final bcell_b>."""
+        nb = len(self.solvers)
+        seeds = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in seeds_bcell]
+        for s, a in zip(self.solvers, seeds):
+            if a is not None and a.shape != s.cfg.shape + (3,):
+                raise ValueError("seed shape does not match its box")
+        grads = [np.zeros(s.cfg.shape + (8,)) for s in self.solvers]
+        sp = (_D * nb)(*[_p(a) if a is not None else None for a in seeds])
+        gp = (_D * nb)(*[_p(g) for g in grads])
+        self._check(lib().cumc_batch_tape_backward(self.h, sp, gp), "cumc_batch_tape_backward")
+        return grads
+
+    def tape_free(self):
+        self._check(lib().cumc_batch_tape_free(self.h), "cumc_batch_tape_free")

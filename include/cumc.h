@@ -178,6 +178,15 @@ int cumc_tape_backward(cumc_ctx *ctx, const double *seed_prim, const double *see
                        double *grad_prim, double *grad_eps);
 int cumc_tape_free(cumc_ctx *ctx);
 
+/* The same for every box of a batch at once (boxes as uploaded, not yet initial_update'd):
+ * dt_in / dt_out are nsteps x nbox, step-major.  Each reverse phase is one launch over all boxes;
+ * a box's gradient equals its cumc_tape_backward gradient bit for bit.  seed_bcell and grad_prim
+ * are arrays of nbox host pointers (cumc_download / cumc_upload layouts; an array or an entry may
+ * be NULL).  cumc_batch_last_ms afterwards gives the device time of the backward pass. */
+int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const double *dt_in, double *dt_out);
+int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed_bcell, double *const *grad_prim);
+int cumc_batch_tape_free(cumc_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
