@@ -455,38 +455,72 @@ __device__ inline void kr_flux_hll_pencil(const Lay &L, const Fields<double> &F,
     const long ps = L.ps;
     const int ax = DIR - 1;
     const double *bnp = F.P + (4 + DIR) * ps;
-    double w[6][7];  // cells f-2 .. f+3
-    for (int c = -2; c <= 2; c++)
-        for (int v = 0; v < 7; v++) w[c + 2][v] = qplane<DIR>(F, ps, v)[pc.at(c)];
-    for (int f = 0; f <= pc.n; f++) {
-        for (int v = 0; v < 7; v++) w[5][v] = qplane<DIR>(F, ps, v)[pc.at(f + 3)];
-        double ql[7], qr[7];
-        for (int v = 0; v < 7; v++) {
-            double s0[5] = {w[0][v], w[1][v], w[2][v], w[3][v], w[4][v]};
-            double s1[5] = {w[1][v], w[2][v], w[3][v], w[4][v], w[5][v]};
-            double vm, vp;
-            recon_cell<RECON>(tb, ax, f, s0, vm, vp);
-            ql[v] = vp;
-            recon_cell<RECON>(tb, ax, f + 1, s1, vm, vp);
-            qr[v] = vm;
+    // sliding window as in the forward walk: each cell's slope and edge values computed once;
+    // w0..w3 = v(c-1 .. c+2), d0, d1 = dm(c-1), dm(c), vpp = vp(c-1)
+    double w0[7], w1[7], w2[7], w3[7], d0[7], d1[7], vpp[7];
+    auto load = [&](int c, double *o) {
+        const unsigned g = (unsigned)pc.at(c);
+        for (int v = 0; v < 7; v++) o[v] = ldi(qplane<DIR>(F, ps, v), g);
+    };
+    {
+        double a[7];
+        load(-2, a);
+        load(-1, w0);
+        load(0, w1);
+        load(1, w2);
+        if (RECON == 4) {
+            const double b0 = tb.at(ax, TB_B0C, -1), b1 = tb.at(ax, TB_B1C, -1);
+            const double c0 = tb.at(ax, TB_B0C, 0), c1 = tb.at(ax, TB_B1C, 0);
+            for (int v = 0; v < 7; v++) {
+                d0[v] = ppm_slope<double>(b0, b1, a[v], w0[v], w1[v]);
+                d1[v] = ppm_slope<double>(c0, c1, w0[v], w1[v], w2[v]);
+            }
         }
-        const long gf = pc.at(f);
-        const double bn = bnp[gf];
-        double o[7], lf[7];
-        for (int i = 0; i < 7; i++) o[i] = A.F[ax][gf + i * ps];
-        flux_to_rot<DIR>(o, lf);
-        FaceState<double> lL{0, 0, 0, 0, 0, 0, 0, 0}, lR{0, 0, 0, 0, 0, 0, 0, 0};
-        hll_face_adj(M.ggas, rot<DIR>(ql, bn), rot<DIR>(qr, bn), lf, lL, lR);
-        double hl[7], hr[7];
-        unrot<DIR>(lL, hl);
-        unrot<DIR>(lR, hr);
-        for (int v = 0; v < 7; v++) {
-            A.H[gf + v * ps] = hl[v];
-            A.H[gf + (7 + v) * ps] = hr[v];
+    }
+    for (int c = 0; c <= pc.n + 1; c++) {
+        load(c + 2, w3);
+        double vm[7], vp[7];
+        if (RECON == 4) {
+            const double b0 = tb.at(ax, TB_B0C, c + 1), b1 = tb.at(ax, TB_B1C, c + 1);
+            const double a0m = tb.at(ax, TB_A0M, c), a1m = tb.at(ax, TB_A1M, c), a2m = tb.at(ax, TB_A2M, c);
+            const double a3m = tb.at(ax, TB_A3M, c), a0p = tb.at(ax, TB_A0P, c), a1p = tb.at(ax, TB_A1P, c);
+            const double a2p = tb.at(ax, TB_A2P, c), a3p = tb.at(ax, TB_A3P, c);
+            for (int v = 0; v < 7; v++) {
+                const double d2 = ppm_slope<double>(b0, b1, w1[v], w2[v], w3[v]);
+                ppm_edges<double>(a0m, a1m, a2m, a3m, a0p, a1p, a2p, a3p, w0[v], w1[v], w2[v], d0[v], d1[v], d2,
+                                  vm[v], vp[v]);
+                d0[v] = d1[v];
+                d1[v] = d2;
+            }
+        } else {
+            const double cf = tb.at(ax, TB_CF, c), cb = tb.at(ax, TB_CB, c);
+            for (int v = 0; v < 7; v++) tvd_edges<double>(cf, cb, w0[v], w1[v], w2[v], vm[v], vp[v]);
         }
-        A.P[gf + (4 + DIR) * ps] += lL.bn + lR.bn;
-        for (int c = 0; c < 5; c++)
-            for (int v = 0; v < 7; v++) w[c][v] = w[c + 1][v];
+        if (c > 0) {
+            // face c-1: left state vp(c-1), right state vm(c)
+            const unsigned gf = (unsigned)pc.at(c - 1);
+            const double bn = ldi(bnp, gf);
+            double o[7], lf[7];
+            for (int i = 0; i < 7; i++) o[i] = ldi(A.F[ax] + i * ps, gf);
+            flux_to_rot<DIR>(o, lf);
+            FaceState<double> lL{0, 0, 0, 0, 0, 0, 0, 0}, lR{0, 0, 0, 0, 0, 0, 0, 0};
+            hll_face_adj(M.ggas, rot<DIR>(vpp, bn), rot<DIR>(vm, bn), lf, lL, lR);
+            double hl[7], hr[7];
+            unrot<DIR>(lL, hl);
+            unrot<DIR>(lR, hr);
+            for (int v = 0; v < 7; v++) {
+                sti(A.H + v * ps, gf, hl[v]);
+                sti(A.H + (7 + v) * ps, gf, hr[v]);
+            }
+            double *pb = A.P + (4 + DIR) * ps;
+            sti(pb, gf, ldi(pb, gf) + (lL.bn + lR.bn));
+        }
+        for (int v = 0; v < 7; v++) {
+            vpp[v] = vp[v];
+            w0[v] = w1[v];
+            w1[v] = w2[v];
+            w2[v] = w3[v];
+        }
     }
 }
 
