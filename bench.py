@@ -17,6 +17,7 @@ launch-shape tuner, with a bit-for-bit check of the batched result against the o
   --workload torus      cylindrical torus 256x128x256, 50 steps (torus_cyn hooks)
   --n 512 --steps 100   the 512^3 box of the mixed-precision workload
   --no-batch            skip the batched workload
+  --workload batch      only the batched workload (value = its hipGraph rate)
 """
 from __future__ import annotations
 
@@ -127,7 +128,7 @@ def fields(state, cfg):
             "|B|": np.linalg.norm(b, axis=-1)}
 
 
-def bench_batch(steps=40, graph_steps=8):
+def bench_batch(steps=40, graph_steps=8, fd_dirs=32):
     """Batched 2D boxes: one at a time vs work-list launches vs hipGraph replay."""
     from cumpic_amd import ics
     from cumpic_amd.cumc import Batch, Solver
@@ -178,7 +179,7 @@ def bench_batch(steps=40, graph_steps=8):
     b.run(steps, graph_steps=0)
     t_direct = time.perf_counter() - t0
     log(f"[bench] batch: direct launches {t_direct * 1e3:.1f} ms")
-    adj = bench_adjoint(probs, boxes, b, np.array(seq_dts).T, steps)
+    adj = bench_adjoint(probs, boxes, b, np.array(seq_dts).T, steps, every=graph_steps, ndir=fd_dirs)
     b.close()
     for s in boxes + seq:
         s.close()
@@ -276,12 +277,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["box", "torus"], default="box")
+    ap.add_argument("--workload", choices=["box", "torus", "batch"], default="box",
+                    help="batch: only the batched 2D boxes (forward and reverse mode), value = batched rate")
     ap.add_argument("--n", type=int, default=256, help="zones per axis of the periodic box")
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and full-size parity check")
     ap.add_argument("--no-error", action="store_true", help="skip the mixed-vs-FP64 error replay")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D-box workload")
+    ap.add_argument("--fd-dirs", type=int, default=32, help="directions of the reverse-mode difference check")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -298,6 +301,19 @@ def main():
     if args.copy_probe_mib:
         copy_gbps = copy_probe(args.copy_probe_mib << 20, reps=3)
         log(f"[bench] copy probe {args.copy_probe_mib} MiB: {copy_gbps:.1f} GB/s")
+
+    if args.workload == "batch":
+        steps = args.steps or 40
+        bt = bench_batch(steps=steps, graph_steps=min(8, steps), fd_dirs=args.fd_dirs)
+        g = bt["batched_graph"]
+        print(json.dumps({
+            "metric": "zone-cycles/s FP64 vs mixed, % HBM roofline, adjoint/forward step cost",
+            "value": g["value"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+            "ms_per_step": g["ms"] / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": bt["workload"], "form_of_value": "batched hipGraph"},
+            "roofline": None, "cpu_baseline": None, "batch": bt}), flush=True)
+        return
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     if args.workload == "torus":
@@ -368,7 +384,7 @@ def main():
 
     batch = None
     if not args.no_batch and args.workload == "box":
-        batch = bench_batch()
+        batch = bench_batch(fd_dirs=args.fd_dirs)
 
     main_form = "fp64" if "fp64" in results else forms[0]
     m = results[main_form]
