@@ -32,6 +32,7 @@ struct cumc_ctx {
     struct cumc_tape *tape = nullptr;  // reverse-mode record (cumc_adjoint.hip)
     int seg = 48;     // target faces per pencil segment of the y / z sweeps
     int zchunk = 32;  // planes per column tile of the EMF and stage-tail kernels
+    int tile_w = 0;   // column tile width of those kernels (0: flattened rows; see col_of)
     double last_ms = 0.0;
     std::string err;
     // per-kernel-class HIP event timing (cumc_profile / cumc_kernel_times)
@@ -182,17 +183,17 @@ template <typename R, int STAGE>
 void launch_tail(cumc_ctx *ctx, const Fields<R> &F, const double *Uin, double *Uout)
 {
     const Lay &L = ctx->L;
-    const int zc = ctx->zchunk;
-    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + zc) / zc;
+    const int zc = ctx->zchunk, tw = ctx->tile_w;
+    const int gx = col_tiles(L, tw), gy = (L.nz + zc) / zc;
     dim3 grid(xcd_blocks(gx, gy));
     Tabs tb = tabs_of(ctx);
     unsigned long long *dtn = (unsigned long long *)(ctx->dt_dev + 1);
     if (ctx->M.coordinate == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((k_tail<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy, zc);
+                           Uin, Uout, dtn, gx, gy, zc, tw);
     else
         hipLaunchKernelGGL((k_tail<R, STAGE, 1>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy, zc);
+                           Uin, Uout, dtn, gx, gy, zc, tw);
 }
 
 // one RK stage: fluxes of the three sweeps, corner EMFs, stage tail, ghost fill.
@@ -203,9 +204,9 @@ void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
     const Lay &L = ctx->L;
     launch_fluxes<R>(ctx, F);
     mark(ctx, KC_ECORN);
-    const int zc = ctx->zchunk;
-    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + zc) / zc;
-    hipLaunchKernelGGL((k_ecorn<R>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, F, gx, gy, zc);
+    const int zc = ctx->zchunk, tw = ctx->tile_w;
+    const int gx = col_tiles(L, tw), gy = (L.nz + zc) / zc;
+    hipLaunchKernelGGL((k_ecorn<R>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, F, gx, gy, zc, tw);
     mark(ctx, -1);
     mark(ctx, KC_TAIL);
     if (st == 1) {

@@ -460,12 +460,15 @@ extern "C" int cumc_get_tuning(cumc_ctx *ctx, cumc_tuning *out)
     std::memset(out, 0, sizeof(*out));
     out->seg = ctx->seg;
     out->zchunk = ctx->zchunk;
+    out->tile_w = ctx->tile_w;
     return CUMC_OK;
 }
 
 extern "C" int cumc_set_tuning(cumc_ctx *ctx, const cumc_tuning *t)
 {
     if (!ctx || !t || t->seg < 1 || t->zchunk < 1) return CUMC_EINVAL;
+    if (t->tile_w != 0 && t->tile_w != 32 && t->tile_w != 64 && t->tile_w != 128) return CUMC_EINVAL;
+    ctx->tile_w = t->tile_w;
     ctx->seg = t->seg;
     ctx->zchunk = t->zchunk;
     return CUMC_OK;
@@ -485,7 +488,7 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
     double dts[2];
     double *u0 = ctx->F64.U0, *u1 = ctx->F64.U1;
     const bool valid = ctx->dt_valid, prof = ctx->prof;
-    const int seg0 = ctx->seg, zc0 = ctx->zchunk;
+    const int seg0 = ctx->seg, zc0 = ctx->zchunk, tw0 = ctx->tile_w;
     hipMemcpyAsync(save, ctx->pool, ctx->pool_bytes, hipMemcpyDeviceToDevice, ctx->stream);
     hipMemcpyAsync(dts, ctx->dt_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     int rc = CUMC_OK;
@@ -504,7 +507,7 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
         return CUMC_OK;
     };
     double cls[KC_N];
-    int best_seg = seg0, best_zc = zc0;
+    int best_seg = seg0, best_zc = zc0, best_tw = tw0;
     if (rc == CUMC_OK) rc = timed_step(cls);  // warm-up
     if (rc == CUMC_OK) {
         const int segs[] = {16, 24, 32, 48, 64, 96, 128};
@@ -526,6 +529,17 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
             const double v = cls[KC_ECORN] + cls[KC_TAIL];
             if (v < bt) { bt = v; best_zc = z; }
         }
+        ctx->zchunk = best_zc;
+    }
+    if (rc == CUMC_OK) {
+        const int tws[] = {0, 32, 64, 128};
+        double bt = 1e300;
+        for (int w : tws) {
+            ctx->tile_w = w;
+            if ((rc = timed_step(cls))) break;
+            const double v = cls[KC_ECORN] + cls[KC_TAIL];
+            if (v < bt) { bt = v; best_tw = w; }
+        }
     }
     hipMemcpyAsync(ctx->pool, save, ctx->pool_bytes, hipMemcpyDeviceToDevice, ctx->stream);
     hipMemcpyAsync(ctx->dt_dev, dts, 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
@@ -545,6 +559,7 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
     if (rc == CUMC_OK && e != hipSuccess) rc = fail(ctx, CUMC_EHIP, std::string("tuner restore: ") + hipGetErrorString(e));
     ctx->seg = rc == CUMC_OK ? best_seg : seg0;
     ctx->zchunk = rc == CUMC_OK ? best_zc : zc0;
+    ctx->tile_w = rc == CUMC_OK ? best_tw : tw0;
     if (rc == CUMC_OK && out) cumc_get_tuning(ctx, out);
     return rc;
 }

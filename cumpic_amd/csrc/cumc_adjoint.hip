@@ -844,9 +844,10 @@ static void recompute_flux_emf(cumc_ctx *ctx)
 {
     const Lay &L = ctx->L;
     launch_fluxes<double>(ctx, ctx->F64);
-    const int zc = ctx->zchunk;
-    const int gx = (int)nblk((long)L.px * (L.ny + 1), 256), gy = (L.nz + zc) / zc;
-    hipLaunchKernelGGL((k_ecorn<double>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, ctx->F64, gx, gy, zc);
+    const int zc = ctx->zchunk, tw = ctx->tile_w;
+    const int gx = col_tiles(L, tw), gy = (L.nz + zc) / zc;
+    hipLaunchKernelGGL((k_ecorn<double>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, ctx->F64, gx, gy, zc,
+                       tw);
 }
 
 template <int DIR>

@@ -33,7 +33,7 @@ struct BoxDesc {
     int fx_gx;             // x sweep: tiles per plane
     int fy_gx, fy_sl;      // y sweep: tiles per segment row, faces per segment
     int fz_gx, fz_sl;      // z sweep
-    int c_gx, zc;          // EMF / stage tail column tiles
+    int c_gx, zc, tw;      // EMF / stage tail column tiles
 };
 
 enum { P_FX, P_FY, P_FZ, P_EC, P_TAIL, P_GX, P_GY, P_GZ, P_DT, P_N };
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void kb_ecorn(const BoxDesc *__restrict__ D, c
         const int b = box_of(pre, nbox, it);
         const int loc = it - pre[b];
         const BoxDesc &d = D[b];
-        ecorn_tile<double>(d.L, d.F, loc % d.c_gx, loc / d.c_gx, d.zc);
+        ecorn_tile<double>(d.L, d.F, loc % d.c_gx, loc / d.c_gx, d.zc, d.tw);
     }
 }
 
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void kb_tail(const BoxDesc *__restrict__ D, co
         const double *Uin = STAGE == 1 ? F.U0 : (STAGE == 2 ? F.U1 : d.Ub);
         double *Uout = STAGE == 2 ? d.Ub : F.U1;
         tail_tile<double, STAGE, COORD>(d.L, F, d.tb, d.M, d.dt, Uin, Uout, (unsigned long long *)(d.dt + 1),
-                                        loc % d.c_gx, loc / d.c_gx, d.zc);
+                                        loc % d.c_gx, loc / d.c_gx, d.zc, d.tw);
     }
 }
 
@@ -260,7 +260,8 @@ static void box_tiles(const cumc_batch *b, const cumc_ctx *c, BoxDesc &d, int cn
     d.fz_gx = (int)nblk((long)L.px * (L.ny + 2), 128);
     cnt[P_FZ] = d.fz_gx * nsz;
     d.zc = zc;
-    d.c_gx = (int)nblk((long)L.px * (L.ny + 1), 256);
+    d.tw = b->tune.tile_w;
+    d.c_gx = col_tiles(L, d.tw);
     cnt[P_EC] = cnt[P_TAIL] = d.c_gx * ((L.nz + zc) / zc);
     cnt[P_GX] = (int)nblk(ghost_threads<0>(L), 256);
     cnt[P_GY] = (int)nblk(ghost_threads<1>(L), 256);
@@ -567,7 +568,8 @@ extern "C" int cumc_batch_get_tuning(cumc_batch *b, cumc_tuning *out)
 extern "C" int cumc_batch_set_tuning(cumc_batch *b, const cumc_tuning *t)
 {
     if (!b || !t) return CUMC_EINVAL;
-    if (t->seg < 1 || t->zchunk < 1) return CUMC_EINVAL;
+    if (t->seg < 1 || t->zchunk < 1 || (t->tile_w != 0 && t->tile_w != 32 && t->tile_w != 64 && t->tile_w != 128))
+        return CUMC_EINVAL;
     for (int p = 0; p < CUMC_PH_N; p++)
         if (t->blocks[p] < 0) return CUMC_EINVAL;
     clear_graphs(b);
@@ -652,7 +654,24 @@ extern "C" int cumc_batch_autotune(cumc_batch *b, cumc_tuning *out)
             }
         }
     }
-    // 2) persistent workgroups per phase
+    // 2) column tile shape of the EMF / tail kernels
+    if (rc == CUMC_OK) {
+        const int tws[] = {0, 32, 64};
+        double bestt = 1e300;
+        for (int w : tws) {
+            cumc_tuning t = best;
+            t.tile_w = w;
+            for (int p = 0; p < CUMC_PH_N; p++) t.blocks[p] = 0;
+            double ph[P_N];
+            if ((rc = time_candidate(b, t, ph))) break;
+            const double v = ph[P_EC] + ph[P_TAIL];
+            if (v < bestt) {
+                bestt = v;
+                best.tile_w = w;
+            }
+        }
+    }
+    // 3) persistent workgroups per phase
     if (rc == CUMC_OK) {
         const int cands[] = {0, 256, 512, 1024, 2048, 4096};
         double bt[P_N];

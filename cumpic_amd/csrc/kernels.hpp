@@ -297,6 +297,29 @@ __device__ inline bool xcd_tile(int gx, int gy, int &bx, int &by)
     return true;
 }
 
+// Column (j,k) of this thread in column tile bx of the [0,nx] x [0,ny] column set.  tw = 0 takes
+// 256 consecutive cells of the (j,k) plane of pitch px (about one row per tile); tw > 0 takes a
+// tw x (256/tw) block of columns, so most of the k-1 / k+1 rows a column reads are loaded by the
+// same workgroup.  Column kernels run 256-thread workgroups.
+__device__ inline void col_of(const Lay &L, int bx, int tw, int &j, int &k)
+{
+    if (tw == 0) {
+        const long t = (long)bx * 256 + threadIdx.x;
+        k = (int)(t / L.px);
+        j = (int)(t - (long)k * L.px);
+    } else {
+        const int ntx = (L.nx + tw) / tw;
+        const int tx = bx % ntx, ty = bx / ntx;
+        j = tx * tw + (int)(threadIdx.x % tw);
+        k = ty * (256 / tw) + (int)(threadIdx.x / tw);
+    }
+}
+__host__ __device__ inline int col_tiles(const Lay &L, int tw)
+{
+    if (tw == 0) return (int)((L.px * (L.ny + 1) + 255) / 256);
+    return ((L.nx + tw) / tw) * ((L.ny + 256 / tw) / (256 / tw));
+}
+
 template <typename R>
 __device__ inline void ecells(const Lay &L, const Fields<R> &F, long g, R &ex, R &ey, R &ez)
 {
@@ -311,11 +334,10 @@ __device__ inline void ecells(const Lay &L, const Fields<R> &F, long g, R &ex, R
 // Column tile (bx, by): 256 (j,k) columns of the flattened [0,nx]x[0,ny] plane (pitch px),
 // planes [by*zc, by*zc+zc-1] of [0,nz].
 template <typename R>
-__device__ inline void ecorn_tile(const Lay &L, const Fields<R> &F, int bx, int by, int zc)
+__device__ inline void ecorn_tile(const Lay &L, const Fields<R> &F, int bx, int by, int zc, int tw)
 {
-    const long t = (long)bx * blockDim.x + threadIdx.x;
-    const int k = (int)(t / L.px);
-    const int j = (int)(t - (long)k * L.px);
+    int j, k;
+    col_of(L, bx, tw, j, k);
     const int l0 = by * zc;
     if (k > L.ny || j > L.nx || l0 > L.nz) return;
     const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
@@ -363,11 +385,11 @@ __device__ inline void ecorn_tile(const Lay &L, const Fields<R> &F, int bx, int 
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_ecorn(Lay L, Fields<R> F, int gx, int gy, int zc)
+__global__ __launch_bounds__(256) void k_ecorn(Lay L, Fields<R> F, int gx, int gy, int zc, int tw)
 {
     int bx, by;
     if (!xcd_tile(gx, gy, bx, by)) return;
-    ecorn_tile<R>(L, F, bx, by, zc);
+    ecorn_tile<R>(L, F, bx, by, zc, tw);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -457,11 +479,10 @@ __device__ inline double block_min(double v)
 template <typename R, int STAGE, int COORD>
 __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, const double *dtp,
                                  const double *__restrict__ Uin, double *__restrict__ Uout,
-                                 unsigned long long *dtnext, int bx, int by, int zc)
+                                 unsigned long long *dtnext, int bx, int by, int zc, int tw)
 {
-    const long t = (long)bx * blockDim.x + threadIdx.x;
-    const int k = (int)(t / L.px);
-    const int j = (int)(t - (long)k * L.px);
+    int j, k;
+    col_of(L, bx, tw, j, k);
     const int l0 = by * zc;
     const bool col = k <= L.ny && j <= L.nx && l0 <= L.nz;
     double dtmin = 1.0e10;
@@ -632,11 +653,10 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
 template <typename R, int STAGE, int COORD>
 __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, const double *dtp,
                                  const double *__restrict__ Uin, double *__restrict__ Uout,
-                                 unsigned long long *dtnext, int bx, int by, int zc)
+                                 unsigned long long *dtnext, int bx, int by, int zc, int tw)
 {
-    const long t = (long)bx * blockDim.x + threadIdx.x;
-    const int k = (int)(t / L.px);
-    const int j = (int)(t - (long)k * L.px);
+    int j, k;
+    col_of(L, bx, tw, j, k);
     const int l0 = by * zc;
     const bool col = k <= L.ny && j <= L.nx && l0 <= L.nz;
     double dtmin = 1.0e10;
@@ -803,12 +823,12 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
 template <typename R, int STAGE, int COORD>
 __global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model M, const double *dtp,
                                               const double *__restrict__ Uin, double *__restrict__ Uout,
-                                              unsigned long long *dtnext, int gx, int gy, int zc)
+                                              unsigned long long *dtnext, int gx, int gy, int zc, int tw)
 {
     int bx, by;
     if (!xcd_tile(gx, gy, bx, by)) return;
-    if constexpr (sizeof(R) == 4) tail_tile_wide<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc);
-    else tail_tile<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc);
+    if constexpr (sizeof(R) == 4) tail_tile_wide<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
+    else tail_tile<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
 }
 
 // initial_update's FROMRVETOU + CUSTOM_CHECKRHO + EOS on [0,n]^3 (initial.f90:103-126)

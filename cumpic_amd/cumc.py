@@ -35,10 +35,12 @@ PHASES = ("flux_x", "flux_y", "flux_z", "ecorn", "tail", "ghost", "finddt")
 
 class Tuning(ctypes.Structure):
     """cumc_tuning: launch shapes (segment length, plane chunk, persistent workgroups per phase)."""
-    _fields_ = [("seg", ctypes.c_int), ("zchunk", ctypes.c_int), ("blocks", ctypes.c_int * len(PHASES))]
+    _fields_ = [("seg", ctypes.c_int), ("zchunk", ctypes.c_int), ("blocks", ctypes.c_int * len(PHASES)),
+                ("tile_w", ctypes.c_int)]
 
     def as_dict(self):
-        return {"seg": self.seg, "zchunk": self.zchunk, "blocks": dict(zip(PHASES, list(self.blocks)))}
+        return {"seg": self.seg, "zchunk": self.zchunk, "blocks": dict(zip(PHASES, list(self.blocks))),
+                "tile_w": self.tile_w}
 
     @classmethod
     def from_dict(cls, d):
@@ -48,6 +50,7 @@ class Tuning(ctypes.Structure):
         b = d.get("blocks", {})
         for i, name in enumerate(PHASES):
             t.blocks[i] = int(b.get(name, 0))
+        t.tile_w = int(d.get("tile_w", 0))
         return t
 
 

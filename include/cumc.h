@@ -120,6 +120,8 @@ typedef struct cumc_tuning {
     int seg;                 /* target faces per pencil segment of the y and z sweeps */
     int zchunk;              /* planes per column tile of the EMF and stage-tail kernels */
     int blocks[CUMC_PH_N];   /* batch only: persistent workgroups per phase, 0 = one per tile */
+    int tile_w;              /* column tile of the EMF / stage-tail kernels: 0 = flattened rows of 256
+                                cells, or 32 / 64 / 128 = tile_w x (256 / tile_w) columns */
 } cumc_tuning;
 
 /* Time candidate shapes on the context's current state (which is saved and restored) and keep

@@ -64,7 +64,7 @@ def test_tuning_struct_layout_matches_header():
     txt = open(os.path.join(ROOT, "include", "cumc.h")).read()
     enum = re.search(r"enum \{ CUMC_PH_FLUX_X = 0,([^}]*)\}", txt).group(0)
     assert enum.count("CUMC_PH_") - 1 == len(PHASES)  # CUMC_PH_N closes the list
-    assert ctypes.sizeof(Tuning) == 4 * (2 + len(PHASES))
+    assert ctypes.sizeof(Tuning) == 4 * (3 + len(PHASES))
     t = Tuning.from_dict({"seg": 5, "zchunk": 6, "blocks": {"tail": 9}})
     assert t.as_dict()["blocks"]["tail"] == 9 and t.as_dict()["seg"] == 5
 

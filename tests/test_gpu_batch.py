@@ -119,8 +119,8 @@ def test_batch_autotune_keeps_state_and_results():
     for i, (x, y) in enumerate(zip(boxes, singles)):
         _same_state(x, y, f"tuned box {i}")
     # explicit shapes round-trip and still give the same results
-    b.set_tuning({"seg": 7, "zchunk": 3, "blocks": {"flux_x": 5, "flux_y": 3, "flux_z": 2, "ecorn": 4,
-                                                    "tail": 6, "ghost": 1, "finddt": 2}})
+    b.set_tuning({"seg": 7, "zchunk": 3, "tile_w": 32, "blocks": {"flux_x": 5, "flux_y": 3, "flux_z": 2,
+                                                                  "ecorn": 4, "tail": 6, "ghost": 1, "finddt": 2}})
     assert b.tuning()["seg"] == 7
     dts = b.run(2, graph_steps=0)
     ref = np.array([s.run(2)[0] for s in singles]).T
@@ -142,7 +142,17 @@ def test_single_autotune_keeps_state_and_results():
     after = a.download()
     for k in ("prim", "eps", "bcell"):
         assert_bitwise(after[k], before[k], f"autotune changed {k}")
-    assert t["seg"] >= 1 and t["zchunk"] >= 1
+    assert t["seg"] >= 1 and t["zchunk"] >= 1 and t["tile_w"] in (0, 32, 64, 128)
+    da, _ = a.run(3)
+    dc, _ = c.run(3)
+    assert_bitwise(da, dc, "dt after autotune")
+    _same_state(a, c, "after autotune")
+    # every column tile shape gives the same bits
+    for w in (32, 64, 128):
+        a.set_tuning({"seg": 16, "zchunk": 5, "tile_w": w})
+        a.run(1)
+        c.run(1)
+        _same_state(a, c, f"tile_w {w}")
     da, _ = a.run(3)
     dc, _ = c.run(3)
     assert_bitwise(da, dc, "dt after autotune")
