@@ -96,18 +96,21 @@ def cpu_baseline(prob, steps=1):
     }, state
 
 
-def timed_run(Solver, prob, prec, steps, warmup):
+def timed_run(Solver, prob, prec, steps, warmup, tune=True):
+    """Upload, initial_update, warm-up steps, then (tune) the library's launch-shape tuner, which
+    leaves the state as it found it, then the timed steps."""
     s = Solver.from_problem(prob, precision=prec)
     s.initial_update()
     if warmup:
         s.run(warmup)
+    tuning = s.autotune() if tune else s.tuning()
     s.profile(True)
     t0 = time.perf_counter()
     dts, _ = s.run(steps)  # synchronous on return (hipStreamSynchronize)
     wall = time.perf_counter() - t0
     kt = s.kernel_times()
     s.profile(False)
-    return s, dts, wall, kt
+    return s, dts, wall, kt, tuning
 
 
 def roofline(kt, cfg, tag):
@@ -264,7 +267,7 @@ def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-5, 1
             fd = dj / (2 * h)
             rel[h].append(abs(fd - ad) / max(abs(fd), abs(ad)))
     summary = {f"{h:.0e}": {"median": float(np.median(v)), "max": float(np.max(v)),
-                            "within_1e-6": int(np.sum(np.array(v) <= 1e-6))} for h, v in rel.items()}
+                            "within_1e-6": int(np.sum(np.array(v) <= 1e-6))} for h, v in rel.items() if v}
     log(f"[bench] adjoint vs central differences (relative, {ndir} unit directions): {summary}")
     return {"objective": "per box sum |B|^2/2 dV after the steps; gradient w.r.t. initial velocity",
             "steps": steps, "checkpoint_every": every, "forward_ms": fwd_ms, "backward_ms": back_ms,
@@ -285,8 +288,10 @@ def main():
     ap.add_argument("--no-error", action="store_true", help="skip the mixed-vs-FP64 error replay")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D-box workload")
     ap.add_argument("--fd-dirs", type=int, default=32, help="directions of the reverse-mode difference check")
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="also run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--copy-probe-mib", type=int, default=1024,
+                    help="plain device copy of this many MiB: the achievable HBM rate, and the counter "
+                         "calibration (0 skips it)")
+    ap.add_argument("--no-tune", action="store_true", help="time the default launch shapes (no autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -350,13 +355,15 @@ def main():
     for form in forms:
         prec = FP64 if form == "fp64" else MIXED
         log(f"[bench] {form}: upload, initial_update, {args.warmup} warm-up, {steps} timed steps")
-        s, dts, wall, kt = timed_run(Solver, prob, prec, steps, args.warmup)
+        s, dts, wall, kt, tuning = timed_run(Solver, prob, prec, steps, args.warmup, tune=not args.no_tune)
         dsig, dabs = s.find_divb()
         st = s.download()
         finite = bool(np.isfinite(st["prim"]).all())
         r = {"value": cfg.zones * steps / wall, "ms_per_step": wall / steps * 1e3, "roofline": roofline(kt, cfg, form),
              "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()}, "divb_max_abs": dabs,
-             "state_finite": finite}
+             "state_finite": finite, "tuning": tuning}
+        if copy_gbps:
+            r["roofline"]["frac_of_copy_probe"] = r["roofline"]["achieved"] / copy_gbps
         s.close()
         if form == "fp64":
             ref_dts = dts

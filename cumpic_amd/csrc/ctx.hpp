@@ -30,8 +30,8 @@ struct cumc_ctx {
     bool dt_valid = false;              // dt_dev[1] holds finddt of the current state
     bool grid_set = false;
     struct cumc_tape *tape = nullptr;  // reverse-mode record (cumc_adjoint.hip)
-    int seg = 48;     // target faces per pencil segment of the y / z sweeps
-    int zchunk = 32;  // planes per column tile of the EMF and stage-tail kernels
+    int seg = 24;    // target faces per pencil segment of the y / z sweeps
+    int zchunk = 4;  // planes per column tile of the EMF and stage-tail kernels
     int tile_w = 0;   // column tile width of those kernels (0: flattened rows; see col_of)
     double last_ms = 0.0;
     std::string err;

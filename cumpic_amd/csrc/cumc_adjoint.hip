@@ -602,110 +602,149 @@ __global__ __launch_bounds__(64) void kr_flux_recon(Lay L, Fields<double> F, Tab
 }
 
 // ---------------------------------------------------------------------------------------------
-// x sweep reversed, wave-parallel like k_flux_x: the lanes of a wavefront hold 64 consecutive
-// cells of the flattened (j,k) plane; each lane recomputes its cell's reconstruction (neighbours
-// by shuffles), reverses the HLL solve of its face, then the edge values and slope of its cell,
-// and gathers the adjoint of its cell value from the +-1 lanes.  Lanes 5..58 own their cells
-// (54 per wave); the others only feed their neighbours.
+// A sweep reversed, wave-parallel like k_flux_x: the lanes of a wavefront hold 64 consecutive
+// cells of the sweep's pencils (one outer index: plane l for the x and y sweeps, row k for the z
+// sweep); each lane recomputes its cell's reconstruction (neighbours by shuffles), reverses the
+// HLL solve of its face, then the edge values and slope of its cell, and gathers the adjoint of
+// its cell value from the +-1 lanes.  Lanes 5..58 own their cells (54 per wave); the others only
+// feed their neighbours.  Pencils are laid end to end with the sweep axis fastest: x pencils are
+// the rows k in [0, ny+1] at pitch px (row padding included), y pencils the columns j in
+// [0, nx+1] of ny+6 cells, z pencils the columns j in [0, nx+1] of nz+6 cells.  Along y and z the
+// lanes of one load are a row or plane apart; the neighbouring pencils that share those cache
+// lines run in the neighbouring tiles.
 // ---------------------------------------------------------------------------------------------
 constexpr int XR_OWN = 54;
+// pencils shorter than this (n < LANE_MIN) are reversed by the per-pencil walks instead
+constexpr int LANE_MIN = 8;
 
-template <int RECON>
-__device__ inline void kr_flux_x_tile(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M,
-                                      const AdjF &A, int bx, int l)
+template <int DIR>
+__host__ __device__ inline long lane_cells(const Lay &L)
+{
+    if (DIR == 1) return (long)(L.ny + 2) * L.px;
+    if (DIR == 2) return (long)(L.nx + 2) * (L.ny + 6);
+    return (long)(L.nx + 2) * (L.nz + 6);
+}
+template <int DIR>
+__host__ __device__ inline int lane_outer(const Lay &L)
+{
+    return DIR == 3 ? L.ny + 2 : L.nz + 2;
+}
+// 256-thread tiles (4 waves of XR_OWN owned cells) per outer index
+template <int DIR>
+__host__ __device__ inline int lane_tiles(const Lay &L)
+{
+    const long waves = (lane_cells<DIR>(L) + XR_OWN - 1) / XR_OWN;
+    return (int)((waves + 3) / 4);
+}
+template <int DIR>
+__host__ __device__ inline bool lane_sweep(const Lay &L)
+{
+    return DIR == 1 || (DIR == 2 ? L.ny : L.nz) >= LANE_MIN;
+}
+
+template <int DIR, int RECON>
+__device__ inline void kr_flux_lane_tile(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M,
+                                         const AdjF &A, int bx, int o)
 {
     const int lane = threadIdx.x & 63;
     const long wave = ((long)bx * blockDim.x + threadIdx.x) >> 6;
-    const long f = wave * XR_OWN + lane - 5;  // flattened (j+2) + k*px over rows k in [0, ny+1]
-    const long nload = (long)(L.ny + 2) * L.px;
-    const bool inr = (f >= 0) && (f < nload);
+    const long f = wave * XR_OWN + lane - 5;
+    const bool inr = (f >= 0) && (f < lane_cells<DIR>(L));
+    const long fc = inr ? f : 0;
+    const int ax = DIR - 1;
+    const int n = DIR == 1 ? L.nx : (DIR == 2 ? L.ny : L.nz);
+    int c;  // cell index along the sweep
+    unsigned g;
+    if (DIR == 1) {
+        const int k = (int)(fc / L.px);
+        c = (int)(fc - (long)k * L.px) - 2;
+        g = (unsigned)((long)(o + 2) * L.py + 2 * L.px + fc);
+    } else {
+        const int pitch = n + 6;
+        const int j = (int)(fc / pitch);
+        c = (int)(fc - (long)j * pitch) - 2;
+        g = (unsigned)(DIR == 2 ? L.id(j, c, o) : L.id(j, o, c));
+    }
+    if (!inr) c = -100;
+    const int cc = c < -2 ? -2 : (c > n + 3 ? n + 3 : c);
     const long ps = L.ps;
-    const unsigned g = (unsigned)((long)(l + 2) * L.py + 2 * L.px + (inr ? f : 0));
-    const int k = inr ? (int)(f / L.px) : 0;
-    const int j = inr ? (int)(f - (long)k * L.px) - 2 : -100;
-    const int jc = j < -2 ? -2 : (j > L.nx + 3 ? L.nx + 3 : j);
-    const bool own = lane >= 5 && lane <= 58 && inr && j <= L.nx + 3;
-    // a face exists for j in [0, nx]; lanes 2..60 have the full forward stencil
-    const bool face = inr && j >= 0 && j <= L.nx && lane >= 2 && lane <= 60;
-    const double *qp[7] = {F.P, F.P + ps, F.P + 2 * ps, F.P + 3 * ps, F.P + 4 * ps, F.BC + ps, F.BC + 2 * ps};
+    const bool own = lane >= 5 && lane <= 58 && inr && c <= n + 3;
+    // a face exists for c in [0, n]; lanes 2..60 have the full forward stencil
+    const bool face = inr && c >= 0 && c <= n && lane >= 2 && lane <= 60;
     double q[7], dm[7], vp[7], vmr[7];
 #pragma unroll
-    for (int v = 0; v < 7; v++) q[v] = ldi(qp[v], g);
-    // forward: edge values of this cell; the right state of face j is vm of lane+1
+    for (int v = 0; v < 7; v++) q[v] = ldi(qplane<DIR>(F, ps, v), g);
+    // forward: edge values of this cell; the right state of face c is vm of lane+1
 #pragma unroll
     for (int v = 0; v < 7; v++) {
-        const double c = q[v], cm1 = __shfl_up(c, 1, 64), cp1 = __shfl_down(c, 1, 64);
+        const double w = q[v], wm1 = __shfl_up(w, 1, 64), wp1 = __shfl_down(w, 1, 64);
         double vm, vpv;
         if (RECON == 4) {
-            dm[v] = ppm_slope<double>(tb.at(0, TB_B0C, jc), tb.at(0, TB_B1C, jc), cm1, c, cp1);
+            dm[v] = ppm_slope<double>(tb.at(ax, TB_B0C, cc), tb.at(ax, TB_B1C, cc), wm1, w, wp1);
             const double dmm1 = __shfl_up(dm[v], 1, 64), dmp1 = __shfl_down(dm[v], 1, 64);
-            ppm_edges<double>(tb.at(0, TB_A0M, jc), tb.at(0, TB_A1M, jc), tb.at(0, TB_A2M, jc), tb.at(0, TB_A3M, jc),
-                              tb.at(0, TB_A0P, jc), tb.at(0, TB_A1P, jc), tb.at(0, TB_A2P, jc), tb.at(0, TB_A3P, jc),
-                              cm1, c, cp1, dmm1, dm[v], dmp1, vm, vpv);
+            ppm_edges<double>(tb.at(ax, TB_A0M, cc), tb.at(ax, TB_A1M, cc), tb.at(ax, TB_A2M, cc),
+                              tb.at(ax, TB_A3M, cc), tb.at(ax, TB_A0P, cc), tb.at(ax, TB_A1P, cc),
+                              tb.at(ax, TB_A2P, cc), tb.at(ax, TB_A3P, cc), wm1, w, wp1, dmm1, dm[v], dmp1, vm, vpv);
         } else {
-            tvd_edges<double>(tb.at(0, TB_CF, jc), tb.at(0, TB_CB, jc), cm1, c, cp1, vm, vpv);
+            tvd_edges<double>(tb.at(ax, TB_CF, cc), tb.at(ax, TB_CB, cc), wm1, w, wp1, vm, vpv);
         }
         vp[v] = vpv;
         vmr[v] = __shfl_down(vm, 1, 64);
     }
-    // HLL reversed at face j
+    // HLL reversed at face c
     double lL[7] = {0, 0, 0, 0, 0, 0, 0}, lR[7] = {0, 0, 0, 0, 0, 0, 0};
     if (face) {
-        const double bn = ldi(F.P + 5 * ps, g);
-        double lf[7];
+        const double bn = ldi(F.P + (4 + DIR) * ps, g);
+        double o7[7], lf[7];
 #pragma unroll
-        for (int i = 0; i < 7; i++) lf[i] = ldi(A.F[0] + i * ps, g);
+        for (int i = 0; i < 7; i++) o7[i] = ldi(A.F[ax] + i * ps, g);
+        flux_to_rot<DIR>(o7, lf);
         FaceState<double> a{0, 0, 0, 0, 0, 0, 0, 0}, b{0, 0, 0, 0, 0, 0, 0, 0};
-        hll_face_adj(M.ggas, rot<1>(vp, bn), rot<1>(vmr, bn), lf, a, b);
-        unrot<1>(a, lL);
-        unrot<1>(b, lR);
-        if (own) sti(A.P + 5 * ps, g, ldi(A.P + 5 * ps, g) + (a.bn + b.bn));
+        hll_face_adj(M.ggas, rot<DIR>(vp, bn), rot<DIR>(vmr, bn), lf, a, b);
+        unrot<DIR>(a, lL);
+        unrot<DIR>(b, lR);
+        double *pb = A.P + (4 + DIR) * ps;
+        if (own) sti(pb, g, ldi(pb, g) + (a.bn + b.bn));
     }
     // reconstruction reversed, one variable at a time
-    double *lp[7] = {A.P, A.P + ps, A.P + 2 * ps, A.P + 3 * ps, A.P + 4 * ps, A.BC + ps, A.BC + 2 * ps};
 #pragma unroll
     for (int v = 0; v < 7; v++) {
         const double lvp = lL[v];
-        const double lvm = __shfl_up(lR[v], 1, 64);  // from face j-1
-        const double c = q[v], cm1 = __shfl_up(c, 1, 64), cp1 = __shfl_down(c, 1, 64);
+        const double lvm = __shfl_up(lR[v], 1, 64);  // from face c-1
+        const double w = q[v], wm1 = __shfl_up(w, 1, 64), wp1 = __shfl_down(w, 1, 64);
         double am1 = 0.0, a0 = 0.0, ap1 = 0.0, em1 = 0.0, e0 = 0.0, ep1 = 0.0;
         double lw;
         // only cells 0..n+1 are reconstructed and only slopes -1..n+2 are used; elsewhere the
         // table entries are not defined and the forward values there are never read
-        const bool recon_cell = j >= 0 && j <= L.nx + 1;
+        const bool recon_cell = c >= 0 && c <= n + 1;
         if (RECON == 4) {
             const double dmm1 = __shfl_up(dm[v], 1, 64), dmp1 = __shfl_down(dm[v], 1, 64);
             if (recon_cell)
-                ppm_edges_adj(tb.at(0, TB_A0M, jc), tb.at(0, TB_A1M, jc), tb.at(0, TB_A2M, jc), tb.at(0, TB_A3M, jc),
-                              tb.at(0, TB_A0P, jc), tb.at(0, TB_A1P, jc), tb.at(0, TB_A2P, jc), tb.at(0, TB_A3P, jc),
-                              cm1, c, cp1, dmm1, dm[v], dmp1, lvm, lvp, am1, a0, ap1, em1, e0, ep1);
+                ppm_edges_adj(tb.at(ax, TB_A0M, cc), tb.at(ax, TB_A1M, cc), tb.at(ax, TB_A2M, cc),
+                              tb.at(ax, TB_A3M, cc), tb.at(ax, TB_A0P, cc), tb.at(ax, TB_A1P, cc),
+                              tb.at(ax, TB_A2P, cc), tb.at(ax, TB_A3P, cc), wm1, w, wp1, dmm1, dm[v], dmp1, lvm, lvp,
+                              am1, a0, ap1, em1, e0, ep1);
             // slope of this cell: its adjoint from the three cells that used it
             const double ldm = e0 + __shfl_down(em1, 1, 64) + __shfl_up(ep1, 1, 64);
             double bm1 = 0.0, b0 = 0.0, bp1 = 0.0;
-            if (j >= -1 && j <= L.nx + 2)
-                ppm_slope_adj(tb.at(0, TB_B0C, jc), tb.at(0, TB_B1C, jc), cm1, c, cp1, ldm, bm1, b0, bp1);
+            if (c >= -1 && c <= n + 2)
+                ppm_slope_adj(tb.at(ax, TB_B0C, cc), tb.at(ax, TB_B1C, cc), wm1, w, wp1, ldm, bm1, b0, bp1);
             lw = a0 + __shfl_down(am1, 1, 64) + __shfl_up(ap1, 1, 64) + b0 + __shfl_down(bm1, 1, 64) +
                  __shfl_up(bp1, 1, 64);
         } else {
             if (recon_cell)
-                tvd_edges_adj(tb.at(0, TB_CF, jc), tb.at(0, TB_CB, jc), cm1, c, cp1, lvm, lvp, am1, a0, ap1);
+                tvd_edges_adj(tb.at(ax, TB_CF, cc), tb.at(ax, TB_CB, cc), wm1, w, wp1, lvm, lvp, am1, a0, ap1);
             lw = a0 + __shfl_down(am1, 1, 64) + __shfl_up(ap1, 1, 64);
         }
-        if (own) sti(lp[v], g, ldi(lp[v], g) + lw);
+        double *lp = qplane_adj<DIR>(A, ps, v);
+        if (own) sti(lp, g, ldi(lp, g) + lw);
     }
 }
 
-template <int RECON>
-__global__ __launch_bounds__(256) void kr_flux_x(Lay L, Fields<double> F, Tabs tb, Model M, AdjF A)
+template <int DIR, int RECON>
+__global__ __launch_bounds__(256) void kr_flux_lane(Lay L, Fields<double> F, Tabs tb, Model M, AdjF A)
 {
-    kr_flux_x_tile<RECON>(L, F, tb, M, A, blockIdx.x, blockIdx.y);
-}
-
-// tiles per plane of kr_flux_x (256 threads = 4 waves of XR_OWN owned cells)
-__host__ __device__ inline int xrev_tiles(const Lay &L)
-{
-    const long waves = ((long)(L.ny + 2) * L.px + XR_OWN - 1) / XR_OWN;
-    return (int)((waves + 3) / 4);
+    kr_flux_lane_tile<DIR, RECON>(L, F, tb, M, A, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -855,19 +894,22 @@ static void reverse_sweep(cumc_ctx *ctx, const AdjF &A)
 {
     const Lay &L = ctx->L;
     Tabs tb = tabs_of(ctx);
-    if (DIR == 1) {
-        dim3 grid(xrev_tiles(L), L.nz + 2);
-        if (ctx->M.recon == CUMC_PPMC) hipLaunchKernelGGL(kr_flux_x<4>, grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M, A);
-        else hipLaunchKernelGGL(kr_flux_x<1>, grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M, A);
+    hipStream_t s = ctx->stream;
+    if (lane_sweep<DIR>(L)) {
+        dim3 grid(lane_tiles<DIR>(L), lane_outer<DIR>(L));
+        if (ctx->M.recon == CUMC_PPMC) hipLaunchKernelGGL((kr_flux_lane<DIR, 4>), grid, dim3(256), 0, s, L, ctx->F64, tb, ctx->M, A);
+        else hipLaunchKernelGGL((kr_flux_lane<DIR, 1>), grid, dim3(256), 0, s, L, ctx->F64, tb, ctx->M, A);
         return;
     }
-    const long np = npencils<DIR>(L);
-    if (ctx->M.recon == CUMC_PPMC) {
-        hipLaunchKernelGGL((kr_flux_hll<DIR, 4>), dim3(nblk(np, 64)), dim3(64), 0, ctx->stream, L, ctx->F64, tb, ctx->M, A);
-        hipLaunchKernelGGL((kr_flux_recon<DIR, 4>), dim3(nblk(np, 64), 7), dim3(64), 0, ctx->stream, L, ctx->F64, tb, A);
-    } else {
-        hipLaunchKernelGGL((kr_flux_hll<DIR, 1>), dim3(nblk(np, 64)), dim3(64), 0, ctx->stream, L, ctx->F64, tb, ctx->M, A);
-        hipLaunchKernelGGL((kr_flux_recon<DIR, 1>), dim3(nblk(np, 64), 7), dim3(64), 0, ctx->stream, L, ctx->F64, tb, A);
+    if constexpr (DIR != 1) {
+        const long np = npencils<DIR>(L);
+        if (ctx->M.recon == CUMC_PPMC) {
+            hipLaunchKernelGGL((kr_flux_hll<DIR, 4>), dim3(nblk(np, 64)), dim3(64), 0, s, L, ctx->F64, tb, ctx->M, A);
+            hipLaunchKernelGGL((kr_flux_recon<DIR, 4>), dim3(nblk(np, 64), 7), dim3(64), 0, s, L, ctx->F64, tb, A);
+        } else {
+            hipLaunchKernelGGL((kr_flux_hll<DIR, 1>), dim3(nblk(np, 64)), dim3(64), 0, s, L, ctx->F64, tb, ctx->M, A);
+            hipLaunchKernelGGL((kr_flux_recon<DIR, 1>), dim3(nblk(np, 64), 7), dim3(64), 0, s, L, ctx->F64, tb, A);
+        }
     }
 }
 
@@ -1123,10 +1165,10 @@ struct AdjBox {
     double *seg, *stA, *stB, *ckpt, *init;
     int gx_c, gx_e;  // 256-cell tiles per plane row set: [0,n]^2 and [0,n+1]^2 column domains
     int gpen[3];     // 64-pencil tiles per sweep
-    int gxr;         // kr_flux_x tiles per plane
+    int glane[3];    // kr_flux_lane tiles per outer index, per sweep
 };
 
-enum { Q_TAIL, Q_GE, Q_GX, Q_GY, Q_GZ, Q_H1, Q_H2, Q_H3, Q_R1, Q_R2, Q_R3, Q_CP, Q_X, Q_N };
+enum { Q_TAIL, Q_GE, Q_GX, Q_GY, Q_GZ, Q_H1, Q_H2, Q_H3, Q_R1, Q_R2, Q_R3, Q_CP, Q_L1, Q_L2, Q_L3, Q_N };
 constexpr int CP_ELEMS = 1024;  // doubles per copy tile (256 threads x 4)
 
 __device__ inline int adj_box_of(const int *pre, int nbox, int it)
@@ -1242,11 +1284,13 @@ __global__ __launch_bounds__(256) void kbr_init_b(const AdjBox *__restrict__ D, 
     }
 }
 
-template <int RECON>
-__global__ __launch_bounds__(256) void kbr_flux_x(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox)
+template <int DIR, int RECON>
+__global__ __launch_bounds__(256) void kbr_flux_lane(const AdjBox *__restrict__ D, const int *__restrict__ pre,
+                                                     int nbox)
 {
-    FOR_ITEMS(Q_X)
-        kr_flux_x_tile<RECON>(d.L, d.F, d.tb, d.M, d.A, loc % d.gxr, loc / d.gxr);
+    FOR_ITEMS(Q_L1 + DIR - 1)
+        const int gl = d.glane[DIR - 1];
+        kr_flux_lane_tile<DIR, RECON>(d.L, d.F, d.tb, d.M, d.A, loc % gl, loc / gl);
     }
 }
 
@@ -1347,13 +1391,17 @@ static int bt_upload(cumc_batch *b, batch_tape *bt)
         cnt[Q_GX] = nblk((long)(L.ny + 1) * (L.nz + 1), 256);
         cnt[Q_GY] = nblk((long)(L.nx + 6) * (L.nz + 1), 256);
         cnt[Q_GZ] = nblk((long)(L.nx + 6) * (L.ny + 6), 256);
+        const bool lane[3] = {lane_sweep<1>(L), lane_sweep<2>(L), lane_sweep<3>(L)};
+        a.glane[0] = lane_tiles<1>(L);
+        a.glane[1] = lane_tiles<2>(L);
+        a.glane[2] = lane_tiles<3>(L);
+        const int outer[3] = {lane_outer<1>(L), lane_outer<2>(L), lane_outer<3>(L)};
         for (int dd = 0; dd < 3; dd++) {
-            cnt[Q_H1 + dd] = a.gpen[dd];
-            cnt[Q_R1 + dd] = 7L * a.gpen[dd];
+            cnt[Q_H1 + dd] = lane[dd] ? 0 : a.gpen[dd];
+            cnt[Q_R1 + dd] = lane[dd] ? 0 : 7L * a.gpen[dd];
+            cnt[Q_L1 + dd] = lane[dd] ? (long)a.glane[dd] * outer[dd] : 0;
         }
         cnt[Q_CP] = (long)((SW * L.ps + CP_ELEMS - 1) / CP_ELEMS);
-        a.gxr = xrev_tiles(L);
-        cnt[Q_X] = (long)a.gxr * (L.nz + 2);
         for (int q = 0; q < Q_N; q++) {
             const long v = (long)pre[(size_t)q * (nb + 1) + i] + cnt[q];
             if (v > 0x7fffffffL) return fail(c, CUMC_EINVAL, "batched reverse work list exceeds 2^31 tiles");
@@ -1384,17 +1432,24 @@ static void bt_sweep(cumc_batch *b, batch_tape *bt)
 {
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
-    if (DIR == 1) {
-        if (bt->recon == CUMC_PPMC) hipLaunchKernelGGL(kbr_flux_x<4>, bt_grid(bt, Q_X), dim3(256), 0, s, bt->d, bt->pre, nb);
-        else hipLaunchKernelGGL(kbr_flux_x<1>, bt_grid(bt, Q_X), dim3(256), 0, s, bt->d, bt->pre, nb);
-        return;
+    // boxes whose pencils along DIR are long enough take the wave-parallel kernel, the others
+    // the per-pencil walks (each work list is empty when no box takes it)
+    if (bt->total[Q_L1 + DIR - 1] > 0) {
+        if (bt->recon == CUMC_PPMC)
+            hipLaunchKernelGGL((kbr_flux_lane<DIR, 4>), bt_grid(bt, Q_L1 + DIR - 1), dim3(256), 0, s, bt->d, bt->pre, nb);
+        else
+            hipLaunchKernelGGL((kbr_flux_lane<DIR, 1>), bt_grid(bt, Q_L1 + DIR - 1), dim3(256), 0, s, bt->d, bt->pre, nb);
     }
-    if (bt->recon == CUMC_PPMC) {
-        hipLaunchKernelGGL((kbr_flux_hll<DIR, 4>), bt_grid(bt, Q_H1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
-        hipLaunchKernelGGL((kbr_flux_recon<DIR, 4>), bt_grid(bt, Q_R1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
-    } else {
-        hipLaunchKernelGGL((kbr_flux_hll<DIR, 1>), bt_grid(bt, Q_H1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
-        hipLaunchKernelGGL((kbr_flux_recon<DIR, 1>), bt_grid(bt, Q_R1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+    if constexpr (DIR != 1) {
+        if (bt->total[Q_H1 + DIR - 1] > 0) {
+            if (bt->recon == CUMC_PPMC) {
+                hipLaunchKernelGGL((kbr_flux_hll<DIR, 4>), bt_grid(bt, Q_H1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+                hipLaunchKernelGGL((kbr_flux_recon<DIR, 4>), bt_grid(bt, Q_R1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+            } else {
+                hipLaunchKernelGGL((kbr_flux_hll<DIR, 1>), bt_grid(bt, Q_H1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+                hipLaunchKernelGGL((kbr_flux_recon<DIR, 1>), bt_grid(bt, Q_R1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+            }
+        }
     }
 }
 

@@ -246,8 +246,8 @@ static void clear_graphs(cumc_batch *b)
 static void box_tiles(const cumc_batch *b, const cumc_ctx *c, BoxDesc &d, int cnt[P_N])
 {
     const Lay &L = c->L;
-    const int seg = b->tune.seg > 0 ? b->tune.seg : 48;
-    const int zc = b->tune.zchunk > 0 ? b->tune.zchunk : 32;
+    const int seg = b->tune.seg > 0 ? b->tune.seg : 24;
+    const int zc = b->tune.zchunk > 0 ? b->tune.zchunk : 4;
     const long waves = ((long)(L.ny + 2) * L.px + XS_FACES - 1) / XS_FACES;
     d.fx_gx = (int)nblk(waves * 64, 256);
     cnt[P_FX] = d.fx_gx * (L.nz + 2);
@@ -468,8 +468,8 @@ extern "C" int cumc_batch_create(cumc_ctx *const *boxes, int nbox, cumc_batch **
     b->nbox = nbox;
     b->coord = boxes[0]->cfg.coordinate;
     b->recon = boxes[0]->cfg.recon;
-    b->tune.seg = 48;
-    b->tune.zchunk = 32;
+    b->tune.seg = 24;
+    b->tune.zchunk = 4;
     for (int p = 0; p < CUMC_PH_N; p++) b->tune.blocks[p] = 0;
     auto bad = [&](int code) {
         cumc_batch_destroy(b);
@@ -639,7 +639,7 @@ extern "C" int cumc_batch_autotune(cumc_batch *b, cumc_tuning *out)
     if (rc == CUMC_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = bfail(b, CUMC_EHIP, "tuner sync");
     // 1) segment length of the pencil sweeps (one workgroup per tile)
     if (rc == CUMC_OK) {
-        const int segs[] = {16, 32, 64, 128, 256};
+        const int segs[] = {16, 24, 32, 64, 128, 256};
         double bestt = 1e300;
         for (int sg : segs) {
             cumc_tuning t = orig;
@@ -654,7 +654,23 @@ extern "C" int cumc_batch_autotune(cumc_batch *b, cumc_tuning *out)
             }
         }
     }
-    // 2) column tile shape of the EMF / tail kernels
+    // 2) plane chunk and column tile shape of the EMF / tail kernels
+    if (rc == CUMC_OK) {
+        const int zcs[] = {1, 2, 4, 8, 32};
+        double bestt = 1e300;
+        for (int z : zcs) {
+            cumc_tuning t = best;
+            t.zchunk = z;
+            for (int p = 0; p < CUMC_PH_N; p++) t.blocks[p] = 0;
+            double ph[P_N];
+            if ((rc = time_candidate(b, t, ph))) break;
+            const double v = ph[P_EC] + ph[P_TAIL];
+            if (v < bestt) {
+                bestt = v;
+                best.zchunk = z;
+            }
+        }
+    }
     if (rc == CUMC_OK) {
         const int tws[] = {0, 32, 64};
         double bestt = 1e300;

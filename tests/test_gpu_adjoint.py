@@ -12,7 +12,9 @@ central difference that straddles a switching surface is off by O(1) in that cel
 contribution; the number of surfaces crossed grows with h.  The difference J(v+) - J(v-) is
 formed as sum (B+ - B-)(B+ + B-)/2 dV, which keeps the rounding floor near 1e-17; h starts at
 5e-8 and a direction that misses is retaken with h/4 (a switching surface within h), see
-_fd_check.
+_fd_check.  A direction along which the contributions g_i d_i largely cancel has a small
+derivative and the same absolute difference noise, so the error is also measured against the
+cancellation-free magnitude sum |g_i d_i|; either measure within 1e-6 passes.
 """
 from __future__ import annotations
 
@@ -90,14 +92,15 @@ def _fd_check(prob, nsteps, every, ndir, seed, h=5e-8, oracle_dirs=0, refine=3):
     for d in range(ndir):
         dirn = rng.uniform(-1.0, 1.0, size=gv.shape)
         ad = float(np.sum(gv * dirn))
+        mag = float(np.sum(np.abs(gv * dirn)))
         hh, tries = h, []
         for r in range(refine + 1):
             pp, pm = prob.prim.copy(), prob.prim.copy()
             pp[inner + (slice(1, 4),)] += hh * dirn
             pm[inner + (slice(1, 4),)] -= hh * dirn
             fd = _dJ(_B_gpu(s, prob, pp, dts), _B_gpu(s, prob, pm, dts), prob) / (2 * hh)
-            rel = abs(fd - ad) / abs(ad)
-            tries.append(f"h={hh:.1e}: {rel:.2e}")
+            rel = min(abs(fd - ad) / abs(ad), abs(fd - ad) / mag)
+            tries.append(f"h={hh:.1e}: {abs(fd - ad) / abs(ad):.2e} ({abs(fd - ad) / mag:.2e} of sum|g d|)")
             if d < oracle_dirs and r == 0:
                 bpo, bmo = _B_oracle(prob, pp, dts), _B_oracle(prob, pm, dts)
                 assert _dJ(bpo, bmo, prob) / (2 * hh) == fd  # the CPU restatement's forward is bit-identical
@@ -113,9 +116,11 @@ def _fd_check(prob, nsteps, every, ndir, seed, h=5e-8, oracle_dirs=0, refine=3):
     return worst
 
 
-def test_adjoint_fd_2d_periodic_ppm():
-    print("[case] adjoint vs FD, 2D 16^2 periodic PPM, 6 steps, checkpoints every 4", flush=True)
-    prob = ics.randomize(ics.spectral_box(n=16, seed=3000, ndim=2, v_rms=0.3, db_rms=0.3), seed=1, amp=0.05)
+@pytest.mark.parametrize("ny", [16, 6])
+def test_adjoint_fd_2d_periodic_ppm(ny):
+    # ny = 16 reverses the y sweep wave-parallel, ny = 6 by per-pencil walks
+    print(f"[case] adjoint vs FD, 2D 16x{ny} periodic PPM, 6 steps, checkpoints every 4", flush=True)
+    prob = ics.randomize(ics.spectral_box(n=16, seed=3000, ndim=2, v_rms=0.3, db_rms=0.3, ny=ny), seed=1, amp=0.05)
     w = _fd_check(prob, nsteps=6, every=4, ndir=8, seed=31, oracle_dirs=2)
     print(f"  worst rel {w:.2e}")
 
@@ -198,6 +203,9 @@ def test_batched_reverse_matches_single_box_tapes():
     probs = [ics.randomize(ics.spectral_box(n=n, seed=3600 + i, ndim=2, v_rms=0.3, db_rms=0.3), seed=i, amp=0.05)
              for i, n in enumerate((12, 20, 9))]
     probs.append(ics.randomize(ics.small_periodic(8, seed=3700), seed=7, amp=0.05))
+    # short y pencils: this box's y sweep takes the per-pencil reverse in the same batch
+    probs.append(ics.randomize(ics.spectral_box(n=14, seed=3800, ndim=2, v_rms=0.3, db_rms=0.3, ny=5), seed=8,
+                               amp=0.05))
     singles = [Solver.from_problem(p) for p in probs]
     ref = []
     for s, p in zip(singles, probs):
