@@ -36,11 +36,19 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 BYTES_PER_ZONE_CYCLE = 512.0  # 64 FP64 state words per RK3 step
 # algorithmic bytes per unit of each kernel class (FP64 form): compulsory loads + stores
 KERNEL_BYTES = {
-    "flux_x": 15 * 8, "flux_y": 15 * 8, "flux_z": 15 * 8,  # 7 stencil vars + normal B in, 7 fluxes out
-    "ecorn": 15 * 8,                                        # 6 B-fluxes + v + cell B in, 3 EMFs out
-    # 15 fluxes + 3 EMFs + U0 (+ Uin in stages 2, 3) in; prim 8, eps, bcell 3, Uout 8 out:
-    # (46 + 54 + 54) / 3 words averaged over the three stage launches
-    "tail": (46 + 54 + 54) / 3 * 8,
+    "fp64": {
+        "flux_x": 15 * 8, "flux_y": 15 * 8, "flux_z": 15 * 8,  # 7 stencil vars + normal B in, 7 fluxes out
+        "ecorn": 15 * 8,                                        # 6 B-fluxes + v + cell B in, 3 EMFs out
+        # 15 fluxes + 3 EMFs + U0 (+ Uin in stages 2, 3) in; prim 8, eps, bcell 3, Uout 8 out:
+        # (46 + 54 + 54) / 3 words averaged over the three stage launches
+        "tail": (46 + 54 + 54) / 3 * 8,
+    },
+    # mixed form: fluxes and EMFs are stored and read as FP32, everything else as FP64
+    "mixed": {
+        "flux_x": 8 * 8 + 7 * 4, "flux_y": 8 * 8 + 7 * 4, "flux_z": 8 * 8 + 7 * 4,
+        "ecorn": 6 * 4 + 6 * 8 + 3 * 4,
+        "tail": ((46 - 18) * 8 + 18 * 4 + 2 * ((54 - 18) * 8 + 18 * 4)) / 3,
+    },
 }
 
 
@@ -114,11 +122,11 @@ def timed_run(Solver, prob, prec, steps, warmup, tune=True):
 
 
 def roofline(kt, cfg, tag):
-    dom = max((k for k in kt if k in KERNEL_BYTES), key=lambda k: kt[k][0])
+    table = KERNEL_BYTES[tag]
+    dom = max((k for k in kt if k in table), key=lambda k: kt[k][0])
     tot_ms, launches = kt[dom]
     avg_s = tot_ms / launches * 1e-3
-    scale = 0.5 if (tag == "mixed" and dom.startswith("flux")) else 1.0  # FP32 flux stores
-    alg = KERNEL_BYTES[dom] * kernel_units(dom, cfg) * scale
+    alg = table[dom] * kernel_units(dom, cfg)
     ach = alg / avg_s
     return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
             "traffic": read_traffic(dom, tag), "kernel": dom, "avg_launch_ms": tot_ms / launches}

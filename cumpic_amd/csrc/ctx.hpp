@@ -24,6 +24,7 @@ struct cumc_ctx {
     Fields<double> F64;
     Fields<float> F32;
     double *tab[3] = {nullptr, nullptr, nullptr};
+    float *tabf[3] = {nullptr, nullptr, nullptr};
     double *dt_dev = nullptr;           // [0] dt of the current step, [1] dt found by stage 3
     unsigned long long *red = nullptr;  // reduction slots
     double *Ub = nullptr;               // third conserved buffer (stage 2 output)
@@ -98,6 +99,7 @@ static inline Tabs tabs_of(const cumc_ctx *ctx)
 {
     Tabs tb;
     tb.t[0] = ctx->tab[0]; tb.t[1] = ctx->tab[1]; tb.t[2] = ctx->tab[2];
+    tb.tf[0] = ctx->tabf[0]; tb.tf[1] = ctx->tabf[1]; tb.tf[2] = ctx->tabf[2];
     tb.len[0] = ctx->L.nx + 7; tb.len[1] = ctx->L.ny + 7; tb.len[2] = ctx->L.nz + 7;
     return tb;
 }

@@ -149,7 +149,10 @@ extern "C" int cumc_destroy(cumc_ctx *ctx)
     hipFree(ctx->fluxpool);
     hipFree(ctx->dt_dev);
     hipFree(ctx->red);
-    for (int a = 0; a < 3; a++) hipFree(ctx->tab[a]);
+    for (int a = 0; a < 3; a++) {
+        hipFree(ctx->tab[a]);
+        hipFree(ctx->tabf[a]);
+    }
     for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
@@ -200,6 +203,9 @@ extern "C" int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, 
         std::vector<double> t = build_tables(fs[a], ns[a]);
         if (!ctx->tab[a]) HIPCHK(ctx, hipMalloc(&ctx->tab[a], t.size() * sizeof(double)));
         HIPCHK(ctx, hipMemcpy(ctx->tab[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<float> tf(t.begin(), t.end());
+        if (!ctx->tabf[a]) HIPCHK(ctx, hipMalloc(&ctx->tabf[a], tf.size() * sizeof(float)));
+        HIPCHK(ctx, hipMemcpy(ctx->tabf[a], tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     if (ctx->cfg.coordinate == CUMC_CYLINDRICAL && !(xf[3] > 0.0))
         return fail(ctx, CUMC_EINVAL, "cylindrical grid needs R > 0 on the inner face");

@@ -43,8 +43,16 @@ __device__ inline void sti(T *base, unsigned i, T v)
 // Per-axis geometry tables: tab[row * (n + 7) + j + 3] for j in [-3, n+3].
 struct Tabs {
     const double *t[3];
-    int len[3];  // n + 7 per axis
+    const float *tf[3];  // the same tables rounded to FP32 (read by the FP32 sweeps)
+    int len[3];          // n + 7 per axis
     __device__ double at(int ax, int row, int j) const { return ldi(t[ax] + (long)row * len[ax], (unsigned)(j + 3)); }
+    // entry in the sweep type: FP32 reads the rounded copy (equal to R(at(...)))
+    template <typename R>
+    __device__ R atr(int ax, int row, int j) const
+    {
+        if constexpr (sizeof(R) == 4) return ldi(tf[ax] + (long)row * len[ax], (unsigned)(j + 3));
+        else return at(ax, row, j);
+    }
 };
 
 struct Model {
@@ -81,14 +89,14 @@ __device__ inline void recon_lane(const Tabs &tb, int ax, int jc, R c, R &vm, R 
     R cm1 = __shfl_up(c, 1, 64);
     R cp1 = __shfl_down(c, 1, 64);
     if (RECON == 4) {
-        R dm = ppm_slope<R>(R(tb.at(ax, TB_B0C, jc)), R(tb.at(ax, TB_B1C, jc)), cm1, c, cp1);
+        R dm = ppm_slope<R>(tb.atr<R>(ax, TB_B0C, jc), tb.atr<R>(ax, TB_B1C, jc), cm1, c, cp1);
         R dmm1 = __shfl_up(dm, 1, 64);
         R dmp1 = __shfl_down(dm, 1, 64);
-        ppm_edges<R>(R(tb.at(ax, TB_A0M, jc)), R(tb.at(ax, TB_A1M, jc)), R(tb.at(ax, TB_A2M, jc)),
-                     R(tb.at(ax, TB_A3M, jc)), R(tb.at(ax, TB_A0P, jc)), R(tb.at(ax, TB_A1P, jc)),
-                     R(tb.at(ax, TB_A2P, jc)), R(tb.at(ax, TB_A3P, jc)), cm1, c, cp1, dmm1, dm, dmp1, vm, vp);
+        ppm_edges<R>(tb.atr<R>(ax, TB_A0M, jc), tb.atr<R>(ax, TB_A1M, jc), tb.atr<R>(ax, TB_A2M, jc),
+                     tb.atr<R>(ax, TB_A3M, jc), tb.atr<R>(ax, TB_A0P, jc), tb.atr<R>(ax, TB_A1P, jc),
+                     tb.atr<R>(ax, TB_A2P, jc), tb.atr<R>(ax, TB_A3P, jc), cm1, c, cp1, dmm1, dm, dmp1, vm, vp);
     } else {
-        tvd_edges<R>(R(tb.at(ax, TB_CF, jc)), R(tb.at(ax, TB_CB, jc)), cm1, c, cp1, vm, vp);
+        tvd_edges<R>(tb.atr<R>(ax, TB_CF, jc), tb.atr<R>(ax, TB_CB, jc), cm1, c, cp1, vm, vp);
     }
 }
 
@@ -196,8 +204,8 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
         load7(s0, w1);
         load7(s0 + 1, w2);
         if (RECON == 4) {
-            R b0 = R(tb.at(ax, TB_B0C, s0 - 1)), b1 = R(tb.at(ax, TB_B1C, s0 - 1));
-            R c0 = R(tb.at(ax, TB_B0C, s0)), c1 = R(tb.at(ax, TB_B1C, s0));
+            R b0 = tb.atr<R>(ax, TB_B0C, s0 - 1), b1 = tb.atr<R>(ax, TB_B1C, s0 - 1);
+            R c0 = tb.atr<R>(ax, TB_B0C, s0), c1 = tb.atr<R>(ax, TB_B1C, s0);
 #pragma unroll
             for (int v = 0; v < 7; v++) {
                 d0[v] = ppm_slope<R>(b0, b1, a[v], w0[v], w1[v]);
@@ -209,10 +217,10 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
         load7(c + 2, w3);
         R vm[7], vp[7];
         if (RECON == 4) {
-            R b0 = R(tb.at(ax, TB_B0C, c + 1)), b1 = R(tb.at(ax, TB_B1C, c + 1));
-            R a0m = R(tb.at(ax, TB_A0M, c)), a1m = R(tb.at(ax, TB_A1M, c)), a2m = R(tb.at(ax, TB_A2M, c));
-            R a3m = R(tb.at(ax, TB_A3M, c)), a0p = R(tb.at(ax, TB_A0P, c)), a1p = R(tb.at(ax, TB_A1P, c));
-            R a2p = R(tb.at(ax, TB_A2P, c)), a3p = R(tb.at(ax, TB_A3P, c));
+            R b0 = tb.atr<R>(ax, TB_B0C, c + 1), b1 = tb.atr<R>(ax, TB_B1C, c + 1);
+            R a0m = tb.atr<R>(ax, TB_A0M, c), a1m = tb.atr<R>(ax, TB_A1M, c), a2m = tb.atr<R>(ax, TB_A2M, c);
+            R a3m = tb.atr<R>(ax, TB_A3M, c), a0p = tb.atr<R>(ax, TB_A0P, c), a1p = tb.atr<R>(ax, TB_A1P, c);
+            R a2p = tb.atr<R>(ax, TB_A2P, c), a3p = tb.atr<R>(ax, TB_A3P, c);
 #pragma unroll
             for (int v = 0; v < 7; v++) {
                 R d2 = ppm_slope<R>(b0, b1, w1[v], w2[v], w3[v]);
@@ -222,7 +230,7 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
                 d1[v] = d2;
             }
         } else {
-            R cf = R(tb.at(ax, TB_CF, c)), cb = R(tb.at(ax, TB_CB, c));
+            R cf = tb.atr<R>(ax, TB_CF, c), cb = tb.atr<R>(ax, TB_CB, c);
 #pragma unroll
             for (int v = 0; v < 7; v++) tvd_edges<R>(cf, cb, w0[v], w1[v], w2[v], vm[v], vp[v]);
         }
