@@ -137,7 +137,6 @@ void batch_set_counter(cumc_batch *b, int v);
 void batch_finddt(cumc_batch *b);
 void batch_step(cumc_batch *b, int par, int frozen);
 void batch_stage(cumc_batch *b, int st, int par);
-void batch_fluxes_ecorn(cumc_batch *b);
 void batch_set_dt(cumc_batch *b, int m);
 void batch_after_run(cumc_batch *b, int nsteps);
 

@@ -117,70 +117,55 @@ __global__ __launch_bounds__(256) void kr_ghost(Lay L, AdjF A, Model M, const do
 }
 
 // ---------------------------------------------------------------------------------------------
-// stage tail reversed, part a: per cell of [0,n]^3 the forward values are recomputed exactly as
-// k_tail does (so the floor branch replays), then the chain
+// stage tail reversed, part a: per cell of [0,n]^3 the chain
 //   Uout, P, EPS, bcell  <-  (rho, v, eps, bcell)  <-  (u, bcell)  <-  (U0, Uin, rhs)
-// is reversed.  Hydro adjoints go to U0 / Uin directly; the face-B parts wait for part b
-// because a face field feeds the cell-centred field of two cells.
+// is reversed.  The forward values it needs come from the stored stage output on interior cells
+// (k_tail writes rho, v, eps and bcell there and the ghost fill leaves them; with the riemann
+// hooks rho is not floored and eps = max(eps0, 0), so eps0 is recovered where eps > 0, and a
+// +0 eps marks the floored branch) and are recomputed on the lower ghost layer (j, k or l = 0),
+// where the tail's right-hand side is zero and its bcell is the stage input's.  Hydro adjoints
+// go to U0 / Uin directly; the face-B parts wait for part b because a face field feeds the
+// cell-centred field of two cells.
 // ---------------------------------------------------------------------------------------------
 template <int STAGE>
-__device__ inline void kr_tail_a_cell(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M, double dt,
-                                      const double *__restrict__ U0, const double *__restrict__ Uin, const AdjF &A,
-                                      double *lU0, double *lUin, double *lUout, long t, int l)
+__device__ inline void kr_tail_a_cell(const Lay &L, const Fields<double> &F, const double *__restrict__ Pout,
+                                      const double *__restrict__ EPSout, const double *__restrict__ BCout,
+                                      const Model &M, double dt, const double *__restrict__ U0,
+                                      const double *__restrict__ Uin, const AdjF &A, double *lU0, double *lUin,
+                                      double *lUout, long t, int l)
 {
     const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
     if (k > L.ny || j > L.nx) return;
-    const long g = L.id(j, k, l), sj = 1, sk = L.px, sl = L.py, ps = L.ps;
+    const long g = L.id(j, k, l), ps = L.ps;
     const bool interior = j >= 1 && k >= 1 && l >= 1;
-    const double *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2], *E = F.E;
-    auto uin = [&](long i) { return STAGE == 1 ? 0.0 : Uin[i]; };
     const double gm1 = M.ggas - 1.0;
-    // forward recompute (k_tail, Cartesian)
-    const double ex0 = E[g], ey0 = E[g + ps], ez0 = E[g + 2 * ps];
-    const double exk = E[g - sk], ezk = E[g - sk + 2 * ps];
-    const double eyj = E[g - sj + ps], ezj = E[g - sj + 2 * ps];
-    const double ezjk = E[g - sj - sk + 2 * ps];
-    const double exL = E[g - sl], eyL = E[g - sl + ps], eyjL = E[g - sl - sj + ps], exkL = E[g - sl - sk];
-    double u[5], bc[3];
-    {
-        double lrk[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (interior) {
-            const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
-            for (int i = 0; i < 5; i++) {
-                double a = 0.0;
-                a = a - (Fx[g + i * ps] * 1.0 - Fx[g - sj + i * ps] * 1.0) / dxl;
-                a = a - (Fy[g + i * ps] * 1.0 - Fy[g - sk + i * ps] * 1.0) / dyl;
-                a = a - (Fz[g + i * ps] * 1.0 - Fz[g - sl + i * ps] * 1.0) / dzl;
-                lrk[i] = a + 0.0;
-            }
-        }
-        for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(U0[g + i * ps], uin(g + i * ps), dt, lrk[i]);
-        if (interior) {
-            const double bxn = rk<STAGE>(U0[g + 5 * ps], uin(g + 5 * ps), dt, rhs_bx<1>(tb, j, k, l, ez0, ezk, ey0, eyL));
-            const double byn = rk<STAGE>(U0[g + 6 * ps], uin(g + 6 * ps), dt, rhs_by<1>(tb, j, l, ex0, exL, ez0, ezj));
-            const double bzn = rk<STAGE>(U0[g + 7 * ps], uin(g + 7 * ps), dt, rhs_bz<1>(tb, j, k, ey0, eyj, ex0, exk));
-            const double bxm = rk<STAGE>(U0[g - sj + 5 * ps], uin(g - sj + 5 * ps), dt,
-                                         rhs_bx<1>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
-            const double bym = rk<STAGE>(U0[g - sk + 6 * ps], uin(g - sk + 6 * ps), dt,
-                                         rhs_by<1>(tb, j, l, exk, exkL, ezk, ezjk));
-            const double bzm = rk<STAGE>(U0[g - sl + 7 * ps], uin(g - sl + 7 * ps), dt,
-                                         rhs_bz<1>(tb, j, k, eyL, eyjL, exL, exkL));
-            bc[0] = 0.5 * (bxn + bxm);
-            bc[1] = 0.5 * (byn + bym);
-            bc[2] = 0.5 * (bzn + bzm);
-        } else {
-            bc[0] = F.BC[g];
-            bc[1] = F.BC[g + ps];
-            bc[2] = F.BC[g + 2 * ps];
-        }
+    double rho, vx, vy, vz, bc[3], eps, ekin;  // ekin = (u4 - B^2/2) / rho = eps0 + v^2/2
+    bool floored;
+    if (interior) {
+        rho = Pout[g];
+        vx = Pout[g + ps];
+        vy = Pout[g + 2 * ps];
+        vz = Pout[g + 3 * ps];
+        for (int c = 0; c < 3; c++) bc[c] = BCout[g + c * ps];
+        eps = EPSout[g];
+        floored = eps == 0.0 && !signbit(eps);
+        ekin = eps + 0.5 * (vx * vx + vy * vy + vz * vz);
+    } else {
+        auto uin = [&](long i) { return STAGE == 1 ? 0.0 : Uin[i]; };
+        double u[5];
+        for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(U0[g + i * ps], uin(g + i * ps), dt, 0.0);
+        for (int c = 0; c < 3; c++) bc[c] = F.BC[g + c * ps];
+        rho = u[0];
+        vx = u[1] / rho;
+        vy = u[2] / rho;
+        vz = u[3] / rho;
+        const double bsq = bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2];
+        const double vsq = vx * vx + vy * vy + vz * vz;
+        ekin = (u[4] - 0.5 * bsq) / rho;
+        eps = ekin - 0.5 * vsq;
+        floored = false;
     }
-    const double rho = u[0];
-    const double vx = u[1] / rho, vy = u[2] / rho, vz = u[3] / rho;
-    const double bsq = bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2];
     const double vsq = vx * vx + vy * vy + vz * vz;
-    const double eps0 = (u[4] - 0.5 * bsq) / rho - 0.5 * vsq;
-    const bool floored = interior && (0.0 > eps0);
-    const double eps = floored ? 0.0 : eps0;
     // adjoints of the outputs
     double lP[8], lUo[8], lBc[3];
     for (int v = 0; v < 8; v++) {
@@ -208,7 +193,7 @@ __device__ inline void kr_tail_a_cell(const Lay &L, const Fields<double> &F, con
     double lu[5];
     lu[4] = leps0 / rho;
     for (int c = 0; c < 3; c++) lbc[c] -= leps0 * bc[c] / rho;
-    lrho -= leps0 * (u[4] - 0.5 * bsq) / (rho * rho);
+    lrho -= leps0 * ekin / rho;
     lv[0] -= leps0 * vx;
     lv[1] -= leps0 * vy;
     lv[2] -= leps0 * vz;
@@ -232,13 +217,22 @@ __device__ inline void kr_tail_a_cell(const Lay &L, const Fields<double> &F, con
     }
 }
 
+// forward fields of a stored state (P 8, EPS 1, bcell 3 planes at s) as the input of a stage
+__host__ __device__ inline Fields<double> fields_of_state(Fields<double> F, const double *s, long ps)
+{
+    F.P = const_cast<double *>(s);
+    F.EPS = const_cast<double *>(s) + 8 * ps;
+    F.BC = const_cast<double *>(s) + 9 * ps;
+    return F;
+}
+
 template <int STAGE>
-__global__ __launch_bounds__(256) void kr_tail_a(Lay L, Fields<double> F, Tabs tb, Model M, double dt,
+__global__ __launch_bounds__(256) void kr_tail_a(Lay L, Fields<double> F, const double *out, Model M, double dt,
                                                  const double *__restrict__ U0, const double *__restrict__ Uin,
                                                  AdjF A, double *lU0, double *lUin, double *lUout)
 {
-    kr_tail_a_cell<STAGE>(L, F, tb, M, dt, U0, Uin, A, lU0, lUin, lUout, (long)blockIdx.x * blockDim.x + threadIdx.x,
-                          blockIdx.y);
+    kr_tail_a_cell<STAGE>(L, F, out, out + 8 * L.ps, out + 9 * L.ps, M, dt, U0, Uin, A, lU0, lUin, lUout,
+                          (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 // part b: face-B adjoints (own cell's share + the upper neighbour's share of its cell field)
@@ -834,8 +828,8 @@ struct cumc_tape {
     double *ckpt = nullptr;    // nck states of SW planes: P 8, EPS 1, BC 3, U 8
     int nck = 0;
     double *seg = nullptr;     // every + 1 states
-    double *stA = nullptr;     // stage-1 output P/EPS/BC (12)
-    double *stB = nullptr;     // stage-2 output rho, eps (2)
+    double *sa = nullptr;      // every states after stage 1: P/EPS/BC and the stage's U (U1)
+    double *sb = nullptr;      // every states after stage 2: P/EPS/BC and the stage's U (Ub)
     double *adj = nullptr;     // NADJ planes
     double *lu[3] = {nullptr, nullptr, nullptr};  // 8 planes each: U adjoints by role U0, U1, Ub
     AdjF A;
@@ -853,12 +847,13 @@ void tape_release(cumc_ctx *ctx)
 static inline size_t plane_bytes(const cumc_ctx *ctx) { return (size_t)ctx->L.ps * sizeof(double); }
 
 // P, EPS, BC are the first 12 planes of the pool; U of the current step start is F64.U0
-static void save_state(cumc_ctx *ctx, double *dst)
+static void save_pool_u(cumc_ctx *ctx, double *dst, const double *U)
 {
     const size_t pb = plane_bytes(ctx);
     hipMemcpyAsync(dst, ctx->pool, 12 * pb, hipMemcpyDeviceToDevice, ctx->stream);
-    hipMemcpyAsync(dst + 12 * ctx->L.ps, ctx->F64.U0, 8 * pb, hipMemcpyDeviceToDevice, ctx->stream);
+    hipMemcpyAsync(dst + 12 * ctx->L.ps, U, 8 * pb, hipMemcpyDeviceToDevice, ctx->stream);
 }
+static void save_state(cumc_ctx *ctx, double *dst) { save_pool_u(ctx, dst, ctx->F64.U0); }
 static void load_state(cumc_ctx *ctx, const double *src, bool with_u = true)
 {
     const size_t pb = plane_bytes(ctx);
@@ -879,36 +874,26 @@ static void forward_step(cumc_ctx *ctx, double dt)
     swap_u(ctx);
 }
 
-static void recompute_flux_emf(cumc_ctx *ctx)
-{
-    const Lay &L = ctx->L;
-    launch_fluxes<double>(ctx, ctx->F64);
-    const int zc = ctx->zchunk, tw = ctx->tile_w;
-    const int gx = col_tiles(L, tw), gy = (L.nz + zc) / zc;
-    hipLaunchKernelGGL((k_ecorn<double>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, ctx->F64, gx, gy, zc,
-                       tw);
-}
-
 template <int DIR>
-static void reverse_sweep(cumc_ctx *ctx, const AdjF &A)
+static void reverse_sweep(cumc_ctx *ctx, const Fields<double> &F, const AdjF &A)
 {
     const Lay &L = ctx->L;
     Tabs tb = tabs_of(ctx);
     hipStream_t s = ctx->stream;
     if (lane_sweep<DIR>(L)) {
         dim3 grid(lane_tiles<DIR>(L), lane_outer<DIR>(L));
-        if (ctx->M.recon == CUMC_PPMC) hipLaunchKernelGGL((kr_flux_lane<DIR, 4>), grid, dim3(256), 0, s, L, ctx->F64, tb, ctx->M, A);
-        else hipLaunchKernelGGL((kr_flux_lane<DIR, 1>), grid, dim3(256), 0, s, L, ctx->F64, tb, ctx->M, A);
+        if (ctx->M.recon == CUMC_PPMC) hipLaunchKernelGGL((kr_flux_lane<DIR, 4>), grid, dim3(256), 0, s, L, F, tb, ctx->M, A);
+        else hipLaunchKernelGGL((kr_flux_lane<DIR, 1>), grid, dim3(256), 0, s, L, F, tb, ctx->M, A);
         return;
     }
     if constexpr (DIR != 1) {
         const long np = npencils<DIR>(L);
         if (ctx->M.recon == CUMC_PPMC) {
-            hipLaunchKernelGGL((kr_flux_hll<DIR, 4>), dim3(nblk(np, 64)), dim3(64), 0, s, L, ctx->F64, tb, ctx->M, A);
-            hipLaunchKernelGGL((kr_flux_recon<DIR, 4>), dim3(nblk(np, 64), 7), dim3(64), 0, s, L, ctx->F64, tb, A);
+            hipLaunchKernelGGL((kr_flux_hll<DIR, 4>), dim3(nblk(np, 64)), dim3(64), 0, s, L, F, tb, ctx->M, A);
+            hipLaunchKernelGGL((kr_flux_recon<DIR, 4>), dim3(nblk(np, 64), 7), dim3(64), 0, s, L, F, tb, A);
         } else {
-            hipLaunchKernelGGL((kr_flux_hll<DIR, 1>), dim3(nblk(np, 64)), dim3(64), 0, s, L, ctx->F64, tb, ctx->M, A);
-            hipLaunchKernelGGL((kr_flux_recon<DIR, 1>), dim3(nblk(np, 64), 7), dim3(64), 0, s, L, ctx->F64, tb, A);
+            hipLaunchKernelGGL((kr_flux_hll<DIR, 1>), dim3(nblk(np, 64)), dim3(64), 0, s, L, F, tb, ctx->M, A);
+            hipLaunchKernelGGL((kr_flux_recon<DIR, 1>), dim3(nblk(np, 64), 7), dim3(64), 0, s, L, F, tb, A);
         }
     }
 }
@@ -923,56 +908,44 @@ static void reverse_ghost(cumc_ctx *ctx, const AdjF &A, const double *rho_out, c
     hipLaunchKernelGGL(kr_ghost<0>, dim3(nblk(n0, 256)), dim3(256), 0, s, L, A, ctx->M, rho_out, eps_out);
 }
 
-// reverse one stage whose forward input (P, BC) is in the context and whose flux / EMF planes
-// have been recomputed from it
+// reverse one stage: forward input state `in` (P, EPS, bcell of the stored state before the
+// stage), forward output state `out` (after the stage and its ghost fill), U0 / Uin its conserved
+// inputs.  No forward kernel runs: the sweeps recompute their reconstruction and Riemann solves
+// from `in`, the tail reads `out`.
 template <int STAGE>
-static void reverse_stage(cumc_ctx *ctx, double dt, const double *rho_out, const double *eps_out, const double *Uin,
-                          double *lU0, double *lUin, double *lUout)
+static void reverse_stage(cumc_ctx *ctx, const double *in, const double *out, double dt, const double *U0,
+                          const double *Uin, double *lU0, double *lUin, double *lUout)
 {
     cumc_tape *tp = ctx->tape;
     const AdjF &A = tp->A;
     const Lay &L = ctx->L;
     hipStream_t s = ctx->stream;
     Tabs tb = tabs_of(ctx);
-    reverse_ghost(ctx, A, rho_out, eps_out);
+    const Fields<double> F = fields_of_state(ctx->F64, in, L.ps);
+    reverse_ghost(ctx, A, out, out + 8 * L.ps);
     dim3 g0(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
-    hipLaunchKernelGGL((kr_tail_a<STAGE>), g0, dim3(256), 0, s, L, ctx->F64, tb, ctx->M, dt, ctx->F64.U0, Uin, A, lU0,
-                       lUin, lUout);
+    hipLaunchKernelGGL((kr_tail_a<STAGE>), g0, dim3(256), 0, s, L, F, out, ctx->M, dt, U0, Uin, A, lU0, lUin, lUout);
     hipLaunchKernelGGL((kr_tail_b<STAGE>), g0, dim3(256), 0, s, L, A, dt, lU0, lUin);
     dim3 g1(nblk((long)L.px * (L.ny + 2), 256), L.nz + 2);
     hipLaunchKernelGGL(kr_gather, g1, dim3(256), 0, s, L, A, tb);
-    hipLaunchKernelGGL(kr_ecorn, g1, dim3(256), 0, s, L, ctx->F64, A);
-    reverse_sweep<3>(ctx, A);
-    reverse_sweep<2>(ctx, A);
-    reverse_sweep<1>(ctx, A);
+    hipLaunchKernelGGL(kr_ecorn, g1, dim3(256), 0, s, L, F, A);
+    reverse_sweep<3>(ctx, F, A);
+    reverse_sweep<2>(ctx, F, A);
+    reverse_sweep<1>(ctx, F, A);
 }
 
-// reverse step m: S_m in seg slot i, S_{m+1} in slot i+1; the adjoint of S_{m+1} in A / lu[U1]
+// reverse step m: S_m in seg slot i, its stage-1 / stage-2 states in sa / sb slot i, S_{m+1} in
+// seg slot i+1; the adjoint of S_{m+1} in A / lu[U1]
 static void reverse_step(cumc_ctx *ctx, int i, double dt)
 {
     cumc_tape *tp = ctx->tape;
-    const long ps = ctx->L.ps;
-    const size_t pb = plane_bytes(ctx);
-    const double *Sm = tp->seg + (size_t)i * SW * ps, *Sn = tp->seg + (size_t)(i + 1) * SW * ps;
+    const size_t st = (size_t)SW * ctx->L.ps;
+    const double *Sm = tp->seg + i * st, *Sn = Sm + st, *Sa = tp->sa + i * st, *Sb = tp->sb + i * st;
+    const double *U0 = Sm + 12 * ctx->L.ps;
     double *&l0 = tp->lu[0], *&l1 = tp->lu[1], *&lb = tp->lu[2];
-    load_state(ctx, Sm);
-    set_dt(ctx, dt);
-    stage<double>(ctx, ctx->F64, 1);
-    hipMemcpyAsync(tp->stA, ctx->pool, 12 * pb, hipMemcpyDeviceToDevice, ctx->stream);
-    stage<double>(ctx, ctx->F64, 2);
-    hipMemcpyAsync(tp->stB, ctx->F64.P, pb, hipMemcpyDeviceToDevice, ctx->stream);
-    hipMemcpyAsync(tp->stB + ps, ctx->F64.EPS, pb, hipMemcpyDeviceToDevice, ctx->stream);
-    // stage 3: input (P2, BC2), Uin = Ub, output S_{m+1}
-    recompute_flux_emf(ctx);
-    reverse_stage<3>(ctx, dt, Sn, Sn + 8 * ps, ctx->Ub, l0, lb, l1);
-    // stage 2: input (P1, BC1), Uin = U1, output (rho2, eps2)
-    hipMemcpyAsync(ctx->pool, tp->stA, 12 * pb, hipMemcpyDeviceToDevice, ctx->stream);
-    recompute_flux_emf(ctx);
-    reverse_stage<2>(ctx, dt, tp->stB, tp->stB + ps, ctx->F64.U1, l0, l1, lb);
-    // stage 1: input S_m, output (rho1, eps1)
-    load_state(ctx, Sm, false);
-    recompute_flux_emf(ctx);
-    reverse_stage<1>(ctx, dt, tp->stA, tp->stA + 8 * ps, nullptr, l0, nullptr, l1);
+    reverse_stage<3>(ctx, Sb, Sn, dt, U0, Sb + 12 * ctx->L.ps, l0, lb, l1);
+    reverse_stage<2>(ctx, Sa, Sb, dt, U0, Sa + 12 * ctx->L.ps, l0, l1, lb);
+    reverse_stage<1>(ctx, Sm, Sa, dt, U0, nullptr, l0, nullptr, l1);
     // the adjoint of S_m's U becomes the output-role adjoint of step m-1
     double *tmp = l0;
     l0 = l1;
@@ -990,7 +963,7 @@ static int tape_alloc(cumc_ctx *ctx, int nsteps, int every)
     tp->every = every;
     tp->nck = nsteps / every + 2;  // step 0, every, 2 every, ... and the final state
     const size_t ps = ctx->L.ps;
-    const size_t planes = 9 + (size_t)SW * tp->nck + (size_t)SW * (every + 1) + 12 + 2 + NADJ + 24;
+    const size_t planes = 9 + (size_t)SW * tp->nck + (size_t)SW * (3 * every + 1) + NADJ + 24;
     if (hipMalloc(&tp->mem, planes * ps * sizeof(double)) != hipSuccess) {
         tape_release(ctx);
         return fail(ctx, CUMC_ENOMEM, "tape allocation");
@@ -1000,8 +973,8 @@ static int tape_alloc(cumc_ctx *ctx, int nsteps, int every)
     tp->init = take(9);
     tp->ckpt = take((size_t)SW * tp->nck);
     tp->seg = take((size_t)SW * (every + 1));
-    tp->stA = take(12);
-    tp->stB = take(2);
+    tp->sa = take((size_t)SW * every);
+    tp->sb = take((size_t)SW * every);
     tp->adj = take(NADJ);
     for (int r = 0; r < 3; r++) tp->lu[r] = take(8);
     AdjF &A = tp->A;
@@ -1114,8 +1087,15 @@ extern "C" int cumc_tape_backward(cumc_ctx *ctx, const double *seed_prim, const 
         load_state(ctx, tp->ckpt + (size_t)SW * ps * (s0 / k));
         save_state(ctx, tp->seg);
         for (int m = s0; m < s1; m++) {
-            forward_step(ctx, tp->dts[m]);
-            save_state(ctx, tp->seg + (size_t)SW * ps * (m - s0 + 1));
+            const size_t at = (size_t)SW * ps * (m - s0);
+            set_dt(ctx, tp->dts[m]);
+            stage<double>(ctx, ctx->F64, 1);
+            save_pool_u(ctx, tp->sa + at, ctx->F64.U1);
+            stage<double>(ctx, ctx->F64, 2);
+            save_pool_u(ctx, tp->sb + at, ctx->Ub);
+            stage<double>(ctx, ctx->F64, 3);
+            swap_u(ctx);
+            save_state(ctx, tp->seg + at + (size_t)SW * ps);
         }
         for (int m = s1 - 1; m >= s0; m--) reverse_step(ctx, m - s0, tp->dts[m]);
         HIPCHK(ctx, hipGetLastError());
@@ -1162,7 +1142,7 @@ struct AdjBox {
     double *pool, *Ub, *dt;
     AdjF A;
     double *lu[3];
-    double *seg, *stA, *stB, *ckpt, *init;
+    double *seg, *sa, *sb, *ckpt, *init;
     int gx_c, gx_e;  // 256-cell tiles per plane row set: [0,n]^2 and [0,n+1]^2 column domains
     int gpen[3];     // 64-pencil tiles per sweep
     int glane[3];    // kr_flux_lane tiles per outer index, per sweep
@@ -1182,15 +1162,23 @@ __device__ inline int adj_box_of(const int *pre, int nbox, int it)
     return lo;
 }
 
-// forward values after a stage, selected per box: 0 segment state `slot`, 1 stage-2 save,
-// 2 stage-1 save, 3 the state after initial_update
-__device__ inline void out_fields(const AdjBox &d, int sel, int slot, const double *&rho, const double *&eps)
+// the stored states around stage `stage` (1..3) of segment step i: forward input (its P, EPS,
+// bcell planes first), forward output, U0 of the step and the stage's Uin; stage 0 stands for
+// initial_update, whose output is checkpoint 0
+struct StageStates {
+    const double *in, *out, *U0, *Uin;
+};
+__device__ inline StageStates stage_states(const AdjBox &d, int stage, int i)
 {
-    const long ps = d.L.ps;
-    if (sel == 0) { rho = d.seg + (size_t)slot * SW * ps; eps = rho + 8 * ps; }
-    else if (sel == 1) { rho = d.stB; eps = d.stB + ps; }
-    else if (sel == 2) { rho = d.stA; eps = d.stA + 8 * ps; }
-    else { rho = d.ckpt; eps = d.ckpt + 8 * ps; }
+    const size_t ps = d.L.ps, st = (size_t)SW * ps;
+    const double *Sm = d.seg + i * st, *Sa = d.sa + i * st, *Sb = d.sb + i * st;
+    StageStates v;
+    v.U0 = Sm + 12 * ps;
+    if (stage == 3) { v.in = Sb; v.out = Sm + st; v.Uin = Sb + 12 * ps; }
+    else if (stage == 2) { v.in = Sa; v.out = Sb; v.Uin = Sa + 12 * ps; }
+    else if (stage == 1) { v.in = Sm; v.out = Sa; v.Uin = nullptr; }
+    else { v.in = d.ckpt; v.out = d.ckpt; v.Uin = nullptr; }
+    return v;
 }
 
 #define FOR_ITEMS(Q)                                                                            \
@@ -1203,26 +1191,27 @@ __device__ inline void out_fields(const AdjBox &d, int sel, int slot, const doub
 
 template <int AX>
 __global__ __launch_bounds__(256) void kbr_ghost(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
-                                                 int sel, int slot)
+                                                 int stage, int i)
 {
     FOR_ITEMS(Q_GX + AX)
-        const double *rho, *eps;
-        out_fields(d, sel, slot, rho, eps);
-        kr_ghost_line<AX>(d.L, d.A, d.M, rho, eps, (long)loc * 256 + threadIdx.x);
+        const double *out = stage_states(d, stage, i).out;
+        kr_ghost_line<AX>(d.L, d.A, d.M, out, out + 8 * d.L.ps, (long)loc * 256 + threadIdx.x);
     }
 }
 
 template <int STAGE>
 __global__ __launch_bounds__(256) void kbr_tail_a(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
-                                                  int q)
+                                                  int q, int i)
 {
     FOR_ITEMS(Q_TAIL)
+        const StageStates v = stage_states(d, STAGE, i);
+        const long ps = d.L.ps;
         double *l0 = d.lu[q], *l1 = d.lu[1 - q], *lb = d.lu[2];
-        const double *Uin = STAGE == 3 ? d.Ub : (STAGE == 2 ? d.F.U1 : nullptr);
         double *lUin = STAGE == 3 ? lb : (STAGE == 2 ? l1 : nullptr);
         double *lUout = STAGE == 2 ? lb : l1;
-        kr_tail_a_cell<STAGE>(d.L, d.F, d.tb, d.M, *d.dt, d.F.U0, Uin, d.A, l0, lUin, lUout,
-                              (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c);
+        kr_tail_a_cell<STAGE>(d.L, fields_of_state(d.F, v.in, ps), v.out, v.out + 8 * ps, v.out + 9 * ps, d.M, *d.dt,
+                              v.U0, v.Uin, d.A, l0, lUin, lUout, (long)(loc % d.gx_c) * 256 + threadIdx.x,
+                              loc / d.gx_c);
     }
 }
 
@@ -1244,28 +1233,33 @@ __global__ __launch_bounds__(256) void kbr_gather(const AdjBox *__restrict__ D, 
     }
 }
 
-__global__ __launch_bounds__(256) void kbr_ecorn(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox)
+__global__ __launch_bounds__(256) void kbr_ecorn(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                                 int stage, int i)
 {
     FOR_ITEMS(Q_GE)
-        kr_ecorn_cell(d.L, d.F, d.A, (long)(loc % d.gx_e) * 256 + threadIdx.x, loc / d.gx_e);
+        const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
+        kr_ecorn_cell(d.L, F, d.A, (long)(loc % d.gx_e) * 256 + threadIdx.x, loc / d.gx_e);
     }
 }
 
 template <int DIR, int RECON>
-__global__ __launch_bounds__(64) void kbr_flux_hll(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox)
+__global__ __launch_bounds__(64) void kbr_flux_hll(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                                   int stage, int i)
 {
     FOR_ITEMS(Q_H1 + DIR - 1)
-        kr_flux_hll_pencil<DIR, RECON>(d.L, d.F, d.tb, d.M, d.A, (long)loc * 64 + threadIdx.x);
+        const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
+        kr_flux_hll_pencil<DIR, RECON>(d.L, F, d.tb, d.M, d.A, (long)loc * 64 + threadIdx.x);
     }
 }
 
 template <int DIR, int RECON>
 __global__ __launch_bounds__(64) void kbr_flux_recon(const AdjBox *__restrict__ D, const int *__restrict__ pre,
-                                                     int nbox)
+                                                     int nbox, int stage, int i)
 {
     FOR_ITEMS(Q_R1 + DIR - 1)
         const int gp = d.gpen[DIR - 1];
-        kr_flux_recon_pencil<DIR, RECON>(d.L, d.F, d.tb, d.A, (long)(loc % gp) * 64 + threadIdx.x, loc / gp);
+        const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
+        kr_flux_recon_pencil<DIR, RECON>(d.L, F, d.tb, d.A, (long)(loc % gp) * 64 + threadIdx.x, loc / gp);
     }
 }
 
@@ -1286,43 +1280,38 @@ __global__ __launch_bounds__(256) void kbr_init_b(const AdjBox *__restrict__ D, 
 
 template <int DIR, int RECON>
 __global__ __launch_bounds__(256) void kbr_flux_lane(const AdjBox *__restrict__ D, const int *__restrict__ pre,
-                                                     int nbox)
+                                                     int nbox, int stage, int i)
 {
     FOR_ITEMS(Q_L1 + DIR - 1)
         const int gl = d.glane[DIR - 1];
-        kr_flux_lane_tile<DIR, RECON>(d.L, d.F, d.tb, d.M, d.A, loc % gl, loc / gl);
+        const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
+        kr_flux_lane_tile<DIR, RECON>(d.L, F, d.tb, d.M, d.A, loc % gl, loc / gl);
     }
 }
 
 // state copies between the context fields and the per-box tape (element e of 20 planes: the
-// first 12 are P, EPS, bcell of the pool, the last 8 the conserved state selected by par)
-enum { CP_SAVE_SEG, CP_LOAD_SEG, CP_LOAD12_SEG, CP_LOAD_CK, CP_SAVE_CK, CP_SAVE_A, CP_LOAD_A, CP_SAVE_B, CP_LOAD_INIT,
-       CP_SAVE_INIT };
+// first 12 are P, EPS, bcell of the pool, the last 8 the conserved state: U0 / U1 by par, or Ub)
+enum { CP_SAVE_SEG, CP_SAVE_SA, CP_SAVE_SB, CP_LOAD_CK, CP_SAVE_CK, CP_LOAD_INIT, CP_SAVE_INIT };
 
 __global__ __launch_bounds__(256) void kbr_copy(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
                                                 int mode, int slot, int par)
 {
     FOR_ITEMS(Q_CP)
         const long ps = d.L.ps;
-        double *U = par ? d.F.U1 : d.F.U0;
+        double *U = mode == CP_SAVE_SB ? d.Ub : (par ? d.F.U1 : d.F.U0);
         double *pool = d.pool;
+        const size_t at = (size_t)slot * SW * ps;
         for (int r = 0; r < CP_ELEMS / 256; r++) {
             const long e = (long)loc * CP_ELEMS + r * 256 + threadIdx.x;
             if (e >= SW * ps) break;
             const bool lo = e < 12 * ps;
             double *ctxp = lo ? pool + e : U + (e - 12 * ps);
             switch (mode) {
-            case CP_SAVE_SEG: d.seg[(size_t)slot * SW * ps + e] = *ctxp; break;
-            case CP_LOAD_SEG: *ctxp = d.seg[(size_t)slot * SW * ps + e]; break;
-            case CP_LOAD12_SEG: if (lo) *ctxp = d.seg[(size_t)slot * SW * ps + e]; break;
-            case CP_LOAD_CK: *ctxp = d.ckpt[(size_t)slot * SW * ps + e]; break;
-            case CP_SAVE_CK: d.ckpt[(size_t)slot * SW * ps + e] = *ctxp; break;
-            case CP_SAVE_A: if (lo) d.stA[e] = *ctxp; break;
-            case CP_LOAD_A: if (lo) *ctxp = d.stA[e]; break;
-            case CP_SAVE_B:
-                if (e < ps) d.stB[e] = pool[e];
-                else if (e < 2 * ps) d.stB[e] = pool[8 * ps + (e - ps)];
-                break;
+            case CP_SAVE_SEG: d.seg[at + e] = *ctxp; break;
+            case CP_SAVE_SA: d.sa[at + e] = *ctxp; break;
+            case CP_SAVE_SB: d.sb[at + e] = *ctxp; break;
+            case CP_LOAD_CK: *ctxp = d.ckpt[at + e]; break;
+            case CP_SAVE_CK: d.ckpt[at + e] = *ctxp; break;
             case CP_LOAD_INIT:
                 if (e < 9 * ps) pool[e] = d.init[e];
                 else if (lo) pool[e] = 0.0;
@@ -1379,7 +1368,7 @@ static int bt_upload(cumc_batch *b, batch_tape *bt)
         a.dt = c->dt_dev;
         a.A = tp->A;
         for (int r = 0; r < 3; r++) a.lu[r] = tp->lu[r];
-        a.seg = tp->seg; a.stA = tp->stA; a.stB = tp->stB; a.ckpt = tp->ckpt; a.init = tp->init;
+        a.seg = tp->seg; a.sa = tp->sa; a.sb = tp->sb; a.ckpt = tp->ckpt; a.init = tp->init;
         a.gx_c = (int)nblk(L.px * (L.ny + 1), 256);
         a.gx_e = (int)nblk(L.px * (L.ny + 2), 256);
         a.gpen[0] = (int)nblk(npencils<1>(L), 64);
@@ -1428,7 +1417,7 @@ static void bt_copy(cumc_batch *b, batch_tape *bt, int mode, int slot, int par)
 }
 
 template <int DIR>
-static void bt_sweep(cumc_batch *b, batch_tape *bt)
+static void bt_sweep(cumc_batch *b, batch_tape *bt, int stage, int i)
 {
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
@@ -1436,45 +1425,48 @@ static void bt_sweep(cumc_batch *b, batch_tape *bt)
     // the per-pencil walks (each work list is empty when no box takes it)
     if (bt->total[Q_L1 + DIR - 1] > 0) {
         if (bt->recon == CUMC_PPMC)
-            hipLaunchKernelGGL((kbr_flux_lane<DIR, 4>), bt_grid(bt, Q_L1 + DIR - 1), dim3(256), 0, s, bt->d, bt->pre, nb);
+            hipLaunchKernelGGL((kbr_flux_lane<DIR, 4>), bt_grid(bt, Q_L1 + DIR - 1), dim3(256), 0, s, bt->d, bt->pre, nb,
+                               stage, i);
         else
-            hipLaunchKernelGGL((kbr_flux_lane<DIR, 1>), bt_grid(bt, Q_L1 + DIR - 1), dim3(256), 0, s, bt->d, bt->pre, nb);
+            hipLaunchKernelGGL((kbr_flux_lane<DIR, 1>), bt_grid(bt, Q_L1 + DIR - 1), dim3(256), 0, s, bt->d, bt->pre, nb,
+                               stage, i);
     }
     if constexpr (DIR != 1) {
         if (bt->total[Q_H1 + DIR - 1] > 0) {
+            const dim3 gh = bt_grid(bt, Q_H1 + DIR - 1), gr = bt_grid(bt, Q_R1 + DIR - 1);
             if (bt->recon == CUMC_PPMC) {
-                hipLaunchKernelGGL((kbr_flux_hll<DIR, 4>), bt_grid(bt, Q_H1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
-                hipLaunchKernelGGL((kbr_flux_recon<DIR, 4>), bt_grid(bt, Q_R1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+                hipLaunchKernelGGL((kbr_flux_hll<DIR, 4>), gh, dim3(64), 0, s, bt->d, bt->pre, nb, stage, i);
+                hipLaunchKernelGGL((kbr_flux_recon<DIR, 4>), gr, dim3(64), 0, s, bt->d, bt->pre, nb, stage, i);
             } else {
-                hipLaunchKernelGGL((kbr_flux_hll<DIR, 1>), bt_grid(bt, Q_H1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
-                hipLaunchKernelGGL((kbr_flux_recon<DIR, 1>), bt_grid(bt, Q_R1 + DIR - 1), dim3(64), 0, s, bt->d, bt->pre, nb);
+                hipLaunchKernelGGL((kbr_flux_hll<DIR, 1>), gh, dim3(64), 0, s, bt->d, bt->pre, nb, stage, i);
+                hipLaunchKernelGGL((kbr_flux_recon<DIR, 1>), gr, dim3(64), 0, s, bt->d, bt->pre, nb, stage, i);
             }
         }
     }
 }
 
-static void bt_ghost(cumc_batch *b, batch_tape *bt, int sel, int slot)
+static void bt_ghost(cumc_batch *b, batch_tape *bt, int stage, int i)
 {
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
-    hipLaunchKernelGGL(kbr_ghost<2>, bt_grid(bt, Q_GZ), dim3(256), 0, s, bt->d, bt->pre, nb, sel, slot);
-    hipLaunchKernelGGL(kbr_ghost<1>, bt_grid(bt, Q_GY), dim3(256), 0, s, bt->d, bt->pre, nb, sel, slot);
-    hipLaunchKernelGGL(kbr_ghost<0>, bt_grid(bt, Q_GX), dim3(256), 0, s, bt->d, bt->pre, nb, sel, slot);
+    hipLaunchKernelGGL(kbr_ghost<2>, bt_grid(bt, Q_GZ), dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
+    hipLaunchKernelGGL(kbr_ghost<1>, bt_grid(bt, Q_GY), dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
+    hipLaunchKernelGGL(kbr_ghost<0>, bt_grid(bt, Q_GX), dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
 }
 
 template <int STAGE>
-static void bt_reverse_stage(cumc_batch *b, batch_tape *bt, int sel, int slot, int q)
+static void bt_reverse_stage(cumc_batch *b, batch_tape *bt, int i, int q)
 {
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
-    bt_ghost(b, bt, sel, slot);
-    hipLaunchKernelGGL((kbr_tail_a<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q);
+    bt_ghost(b, bt, STAGE, i);
+    hipLaunchKernelGGL((kbr_tail_a<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q, i);
     hipLaunchKernelGGL((kbr_tail_b<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q);
     hipLaunchKernelGGL(kbr_gather, bt_grid(bt, Q_GE), dim3(256), 0, s, bt->d, bt->pre, nb);
-    hipLaunchKernelGGL(kbr_ecorn, bt_grid(bt, Q_GE), dim3(256), 0, s, bt->d, bt->pre, nb);
-    bt_sweep<3>(b, bt);
-    bt_sweep<2>(b, bt);
-    bt_sweep<1>(b, bt);
+    hipLaunchKernelGGL(kbr_ecorn, bt_grid(bt, Q_GE), dim3(256), 0, s, bt->d, bt->pre, nb, STAGE, i);
+    bt_sweep<3>(b, bt, STAGE, i);
+    bt_sweep<2>(b, bt, STAGE, i);
+    bt_sweep<1>(b, bt, STAGE, i);
 }
 
 #define BTCHK(b, call)                                                                          \
@@ -1595,34 +1587,30 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
         const int s1 = s0 + k < N ? s0 + k : N;
         bt_copy(b, bt, CP_LOAD_CK, s0 / k, 0);
         bt_copy(b, bt, CP_SAVE_SEG, 0, 0);
+        // segment forward: every stage's output state is stored, so the reverse below runs
+        // no forward kernel (stage-1 output U is in U1 for even steps, U0 for odd ones)
         for (int m = s0; m < s1; m++) {
-            const int i = m - s0;
+            const int i = m - s0, par = i & 1;
             batch_set_dt(b, m);
-            for (int st = 1; st <= 3; st++) batch_stage(b, st, i & 1);
-            bt_copy(b, bt, CP_SAVE_SEG, i + 1, (i + 1) & 1);
+            batch_stage(b, 1, par);
+            bt_copy(b, bt, CP_SAVE_SA, i, 1 - par);
+            batch_stage(b, 2, par);
+            bt_copy(b, bt, CP_SAVE_SB, i, 0);
+            batch_stage(b, 3, par);
+            bt_copy(b, bt, CP_SAVE_SEG, i + 1, 1 - par);
         }
         for (int m = s1 - 1; m >= s0; m--) {
             const int i = m - s0;
-            bt_copy(b, bt, CP_LOAD_SEG, i, 0);
             batch_set_dt(b, m);
-            batch_stage(b, 1, 0);
-            bt_copy(b, bt, CP_SAVE_A, 0, 0);
-            batch_stage(b, 2, 0);
-            bt_copy(b, bt, CP_SAVE_B, 0, 0);
-            batch_fluxes_ecorn(b);
-            bt_reverse_stage<3>(b, bt, 0, i + 1, q);
-            bt_copy(b, bt, CP_LOAD_A, 0, 0);
-            batch_fluxes_ecorn(b);
-            bt_reverse_stage<2>(b, bt, 1, 0, q);
-            bt_copy(b, bt, CP_LOAD12_SEG, i, 0);
-            batch_fluxes_ecorn(b);
-            bt_reverse_stage<1>(b, bt, 2, 0, q);
+            bt_reverse_stage<3>(b, bt, i, q);
+            bt_reverse_stage<2>(b, bt, i, q);
+            bt_reverse_stage<1>(b, bt, i, q);
             q ^= 1;
         }
         BTCHK(b, hipGetLastError());
     }
     // initial_update: its ghost fill, then k_init on the uploaded state
-    bt_ghost(b, bt, 3, 0);
+    bt_ghost(b, bt, 0, 0);
     bt_copy(b, bt, CP_LOAD_INIT, 0, 0);
     hipLaunchKernelGGL(kbr_init_a, bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q);
     hipLaunchKernelGGL(kbr_init_b, bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb);

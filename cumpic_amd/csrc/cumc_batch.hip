@@ -416,7 +416,6 @@ void batch_set_counter(cumc_batch *b, int v)
 void batch_finddt(cumc_batch *b) { enqueue_finddt_b(b); }
 void batch_step(cumc_batch *b, int par, int frozen) { enqueue_step_b(b, par, frozen); }
 void batch_stage(cumc_batch *b, int st, int par) { enqueue_stage_b(b, st, par); }
-void batch_fluxes_ecorn(cumc_batch *b) { enqueue_fluxes_ecorn_b(b); }
 // current dt of every box := history entry of step m (the counter is left at m + 1)
 void batch_set_dt(cumc_batch *b, int m)
 {
