@@ -623,10 +623,12 @@ __host__ __device__ inline int lane_outer(const Lay &L)
 {
     return DIR == 3 ? L.ny + 2 : L.nz + 2;
 }
-// 256-thread tiles (4 waves of XR_OWN owned cells) per outer index
+// 256-thread tiles per outer index: along x, 4 waves of XR_OWN owned cells of the row run; along
+// y and z, 4 pencils x one XR_OWN-cell segment (kr_flux_lane_t_tile)
 template <int DIR>
 __host__ __device__ inline int lane_tiles(const Lay &L)
 {
+    if (DIR != 1) return ((L.nx + 2 + 3) / 4) * (((DIR == 2 ? L.ny : L.nz) + 6 + XR_OWN - 1) / XR_OWN);
     const long waves = (lane_cells<DIR>(L) + XR_OWN - 1) / XR_OWN;
     return (int)((waves + 3) / 4);
 }
@@ -636,38 +638,18 @@ __host__ __device__ inline bool lane_sweep(const Lay &L)
     return DIR == 1 || (DIR == 2 ? L.ny : L.nz) >= LANE_MIN;
 }
 
+// The lane-parallel reverse of one wavefront: inputs per lane are its cell's 7 reconstructed
+// values q, its face's 7 flux adjoints lfo (stored xyz order; read on face lanes only) and normal
+// field bn; outputs are the adjoint increments lw of its cell values and lbn of its face field
+// (meaningful on the lanes that own them).  c is the lane's cell along the sweep (-100 off the
+// pencils), n the interior cell count along it.  Every lane of the wavefront must call it.
 template <int DIR, int RECON>
-__device__ inline void kr_flux_lane_tile(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M,
-                                         const AdjF &A, int bx, int o)
+__device__ inline void lane_reverse(const Tabs &tb, const Model &M, int n, int lane, int c, bool face,
+                                    const double *q, const double *lfo, double bn, double *lw, double &lbn)
 {
-    const int lane = threadIdx.x & 63;
-    const long wave = ((long)bx * blockDim.x + threadIdx.x) >> 6;
-    const long f = wave * XR_OWN + lane - 5;
-    const bool inr = (f >= 0) && (f < lane_cells<DIR>(L));
-    const long fc = inr ? f : 0;
     const int ax = DIR - 1;
-    const int n = DIR == 1 ? L.nx : (DIR == 2 ? L.ny : L.nz);
-    int c;  // cell index along the sweep
-    unsigned g;
-    if (DIR == 1) {
-        const int k = (int)(fc / L.px);
-        c = (int)(fc - (long)k * L.px) - 2;
-        g = (unsigned)((long)(o + 2) * L.py + 2 * L.px + fc);
-    } else {
-        const int pitch = n + 6;
-        const int j = (int)(fc / pitch);
-        c = (int)(fc - (long)j * pitch) - 2;
-        g = (unsigned)(DIR == 2 ? L.id(j, c, o) : L.id(j, o, c));
-    }
-    if (!inr) c = -100;
     const int cc = c < -2 ? -2 : (c > n + 3 ? n + 3 : c);
-    const long ps = L.ps;
-    const bool own = lane >= 5 && lane <= 58 && inr && c <= n + 3;
-    // a face exists for c in [0, n]; lanes 2..60 have the full forward stencil
-    const bool face = inr && c >= 0 && c <= n && lane >= 2 && lane <= 60;
-    double q[7], dm[7], vp[7], vmr[7];
-#pragma unroll
-    for (int v = 0; v < 7; v++) q[v] = ldi(qplane<DIR>(F, ps, v), g);
+    double dm[7], vp[7], vmr[7];
     // forward: edge values of this cell; the right state of face c is vm of lane+1
 #pragma unroll
     for (int v = 0; v < 7; v++) {
@@ -687,18 +669,15 @@ __device__ inline void kr_flux_lane_tile(const Lay &L, const Fields<double> &F, 
     }
     // HLL reversed at face c
     double lL[7] = {0, 0, 0, 0, 0, 0, 0}, lR[7] = {0, 0, 0, 0, 0, 0, 0};
+    lbn = 0.0;
     if (face) {
-        const double bn = ldi(F.P + (4 + DIR) * ps, g);
-        double o7[7], lf[7];
-#pragma unroll
-        for (int i = 0; i < 7; i++) o7[i] = ldi(A.F[ax] + i * ps, g);
-        flux_to_rot<DIR>(o7, lf);
+        double lf[7];
+        flux_to_rot<DIR>(lfo, lf);
         FaceState<double> a{0, 0, 0, 0, 0, 0, 0, 0}, b{0, 0, 0, 0, 0, 0, 0, 0};
         hll_face_adj(M.ggas, rot<DIR>(vp, bn), rot<DIR>(vmr, bn), lf, a, b);
         unrot<DIR>(a, lL);
         unrot<DIR>(b, lR);
-        double *pb = A.P + (4 + DIR) * ps;
-        if (own) sti(pb, g, ldi(pb, g) + (a.bn + b.bn));
+        lbn = a.bn + b.bn;
     }
     // reconstruction reversed, one variable at a time
 #pragma unroll
@@ -707,7 +686,6 @@ __device__ inline void kr_flux_lane_tile(const Lay &L, const Fields<double> &F, 
         const double lvm = __shfl_up(lR[v], 1, 64);  // from face c-1
         const double w = q[v], wm1 = __shfl_up(w, 1, 64), wp1 = __shfl_down(w, 1, 64);
         double am1 = 0.0, a0 = 0.0, ap1 = 0.0, em1 = 0.0, e0 = 0.0, ep1 = 0.0;
-        double lw;
         // only cells 0..n+1 are reconstructed and only slopes -1..n+2 are used; elsewhere the
         // table entries are not defined and the forward values there are never read
         const bool recon_cell = c >= 0 && c <= n + 1;
@@ -723,22 +701,132 @@ __device__ inline void kr_flux_lane_tile(const Lay &L, const Fields<double> &F, 
             double bm1 = 0.0, b0 = 0.0, bp1 = 0.0;
             if (c >= -1 && c <= n + 2)
                 ppm_slope_adj(tb.at(ax, TB_B0C, cc), tb.at(ax, TB_B1C, cc), wm1, w, wp1, ldm, bm1, b0, bp1);
-            lw = a0 + __shfl_down(am1, 1, 64) + __shfl_up(ap1, 1, 64) + b0 + __shfl_down(bm1, 1, 64) +
-                 __shfl_up(bp1, 1, 64);
+            lw[v] = a0 + __shfl_down(am1, 1, 64) + __shfl_up(ap1, 1, 64) + b0 + __shfl_down(bm1, 1, 64) +
+                    __shfl_up(bp1, 1, 64);
         } else {
             if (recon_cell)
                 tvd_edges_adj(tb.at(ax, TB_CF, cc), tb.at(ax, TB_CB, cc), wm1, w, wp1, lvm, lvp, am1, a0, ap1);
-            lw = a0 + __shfl_down(am1, 1, 64) + __shfl_up(ap1, 1, 64);
+            lw[v] = a0 + __shfl_down(am1, 1, 64) + __shfl_up(ap1, 1, 64);
         }
-        double *lp = qplane_adj<DIR>(A, ps, v);
-        if (own) sti(lp, g, ldi(lp, g) + lw);
     }
+}
+
+// x sweep: a wavefront's lanes load 64 consecutive cells of a row (coalesced) straight from HBM
+template <int RECON>
+__device__ inline void kr_flux_lane_x_tile(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M,
+                                           const AdjF &A, int bx, int o)
+{
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)bx * blockDim.x + threadIdx.x) >> 6;
+    const long f = wave * XR_OWN + lane - 5;
+    const bool inr = (f >= 0) && (f < lane_cells<1>(L));
+    const long fc = inr ? f : 0;
+    const int n = L.nx;
+    const int k = (int)(fc / L.px);
+    const int c = inr ? (int)(fc - (long)k * L.px) - 2 : -100;
+    const unsigned g = (unsigned)((long)(o + 2) * L.py + 2 * L.px + fc);
+    const long ps = L.ps;
+    const bool own = lane >= 5 && lane <= 58 && inr && c <= n + 3;
+    // a face exists for c in [0, n]; lanes 2..60 have the full forward stencil
+    const bool face = inr && c >= 0 && c <= n && lane >= 2 && lane <= 60;
+    double q[7], lfo[7] = {0, 0, 0, 0, 0, 0, 0}, bn = 0.0;
+#pragma unroll
+    for (int v = 0; v < 7; v++) q[v] = ldi(qplane<1>(F, ps, v), g);
+    if (face) {
+        bn = ldi(F.P + 5 * ps, g);
+#pragma unroll
+        for (int i = 0; i < 7; i++) lfo[i] = ldi(A.F[0] + i * ps, g);
+    }
+    double lw[7], lbn;
+    lane_reverse<1, RECON>(tb, M, n, lane, c, face, q, lfo, bn, lw, lbn);
+    if (own) {
+        if (face) sti(A.P + 5 * ps, g, ldi(A.P + 5 * ps, g) + lbn);
+#pragma unroll
+        for (int v = 0; v < 7; v++) {
+            double *lp = qplane_adj<1>(A, ps, v);
+            sti(lp, g, ldi(lp, g) + lw[v]);
+        }
+    }
+}
+
+// y / z sweeps: a 256-thread workgroup takes 4 neighbouring pencils (columns j0..j0+3 at outer
+// index o) over 64 consecutive cells.  The 15 input planes of that 64 x 4 tile are loaded row by
+// row (4 consecutive cells of an x row per load) into LDS, each wavefront runs the lane-parallel
+// reverse on one pencil from LDS, and the cell adjoints return through LDS to row-wise
+// read-add-writes of the owned cells.  Every thread reaches the barriers.
+constexpr int LT_PITCH = 5;  // LDS row pitch of a 4-pencil tile (one pad double)
+constexpr int LT_PLANES = 15;
+
+template <int DIR>
+__host__ __device__ inline int lane_t_segs(const Lay &L)
+{
+    return ((DIR == 2 ? L.ny : L.nz) + 6 + XR_OWN - 1) / XR_OWN;
+}
+
+template <int DIR, int RECON>
+__device__ inline void kr_flux_lane_t_tile(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M,
+                                           const AdjF &A, int bx, int o, double (*buf)[64][LT_PITCH])
+{
+    const int n = DIR == 2 ? L.ny : L.nz;
+    const int nseg = lane_t_segs<DIR>(L);
+    const int seg = bx % nseg, j0 = (bx / nseg) * 4;
+    const long ps = L.ps;
+    auto cell = [&](int j, int c) { return (unsigned)(DIR == 2 ? L.id(j, c, o) : L.id(j, o, c)); };
+    // row r of the tile is pencil cell f = seg * XR_OWN + r - 5 (c = f - 2)
+    const int tr = threadIdx.x >> 2, tp = threadIdx.x & 3;
+    const int tf = seg * XR_OWN + tr - 5, tj = j0 + tp;
+    const bool tin = tf >= 0 && tf < n + 6 && tj <= L.nx + 1;
+    const unsigned tg = tin ? cell(tj, tf - 2) : 0u;
+#pragma unroll
+    for (int v = 0; v < 7; v++) buf[v][tr][tp] = tin ? ldi(qplane<DIR>(F, ps, v), tg) : 0.0;
+#pragma unroll
+    for (int i = 0; i < 7; i++) buf[7 + i][tr][tp] = tin ? ldi(A.F[DIR - 1] + i * ps, tg) : 0.0;
+    buf[14][tr][tp] = tin ? ldi(F.P + (4 + DIR) * ps, tg) : 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int f = seg * XR_OWN + lane - 5;
+    const bool inr = f >= 0 && f < n + 6 && j0 + w <= L.nx + 1;
+    const int c = inr ? f - 2 : -100;
+    const bool face = inr && c >= 0 && c <= n && lane >= 2 && lane <= 60;
+    double q[7], lfo[7];
+#pragma unroll
+    for (int v = 0; v < 7; v++) {
+        q[v] = buf[v][lane][w];
+        lfo[v] = buf[7 + v][lane][w];
+    }
+    const double bn = buf[14][lane][w];
+    double lw[7], lbn;
+    lane_reverse<DIR, RECON>(tb, M, n, lane, c, face, q, lfo, bn, lw, lbn);
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 7; v++) buf[v][lane][w] = lw[v];
+    buf[7][lane][w] = lbn;
+    __syncthreads();
+    // owned rows 5..58 of the tile; a face exists at c in [0, n]
+    if (tin && tr >= 5 && tr <= 58) {
+        const int tc = tf - 2;
+        if (tc >= 0 && tc <= n) {
+            double *pb = A.P + (4 + DIR) * ps;
+            sti(pb, tg, ldi(pb, tg) + buf[7][tr][tp]);
+        }
+#pragma unroll
+        for (int v = 0; v < 7; v++) {
+            double *lp = qplane_adj<DIR>(A, ps, v);
+            sti(lp, tg, ldi(lp, tg) + buf[v][tr][tp]);
+        }
+    }
+    __syncthreads();  // the next tile of a persistent loop reuses buf
 }
 
 template <int DIR, int RECON>
 __global__ __launch_bounds__(256) void kr_flux_lane(Lay L, Fields<double> F, Tabs tb, Model M, AdjF A)
 {
-    kr_flux_lane_tile<DIR, RECON>(L, F, tb, M, A, blockIdx.x, blockIdx.y);
+    if constexpr (DIR == 1) {
+        kr_flux_lane_x_tile<RECON>(L, F, tb, M, A, blockIdx.x, blockIdx.y);
+    } else {
+        __shared__ double buf[LT_PLANES][64][LT_PITCH];
+        kr_flux_lane_t_tile<DIR, RECON>(L, F, tb, M, A, blockIdx.x, blockIdx.y, buf);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1282,10 +1370,19 @@ template <int DIR, int RECON>
 __global__ __launch_bounds__(256) void kbr_flux_lane(const AdjBox *__restrict__ D, const int *__restrict__ pre,
                                                      int nbox, int stage, int i)
 {
-    FOR_ITEMS(Q_L1 + DIR - 1)
-        const int gl = d.glane[DIR - 1];
-        const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
-        kr_flux_lane_tile<DIR, RECON>(d.L, F, d.tb, d.M, d.A, loc % gl, loc / gl);
+    if constexpr (DIR == 1) {
+        FOR_ITEMS(Q_L1)
+            const int gl = d.glane[0];
+            const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
+            kr_flux_lane_x_tile<RECON>(d.L, F, d.tb, d.M, d.A, loc % gl, loc / gl);
+        }
+    } else {
+        __shared__ double buf[LT_PLANES][64][LT_PITCH];
+        FOR_ITEMS(Q_L1 + DIR - 1)
+            const int gl = d.glane[DIR - 1];
+            const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
+            kr_flux_lane_t_tile<DIR, RECON>(d.L, F, d.tb, d.M, d.A, loc % gl, loc / gl, buf);
+        }
     }
 }
 
