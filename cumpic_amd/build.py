@@ -63,16 +63,28 @@ FHOST_DIR = os.path.join(PKG, "fhost")
 FHOST_EXE = os.path.join(FHOST_DIR, "cumc3d")
 
 
+H5_LIB = os.path.join(FHOST_DIR, "libcumc_h5.so")
+
+
+def build_h5profile() -> str:
+    """HDF5 profile writer of the host (h5profile.c; opens libhdf5 at run time)."""
+    tmp = H5_LIB + ".tmp"
+    _run(["gcc", "-O2", "-shared", "-fPIC", "-Wall", "-o", tmp, os.path.join(FHOST_DIR, "h5profile.c"), "-ldl"])
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
+
+
 def build_fortran_host() -> str | None:
     """Fortran driver over the C ABI (flang); returns the executable path or None without flang."""
     fc = flang()
     if fc is None:
         return None
+    build_h5profile()
     srcs = [os.path.join(FHOST_DIR, f) for f in ("cumc_iface.f90", "model_file.f90", "cumc3d.f90")]
     mod = os.path.join(FHOST_DIR, "_mod")
     os.makedirs(mod, exist_ok=True)
-    cmd = [fc, "-O2", "-J", mod, *srcs, "-L", PKG, "-lcumc", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/..",
-           "-o", FHOST_EXE + ".tmp"]
+    cmd = [fc, "-O2", "-J", mod, *srcs, "-L", PKG, "-lcumc", "-L", FHOST_DIR, "-lcumc_h5",
+           "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,$ORIGIN", "-o", FHOST_EXE + ".tmp"]
     _run(cmd)
     os.replace(FHOST_EXE + ".tmp", FHOST_EXE)
     return FHOST_EXE

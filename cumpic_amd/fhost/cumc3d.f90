@@ -6,9 +6,13 @@
 ! print "n_step, dt, global_time"; finally report the divergence of B.  The hot path runs in
 ! libcumc.so through the ISO_C_BINDING interface of cumc_iface.f90.
 !
-! Usage: cumc3d <run file> <output file>
+! Usage: cumc3d <run file> <output file> [<profile directory>]
 ! Output (stream): int32 n_step ; float64 global_time, divb_signed, divb_abs ;
 !                  prim(8,nx+6,ny+6,nz+6) ; eps(nx+6,ny+6,nz+6) ; dt(n_step)
+! With a profile directory, HDF5 profiles star_weno_data_<n_iter>.hdf5 in the reference's layout
+! (dataio.f90) are written there at the reference's points (main.f90:107-110, 150-160, 165-168):
+! after initial_update, whenever |global_time - time of the last profile| >= output_profiletime,
+! and at the end.
 !-----------------------------------------------------------------------------------------------
 PROGRAM CUMC3D
 USE ISO_C_BINDING
@@ -20,20 +24,21 @@ TYPE(cumc_config) :: cfg
 TYPE(C_PTR) :: ctx
 REAL(C_DOUBLE), ALLOCATABLE :: xf(:), yf(:), zf(:), prim(:,:,:,:), eps(:,:,:), bcell(:,:,:,:)
 REAL(C_DOUBLE), ALLOCATABLE :: dts(:)
-REAL(C_DOUBLE) :: dt, global_time, total_time, maxs, maxa
-INTEGER :: max_steps, n_step, u
+REAL(C_DOUBLE) :: dt, global_time, total_time, maxs, maxa, output_time, output_last
+INTEGER :: max_steps, n_step, u, n_iter
 INTEGER(C_INT) :: ierr
-CHARACTER(LEN=1024) :: infile, outfile
+CHARACTER(LEN=1024) :: infile, outfile, profdir
 
 CALL GET_COMMAND_ARGUMENT(1, infile)
 CALL GET_COMMAND_ARGUMENT(2, outfile)
+CALL GET_COMMAND_ARGUMENT(3, profdir)
 IF (LEN_TRIM(infile) == 0 .OR. LEN_TRIM(outfile) == 0) THEN
   WRITE(*,*) 'usage: cumc3d <run file> <output file>'
   STOP 2
 END IF
 
 ! setup: model, grid, device residency (main.f90:57-99)
-CALL GET_MODEL(TRIM(infile), cfg, xf, yf, zf, prim, eps, max_steps, total_time)
+CALL GET_MODEL(TRIM(infile), cfg, xf, yf, zf, prim, eps, max_steps, total_time, output_time)
 ctx = C_NULL_PTR
 ierr = cumc_create(cfg, ctx)
 CALL CUMC_CHECK(ierr, 'cumc_create', ctx)
@@ -41,6 +46,12 @@ CALL CUMC_CHECK(cumc_set_grid(ctx, xf, yf, zf), 'cumc_set_grid', ctx)
 CALL CUMC_CHECK(cumc_upload(ctx, prim, eps), 'cumc_upload', ctx)
 CALL CUMC_CHECK(cumc_initial_update(ctx), 'cumc_initial_update', ctx)
 ALLOCATE(dts(MAX(max_steps, 1)))
+ALLOCATE(bcell(1:3, -2:cfg%nx+3, -2:cfg%ny+3, -2:cfg%nz+3))
+
+! print primitive profiles (main.f90:107-110)
+n_iter = 0
+output_last = 0.0D0
+CALL PRINT_PROFILE()
 
 WRITE(*,*) 'We now start the time evolution'
 WRITE(*,*) '------------------------------------'
@@ -60,9 +71,22 @@ DO WHILE (global_time < total_time .AND. n_step < max_steps)
   global_time = global_time + dt
   dts(n_step) = dt
   WRITE(*,*) n_step, dt, global_time
+  ! data I/O (main.f90:150-160)
+  IF (ABS(global_time - output_last) >= output_time) THEN
+    output_last = global_time
+    CALL PRINT_PROFILE()
+  END IF
 END DO
 
-ALLOCATE(bcell(1:3, -2:cfg%nx+3, -2:cfg%ny+3, -2:cfg%nz+3))
+! output again (main.f90:165-168), without advancing the counter
+IF (LEN_TRIM(profdir) > 0) THEN
+  CALL CUMC_CHECK(cumc_download(ctx, prim, eps, bcell), 'cumc_download', ctx)
+  IF (cumc_h5_profile(TRIM(profdir) // C_NULL_CHAR, INT(n_iter, C_INT), cfg%nx, cfg%ny, cfg%nz, &
+                      xf, yf, zf, prim, eps) /= 0) THEN
+    WRITE(*,*) 'cumc3d: HDF5 profile output failed (libhdf5 not found or directory not writable)'
+    STOP 1
+  END IF
+END IF
 CALL CUMC_CHECK(cumc_download(ctx, prim, eps, bcell), 'cumc_download', ctx)
 CALL CUMC_CHECK(cumc_find_divb(ctx, maxs, maxa), 'cumc_find_divb', ctx)
 WRITE(*,*)
@@ -81,4 +105,18 @@ ierr = cumc_destroy(ctx)
 WRITE(*,*) '----------------'
 WRITE(*,*) '-Simulation End-'
 WRITE(*,*) '----------------'
+
+CONTAINS
+
+! print_hydroprofile (dataio.f90:7-77): the state to the host, one HDF5 file, n_iter + 1
+SUBROUTINE PRINT_PROFILE()
+  IF (LEN_TRIM(profdir) == 0) RETURN
+  CALL CUMC_CHECK(cumc_download(ctx, prim, eps, bcell), 'cumc_download', ctx)
+  IF (cumc_h5_profile(TRIM(profdir) // C_NULL_CHAR, INT(n_iter, C_INT), cfg%nx, cfg%ny, cfg%nz, &
+                      xf, yf, zf, prim, eps) /= 0) THEN
+    WRITE(*,*) 'cumc3d: HDF5 profile output failed (libhdf5 not found or directory not writable)'
+    STOP 1
+  END IF
+  n_iter = n_iter + 1
+END SUBROUTINE
 END PROGRAM CUMC3D

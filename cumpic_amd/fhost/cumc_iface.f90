@@ -76,6 +76,14 @@ INTERFACE
     IMPORT :: C_PTR
     TYPE(C_PTR), VALUE :: ctx
   END FUNCTION
+  ! profile output in the reference's HDF5 layout (h5profile.c; replaces dataio.f90:7-77)
+  INTEGER(C_INT) FUNCTION cumc_h5_profile(dir, n_iter, nx, ny, nz, xf, yf, zf, prim, eps) &
+      BIND(C, NAME="cumc_h5_profile")
+    IMPORT :: C_INT, C_CHAR, C_DOUBLE
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: dir(*)
+    INTEGER(C_INT), VALUE :: n_iter, nx, ny, nz
+    REAL(C_DOUBLE), INTENT(IN) :: xf(*), yf(*), zf(*), prim(*), eps(*)
+  END FUNCTION
 END INTERFACE
 
 CONTAINS

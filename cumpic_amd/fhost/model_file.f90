@@ -6,8 +6,9 @@
 ! the GPU and are selected by cfg%model.
 !
 ! Run file (little-endian stream): 16 int32  nx ny nz coordinate bflag(6) recon model
-!   geom_source_repeat precision max_steps pad ; 6 float64 ggas cfl rho_floor eps_floor r_sh
-!   total_time ; xf(nx+7) yf(ny+7) zf(nz+7) ; prim(8,nx+6,ny+6,nz+6) ; eps(nx+6,ny+6,nz+6)
+!   geom_source_repeat precision max_steps pad ; 7 float64 ggas cfl rho_floor eps_floor r_sh
+!   total_time output_profiletime ; xf(nx+7) yf(ny+7) zf(nz+7) ; prim(8,nx+6,ny+6,nz+6) ;
+!   eps(nx+6,ny+6,nz+6)
 !-----------------------------------------------------------------------------------------------
 MODULE MODEL_FILE
 USE ISO_C_BINDING
@@ -16,15 +17,15 @@ IMPLICIT NONE
 
 CONTAINS
 
-SUBROUTINE GET_MODEL(path, cfg, xf, yf, zf, prim, eps, max_steps, total_time)
+SUBROUTINE GET_MODEL(path, cfg, xf, yf, zf, prim, eps, max_steps, total_time, output_time)
   CHARACTER(*), INTENT(IN) :: path
   TYPE(cumc_config), INTENT(OUT) :: cfg
   REAL(C_DOUBLE), ALLOCATABLE, INTENT(OUT) :: xf(:), yf(:), zf(:)
   REAL(C_DOUBLE), ALLOCATABLE, INTENT(OUT) :: prim(:,:,:,:), eps(:,:,:)
   INTEGER, INTENT(OUT) :: max_steps
-  REAL(C_DOUBLE), INTENT(OUT) :: total_time
+  REAL(C_DOUBLE), INTENT(OUT) :: total_time, output_time
   INTEGER(C_INT) :: hdr(16)
-  REAL(C_DOUBLE) :: rh(6)
+  REAL(C_DOUBLE) :: rh(7)
   INTEGER :: u, nx, ny, nz
 
   OPEN(NEWUNIT=u, FILE=path, ACCESS='STREAM', FORM='UNFORMATTED', STATUS='OLD', ACTION='READ')
@@ -42,6 +43,7 @@ SUBROUTINE GET_MODEL(path, cfg, xf, yf, zf, prim, eps, max_steps, total_time)
   cfg%ggas = rh(1); cfg%cfl = rh(2)
   cfg%rho_floor = rh(3); cfg%eps_floor = rh(4); cfg%r_sh = rh(5)
   total_time = rh(6)
+  output_time = rh(7)
   ALLOCATE(xf(-3:nx+3), yf(-3:ny+3), zf(-3:nz+3))
   ALLOCATE(prim(1:8, -2:nx+3, -2:ny+3, -2:nz+3))
   ALLOCATE(eps(-2:nx+3, -2:ny+3, -2:nz+3))

@@ -36,3 +36,50 @@ def test_fortran_host_riemann_1d(tmp_path):
     assert np.array_equal(res["dts"], dts)
     assert np.array_equal(res["prim"], so["prim"])
     assert np.array_equal(res["eps"], so["eps"])
+
+
+def test_fortran_host_hdf5_profiles(tmp_path):
+    """HDF5 profiles written by the host at the reference's points (main.f90:107-110, 150-160,
+    165-168): after initial_update as profile 0, whenever |t - t_last| >= output_profiletime
+    (n_iter advancing), and at the end under the next n_iter; each equal bit for bit to the CPU
+    restatement's state at that step, in the dataio.f90 layout."""
+    from cumpic_amd import dataio
+    print("[case] Fortran host HDF5 profiles, 1D Riemann, output every 0.03", flush=True)
+    exe = build.FHOST_EXE
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (no flang)")
+    try:
+        dataio._hdf5()
+    except RuntimeError:
+        pytest.skip("the HDF5 C library is not available")
+    p = ics.riemann_1d()
+    out_time = 0.03
+    run, out, prof = tmp_path / "c0.run", tmp_path / "c0.out", tmp_path / "outfile"
+    prof.mkdir()
+    write_runfile(str(run), p, max_steps=100000, total_time=p.total_time, output_time=out_time)
+    r = subprocess.run([exe, str(run), str(out), str(prof)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    res = read_output(str(out), p.cfg)
+    o = Oracle.from_problem(p)
+    o.initial_update()
+    expect = {0: o.get_state()}
+    n_iter, t, last = 1, 0.0, 0.0
+    for dt in res["dts"]:
+        o.rungekutta(float(dt))
+        t += float(dt)
+        if abs(t - last) >= out_time:
+            last = t
+            expect[n_iter] = o.get_state()
+            n_iter += 1
+    expect[n_iter] = o.get_state()
+    assert len(expect) >= 4
+    assert sorted(os.listdir(prof)) == sorted(dataio.profile_name(k) for k in expect)
+    c = p.cfg
+    inner = (slice(3, 3 + c.nz), slice(3, 3 + c.ny), slice(3, 3 + c.nx))
+    for k, st in expect.items():
+        got = dataio.read_profile(str(prof / dataio.profile_name(k)))
+        assert set(got) == {"xf", "prim", "eps"}
+        assert np.array_equal(got["prim"], st["prim"][inner]), k
+        assert np.array_equal(got["eps"], st["eps"][inner]), k
+        assert np.array_equal(got["xf"], p.xf[3:3 + c.nx + 1])
