@@ -425,6 +425,7 @@ def main():
         "step_roofline": {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE, "achieved_GBps": step_alg / 1e9,
                           "frac": step_alg / HBM_PEAK},
         "kernel_ms_per_step": m["kernel_ms_per_step"],
+        "tuning": m["tuning"],
         "divb_max_abs": m["divb_max_abs"],
         "state_finite": m["state_finite"],
         "parity": parity,
@@ -435,7 +436,8 @@ def main():
         out["mixed"] = {"value": mx["value"], "ms_per_step": mx["ms_per_step"],
                         "speedup_vs_fp64": mx["value"] / m["value"], "rel_l1_vs_fp64": mx.get("rel_l1_vs_fp64"),
                         "divb_max_abs": mx["divb_max_abs"], "state_finite": mx["state_finite"],
-                        "roofline": mx["roofline"], "kernel_ms_per_step": mx["kernel_ms_per_step"]}
+                        "roofline": mx["roofline"], "kernel_ms_per_step": mx["kernel_ms_per_step"],
+                        "tuning": mx["tuning"]}
     out["batch"] = batch
     print(json.dumps(out), flush=True)
 

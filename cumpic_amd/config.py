@@ -16,7 +16,7 @@ NGHOST = 3  # solver/definition.f90:54
 NV = 8  # rho, vx, vy, vz, p, bx, by, bz  (solver/setup.f90:49-116)
 IRHO, IVX, IVY, IVZ, ITAU, IBX, IBY, IBZ = range(8)
 
-CARTESIAN, CYLINDRICAL = 1, 2  # definition.f90:65-66
+CARTESIAN, CYLINDRICAL, SPHERICAL = 1, 2, 3  # definition.f90:65-67
 TVDMM, PPMC = 1, 4  # definition.f90:158,167
 MODEL_RIEMANN, MODEL_TORUS_CYN = 0, 1
 FP64, MIXED = 0, 1

@@ -189,7 +189,10 @@ void launch_tail(cumc_ctx *ctx, const Fields<R> &F, const double *Uin, double *U
     dim3 grid(xcd_blocks(gx, gy));
     Tabs tb = tabs_of(ctx);
     unsigned long long *dtn = (unsigned long long *)(ctx->dt_dev + 1);
-    if (ctx->M.coordinate == CUMC_CYLINDRICAL)
+    if (ctx->M.coordinate == CUMC_SPHERICAL)
+        hipLaunchKernelGGL((k_tail<R, STAGE, 3>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
+                           Uin, Uout, dtn, gx, gy, zc, tw);
+    else if (ctx->M.coordinate == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((k_tail<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
                            Uin, Uout, dtn, gx, gy, zc, tw);
     else

@@ -57,7 +57,8 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     *out = nullptr;
     const cumc_config &c = *cfg;
     if (c.nx < 1 || c.ny < 1 || c.nz < 1) return CUMC_EINVAL;
-    if (c.coordinate != CUMC_CARTESIAN && c.coordinate != CUMC_CYLINDRICAL) return CUMC_EINVAL;
+    if (c.coordinate != CUMC_CARTESIAN && c.coordinate != CUMC_CYLINDRICAL && c.coordinate != CUMC_SPHERICAL)
+        return CUMC_EINVAL;
     if (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM) return CUMC_EINVAL;
     if (c.model != CUMC_MODEL_RIEMANN && c.model != CUMC_MODEL_TORUS_CYN) return CUMC_EINVAL;
     if (c.precision != CUMC_FP64 && c.precision != CUMC_MIXED) return CUMC_EINVAL;
@@ -175,6 +176,22 @@ static std::vector<double> build_tables(const double *f_in, int n)
         T(TB_D, j) = D(j);
         T(TB_C, j) = 0.5 * (F(j) + F(j - 1));
     }
+    // spherical factors, each with the expression coords.f90 evaluates per call (x**2 = x*x,
+    // x**3 = x*x*x), so the kernels combine bit-identical values
+    auto p2 = [](double x) { return x * x; };
+    auto p3 = [](double x) { return x * x * x; };
+    for (int j = -3; j <= n + 3; j++) {
+        T(TB_F2, j) = p2(F(j));
+        T(TB_SIN, j) = std::sin(F(j));
+    }
+    for (int j = -2; j <= n + 3; j++) {
+        T(TB_V3, j) = (p3(F(j)) - p3(F(j - 1))) / 3.0;
+        T(TB_RA, j) = 2.0 * (p3(F(j)) - p3(F(j - 1))) / 3.0 / (p2(F(j)) - p2(F(j - 1)));
+        T(TB_D2, j) = p2(F(j)) - p2(F(j - 1));
+        T(TB_CD, j) = std::cos(F(j - 1)) - std::cos(F(j));
+        T(TB_TANC, j) = std::tan(0.5 * (F(j) + F(j - 1)));
+        T(TB_SINC, j) = std::sin(0.5 * (F(j) + F(j - 1)));
+    }
     for (int j = -1; j <= n + 2; j++) {
         double b0, b1;
         ppm_slope_coef(D(j - 1), D(j), D(j + 1), &b0, &b1);
@@ -209,6 +226,8 @@ extern "C" int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, 
     }
     if (ctx->cfg.coordinate == CUMC_CYLINDRICAL && !(xf[3] > 0.0))
         return fail(ctx, CUMC_EINVAL, "cylindrical grid needs R > 0 on the inner face");
+    if (ctx->cfg.coordinate == CUMC_SPHERICAL && !(xf[3] > 0.0 && yf[3] > 0.0 && yf[ctx->L.ny + 3] < 3.14159265358979323846))
+        return fail(ctx, CUMC_EINVAL, "spherical grid needs r > 0 and 0 < theta < pi on the boundary faces");
     ctx->grid_set = true;
     return CUMC_OK;
 }
@@ -334,7 +353,9 @@ static void enqueue_finddt(cumc_ctx *ctx)
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, out, bits, 1);
     dim3 grid(nblk((long)L.px * L.ny, 256));
     Tabs tb = tabs_of(ctx);
-    if (ctx->M.coordinate == CUMC_CYLINDRICAL)
+    if (ctx->M.coordinate == CUMC_SPHERICAL)
+        hipLaunchKernelGGL((k_finddt<3>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M, out);
+    else if (ctx->M.coordinate == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((k_finddt<2>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M, out);
     else
         hipLaunchKernelGGL((k_finddt<1>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->M, out);
@@ -425,7 +446,9 @@ extern "C" int cumc_find_divb(cumc_ctx *ctx, double *max_signed, double *max_abs
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->red, 0ull, 2);
     dim3 grid(nblk((long)L.px * L.ny, 256));
     Tabs tb = tabs_of(ctx);
-    if (ctx->M.coordinate == CUMC_CYLINDRICAL)
+    if (ctx->M.coordinate == CUMC_SPHERICAL)
+        hipLaunchKernelGGL((k_divb<3>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->red);
+    else if (ctx->M.coordinate == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((k_divb<2>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->red);
     else
         hipLaunchKernelGGL((k_divb<1>), grid, dim3(256), 0, ctx->stream, L, ctx->F64, tb, ctx->red);

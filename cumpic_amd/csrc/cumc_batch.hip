@@ -317,7 +317,9 @@ static void enqueue_finddt_b(cumc_batch *b)
     hipStream_t s = b->stream;
     hipLaunchKernelGGL(kb_dt_reset, dim3(1), dim3(256), 0, s, b->d_desc, b->nbox);
     bmark(b, P_DT);
-    if (b->coord == CUMC_CYLINDRICAL)
+    if (b->coord == CUMC_SPHERICAL)
+        hipLaunchKernelGGL((kb_finddt<3>), dim3(grid_of(b, P_DT)), dim3(256), 0, s, b->d_desc, pre_of(b, P_DT), b->nbox);
+    else if (b->coord == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((kb_finddt<2>), dim3(grid_of(b, P_DT)), dim3(256), 0, s, b->d_desc, pre_of(b, P_DT), b->nbox);
     else
         hipLaunchKernelGGL((kb_finddt<1>), dim3(grid_of(b, P_DT)), dim3(256), 0, s, b->d_desc, pre_of(b, P_DT), b->nbox);
@@ -328,7 +330,9 @@ template <int STAGE>
 static void launch_tail_b(cumc_batch *b, int par)
 {
     const unsigned g = grid_of(b, P_TAIL);
-    if (b->coord == CUMC_CYLINDRICAL)
+    if (b->coord == CUMC_SPHERICAL)
+        hipLaunchKernelGGL((kb_tail<STAGE, 3>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
+    else if (b->coord == CUMC_CYLINDRICAL)
         hipLaunchKernelGGL((kb_tail<STAGE, 2>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
     else
         hipLaunchKernelGGL((kb_tail<STAGE, 1>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);

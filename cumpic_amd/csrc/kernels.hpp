@@ -14,9 +14,13 @@
 namespace cumc {
 
 constexpr int NG = 3;
-constexpr int NTAB = 15;  // per-axis table rows
+constexpr int NTAB = 23;  // per-axis table rows
 enum { TB_A0M = 0, TB_A1M, TB_A2M, TB_A3M, TB_A0P, TB_A1P, TB_A2P, TB_A3P, TB_B0C, TB_B1C, TB_CF, TB_CB,
-       TB_F, TB_D, TB_C };
+       TB_F, TB_D, TB_C,
+       // spherical factors of coords.f90:139-160, 213-219, 284-298, 388-411 (Fortran x**2 = x*x,
+       // x**3 = x*x*x): F^2; (F_j^3 - F_{j-1}^3)/3; 2 (F_j^3 - F_{j-1}^3)/3/(F_j^2 - F_{j-1}^2);
+       // F_j^2 - F_{j-1}^2; sin F; cos F_{j-1} - cos F_j; tan and sin of the cell centre
+       TB_F2, TB_V3, TB_RA, TB_D2, TB_SIN, TB_CD, TB_TANC, TB_SINC };
 
 struct Lay {
     int nx, ny, nz;
@@ -426,18 +430,28 @@ __device__ inline double rk(double u0, double u1, double dt, double lrk)
 template <int COORD>
 __device__ inline double rhs_bx(const Tabs &tb, int j, int k, int l, double ez, double ez_km, double ey, double ey_lm)
 {
+    if (COORD == 3) {
+        const double cez = tb.at(0, TB_F, j) * tb.at(1, TB_CD, k);
+        const double cey = tb.at(0, TB_F, j) * tb.at(1, TB_CD, k) * tb.at(2, TB_D, l) / tb.at(1, TB_D, k);
+        return -((ez * tb.at(1, TB_SIN, k) - ez_km * tb.at(1, TB_SIN, k - 1)) / cez - (ey * 1.0 - ey_lm * 1.0) / cey);
+    }
     const double cez = COORD == 2 ? tb.at(1, TB_D, k) * tb.at(0, TB_F, j) : tb.at(1, TB_D, k);
     return -((ez * 1.0 - ez_km * 1.0) / cez - (ey * 1.0 - ey_lm * 1.0) / tb.at(2, TB_D, l));
 }
 template <int COORD>
-__device__ inline double rhs_by(const Tabs &tb, int j, int l, double ex, double ex_lm, double ez, double ez_jm)
+__device__ inline double rhs_by(const Tabs &tb, int j, int k, int l, double ex, double ex_lm, double ez, double ez_jm)
 {
+    if (COORD == 3) {
+        const double xl = tb.at(0, TB_C, j);
+        return -((ex * 1.0 - ex_lm * 1.0) / (xl * tb.at(1, TB_SIN, k) * tb.at(2, TB_D, l)) -
+                 (ez * tb.at(0, TB_F, j) - ez_jm * tb.at(0, TB_F, j - 1)) / (xl * tb.at(0, TB_D, j)));
+    }
     return -((ex * 1.0 - ex_lm * 1.0) / tb.at(2, TB_D, l) - (ez * 1.0 - ez_jm * 1.0) / tb.at(0, TB_D, j));
 }
 template <int COORD>
 __device__ inline double rhs_bz(const Tabs &tb, int j, int k, double ey, double ey_jm, double ex, double ex_km)
 {
-    if (COORD == 2) {
+    if (COORD == 2 || COORD == 3) {
         const double xl = tb.at(0, TB_C, j);
         return -((ey * tb.at(0, TB_F, j) - ey_jm * tb.at(0, TB_F, j - 1)) / (xl * tb.at(0, TB_D, j)) -
                  (ex * 1.0 - ex_km * 1.0) / (xl * tb.at(1, TB_D, k)));
@@ -463,6 +477,10 @@ __device__ inline double cell_dt(const Tabs &tb, const Model &M, int j, int k, i
     const double lambda = fmax2<double>(fmax2<double>(l1, l2), l3);
     double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
     if (COORD == 2) dyl = tb.at(0, TB_C, j) * dyl;
+    if (COORD == 3) {
+        dyl = tb.at(0, TB_C, j) * dyl;
+        dzl = tb.at(0, TB_C, j) * tb.at(1, TB_SINC, k) * dzl;
+    }
     const double dt_out = fmin2<double>(fmin2<double>(dxl, dyl), dzl) * M.cfl / lambda;
     return fmin2<double>(dt_out, 1.0e10);
 }
@@ -512,13 +530,24 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
         const double gm1 = M.ggas - 1.0;
         // flux-divergence geometry of the column (coords.f90:255-282)
         const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k);
-        double gxp = 1.0, gxm = 1.0, gxc = dxl, gyc = dyl, xloc = 0.0;
+        double gxp = 1.0, gxm = 1.0, gxc = dxl, gyp = 1.0, gym = 1.0, gyc = dyl, gac = 0.0, xloc = 0.0, tanc = 1.0;
         if (COORD == 2) {
             xloc = tb.at(0, TB_C, j);
             gxp = tb.at(0, TB_F, j);
             gxm = tb.at(0, TB_F, j - 1);
             gxc = xloc * dxl;
             gyc = xloc * dyl;
+        }
+        if (COORD == 3) {
+            xloc = tb.at(0, TB_C, j);
+            tanc = tb.at(1, TB_TANC, k);
+            gxp = tb.at(0, TB_F2, j);
+            gxm = tb.at(0, TB_F2, j - 1);
+            gxc = tb.at(0, TB_V3, j);
+            gyp = tb.at(1, TB_SIN, k);
+            gym = tb.at(1, TB_SIN, k - 1);
+            gac = tb.at(0, TB_RA, j) * tb.at(1, TB_CD, k);
+            gyc = gac;
         }
         // carries from plane l0-1
         unsigned g = (unsigned)L.id(j, k, l0);
@@ -538,7 +567,7 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             const double ezjk = e(2, g - sj - sk);
             // new face fields: own three, and the x / y faces of the lower neighbours
             const double bxn = rk<STAGE>(u0(5, g), ui(5, g), dt, rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, eyL));
-            const double byn = rk<STAGE>(u0(6, g), ui(6, g), dt, rhs_by<COORD>(tb, j, l, ex0, exL, ez0, ezj));
+            const double byn = rk<STAGE>(u0(6, g), ui(6, g), dt, rhs_by<COORD>(tb, j, k, l, ex0, exL, ez0, ezj));
             const double bzn = rk<STAGE>(u0(7, g), ui(7, g), dt, rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
             // hydro right-hand side on interior cells
             double lrk[5];
@@ -547,7 +576,7 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             for (int i = 0; i < 5; i++) fz[i] = fzz(i, g);
             if (interior) {
                 double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                if (COORD == 2 || M.model == 1) {
+                if (COORD == 2 || COORD == 3 || M.model == 1) {
                     const double rho = pp(0, g), vx = pp(1, g), vy = pp(2, g), vz = pp(3, g);
                     const double p = pp(4, g);
                     const double bx = bcl(0, g), by = bcl(1, g), bz = bcl(2, g);
@@ -558,6 +587,20 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
                         for (int r = 0; r < M.geom_repeat; r++) {
                             sc[IVX] = sc[IVX] + tx;
                             sc[IVY] = sc[IVY] - ty;
+                        }
+                    }
+                    if (COORD == 3) {
+                        // coords.f90:139-156
+                        const double bsq = bx * bx + by * by + bz * bz;
+                        const double tx = (2.0 * p + bx * bx + rho * (vy * vy + vz * vz)) / (xloc);
+                        const double ty1 = (p + rho * vz * vz + 0.5 * bsq - bz * bz) / (xloc * tanc);
+                        const double ty2 = (rho * vx * vy - bx * by) / (xloc);
+                        const double tz1 = (rho * vx * vz - bx * bz) / (xloc);
+                        const double tz2 = (rho * vy * vz - by * bz) / (xloc * tanc);
+                        for (int r = 0; r < M.geom_repeat; r++) {
+                            sc[IVX] = sc[IVX] + tx;
+                            sc[IVY] = sc[IVY] + ty1 - ty2;
+                            sc[IVZ] = sc[IVZ] - tz1 - tz2;
                         }
                     }
                     if (M.model == 1) {
@@ -572,12 +615,13 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
                     }
                 }
                 const double dzl = tb.at(2, TB_D, l);
+                const double gzc = COORD == 3 ? gac * dzl / dyl : dzl;
 #pragma unroll
                 for (int i = 0; i < 5; i++) {
                     double a = 0.0;
                     a = a - (fx(i, g) * gxp - fx(i, g - sj) * gxm) / gxc;
-                    a = a - (fy(i, g) * 1.0 - fy(i, g - sk) * 1.0) / gyc;
-                    a = a - (fz[i] * 1.0 - fzL[i] * 1.0) / dzl;
+                    a = a - (fy(i, g) * gyp - fy(i, g - sk) * gym) / gyc;
+                    a = a - (fz[i] * 1.0 - fzL[i] * 1.0) / gzc;
                     lrk[i] = a + sc[i];
                 }
             } else {
@@ -593,7 +637,7 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
                 const double bxm = rk<STAGE>(u0(5, g - sj), ui(5, g - sj), dt,
                                              rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
                 const double bym = rk<STAGE>(u0(6, g - sk), ui(6, g - sk), dt,
-                                             rhs_by<COORD>(tb, j, l, exk, exkL, ezk, ezjk));
+                                             rhs_by<COORD>(tb, j, k - 1, l, exk, exkL, ezk, ezjk));
                 bcx = 0.5 * (bxn + bxm);
                 bcy = 0.5 * (byn + bym);
                 bcz = 0.5 * (bzn + bzL);
@@ -678,13 +722,24 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
         const double gm1 = M.ggas - 1.0;
         // flux-divergence geometry of the column (coords.f90:255-282)
         const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k);
-        double gxp = 1.0, gxm = 1.0, gxc = dxl, gyc = dyl, xloc = 0.0;
+        double gxp = 1.0, gxm = 1.0, gxc = dxl, gyp = 1.0, gym = 1.0, gyc = dyl, gac = 0.0, xloc = 0.0, tanc = 1.0;
         if (COORD == 2) {
             xloc = tb.at(0, TB_C, j);
             gxp = tb.at(0, TB_F, j);
             gxm = tb.at(0, TB_F, j - 1);
             gxc = xloc * dxl;
             gyc = xloc * dyl;
+        }
+        if (COORD == 3) {
+            xloc = tb.at(0, TB_C, j);
+            tanc = tb.at(1, TB_TANC, k);
+            gxp = tb.at(0, TB_F2, j);
+            gxm = tb.at(0, TB_F2, j - 1);
+            gxc = tb.at(0, TB_V3, j);
+            gyp = tb.at(1, TB_SIN, k);
+            gym = tb.at(1, TB_SIN, k - 1);
+            gac = tb.at(0, TB_RA, j) * tb.at(1, TB_CD, k);
+            gyc = gac;
         }
         // carries from plane l0-1
         long g = L.id(j, k, l0);
@@ -708,7 +763,7 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
             const double bxn = rk<STAGE>(U0[g + 5 * ps], STAGE == 1 ? 0.0 : Uin[g + 5 * ps], dt,
                                          rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, eyL));
             const double byn = rk<STAGE>(U0[g + 6 * ps], STAGE == 1 ? 0.0 : Uin[g + 6 * ps], dt,
-                                         rhs_by<COORD>(tb, j, l, ex0, exL, ez0, ezj));
+                                         rhs_by<COORD>(tb, j, k, l, ex0, exL, ez0, ezj));
             const double bzn = rk<STAGE>(U0[g + 7 * ps], STAGE == 1 ? 0.0 : Uin[g + 7 * ps], dt,
                                          rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
             // hydro right-hand side on interior cells
@@ -718,7 +773,7 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
             for (int i = 0; i < 5; i++) fz[i] = (double)Fz[g + i * ps];
             if (interior) {
                 double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                if (COORD == 2 || M.model == 1) {
+                if (COORD == 2 || COORD == 3 || M.model == 1) {
                     const double rho = F.P[g], vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
                     const double p = F.P[g + 4 * ps];
                     const double bx = F.BC[g], by = F.BC[g + ps], bz = F.BC[g + 2 * ps];
@@ -729,6 +784,20 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
                         for (int r = 0; r < M.geom_repeat; r++) {
                             sc[IVX] = sc[IVX] + tx;
                             sc[IVY] = sc[IVY] - ty;
+                        }
+                    }
+                    if (COORD == 3) {
+                        // coords.f90:139-156
+                        const double bsq = bx * bx + by * by + bz * bz;
+                        const double tx = (2.0 * p + bx * bx + rho * (vy * vy + vz * vz)) / (xloc);
+                        const double ty1 = (p + rho * vz * vz + 0.5 * bsq - bz * bz) / (xloc * tanc);
+                        const double ty2 = (rho * vx * vy - bx * by) / (xloc);
+                        const double tz1 = (rho * vx * vz - bx * bz) / (xloc);
+                        const double tz2 = (rho * vy * vz - by * bz) / (xloc * tanc);
+                        for (int r = 0; r < M.geom_repeat; r++) {
+                            sc[IVX] = sc[IVX] + tx;
+                            sc[IVY] = sc[IVY] + ty1 - ty2;
+                            sc[IVZ] = sc[IVZ] - tz1 - tz2;
                         }
                     }
                     if (M.model == 1) {
@@ -743,12 +812,13 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
                     }
                 }
                 const double dzl = tb.at(2, TB_D, l);
+                const double gzc = COORD == 3 ? gac * dzl / dyl : dzl;
 #pragma unroll
                 for (int i = 0; i < 5; i++) {
                     double a = 0.0;
                     a = a - ((double)Fx[g + i * ps] * gxp - (double)Fx[g - sj + i * ps] * gxm) / gxc;
-                    a = a - ((double)Fy[g + i * ps] * 1.0 - (double)Fy[g - sk + i * ps] * 1.0) / gyc;
-                    a = a - (fz[i] * 1.0 - fzL[i] * 1.0) / dzl;
+                    a = a - ((double)Fy[g + i * ps] * gyp - (double)Fy[g - sk + i * ps] * gym) / gyc;
+                    a = a - (fz[i] * 1.0 - fzL[i] * 1.0) / gzc;
                     lrk[i] = a + sc[i];
                 }
             } else {
@@ -764,7 +834,7 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
                 const double bxm = rk<STAGE>(U0[g - sj + 5 * ps], STAGE == 1 ? 0.0 : Uin[g - sj + 5 * ps], dt,
                                              rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
                 const double bym = rk<STAGE>(U0[g - sk + 6 * ps], STAGE == 1 ? 0.0 : Uin[g - sk + 6 * ps], dt,
-                                             rhs_by<COORD>(tb, j, l, exk, exkL, ezk, ezjk));
+                                             rhs_by<COORD>(tb, j, k - 1, l, exk, exkL, ezk, ezjk));
                 bcx = 0.5 * (bxn + bxm);
                 bcy = 0.5 * (byn + bym);
                 bcz = 0.5 * (bzn + bzL);
@@ -1094,7 +1164,16 @@ __global__ __launch_bounds__(256) void k_divb(Lay L, Fields<double> F, Tabs tb, 
             const long g = L.id(j, k, l), ps = L.ps;
             const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k), dzl = tb.at(2, TB_D, l);
             double axp, axm, ayp, aym, azp, azm;
-            if (COORD == 2) {
+            if (COORD == 3) {
+                const double xp = tb.at(0, TB_F, j), xm = tb.at(0, TB_F, j - 1), cd = tb.at(1, TB_CD, k);
+                const double d2 = tb.at(0, TB_D2, j);
+                axp = xp * xp * cd * dzl;
+                axm = xm * xm * cd * dzl;
+                ayp = 0.5 * d2 * tb.at(1, TB_SIN, k) * dzl;
+                aym = 0.5 * d2 * tb.at(1, TB_SIN, k - 1) * dzl;
+                azp = 0.5 * d2 * dyl;
+                azm = azp;
+            } else if (COORD == 2) {
                 const double xp = tb.at(0, TB_F, j), xm = tb.at(0, TB_F, j - 1);
                 axp = xp * dyl * dzl;
                 axm = xm * dyl * dzl;

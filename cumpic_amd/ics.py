@@ -18,7 +18,8 @@ import math
 import numpy as np
 
 from .config import (CARTESIAN, CYLINDRICAL, IBX, IBY, IBZ, IRHO, ITAU, IVX, IVY, IVZ, MODEL_RIEMANN,
-                     MODEL_TORUS_CYN, NGHOST, PPMC, Config, log_faces, periodic_angle_faces, uniform_faces)
+                     MODEL_TORUS_CYN, NGHOST, PPMC, SPHERICAL, Config, log_faces, periodic_angle_faces,
+                     uniform_faces)
 
 G = NGHOST
 
@@ -277,6 +278,82 @@ def torus_cyn(nr: int = 256, nphi: int = 128, nzz: int = 256, seed: int = 1003, 
     prim[:, :, :, IBX] = (bx_face * norm)[:, None, :]
     prim[:, :, :, IBZ] = (bz_face * norm)[:, None, :]
     return Problem(cfg, xf, yf, zf, prim, eps, steps=steps, name="torus_cyn")
+
+
+def spherical_areas(xf, yf, zf):
+    """Face areas of coords.f90:213-219 (GEOM_AREA, spherical) on every face of the
+    ghost-inclusive grid, as arrays [l, k, j] (entry j = face xF(j) for ax, the upper face of
+    cell j for ay / az; the first entry along an axis without a lower face is left at 1)."""
+    def d(a):
+        out = np.ones(len(a) - 1)
+        out[1:] = a[1:-1] - a[:-2]
+        return out
+    xF, xFm = xf[1:], xf[:-1]       # cell j: faces j (upper) and j-1
+    yF, yFm = yf[1:], yf[:-1]
+    dz = zf[1:] - zf[:-1]
+    dy = yf[1:] - yf[:-1]
+    x2d = xF * xF - xFm * xFm
+    cd = np.cos(yFm) - np.cos(yF)
+    ax = (xF * xF)[None, None, :] * cd[None, :, None] * dz[:, None, None]
+    ay = (0.5 * x2d)[None, None, :] * np.sin(yF)[None, :, None] * dz[:, None, None]
+    az = (0.5 * x2d)[None, None, :] * dy[None, :, None] * np.ones(len(dz))[:, None, None]
+    return ax, ay, az
+
+
+def spherical_shell(nr: int = 16, nth: int = 12, nph: int = 8, seed: int = 1006, r=(1.0, 2.0),
+                    theta=(0.25 * math.pi, 0.75 * math.pi), phi=(0.0, 0.5 * math.pi),
+                    boundary_flag=(1, 1, 1, 1, 0, 0), recon: int = PPMC, gamma: float = 5.0 / 3.0,
+                    cfl: float = 0.4, noise: float = 0.1, v_amp: float = 0.1, b_amp: float = 0.3,
+                    static: bool = False, steps: int = 4) -> Problem:
+    """Spherical (r, theta, phi) shell with uniform faces on r x theta x phi, riemann hooks.
+
+    rho = 1, p = 0.5 with seeded relative noise, seeded velocities of size v_amp, and face B
+    = (discrete curl of seeded edge values) / (face area of coords.f90:213-219), scaled to rms
+    b_amp, so the area-weighted divergence of find_divb (mhd.f90:15-66) is zero to round-off.
+    static: rho = 1, p = 0.5, v = 0, B = 0 exactly (the geometric-source balance state)."""
+    rng = np.random.default_rng(seed)
+    xf = uniform_faces(nr, *r)
+    yf = uniform_faces(nth, *theta)
+    zf = uniform_faces(nph, *phi)
+    cfg = Config(nx=nr, ny=nth, nz=nph, coordinate=SPHERICAL, boundary_flag=tuple(boundary_flag), recon=recon,
+                 model=MODEL_RIEMANN, ggas=gamma, cfl=cfl)
+    prim, eps = _alloc(cfg)
+    inner = (slice(ix(1), ix(nph) + 1), slice(ix(1), ix(nth) + 1), slice(ix(1), ix(nr) + 1))
+    shp = (nph, nth, nr)
+    rho = np.ones(shp)
+    p = 0.5 * np.ones(shp)
+    if not static:
+        rho *= 1.0 + noise * rng.uniform(-1, 1, shp)
+        p *= 1.0 + noise * rng.uniform(-1, 1, shp)
+        for v in (IVX, IVY, IVZ):
+            prim[inner + (v,)] = v_amp * rng.uniform(-1, 1, shp)
+        full = cfg.shape
+        psi = [rng.standard_normal(full) for _ in range(3)]  # edge values along x, y, z
+        if boundary_flag[4] == 0 and boundary_flag[5] == 0:
+            # periodic in phi: the same edge values one period apart, so the face fields of the
+            # two periodic boundary faces agree
+            wrap = (np.arange(full[0]) - G) % nph + G
+            psi = [a[wrap] for a in psi]
+
+        def dm(a, axis):
+            out = np.zeros_like(a)
+            sl, sm = [slice(None)] * 3, [slice(None)] * 3
+            sl[axis], sm[axis] = slice(1, None), slice(None, -1)
+            out[tuple(sl)] = a[tuple(sl)] - a[tuple(sm)]
+            return out
+        # axes of the [l, k, j] arrays: 0 = z, 1 = y, 2 = x
+        fx = dm(psi[2], 1) - dm(psi[1], 0)
+        fy = dm(psi[0], 0) - dm(psi[2], 2)
+        fz = dm(psi[1], 2) - dm(psi[0], 1)
+        ax, ay, az = spherical_areas(xf, yf, zf)
+        b = [fx / ax, fy / ay, fz / az]
+        rms = math.sqrt(float(np.mean(b[0] ** 2 + b[1] ** 2 + b[2] ** 2)))
+        for c, arr in enumerate(b):
+            prim[:, :, :, IBX + c] = arr * (b_amp / rms)
+    prim[inner + (IRHO,)] = rho
+    prim[inner + (ITAU,)] = p
+    eps[inner] = p / rho / (gamma - 1.0)
+    return Problem(cfg, xf, yf, zf, prim, eps, steps=steps, name="spherical_shell")
 
 
 def small_periodic(n: int = 16, seed: int = 1005, ndim: int = 3) -> Problem:

@@ -29,7 +29,7 @@ enum {
     CUMC_ESTATE = 5       /* call out of order (e.g. no grid set) */
 };
 
-enum { CUMC_CARTESIAN = 1, CUMC_CYLINDRICAL = 2 };      /* solver/definition.f90:65-66 */
+enum { CUMC_CARTESIAN = 1, CUMC_CYLINDRICAL = 2, CUMC_SPHERICAL = 3 };  /* solver/definition.f90:65-67 */
 enum { CUMC_TVDMM = 1, CUMC_PPMC = 4 };                  /* solver/definition.f90:158,167 */
 enum { CUMC_MODEL_RIEMANN = 0, CUMC_MODEL_TORUS_CYN = 1 };
 enum { CUMC_FP64 = 0, CUMC_MIXED = 1 };
@@ -38,7 +38,7 @@ enum { CUMC_FP64 = 0, CUMC_MIXED = 1 };
  * model plugin's defs.f90 (ggas, rho_floor, eps_floor, r_sh). */
 typedef struct cumc_config {
     int nx, ny, nz;               /* interior cells of this subdomain (definition.f90:110-112) */
-    int coordinate;               /* CUMC_CARTESIAN | CUMC_CYLINDRICAL (definition.f90:72) */
+    int coordinate;               /* CUMC_CARTESIAN | CUMC_CYLINDRICAL | CUMC_SPHERICAL (definition.f90:72) */
     int boundary_flag[6];         /* x-in, x-out, y-in, y-out, z-in, z-out (definition.f90:87) */
     int recon;                    /* CUMC_PPMC | CUMC_TVDMM (definition.f90:173) */
     int model;                    /* which model plugin's in-loop hooks to run */

@@ -188,6 +188,17 @@ static inline double cdz(const orc *o, int l) { return o->zf[l] - o->zf[l - 1]; 
 /* coords.f90:49-69 */
 static inline double cxc(const orc *o, int j) { return 0.5 * (o->xf[j] + o->xf[j - 1]); }
 static inline double czc(const orc *o, int l) { return 0.5 * (o->zf[l] + o->zf[l - 1]); }
+static inline double cyc(const orc *o, int k) { return 0.5 * (o->yf[k] + o->yf[k - 1]); }
+/* Fortran x**2 and x**3 with integer exponents: x*x and (x*x)*x (coords.f90 spherical branches) */
+static inline double pw2(double x) { return x * x; }
+static inline double pw3(double x) { return x * x * x; }
+/* spherical radial factor 2 (r_j^3 - r_{j-1}^3) / 3 / (r_j^2 - r_{j-1}^2) (coords.f90:290, 295) */
+static inline double sph_a(const orc *o, int j)
+{
+    return 2.0 * (pw3(o->xf[j]) - pw3(o->xf[j - 1])) / 3.0 / (pw2(o->xf[j]) - pw2(o->xf[j - 1]));
+}
+/* cos(theta_{k-1}) - cos(theta_k) (coords.f90:215-216, 290, 389) */
+static inline double sph_cd(const orc *o, int k) { return cos(o->yf[k - 1]) - cos(o->yf[k]); }
 
 /* reconstruct.f90:18-51 INTERPOLATE: widths along `dir` around index `c` */
 static void interpolate(const orc *o, int dir, int c, const double v[5], double *vm, double *vp)
@@ -642,9 +653,27 @@ void orc_initial_update(orc *o)
 /* ---------------------------------------------------------------------------------------------- */
 /* Spatial operator                                                                               */
 
-/* coords.f90:105-137 GEOM_SOURCE, cylindrical branch */
+/* coords.f90:105-160 GEOM_SOURCE, cylindrical and spherical branches */
 static inline void geom_source(orc *o, int j, int k, int l)
 {
+    if (o->c.coordinate == 3) {
+        double x_loc = cxc(o, j), y_loc = cyc(o, k);
+        double bsquare = dotp(BC(0, j, k, l), BC(1, j, k, l), BC(2, j, k, l), BC(0, j, k, l), BC(1, j, k, l),
+                              BC(2, j, k, l));
+        SC(IVX, j, k, l) = SC(IVX, j, k, l) +
+            (2.0 * PR(ITAU, j, k, l) + BC(0, j, k, l) * BC(0, j, k, l) +
+             PR(IRHO, j, k, l) * (PR(IVY, j, k, l) * PR(IVY, j, k, l) + PR(IVZ, j, k, l) * PR(IVZ, j, k, l))) /
+                (x_loc);
+        SC(IVY, j, k, l) = SC(IVY, j, k, l) +
+            (PR(ITAU, j, k, l) + PR(IRHO, j, k, l) * PR(IVZ, j, k, l) * PR(IVZ, j, k, l) + 0.5 * bsquare -
+             BC(2, j, k, l) * BC(2, j, k, l)) / (x_loc * tan(y_loc)) -
+            (PR(IRHO, j, k, l) * PR(IVX, j, k, l) * PR(IVY, j, k, l) - BC(0, j, k, l) * BC(1, j, k, l)) / (x_loc);
+        SC(IVZ, j, k, l) = SC(IVZ, j, k, l) -
+            (PR(IRHO, j, k, l) * PR(IVX, j, k, l) * PR(IVZ, j, k, l) - BC(0, j, k, l) * BC(2, j, k, l)) / (x_loc) -
+            (PR(IRHO, j, k, l) * PR(IVY, j, k, l) * PR(IVZ, j, k, l) - BC(1, j, k, l) * BC(2, j, k, l)) /
+                (x_loc * tan(y_loc));
+        return;
+    }
     if (o->c.coordinate != 2) return;
     double x_loc = cxc(o, j);
     double bsquare = dotp(BC(0, j, k, l), BC(1, j, k, l), BC(2, j, k, l), BC(0, j, k, l), BC(1, j, k, l), BC(2, j, k, l));
@@ -658,7 +687,7 @@ static inline void geom_source(orc *o, int j, int k, int l)
 /* spatial.f90:75-102 GET_SOURCE (Q1: GEOM_SOURCE runs once per equation index i) */
 static void get_source(orc *o)
 {
-    if (o->c.coordinate != 2) return;
+    if (o->c.coordinate != 2 && o->c.coordinate != 3) return;
     const int nx = o->nx, ny = o->ny, nz = o->nz;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 1; l <= nz; l++)
@@ -667,10 +696,26 @@ static void get_source(orc *o)
                 for (int i = 0; i < o->c.geom_source_repeat; i++) geom_source(o, j, k, l);
 }
 
-/* coords.f90:231-302 GEOM_FLUX (cartesian / cylindrical) */
+/* coords.f90:231-302 GEOM_FLUX (cartesian / cylindrical / spherical) */
 static inline void geom_flux(const orc *o, int dir, int j, int k, int l, double *gp, double *gc, double *gm)
 {
     (void)l;
+    if (o->c.coordinate == 3) {
+        if (dir == 1) {
+            *gp = pw2(o->xf[j]);
+            *gm = pw2(o->xf[j - 1]);
+            *gc = (pw3(o->xf[j]) - pw3(o->xf[j - 1])) / 3.0;
+        } else if (dir == 2) {
+            *gp = sin(o->yf[k]);
+            *gm = sin(o->yf[k - 1]);
+            *gc = sph_a(o, j) * sph_cd(o, k);
+        } else {
+            *gp = 1.0;
+            *gm = 1.0;
+            *gc = sph_a(o, j) * sph_cd(o, k) * cdz(o, l) / cdy(o, k);
+        }
+        return;
+    }
     if (o->c.coordinate == 2) {
         double x_loc = cxc(o, j);
         if (dir == 1) { *gp = o->xf[j]; *gm = o->xf[j - 1]; *gc = x_loc * cdx(o, j); }
@@ -754,7 +799,7 @@ static void sweep(orc *o, int dir)
             }
 }
 
-/* coords.f90:310-415 GEOM_CT (cartesian / cylindrical) */
+/* coords.f90:310-415 GEOM_CT (cartesian / cylindrical / spherical) */
 typedef struct { double bx_ez[3], bx_ey[3], by_ex[3], by_ez[3], bz_ex[3], bz_ey[3]; } ctgeom; /* m, c, p */
 static inline void geom_ct(const orc *o, int j, int k, int l, ctgeom *g)
 {
@@ -762,7 +807,21 @@ static inline void geom_ct(const orc *o, int j, int k, int l, ctgeom *g)
     for (int i = 0; i < 3; i++) {
         g->bx_ez[i] = g->bx_ey[i] = g->by_ex[i] = g->by_ez[i] = g->bz_ex[i] = g->bz_ey[i] = 1.0;
     }
-    if (o->c.coordinate == 2) {
+    if (o->c.coordinate == 3) {
+        double x_loc = cxc(o, j);
+        g->bx_ez[0] = sin(o->yf[k - 1]);
+        g->bx_ez[1] = o->xf[j] * sph_cd(o, k);
+        g->bx_ez[2] = sin(o->yf[k]);
+        g->bx_ey[1] = o->xf[j] * sph_cd(o, k) * dz_loc / dy_loc;
+        g->by_ex[1] = x_loc * sin(o->yf[k]) * dz_loc;
+        g->by_ez[0] = o->xf[j - 1];
+        g->by_ez[1] = x_loc * dx_loc;
+        g->by_ez[2] = o->xf[j];
+        g->bz_ey[0] = o->xf[j - 1];
+        g->bz_ey[1] = x_loc * dx_loc;
+        g->bz_ey[2] = o->xf[j];
+        g->bz_ex[1] = x_loc * dy_loc;
+    } else if (o->c.coordinate == 2) {
         double x_loc = cxc(o, j);
         g->bx_ez[1] = dy_loc * o->xf[j];
         g->bx_ey[1] = dz_loc;
@@ -944,6 +1003,10 @@ double orc_finddt(orc *o)
                                       fabs(PR(IVZ, j, k, l)) + cfz);
                 double dx_loc = cdx(o, j), dy_loc = cdy(o, k), dz_loc = cdz(o, l);
                 if (o->c.coordinate == 2) dy_loc = cxc(o, j) * dy_loc;
+                if (o->c.coordinate == 3) {
+                    dy_loc = cxc(o, j) * dy_loc;
+                    dz_loc = cxc(o, j) * sin(cyc(o, k)) * dz_loc;
+                }
                 double dt_out = fmin3(dx_loc, dy_loc, dz_loc) * cfl / lambda;
                 dt_temp = fmin2(dt_out, dt_temp);
             }
@@ -961,7 +1024,14 @@ void orc_find_divb(const orc *o, double *max_signed, double *max_abs)
             for (int j = 1; j <= nx; j++) {
                 double dx_loc = cdx(o, j), dy_loc = cdy(o, k), dz_loc = cdz(o, l);
                 double axp, axm, ayp, aym, azp, azm;
-                if (o->c.coordinate == 2) {
+                if (o->c.coordinate == 3) {
+                    axp = o->xf[j] * o->xf[j] * sph_cd(o, k) * dz_loc;
+                    axm = o->xf[j - 1] * o->xf[j - 1] * sph_cd(o, k) * dz_loc;
+                    ayp = 0.5 * (pw2(o->xf[j]) - pw2(o->xf[j - 1])) * sin(o->yf[k]) * dz_loc;
+                    aym = 0.5 * (pw2(o->xf[j]) - pw2(o->xf[j - 1])) * sin(o->yf[k - 1]) * dz_loc;
+                    azp = 0.5 * (pw2(o->xf[j]) - pw2(o->xf[j - 1])) * dy_loc;
+                    azm = azp;
+                } else if (o->c.coordinate == 2) {
                     axp = o->xf[j] * dy_loc * dz_loc;
                     axm = o->xf[j - 1] * dy_loc * dz_loc;
                     ayp = dx_loc * dz_loc;

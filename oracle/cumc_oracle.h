@@ -23,7 +23,7 @@ extern "C" {
 
 typedef struct orc_config {
     int nx, ny, nz;            /* interior cells per axis (definition.f90:100-112) */
-    int coordinate;            /* 1 cartesian, 2 cylindrical (definition.f90:65-72) */
+    int coordinate;            /* 1 cartesian, 2 cylindrical, 3 spherical (definition.f90:65-72) */
     int boundary_flag[6];      /* 0 periodic, 1 outflow, >=2 parity (definition.f90:87) */
     int recon;                 /* 1 TVDMM, 4 PPMC (definition.f90:157-173) */
     int model;                 /* 0 riemann plugin, 1 torus_cyn plugin */
