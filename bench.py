@@ -121,7 +121,9 @@ def timed_run(Solver, prob, prec, steps, warmup, tune=True):
     return s, dts, wall, kt, tuning
 
 
-def roofline(kt, cfg, tag):
+def roofline(kt, cfg, tag, profiled=True):
+    """profiled: the workload is the one profiles/pmc_traffic.txt was measured on (the 256^3 box);
+    elsewhere `traffic` is null."""
     table = KERNEL_BYTES[tag]
     dom = max((k for k in kt if k in table), key=lambda k: kt[k][0])
     tot_ms, launches = kt[dom]
@@ -129,7 +131,8 @@ def roofline(kt, cfg, tag):
     alg = table[dom] * kernel_units(dom, cfg)
     ach = alg / avg_s
     return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
-            "traffic": read_traffic(dom, tag), "kernel": dom, "avg_launch_ms": tot_ms / launches}
+            "traffic": read_traffic(dom, tag) if profiled else None, "kernel": dom,
+            "avg_launch_ms": tot_ms / launches}
 
 
 def fields(state, cfg):
@@ -288,7 +291,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["box", "torus", "batch"], default="box",
+    ap.add_argument("--workload", choices=["box", "torus", "sphere", "batch"], default="box",
                     help="batch: only the batched 2D boxes (forward and reverse mode), value = batched rate")
     ap.add_argument("--n", type=int, default=256, help="zones per axis of the periodic box")
     ap.add_argument("--forms", default="fp64,mixed", help="comma list of fp64, mixed")
@@ -329,7 +332,15 @@ def main():
         return
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    if args.workload == "torus":
+    if args.workload == "sphere":
+        steps = args.steps or 50
+        log("[bench] building 256x128x128 spherical shell")
+        prob = ics.spherical_shell(256, 128, 128, seed=1006, smooth=True, noise=0.1, v_amp=0.1, b_amp=0.1,
+                                   boundary_flag=(2, 2, 2, 2, 0, 0), steps=steps)
+        workload = ("spherical shell 256x128x128 (r in [1,2], theta in [pi/4,3pi/4], phi in [0,pi/2] periodic), "
+                    "reflecting r/theta walls, smooth seeded state, geometric source added 8 times as the "
+                    "reference does, PPM+HLL+upwind CT+SSP-RK3")
+    elif args.workload == "torus":
         steps = args.steps or 50
         log("[bench] building 256x128x256 cylindrical torus")
         prob = ics.torus_cyn(256, 128, 256, steps=steps)
@@ -367,7 +378,9 @@ def main():
         dsig, dabs = s.find_divb()
         st = s.download()
         finite = bool(np.isfinite(st["prim"]).all())
-        r = {"value": cfg.zones * steps / wall, "ms_per_step": wall / steps * 1e3, "roofline": roofline(kt, cfg, form),
+        profiled = args.workload == "box" and args.n == 256
+        r = {"value": cfg.zones * steps / wall, "ms_per_step": wall / steps * 1e3,
+             "roofline": roofline(kt, cfg, form, profiled=profiled),
              "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()}, "divb_max_abs": dabs,
              "state_finite": finite, "tuning": tuning}
         if copy_gbps:
@@ -418,7 +431,8 @@ def main():
         "dtype": "f64" if main_form == "fp64" else "f64-state/f32-flux",
         "data": "synthetic",
         "config": {"workload": workload, "zones": cfg.zones, "grid": [cfg.nx, cfg.ny, cfg.nz],
-                   "gamma": cfg.ggas, "cfl": cfg.cfl, "seed": 1001 if args.workload == "box" else 1003,
+                   "gamma": cfg.ggas, "cfl": cfg.cfl,
+                   "seed": {"box": 1001, "torus": 1003, "sphere": 1006}[args.workload],
                    "form_of_value": main_form},
         "roofline": m["roofline"],
         "cpu_baseline": cpu,

@@ -304,13 +304,15 @@ def spherical_shell(nr: int = 16, nth: int = 12, nph: int = 8, seed: int = 1006,
                     theta=(0.25 * math.pi, 0.75 * math.pi), phi=(0.0, 0.5 * math.pi),
                     boundary_flag=(1, 1, 1, 1, 0, 0), recon: int = PPMC, gamma: float = 5.0 / 3.0,
                     cfl: float = 0.4, noise: float = 0.1, v_amp: float = 0.1, b_amp: float = 0.3,
-                    static: bool = False, steps: int = 4) -> Problem:
+                    static: bool = False, smooth: bool = False, steps: int = 4) -> Problem:
     """Spherical (r, theta, phi) shell with uniform faces on r x theta x phi, riemann hooks.
 
     rho = 1, p = 0.5 with seeded relative noise, seeded velocities of size v_amp, and face B
     = (discrete curl of seeded edge values) / (face area of coords.f90:213-219), scaled to rms
     b_amp, so the area-weighted divergence of find_divb (mhd.f90:15-66) is zero to round-off.
-    static: rho = 1, p = 0.5, v = 0, B = 0 exactly (the geometric-source balance state)."""
+    static: rho = 1, p = 0.5, v = 0, B = 0 exactly (the geometric-source balance state).
+    smooth: the noise, velocities and edge values are smooth random fields (modes 1 <= |k| <= 4
+    of the index space, periodic) instead of cell-to-cell noise, for long runs."""
     rng = np.random.default_rng(seed)
     xf = uniform_faces(nr, *r)
     yf = uniform_faces(nth, *theta)
@@ -323,12 +325,16 @@ def spherical_shell(nr: int = 16, nth: int = 12, nph: int = 8, seed: int = 1006,
     rho = np.ones(shp)
     p = 0.5 * np.ones(shp)
     if not static:
-        rho *= 1.0 + noise * rng.uniform(-1, 1, shp)
-        p *= 1.0 + noise * rng.uniform(-1, 1, shp)
+        def field(shape):
+            if smooth:
+                return _spectral_field(shape, 4.0, -2.0, rng)
+            return rng.uniform(-1, 1, shape)
+        rho *= 1.0 + noise * field(shp)
+        p *= 1.0 + noise * field(shp)
         for v in (IVX, IVY, IVZ):
-            prim[inner + (v,)] = v_amp * rng.uniform(-1, 1, shp)
+            prim[inner + (v,)] = v_amp * field(shp)
         full = cfg.shape
-        psi = [rng.standard_normal(full) for _ in range(3)]  # edge values along x, y, z
+        psi = [field(full) if smooth else rng.standard_normal(full) for _ in range(3)]  # edge values along x, y, z
         if boundary_flag[4] == 0 and boundary_flag[5] == 0:
             # periodic in phi: the same edge values one period apart, so the face fields of the
             # two periodic boundary faces agree
