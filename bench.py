@@ -334,12 +334,11 @@ def main():
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     if args.workload == "sphere":
         steps = args.steps or 50
-        log("[bench] building 256x128x128 spherical shell")
-        prob = ics.spherical_shell(256, 128, 128, seed=1006, smooth=True, noise=0.1, v_amp=0.1, b_amp=0.1,
-                                   boundary_flag=(2, 2, 2, 2, 0, 0), steps=steps)
-        workload = ("spherical shell 256x128x128 (r in [1,2], theta in [pi/4,3pi/4], phi in [0,pi/2] periodic), "
-                    "reflecting r/theta walls, smooth seeded state, geometric source added 8 times as the "
-                    "reference does, PPM+HLL+upwind CT+SSP-RK3")
+        log("[bench] building 256x128x128 spherical torus")
+        prob = ics.torus_sph(256, 128, 128, steps=steps)
+        workload = ("spherical torus 256x128x128 (log r in [10,110], theta in [0.2,pi-0.2], phi periodic), "
+                    "torus_sph hooks: PW gravity + floors + radial inflow clamp, beta=100, "
+                    "PPM+HLL+upwind CT+SSP-RK3")
     elif args.workload == "torus":
         steps = args.steps or 50
         log("[bench] building 256x128x256 cylindrical torus")
@@ -432,7 +431,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "zones": cfg.zones, "grid": [cfg.nx, cfg.ny, cfg.nz],
                    "gamma": cfg.ggas, "cfl": cfg.cfl,
-                   "seed": {"box": 1001, "torus": 1003, "sphere": 1006}[args.workload],
+                   "seed": {"box": 1001, "torus": 1003, "sphere": 1007}[args.workload],
                    "form_of_value": main_form},
         "roofline": m["roofline"],
         "cpu_baseline": cpu,

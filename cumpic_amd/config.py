@@ -18,7 +18,7 @@ IRHO, IVX, IVY, IVZ, ITAU, IBX, IBY, IBZ = range(8)
 
 CARTESIAN, CYLINDRICAL, SPHERICAL = 1, 2, 3  # definition.f90:65-67
 TVDMM, PPMC = 1, 4  # definition.f90:158,167
-MODEL_RIEMANN, MODEL_TORUS_CYN = 0, 1
+MODEL_RIEMANN, MODEL_TORUS_CYN, MODEL_TORUS_SPH = 0, 1, 2
 FP64, MIXED = 0, 1
 
 

@@ -60,7 +60,8 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     if (c.coordinate != CUMC_CARTESIAN && c.coordinate != CUMC_CYLINDRICAL && c.coordinate != CUMC_SPHERICAL)
         return CUMC_EINVAL;
     if (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM) return CUMC_EINVAL;
-    if (c.model != CUMC_MODEL_RIEMANN && c.model != CUMC_MODEL_TORUS_CYN) return CUMC_EINVAL;
+    if (c.model != CUMC_MODEL_RIEMANN && c.model != CUMC_MODEL_TORUS_CYN && c.model != CUMC_MODEL_TORUS_SPH)
+        return CUMC_EINVAL;
     if (c.precision != CUMC_FP64 && c.precision != CUMC_MIXED) return CUMC_EINVAL;
     if (!(c.ggas > 1.0) || !(c.cfl > 0.0)) return CUMC_EINVAL;
     const int n3[3] = {c.nx, c.ny, c.nz};
@@ -299,10 +300,12 @@ void launch_ghost(cumc_ctx *ctx)
     hipLaunchKernelGGL(k_ghost<0>, dim3(nblk(n0, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
     hipLaunchKernelGGL(k_ghost<1>, dim3(nblk(n1, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
     hipLaunchKernelGGL(k_ghost<2>, dim3(nblk(n2, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
-    if (ctx->M.model == CUMC_MODEL_TORUS_CYN) {
+    if (ctx->M.model == CUMC_MODEL_TORUS_CYN || ctx->M.model == CUMC_MODEL_TORUS_SPH) {
+        // inflow clamp on the R / r ghosts; torus_cyn also clamps the z ghosts
         const long nyz = (long)(L.ny + 6) * (L.nz + 6) * 3, nxy = nx6 * ny6 * 3;
         hipLaunchKernelGGL(k_custom_boundary_x, dim3(nblk(nyz, 256)), dim3(256), 0, s, L, ctx->F64);
-        hipLaunchKernelGGL(k_custom_boundary_z, dim3(nblk(nxy, 256)), dim3(256), 0, s, L, ctx->F64);
+        if (ctx->M.model == CUMC_MODEL_TORUS_CYN)
+            hipLaunchKernelGGL(k_custom_boundary_z, dim3(nblk(nxy, 256)), dim3(256), 0, s, L, ctx->F64);
     }
 }
 

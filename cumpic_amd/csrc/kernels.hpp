@@ -576,7 +576,7 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             for (int i = 0; i < 5; i++) fz[i] = fzz(i, g);
             if (interior) {
                 double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                if (COORD == 2 || COORD == 3 || M.model == 1) {
+                if (COORD == 2 || COORD == 3 || M.model != 0) {
                     const double rho = pp(0, g), vx = pp(1, g), vy = pp(2, g), vz = pp(3, g);
                     const double p = pp(4, g);
                     const double bx = bcl(0, g), by = bcl(1, g), bz = bcl(2, g);
@@ -602,6 +602,14 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
                             sc[IVY] = sc[IVY] + ty1 - ty2;
                             sc[IVZ] = sc[IVZ] - tz1 - tz2;
                         }
+                    }
+                    if (M.model == 2) {
+                        // torus_sph/src/custom.f90:121-158: radial Paczynski-Wiita gravity
+                        const double xl = tb.at(0, TB_C, j);
+                        const double factor = fmax2<double>(sign1<double>(rho - M.rho_floor), 0.0);
+                        const double dphidr = 1.0 / ((xl - M.r_sh) * (xl - M.r_sh));
+                        sc[IVX] = sc[IVX] + (-factor * rho * dphidr);
+                        sc[ITAU] = sc[ITAU] + (-factor * rho * vx * dphidr);
                     }
                     if (M.model == 1) {
                         const double xl = tb.at(0, TB_C, j), zl = tb.at(2, TB_C, l);
@@ -657,7 +665,7 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             double eps = (u[4] - 0.5 * bsq) / rho0 - 0.5 * vsq;
             double rho = rho0;
             if (interior) {
-                if (M.model == 1) {
+                if (M.model != 0) {
                     if (rho < M.rho_floor) rho = M.rho_floor;
                     if (eps < M.eps_floor) eps = M.eps_floor;
                 } else {
@@ -773,7 +781,7 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
             for (int i = 0; i < 5; i++) fz[i] = (double)Fz[g + i * ps];
             if (interior) {
                 double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                if (COORD == 2 || COORD == 3 || M.model == 1) {
+                if (COORD == 2 || COORD == 3 || M.model != 0) {
                     const double rho = F.P[g], vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
                     const double p = F.P[g + 4 * ps];
                     const double bx = F.BC[g], by = F.BC[g + ps], bz = F.BC[g + 2 * ps];
@@ -799,6 +807,14 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
                             sc[IVY] = sc[IVY] + ty1 - ty2;
                             sc[IVZ] = sc[IVZ] - tz1 - tz2;
                         }
+                    }
+                    if (M.model == 2) {
+                        // torus_sph/src/custom.f90:121-158: radial Paczynski-Wiita gravity
+                        const double xl = tb.at(0, TB_C, j);
+                        const double factor = fmax2<double>(sign1<double>(rho - M.rho_floor), 0.0);
+                        const double dphidr = 1.0 / ((xl - M.r_sh) * (xl - M.r_sh));
+                        sc[IVX] = sc[IVX] + (-factor * rho * dphidr);
+                        sc[ITAU] = sc[ITAU] + (-factor * rho * vx * dphidr);
                     }
                     if (M.model == 1) {
                         const double xl = tb.at(0, TB_C, j), zl = tb.at(2, TB_C, l);
@@ -854,7 +870,7 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
             double eps = (u[4] - 0.5 * bsq) / rho0 - 0.5 * vsq;
             double rho = rho0;
             if (interior) {
-                if (M.model == 1) {
+                if (M.model != 0) {
                     if (rho < M.rho_floor) rho = M.rho_floor;
                     if (eps < M.eps_floor) eps = M.eps_floor;
                 } else {
@@ -938,7 +954,7 @@ static __global__ __launch_bounds__(256) void k_init(Lay L, Fields<double> F, Mo
     const double vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
     double eps = F.EPS[g];
     if (interior) {
-        if (M.model == 1) {
+        if (M.model != 0) {
             if (rho < M.rho_floor) rho = M.rho_floor;
             if (eps < M.eps_floor) eps = M.eps_floor;
         } else {

@@ -18,8 +18,8 @@ import math
 import numpy as np
 
 from .config import (CARTESIAN, CYLINDRICAL, IBX, IBY, IBZ, IRHO, ITAU, IVX, IVY, IVZ, MODEL_RIEMANN,
-                     MODEL_TORUS_CYN, NGHOST, PPMC, SPHERICAL, Config, log_faces, periodic_angle_faces,
-                     uniform_faces)
+                     MODEL_TORUS_CYN, MODEL_TORUS_SPH, NGHOST, PPMC, SPHERICAL, Config, log_faces,
+                     periodic_angle_faces, uniform_faces)
 
 G = NGHOST
 
@@ -360,6 +360,82 @@ def spherical_shell(nr: int = 16, nth: int = 12, nph: int = 8, seed: int = 1006,
     prim[inner + (ITAU,)] = p
     eps[inner] = p / rho / (gamma - 1.0)
     return Problem(cfg, xf, yf, zf, prim, eps, steps=steps, name="spherical_shell")
+
+
+def torus_sph(nr: int = 256, nth: int = 128, nph: int = 128, seed: int = 1007, gamma: float = 5.0 / 3.0,
+              cfl: float = 0.4, beta: float = 100.0, rho_cut: float = 0.2, dp_frac: float = 0.01,
+              r0: float = 10.0, r1: float = 110.0, theta0: float = 0.2, steps: int = 50) -> Problem:
+    """Constant-angular-momentum torus in a Paczynski-Wiita potential on a spherical (r, theta,
+    phi) grid with log-spaced r faces, theta in [theta0, pi - theta0], phi periodic.
+
+    Constants of the torus_sph plugin (model/torus_sph/src/defs.f90: q = 2, r_sh = 2, s_in = 21,
+    s_max = 40, rho_max = 1, floor rho = 1e-4 rho_max) and its density solution with the
+    equatorial radius s = r sin(theta) (model/torus_sph/src/model.f90:71-131); v_phi = Omega s
+    inside the torus.  The seeded field follows the plugin's vector potential
+    A_phi = (rho/rho_max) (r sin(theta)/3)^3 exp(-r/400) - rho_cut where rho >= rho_cut, averaged
+    to the (r, theta) corners and curled onto the faces with the spherical factors, then scaled
+    to max(p)/max(B^2/2) = beta.  A seeded U[-1,1] relative pressure perturbation of size
+    dp_frac is applied where rho exceeds the floor.  Boundary flags (1,1,1,1,0,0)."""
+    rng = np.random.default_rng(seed)
+    q, r_sh, s_in, s_max, rho_max, rho_fac = 2.0, 2.0, 21.0, 40.0, 1.0, 1.0e-4
+    xf = log_faces(nr, r0, r1)
+    yf = uniform_faces(nth, theta0, math.pi - theta0)
+    zf = periodic_angle_faces(nph)
+
+    def phi_pw(r):
+        return -1.0 / (r - r_sh)
+
+    lkep2 = s_max ** (2.0 * q - 1.0) / (s_max - r_sh) ** 2
+    c_const = phi_pw(s_in) - lkep2 * s_in ** (2.0 - 2.0 * q) / (2.0 - 2.0 * q)
+    h_max = -phi_pw(s_max) + lkep2 * s_max ** (2.0 - 2.0 * q) / (2.0 - 2.0 * q) + c_const
+    k_poly = h_max * (gamma - 1.0) / gamma / rho_max ** (gamma - 1.0)
+    rho_a = rho_fac * rho_max
+    eps_floor = k_poly * rho_a ** (gamma - 1.0) / (gamma - 1.0)
+    cfg = Config(nx=nr, ny=nth, nz=nph, coordinate=SPHERICAL, boundary_flag=(1, 1, 1, 1, 0, 0), recon=PPMC,
+                 model=MODEL_TORUS_SPH, ggas=gamma, cfl=cfl, rho_floor=rho_a, eps_floor=eps_floor, r_sh=r_sh)
+    prim, eps = _alloc(cfg)
+    xc = 0.5 * (xf[1:] + xf[:-1])
+    yc = 0.5 * (yf[1:] + yf[:-1])
+    TH, R = np.meshgrid(yc, xc, indexing="ij")  # [k, j] over all cells
+    s_eq = R * np.sin(TH)
+    h = -phi_pw(R) + lkep2 * s_eq ** (2.0 - 2.0 * q) / (2.0 - 2.0 * q) + c_const
+    h = np.maximum(h, 0.0)
+    rho_loc = (h * (gamma - 1.0) / gamma / k_poly) ** (1.0 / (gamma - 1.0))
+    inside = (rho_loc > rho_a) & (s_eq >= s_in)
+    rho2 = np.where(inside, rho_loc, rho_a)
+    vphi2 = np.where(inside, np.sqrt(lkep2) * s_eq ** (-q) * s_eq, 0.0)
+    p2 = k_poly * rho2 ** gamma
+    a_c = np.where(rho2 >= rho_cut, rho2 / rho_max * (R / 3.0 * np.sin(TH)) ** 3 * np.exp(-R / 400.0) - rho_cut,
+                   0.0)
+    # corners (xF(j), yF(k)) from the four surrounding cells; face fields by the discrete curl
+    ac = np.zeros_like(a_c)
+    ac[:-1, :-1] = 0.25 * (a_c[:-1, :-1] + a_c[:-1, 1:] + a_c[1:, :-1] + a_c[1:, 1:])  # ac[ix(k), ix(j)]
+    xF, xFm = xf[1:], xf[:-1]
+    yF, yFm = yf[1:], yf[:-1]
+    ac_km = np.zeros_like(ac)
+    ac_km[1:, :] = ac[:-1, :]
+    ac_jm = np.zeros_like(ac)
+    ac_jm[:, 1:] = ac[:, :-1]
+    bx_face = (np.sin(yF)[:, None] * ac - np.sin(yFm)[:, None] * ac_km) / (xF[None, :] * (np.cos(yFm) - np.cos(yF))[:, None])
+    by_face = -(xF[None, :] * ac - xFm[None, :] * ac_jm) / (xc * (xF - xFm))[None, :]
+    ik = slice(ix(1), ix(nth) + 1)
+    ij = slice(ix(1), ix(nr) + 1)
+    bcx = 0.5 * (bx_face[ik, ij] + bx_face[ik, ix(0):ix(nr - 1) + 1])
+    bcy = 0.5 * (by_face[ik, ij] + by_face[ix(0):ix(nth - 1) + 1, ij])
+    pmag_max = float(np.max(0.5 * (bcx ** 2 + bcy ** 2)))
+    pgas_max = float(np.max(p2[ik, ij]))
+    norm = math.sqrt(pgas_max / pmag_max / beta) if pmag_max > 0 else 0.0
+    pert = rng.uniform(-1.0, 1.0, size=(nph, nth, nr))
+    rho3 = np.broadcast_to(rho2[None, ik, ij], (nph, nth, nr))
+    p3 = p2[None, ik, ij] * (1.0 + dp_frac * pert * (rho3 > rho_a))
+    inner = (slice(ix(1), ix(nph) + 1), ik, ij)
+    prim[inner + (IRHO,)] = rho3
+    prim[inner + (IVZ,)] = np.broadcast_to(vphi2[None, ik, ij], (nph, nth, nr))
+    prim[inner + (ITAU,)] = p3
+    eps[inner] = p3 / rho3 / (gamma - 1.0)
+    prim[:, :, :, IBX] = (bx_face * norm)[None, :, :]
+    prim[:, :, :, IBY] = (by_face * norm)[None, :, :]
+    return Problem(cfg, xf, yf, zf, prim, eps, steps=steps, name="torus_sph")
 
 
 def small_periodic(n: int = 16, seed: int = 1005, ndim: int = 3) -> Problem:

@@ -31,7 +31,7 @@ enum {
 
 enum { CUMC_CARTESIAN = 1, CUMC_CYLINDRICAL = 2, CUMC_SPHERICAL = 3 };  /* solver/definition.f90:65-67 */
 enum { CUMC_TVDMM = 1, CUMC_PPMC = 4 };                  /* solver/definition.f90:158,167 */
-enum { CUMC_MODEL_RIEMANN = 0, CUMC_MODEL_TORUS_CYN = 1 };
+enum { CUMC_MODEL_RIEMANN = 0, CUMC_MODEL_TORUS_CYN = 1, CUMC_MODEL_TORUS_SPH = 2 };
 enum { CUMC_FP64 = 0, CUMC_MIXED = 1 };
 
 /* Runtime form of the compile-time PARAMETERs of solver/definition.f90:29-185 and the

@@ -417,7 +417,7 @@ void orc_get_lrk(const orc *o, double *l_rk) { memcpy(l_rk, o->l_rk, sizeof(doub
 /* ---------------------------------------------------------------------------------------------- */
 /* Model plugin hooks                                                                             */
 
-/* riemann/src/custom.f90:54-76 ; torus_cyn/src/custom.f90:86-116 */
+/* riemann/src/custom.f90:54-76 ; torus_cyn/src/custom.f90:86-116 ; torus_sph/src/custom.f90:86-116 */
 static void custom_checkrho(orc *o)
 {
     const int nx = o->nx, ny = o->ny, nz = o->nz;
@@ -425,7 +425,7 @@ static void custom_checkrho(orc *o)
     for (int l = 1; l <= nz; l++)
         for (int k = 1; k <= ny; k++)
             for (int j = 1; j <= nx; j++) {
-                if (o->c.model == 1) {
+                if (o->c.model != 0) {
                     if (PR(IRHO, j, k, l) < o->c.rho_floor) PR(IRHO, j, k, l) = o->c.rho_floor;
                     if (EPS(j, k, l) < o->c.eps_floor) EPS(j, k, l) = o->c.eps_floor;
                 } else {
@@ -434,12 +434,27 @@ static void custom_checkrho(orc *o)
             }
 }
 
-/* torus_cyn/src/custom.f90:121-166 (Paczynski-Wiita gravity) */
+/* torus_cyn/src/custom.f90:121-166 and torus_sph/src/custom.f90:121-158 (Paczynski-Wiita gravity:
+ * cylindrical R and z components; spherical radial component) */
 static void custom_source(orc *o)
 {
-    if (o->c.model != 1) return;
+    if (o->c.model == 0) return;
     const int nx = o->nx, ny = o->ny, nz = o->nz;
     const double r_sh = o->c.r_sh, rho_floor = o->c.rho_floor;
+    if (o->c.model == 2) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int l = 1; l <= nz; l++)
+            for (int k = 1; k <= ny; k++)
+                for (int j = 1; j <= nx; j++) {
+                    double x_loc = cxc(o, j);
+                    double diff = PR(IRHO, j, k, l) - rho_floor;
+                    double factor = fmax2(fsign1(diff), 0.0);
+                    double dphidr = 1.0 / ((x_loc - r_sh) * (x_loc - r_sh));
+                    SC(IVX, j, k, l) = SC(IVX, j, k, l) + (-factor * PR(IRHO, j, k, l) * dphidr);
+                    SC(ITAU, j, k, l) = SC(ITAU, j, k, l) + (-factor * PR(IRHO, j, k, l) * PR(IVX, j, k, l) * dphidr);
+                }
+        return;
+    }
 #pragma omp parallel for collapse(2) schedule(static)
     for (int l = 1; l <= nz; l++)
         for (int k = 1; k <= ny; k++)
@@ -457,10 +472,11 @@ static void custom_source(orc *o)
             }
 }
 
-/* torus_cyn/src/custom.f90:47-81 (ghost inflow clamp) */
+/* torus_cyn/src/custom.f90:47-81, torus_sph/src/custom.f90:47-70 (ghost inflow clamp: radial
+ * ghosts, and for torus_cyn the z ghosts) */
 static void custom_boundary(orc *o)
 {
-    if (o->c.model != 1) return;
+    if (o->c.model == 0) return;
     const int nx = o->nx, ny = o->ny, nz = o->nz;
     for (int l = -2; l <= nz + 3; l++)
         for (int k = -2; k <= ny + 3; k++)
@@ -468,6 +484,7 @@ static void custom_boundary(orc *o)
                 PR(IVX, 1 - j, k, l) = fmin2(PR(IVX, 1 - j, k, l), 0.0);
                 PR(IVX, nx + j, k, l) = fmax2(PR(IVX, nx + j, k, l), 0.0);
             }
+    if (o->c.model != 1) return;
     for (int l = 1; l <= 3; l++)
         for (int k = -2; k <= ny + 3; k++)
             for (int j = -2; j <= nx + 3; j++) {
