@@ -41,7 +41,7 @@ typedef struct cumc_config {
     int coordinate;               /* CUMC_CARTESIAN | CUMC_CYLINDRICAL | CUMC_SPHERICAL (definition.f90:72) */
     int boundary_flag[6];         /* x-in, x-out, y-in, y-out, z-in, z-out (definition.f90:87) */
     int recon;                    /* CUMC_PPMC | CUMC_TVDMM (definition.f90:173) */
-    int model;                    /* which model plugin's in-loop hooks to run */
+    int model;                    /* which model plugin's in-loop hooks to run: CUMC_MODEL_* */
     int geom_source_repeat;       /* 8 reproduces spatial.f90:86-97; 1 adds the source once */
     double ggas;                  /* adiabatic index */
     double cfl;                   /* definition.f90:116 */
