@@ -597,27 +597,20 @@ __global__ __launch_bounds__(64) void kr_flux_recon(Lay L, Fields<double> F, Tab
 
 // ---------------------------------------------------------------------------------------------
 // A sweep reversed, wave-parallel like k_flux_x: the lanes of a wavefront hold 64 consecutive
-// cells of the sweep's pencils (one outer index: plane l for the x and y sweeps, row k for the z
+// cells along the sweep (one outer index: plane l for the x and y sweeps, row k for the z
 // sweep); each lane recomputes its cell's reconstruction (neighbours by shuffles), reverses the
 // HLL solve of its face, then the edge values and slope of its cell, and gathers the adjoint of
 // its cell value from the +-1 lanes.  Lanes 5..58 own their cells (54 per wave); the others only
-// feed their neighbours.  Pencils are laid end to end with the sweep axis fastest: x pencils are
-// the rows k in [0, ny+1] at pitch px (row padding included), y pencils the columns j in
-// [0, nx+1] of ny+6 cells, z pencils the columns j in [0, nx+1] of nz+6 cells.  Along y and z the
-// lanes of one load are a row or plane apart; the neighbouring pencils that share those cache
-// lines run in the neighbouring tiles.
+// feed their neighbours.  Along x the rows k in [0, ny+1] are laid end to end at pitch px (row
+// padding included) and loaded straight from HBM; along y and z each wavefront takes one pencil
+// segment and the workgroup stages a 4-pencil tile through LDS (kr_flux_lane_t_tile).
 // ---------------------------------------------------------------------------------------------
 constexpr int XR_OWN = 54;
 // pencils shorter than this (n < LANE_MIN) are reversed by the per-pencil walks instead
 constexpr int LANE_MIN = 8;
 
-template <int DIR>
-__host__ __device__ inline long lane_cells(const Lay &L)
-{
-    if (DIR == 1) return (long)(L.ny + 2) * L.px;
-    if (DIR == 2) return (long)(L.nx + 2) * (L.ny + 6);
-    return (long)(L.nx + 2) * (L.nz + 6);
-}
+// cells of the end-to-end x rows of one plane
+__host__ __device__ inline long lane_cells_x(const Lay &L) { return (long)(L.ny + 2) * L.px; }
 template <int DIR>
 __host__ __device__ inline int lane_outer(const Lay &L)
 {
@@ -629,7 +622,7 @@ template <int DIR>
 __host__ __device__ inline int lane_tiles(const Lay &L)
 {
     if (DIR != 1) return ((L.nx + 2 + 3) / 4) * (((DIR == 2 ? L.ny : L.nz) + 6 + XR_OWN - 1) / XR_OWN);
-    const long waves = (lane_cells<DIR>(L) + XR_OWN - 1) / XR_OWN;
+    const long waves = (lane_cells_x(L) + XR_OWN - 1) / XR_OWN;
     return (int)((waves + 3) / 4);
 }
 template <int DIR>
@@ -719,7 +712,7 @@ __device__ inline void kr_flux_lane_x_tile(const Lay &L, const Fields<double> &F
     const int lane = threadIdx.x & 63;
     const long wave = ((long)bx * blockDim.x + threadIdx.x) >> 6;
     const long f = wave * XR_OWN + lane - 5;
-    const bool inr = (f >= 0) && (f < lane_cells<1>(L));
+    const bool inr = (f >= 0) && (f < lane_cells_x(L));
     const long fc = inr ? f : 0;
     const int n = L.nx;
     const int k = (int)(fc / L.px);
