@@ -43,11 +43,13 @@ KERNEL_BYTES = {
         # (46 + 54 + 54) / 3 words averaged over the three stage launches
         "tail": (46 + 54 + 54) / 3 * 8,
     },
-    # mixed form: fluxes and EMFs are stored and read as FP32, everything else as FP64
+    # mixed form: fluxes, EMFs and the primitive planes the sweeps read are FP32; the conserved
+    # state is FP64.  Tail: 18 FP32 flux/EMF words + U0 (+ Uin) in; Uout 8 FP64 + P32 8, EPS32 1,
+    # BC32 3 FP32 out
     "mixed": {
-        "flux_x": 8 * 8 + 7 * 4, "flux_y": 8 * 8 + 7 * 4, "flux_z": 8 * 8 + 7 * 4,
-        "ecorn": 6 * 4 + 6 * 8 + 3 * 4,
-        "tail": ((46 - 18) * 8 + 18 * 4 + 2 * ((54 - 18) * 8 + 18 * 4)) / 3,
+        "flux_x": 8 * 4 + 7 * 4, "flux_y": 8 * 4 + 7 * 4, "flux_z": 8 * 4 + 7 * 4,
+        "ecorn": 6 * 4 + 6 * 4 + 3 * 4,
+        "tail": ((18 * 4 + 8 * 8 + 8 * 8 + 12 * 4) + 2 * (18 * 4 + 16 * 8 + 8 * 8 + 12 * 4)) / 3,
     },
 }
 

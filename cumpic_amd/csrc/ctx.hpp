@@ -29,6 +29,7 @@ struct cumc_ctx {
     unsigned long long *red = nullptr;  // reduction slots
     double *Ub = nullptr;               // third conserved buffer (stage 2 output)
     bool dt_valid = false;              // dt_dev[1] holds finddt of the current state
+    bool p64_stale = false;             // mixed form: P / EPS / BC lag the FP32 planes and U0
     bool grid_set = false;
     struct cumc_tape *tape = nullptr;  // reverse-mode record (cumc_adjoint.hip)
     int seg = 24;    // target faces per pencil segment of the y / z sweeps
@@ -140,8 +141,10 @@ void batch_stage(cumc_batch *b, int st, int par);
 void batch_set_dt(cumc_batch *b, int m);
 void batch_after_run(cumc_batch *b, int nsteps);
 
-// ghost passes + torus clamp (defined in cumc.hip)
+// ghost passes + torus clamp on the FP64 primitive planes (defined in cumc.hip) and on the
+// mixed form's FP32 planes (cumc_mixed.hip)
 void launch_ghost(cumc_ctx *ctx);
+void launch_ghost32(cumc_ctx *ctx);
 
 template <typename R>
 void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
@@ -226,6 +229,11 @@ void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
     }
     mark(ctx, -1);
     mark(ctx, KC_GHOST);
-    launch_ghost(ctx);
+    if constexpr (sizeof(R) == 4) {
+        launch_ghost32(ctx);
+        ctx->p64_stale = true;
+    } else {
+        launch_ghost(ctx);
+    }
     mark(ctx, -1);
 }

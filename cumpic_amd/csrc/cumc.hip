@@ -15,6 +15,8 @@
 
 // float instantiations live in cumc_mixed.hip (built with FP32 contraction and fast division)
 void stage_mixed(cumc_ctx *ctx, int st);
+static void prim_to_f32(cumc_ctx *ctx);
+static void sync_prim64(cumc_ctx *ctx);
 void fluxes_mixed(cumc_ctx *ctx, int dir, double *aos_tmp);
 
 
@@ -100,11 +102,13 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     M.eps_floor = c.eps_floor;
     M.r_sh = c.r_sh;
 
-    // FP64 state: P 8 + EPS 1 + BC 3 + U0 8 + U1 8 + Ub 8 = 36 planes ; fluxes 21 + EMF 3 planes in R
+    // FP64 state: P 8 + EPS 1 + BC 3 + U0 8 + U1 8 + Ub 8 = 36 planes ; fluxes 21 + EMF 3 planes in R,
+    // and in the mixed form 12 more FP32 planes (P32 8, EPS32 1, BC32 3)
     const size_t nplanes = 36;
     ctx->pool_bytes = nplanes * (size_t)L.ps * sizeof(double);
-    const size_t rsz = c.precision == CUMC_MIXED ? sizeof(float) : sizeof(double);
-    const size_t fbytes = 24 * (size_t)L.ps * rsz;
+    const bool mixed = c.precision == CUMC_MIXED;
+    const size_t rsz = mixed ? sizeof(float) : sizeof(double);
+    const size_t fbytes = (mixed ? 36 : 24) * (size_t)L.ps * rsz;
     hipError_t e = hipMalloc(&ctx->pool, ctx->pool_bytes);
     if (e != hipSuccess) { delete ctx; return CUMC_ENOMEM; }
     e = hipMalloc(&ctx->fluxpool, fbytes);
@@ -137,6 +141,11 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     }
     F.E = (double *)ctx->fluxpool + (size_t)21 * L.ps;
     G.E = (float *)ctx->fluxpool + (size_t)21 * L.ps;
+    if (mixed) {
+        G.P32 = (float *)ctx->fluxpool + (size_t)24 * L.ps;
+        G.EPS32 = (float *)ctx->fluxpool + (size_t)32 * L.ps;
+        G.BC32 = (float *)ctx->fluxpool + (size_t)33 * L.ps;
+    }
     if (hipDeviceSynchronize() != hipSuccess) { cumc_destroy(ctx); return CUMC_EHIP; }
     *out = ctx;
     return CUMC_OK;
@@ -273,13 +282,19 @@ extern "C" int cumc_upload(cumc_ctx *ctx, const double *prim, const double *eps)
     HIPCHK(ctx, hipMemsetAsync(ctx->F64.BC, 0, 3 * (size_t)ctx->L.ps * sizeof(double), ctx->stream));
     int r = conv_in(ctx, prim, 8, ctx->F64.P);
     if (r) return r;
-    return conv_in(ctx, eps, 1, ctx->F64.EPS);
+    if ((r = conv_in(ctx, eps, 1, ctx->F64.EPS))) return r;
+    ctx->p64_stale = false;
+    prim_to_f32(ctx);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CUMC_OK;
 }
 
 extern "C" int cumc_download(cumc_ctx *ctx, double *prim, double *eps, double *bcell)
 {
     if (!ctx) return CUMC_EINVAL;
     int r;
+    sync_prim64(ctx);
     if (prim && (r = conv_out(ctx, ctx->F64.P, 8, prim))) return r;
     if (eps && (r = conv_out(ctx, ctx->F64.EPS, 1, eps))) return r;
     if (bcell && (r = conv_out(ctx, ctx->F64.BC, 3, bcell))) return r;
@@ -303,10 +318,33 @@ void launch_ghost(cumc_ctx *ctx)
     if (ctx->M.model == CUMC_MODEL_TORUS_CYN || ctx->M.model == CUMC_MODEL_TORUS_SPH) {
         // inflow clamp on the R / r ghosts; torus_cyn also clamps the z ghosts
         const long nyz = (long)(L.ny + 6) * (L.nz + 6) * 3, nxy = nx6 * ny6 * 3;
-        hipLaunchKernelGGL(k_custom_boundary_x, dim3(nblk(nyz, 256)), dim3(256), 0, s, L, ctx->F64);
+        hipLaunchKernelGGL(k_custom_boundary_x<double>, dim3(nblk(nyz, 256)), dim3(256), 0, s, L, ctx->F64.P);
         if (ctx->M.model == CUMC_MODEL_TORUS_CYN)
-            hipLaunchKernelGGL(k_custom_boundary_z, dim3(nblk(nxy, 256)), dim3(256), 0, s, L, ctx->F64);
+            hipLaunchKernelGGL(k_custom_boundary_z<double>, dim3(nblk(nxy, 256)), dim3(256), 0, s, L, ctx->F64.P);
     }
+}
+
+// mixed form: the FP32 primitive planes from the FP64 ones (after the host or an FP64 pass set
+// P / EPS / BC)
+static void prim_to_f32(cumc_ctx *ctx)
+{
+    if (ctx->cfg.precision != CUMC_MIXED) return;
+    const Fields<double> &F = ctx->F64;
+    const Fields<float> &G = ctx->F32;
+    hipLaunchKernelGGL(k_prim_to_f32, dim3(nblk(ctx->L.ps, 256)), dim3(256), 0, ctx->stream, ctx->L.ps, F.P, F.EPS,
+                       F.BC, G.P32, G.EPS32, G.BC32);
+}
+
+// mixed form: bring P / EPS / BC up to date with the conserved state U0 after steps (the stage
+// tail only writes the FP32 planes), ghost cells included
+static void sync_prim64(cumc_ctx *ctx)
+{
+    if (!ctx->p64_stale) return;
+    const Lay &L = ctx->L;
+    dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
+    hipLaunchKernelGGL(k_prim_from_u, grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, (const double *)ctx->F64.U0);
+    launch_ghost(ctx);
+    ctx->p64_stale = false;
 }
 
 
@@ -330,6 +368,8 @@ extern "C" int cumc_initial_update(cumc_ctx *ctx)
     dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
     hipLaunchKernelGGL(k_init, grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, ctx->F64.U0);
     launch_ghost(ctx);
+    ctx->p64_stale = false;
+    prim_to_f32(ctx);
     ctx->dt_valid = false;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -339,7 +379,9 @@ extern "C" int cumc_initial_update(cumc_ctx *ctx)
 extern "C" int cumc_boundary(cumc_ctx *ctx)
 {
     if (!ctx) return CUMC_EINVAL;
+    sync_prim64(ctx);
     launch_ghost(ctx);
+    prim_to_f32(ctx);
     ctx->dt_valid = false;
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -370,6 +412,7 @@ static void enqueue_finddt(cumc_ctx *ctx)
 static int finddt_to_current(cumc_ctx *ctx, double *dt)
 {
     if (!ctx->dt_valid) {
+        sync_prim64(ctx);
         mark(ctx, KC_FINDDT);
         enqueue_finddt(ctx);
         mark(ctx, -1);
@@ -446,6 +489,7 @@ extern "C" int cumc_find_divb(cumc_ctx *ctx, double *max_signed, double *max_abs
     if (!ctx || !max_signed || !max_abs) return CUMC_EINVAL;
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
     const Lay &L = ctx->L;
+    sync_prim64(ctx);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->red, 0ull, 2);
     dim3 grid(nblk((long)L.px * L.ny, 256));
     Tabs tb = tabs_of(ctx);
@@ -508,20 +552,24 @@ extern "C" int cumc_set_tuning(cumc_ctx *ctx, const cumc_tuning *t)
 
 // Tuner: one warm-up step, then one timed step per candidate with per-kernel-class HIP events:
 // the segment length is scored by the y + z sweeps, the plane chunk by the EMF + stage-tail
-// kernels.  The FP64 fields, the dt slots and the U0/U1 assignment are saved and restored, so
-// the state is as before; the per-class timing totals are cleared.
+// kernels.  The FP64 fields (and the mixed form's FP32 primitive planes), the dt slots and the
+// U0/U1 assignment are saved and restored, so the state is as before; the per-class timing
+// totals are cleared.
 extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
 {
     if (!ctx) return CUMC_EINVAL;
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     void *save = nullptr;
-    if (hipMalloc(&save, ctx->pool_bytes) != hipSuccess) return fail(ctx, CUMC_ENOMEM, "tuner state copy");
+    const size_t p32_bytes = ctx->F32.P32 ? 12 * (size_t)ctx->L.ps * sizeof(float) : 0;
+    if (hipMalloc(&save, ctx->pool_bytes + p32_bytes) != hipSuccess) return fail(ctx, CUMC_ENOMEM, "tuner state copy");
     double dts[2];
     double *u0 = ctx->F64.U0, *u1 = ctx->F64.U1;
-    const bool valid = ctx->dt_valid, prof = ctx->prof;
+    const bool valid = ctx->dt_valid, prof = ctx->prof, stale = ctx->p64_stale;
     const int seg0 = ctx->seg, zc0 = ctx->zchunk, tw0 = ctx->tile_w;
     hipMemcpyAsync(save, ctx->pool, ctx->pool_bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    if (p32_bytes)
+        hipMemcpyAsync((char *)save + ctx->pool_bytes, ctx->F32.P32, p32_bytes, hipMemcpyDeviceToDevice, ctx->stream);
     hipMemcpyAsync(dts, ctx->dt_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     int rc = CUMC_OK;
     double dt;
@@ -574,6 +622,8 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
         }
     }
     hipMemcpyAsync(ctx->pool, save, ctx->pool_bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    if (p32_bytes)
+        hipMemcpyAsync(ctx->F32.P32, (char *)save + ctx->pool_bytes, p32_bytes, hipMemcpyDeviceToDevice, ctx->stream);
     hipMemcpyAsync(ctx->dt_dev, dts, 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     hipFree(save);
@@ -582,6 +632,7 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
     ctx->F32.U0 = u0;
     ctx->F32.U1 = u1;
     ctx->dt_valid = valid;
+    ctx->p64_stale = stale;
     ctx->prof = prof;
     ctx->evn = 0;
     for (int c = 0; c < KC_N; c++) {

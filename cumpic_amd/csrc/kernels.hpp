@@ -76,7 +76,20 @@ struct Fields {
     double *U1;   // 8 planes: conserved state of the current stage (cons)
     R *F[3];      // 7 planes per sweep direction
     R *E;         // 3 planes: corner EMFs
+    // mixed form only: the primitive state as FP32 planes, written by the stage tail and the
+    // FP32 ghost fill and read by the sweeps and the EMF kernel (which compute in FP32 anyway);
+    // P / EPS / BC are then brought up to date from the conserved state when the host asks
+    float *P32 = nullptr;    // 8 planes, as P
+    float *EPS32 = nullptr;  // 1 plane
+    float *BC32 = nullptr;   // 3 planes
 };
+
+// primitive / cell-B planes the sweeps and the EMF kernel read: the FP64 state in the FP64
+// form, its FP32 copy in the mixed form
+__host__ __device__ inline const double *sweep_p(const Fields<double> &F) { return F.P; }
+__host__ __device__ inline const float *sweep_p(const Fields<float> &F) { return F.P32; }
+__host__ __device__ inline const double *sweep_bc(const Fields<double> &F) { return F.BC; }
+__host__ __device__ inline const float *sweep_bc(const Fields<float> &F) { return F.BC32; }
 
 __device__ inline double ld(const double *p) { return __builtin_nontemporal_load(p); }
 
@@ -123,15 +136,16 @@ __device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs 
 
     const unsigned gu = (unsigned)g;
     const long ps = L.ps;
+    const R *SP = sweep_p(F), *SB = sweep_bc(F);
     R q[7];
-    q[0] = R(ldi(F.P, gu));
-    q[1] = R(ldi(F.P + ps, gu));
-    q[2] = R(ldi(F.P + 2 * ps, gu));
-    q[3] = R(ldi(F.P + 3 * ps, gu));
-    q[4] = R(ldi(F.P + 4 * ps, gu));
-    q[5] = R(ldi(F.BC + ps, gu));       // By cell
-    q[6] = R(ldi(F.BC + 2 * ps, gu));   // Bz cell
-    const R bn = R(ldi(F.P + 5 * ps, gu));  // Bx on face j
+    q[0] = ldi(SP, gu);
+    q[1] = ldi(SP + ps, gu);
+    q[2] = ldi(SP + 2 * ps, gu);
+    q[3] = ldi(SP + 3 * ps, gu);
+    q[4] = ldi(SP + 4 * ps, gu);
+    q[5] = ldi(SB + ps, gu);       // By cell
+    q[6] = ldi(SB + 2 * ps, gu);   // Bz cell
+    const R bn = ldi(SP + 5 * ps, gu);  // Bx on face j
 
     R vl[7], vr[7];
 #pragma unroll
@@ -190,13 +204,14 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
 
     auto load7 = [&](int c, R *o) {
         const unsigned g = at(c);
-        o[0] = R(ldi(F.P, g));
-        o[1] = R(ldi(F.P + ps, g));
-        o[2] = R(ldi(F.P + 2 * ps, g));
-        o[3] = R(ldi(F.P + 3 * ps, g));
-        o[4] = R(ldi(F.P + 4 * ps, g));
-        o[5] = R(ldi(F.BC + bA * ps, g));
-        o[6] = R(ldi(F.BC + bB * ps, g));
+        const R *SP = sweep_p(F), *SB = sweep_bc(F);
+        o[0] = ldi(SP, g);
+        o[1] = ldi(SP + ps, g);
+        o[2] = ldi(SP + 2 * ps, g);
+        o[3] = ldi(SP + 3 * ps, g);
+        o[4] = ldi(SP + 4 * ps, g);
+        o[5] = ldi(SB + bA * ps, g);
+        o[6] = ldi(SB + bB * ps, g);
     };
 
     // window: w0 = v(c-1), w1 = v(c), w2 = v(c+1), w3 = v(c+2) ; slopes d0 = dm(c-1), d1 = dm(c), d2 = dm(c+1)
@@ -241,7 +256,7 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
         if (c > s0) {
             // face c-1: left state = vp(c-1), right state = vm(c)
             const unsigned gf = at(c - 1);
-            const R bn = R(ldi(F.P + ibn * ps, gf));
+            const R bn = ldi(sweep_p(F) + ibn * ps, gf);
             R fl[7];
             if (DIR == 2) {
                 // rotated frame n=y, t1=z, t2=x ; reconstructed cell B order (x, z)
@@ -336,8 +351,9 @@ template <typename R>
 __device__ inline void ecells(const Lay &L, const Fields<R> &F, long g, R &ex, R &ey, R &ez)
 {
     const long ps = L.ps;
-    const R vx = R(F.P[g + ps]), vy = R(F.P[g + 2 * ps]), vz = R(F.P[g + 3 * ps]);
-    const R bx = R(F.BC[g]), by = R(F.BC[g + ps]), bz = R(F.BC[g + 2 * ps]);
+    const R *SP = sweep_p(F), *SB = sweep_bc(F);
+    const R vx = SP[g + ps], vy = SP[g + 2 * ps], vz = SP[g + 3 * ps];
+    const R bx = SB[g], by = SB[g + ps], bz = SB[g + 2 * ps];
     ex = vz * by - vy * bz;
     ey = vx * bz - vz * bx;
     ez = vy * bx - vx * by;
@@ -708,8 +724,11 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
         if (threadIdx.x == 0) atomicMin(dtnext, (unsigned long long)__double_as_longlong(m));
     }
 }
-// Same stage tail with 64-bit per-lane addresses: measured faster than the 32-bit-offset form
-// when the flux and EMF planes are FP32 (the mixed form), slower when they are FP64.
+// The stage tail of the mixed form: the same arithmetic in FP64 with 64-bit per-lane addresses
+// (measured faster than the 32-bit-offset form when the flux and EMF planes are FP32).  The new
+// primitive state goes to the FP32 planes P32 / EPS32 / BC32 that the next sweeps read; the
+// FP64 P / EPS / BC are rebuilt from the conserved state only when the host needs them, which
+// takes 48 of the 96 bytes per cell this kernel would otherwise write.
 template <typename R, int STAGE, int COORD>
 __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, const double *dtp,
                                  const double *__restrict__ Uin, double *__restrict__ Uout,
@@ -782,9 +801,13 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
             if (interior) {
                 double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
                 if (COORD == 2 || COORD == 3 || M.model != 0) {
-                    const double rho = F.P[g], vx = F.P[g + ps], vy = F.P[g + 2 * ps], vz = F.P[g + 3 * ps];
-                    const double p = F.P[g + 4 * ps];
-                    const double bx = F.BC[g], by = F.BC[g + ps], bz = F.BC[g + 2 * ps];
+                    // the stage-input density exactly (U holds the same FP64 value as the primitive
+                    // rho), so the density-floor switch of the model gravity sees what the FP64
+                    // form sees; velocities and pressure from the FP32 planes
+                    const double rho = STAGE == 1 ? U0[g] : Uin[g];
+                    const double vx = F.P32[g + ps], vy = F.P32[g + 2 * ps], vz = F.P32[g + 3 * ps];
+                    const double p = F.P32[g + 4 * ps];
+                    const double bx = F.BC32[g], by = F.BC32[g + ps], bz = F.BC32[g + 2 * ps];
                     if (COORD == 2) {
                         const double bsq = bx * bx + by * by + bz * bz;
                         const double tx = (p + rho * vy * vy + 0.5 * bsq - by * by) / (xloc);
@@ -854,13 +877,13 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
                 bcx = 0.5 * (bxn + bxm);
                 bcy = 0.5 * (byn + bym);
                 bcz = 0.5 * (bzn + bzL);
-                F.BC[g] = bcx;
-                F.BC[g + ps] = bcy;
-                F.BC[g + 2 * ps] = bcz;
+                F.BC32[g] = (float)bcx;
+                F.BC32[g + ps] = (float)bcy;
+                F.BC32[g + 2 * ps] = (float)bcz;
             } else {
-                bcx = F.BC[g];
-                bcy = F.BC[g + ps];
-                bcz = F.BC[g + 2 * ps];
+                bcx = F.BC32[g];
+                bcy = F.BC32[g + ps];
+                bcz = F.BC32[g + 2 * ps];
             }
             // U_to_P, floors, P_to_U, EOS
             const double rho0 = u[0];
@@ -887,15 +910,15 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
             Uout[g + 5 * ps] = bxn;
             Uout[g + 6 * ps] = byn;
             Uout[g + 7 * ps] = bzn;
-            F.P[g] = rho;
-            F.P[g + ps] = vx;
-            F.P[g + 2 * ps] = vy;
-            F.P[g + 3 * ps] = vz;
-            F.P[g + 4 * ps] = p;
-            F.P[g + 5 * ps] = bxn;
-            F.P[g + 6 * ps] = byn;
-            F.P[g + 7 * ps] = bzn;
-            F.EPS[g] = eps;
+            F.P32[g] = (float)rho;
+            F.P32[g + ps] = (float)vx;
+            F.P32[g + 2 * ps] = (float)vy;
+            F.P32[g + 3 * ps] = (float)vz;
+            F.P32[g + 4 * ps] = (float)p;
+            F.P32[g + 5 * ps] = (float)bxn;
+            F.P32[g + 6 * ps] = (float)byn;
+            F.P32[g + 7 * ps] = (float)bzn;
+            F.EPS32[g] = (float)eps;
             if (STAGE == 3 && interior)
                 dtmin = fmin2<double>(cell_dt<COORD>(tb, M, j, k, l, rho, vx, vy, vz, p, bcx, bcy, bcz), dtmin);
             // carries
@@ -976,6 +999,66 @@ static __global__ __launch_bounds__(256) void k_init(Lay L, Fields<double> F, Mo
     F.EPS[g] = eps;
 }
 
+// Mixed form: the FP32 primitive planes from the FP64 ones, every cell of the padded volume.
+static __global__ __launch_bounds__(256) void k_prim_to_f32(long ps, const double *__restrict__ P,
+                                                            const double *__restrict__ EPS,
+                                                            const double *__restrict__ BC, float *__restrict__ P32,
+                                                            float *__restrict__ EPS32, float *__restrict__ BC32)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ps) return;
+#pragma unroll
+    for (int c = 0; c < 8; c++) P32[g + c * ps] = (float)P[g + c * ps];
+#pragma unroll
+    for (int c = 0; c < 3; c++) BC32[g + c * ps] = (float)BC[g + c * ps];
+    EPS32[g] = (float)EPS[g];
+}
+
+// Mixed form: the FP64 primitive state rebuilt from the conserved state U (the last stage
+// output) on [0,n]^3 -- face B copied on the whole range, cell-centred B, U_to_P with the
+// floors and the EOS on the interior, as the stage tail forms them (the ghost fill then
+// completes the rest).  Rounding can differ from the tail's own values in the last bit, since
+// U already went through P_to_U.
+static __global__ __launch_bounds__(256) void k_prim_from_u(Lay L, Fields<double> F, Model M, const double *U)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int k = (int)(t / L.px);
+    const int j = (int)(t - (long)k * L.px);
+    if (k > L.ny || j > L.nx) return;
+    const long g = L.id(j, k, l);
+    const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
+    const double bxn = U[g + 5 * ps], byn = U[g + 6 * ps], bzn = U[g + 7 * ps];
+    F.P[g + 5 * ps] = bxn;
+    F.P[g + 6 * ps] = byn;
+    F.P[g + 7 * ps] = bzn;
+    if (!(j >= 1 && k >= 1 && l >= 1)) return;
+    const double bcx = 0.5 * (bxn + U[g - sj + 5 * ps]);
+    const double bcy = 0.5 * (byn + U[g - sk + 6 * ps]);
+    const double bcz = 0.5 * (bzn + U[g - sl + 7 * ps]);
+    F.BC[g] = bcx;
+    F.BC[g + ps] = bcy;
+    F.BC[g + 2 * ps] = bcz;
+    const double rho0 = U[g];
+    const double vx = U[g + ps] / rho0, vy = U[g + 2 * ps] / rho0, vz = U[g + 3 * ps] / rho0;
+    const double bsq = bcx * bcx + bcy * bcy + bcz * bcz;
+    const double vsq = vx * vx + vy * vy + vz * vz;
+    double eps = (U[g + 4 * ps] - 0.5 * bsq) / rho0 - 0.5 * vsq;
+    double rho = rho0;
+    if (M.model != 0) {
+        if (rho < M.rho_floor) rho = M.rho_floor;
+        if (eps < M.eps_floor) eps = M.eps_floor;
+    } else {
+        eps = fmax2<double>(eps, 0.0);
+    }
+    F.P[g] = rho;
+    F.P[g + ps] = vx;
+    F.P[g + 2 * ps] = vy;
+    F.P[g + 3 * ps] = vz;
+    F.P[g + 4 * ps] = rho * eps * (M.ggas - 1.0);
+    F.EPS[g] = eps;
+}
+
 // ------------------------------------------------------------------------------------------
 // Ghost fill, one launch per pass in the reference's x -> y -> z order (boundary.f90:349-672 for
 // prim/bcell/transverse face B, boundary.f90:29-341 for eps), followed in each pass by UPDATE's
@@ -1011,8 +1094,12 @@ __host__ __device__ inline long ghost_threads(const Lay &L)
     return (long)(L.nx + 6) * (L.ny + 6) * 6;
 }
 
-template <int AX>
-__device__ inline void ghost_thread(const Lay &L, const Fields<double> &F, const Model &M, long t)
+// One ghost cell of pass AX on the primitive planes P (8), EPS, BC (3) of element type T: the
+// FP64 state, or the FP32 copy of the mixed form (whose ghost pressure is formed in FP64 and
+// rounded, as the stage tail forms the interior one).
+template <int AX, typename T>
+__device__ inline void ghost_cells(const Lay &L, T *__restrict__ P, T *__restrict__ EPS, T *__restrict__ BC,
+                                   const Model &M, long t)
 {
     // thread domain: (a, b) over the two other axes, g in 1..3, side in {0,1}
     int na, nb, a0, b0;  // ranges [a0, a0+na) x [b0, b0+nb)
@@ -1047,13 +1134,13 @@ __device__ inline void ghost_thread(const Lay &L, const Fields<double> &F, const
     const bool usef = flag >= 2;
     // prim(1:5), bcell(1:3), transverse face B
 #pragma unroll
-    for (int i = 0; i < 5; i++) F.P[dst + i * ps] = usef ? (double)fac[i] * F.P[src + i * ps] : F.P[src + i * ps];
+    for (int i = 0; i < 5; i++) P[dst + i * ps] = usef ? (T)fac[i] * P[src + i * ps] : P[src + i * ps];
 #pragma unroll
     for (int c = 0; c < 3; c++)
-        F.BC[dst + c * ps] = usef ? (double)fac[5 + c] * F.BC[src + c * ps] : F.BC[src + c * ps];
+        BC[dst + c * ps] = usef ? (T)fac[5 + c] * BC[src + c * ps] : BC[src + c * ps];
     const int tb1 = AX == 0 ? 6 : 5, tb2 = AX == 2 ? 6 : 7;
-    F.P[dst + tb1 * ps] = usef ? (double)fac[tb1] * F.P[src + tb1 * ps] : F.P[src + tb1 * ps];
-    F.P[dst + tb2 * ps] = usef ? (double)fac[tb2] * F.P[src + tb2 * ps] : F.P[src + tb2 * ps];
+    P[dst + tb1 * ps] = usef ? (T)fac[tb1] * P[src + tb1 * ps] : P[src + tb1 * ps];
+    P[dst + tb2 * ps] = usef ? (T)fac[tb2] * P[src + tb2 * ps] : P[src + tb2 * ps];
     // eps (BOUNDARY1D ranges: x pass k,l in [1,n]; y pass l in [1,n]; z pass all)
     bool final_region;
     if (AX == 0) final_region = a >= 1 && a <= L.ny && b >= 1 && b <= L.nz;
@@ -1063,16 +1150,26 @@ __device__ inline void ghost_thread(const Lay &L, const Fields<double> &F, const
         int ef = flag;
         bool doit = true;
         if (AX == 1 && side == 1) doit = eps_y_outer(M.bflag, &ef);
+        bool same_src = false;  // eps comes from the cell prim(1:5) came from
         if (doit) {
             const int es = ghost_src(ef, side, g, n);
+            same_src = es == srci;
             long esrc;
             if (AX == 0) esrc = L.id(es, a, b);
             else if (AX == 1) esrc = L.id(a, es, b);
             else esrc = L.id(a, b, es);
-            F.EPS[dst] = F.EPS[esrc];
+            EPS[dst] = EPS[esrc];
         }
-        F.P[dst + 4 * ps] = F.P[dst] * F.EPS[dst] * (M.ggas - 1.0);
+        // EOS.  With eps from the prim source cell the FP64 product equals the source pressure
+        // (formed by the same expression), which the FP32 planes then keep as copied.
+        if (sizeof(T) == 8 || !same_src) P[dst + 4 * ps] = (T)((double)P[dst] * (double)EPS[dst] * (M.ggas - 1.0));
     }
+}
+
+template <int AX>
+__device__ inline void ghost_thread(const Lay &L, const Fields<double> &F, const Model &M, long t)
+{
+    ghost_cells<AX, double>(L, F.P, F.EPS, F.BC, M, t);
 }
 
 template <int AX>
@@ -1081,8 +1178,16 @@ __global__ __launch_bounds__(256) void k_ghost(Lay L, Fields<double> F, Model M)
     ghost_thread<AX>(L, F, M, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// ghost fill of the mixed form's FP32 primitive planes
+template <int AX>
+__global__ __launch_bounds__(256) void k_ghost32(Lay L, float *P32, float *EPS32, float *BC32, Model M)
+{
+    ghost_cells<AX, float>(L, P32, EPS32, BC32, M, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // torus_cyn CUSTOM_BOUNDARY (custom.f90:47-81): no inflow through the R ghost zones ...
-static __global__ __launch_bounds__(256) void k_custom_boundary_x(Lay L, Fields<double> F)
+template <typename T>
+static __global__ __launch_bounds__(256) void k_custom_boundary_x(Lay L, T *P)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long nyz = (long)(L.ny + 6) * (L.nz + 6);
@@ -1090,14 +1195,15 @@ static __global__ __launch_bounds__(256) void k_custom_boundary_x(Lay L, Fields<
     const int g = (int)(t / nyz) + 1;
     const long r = t % nyz;
     const int k = (int)(r % (L.ny + 6)) - 2, l = (int)(r / (L.ny + 6)) - 2;
-    double *vx = F.P + L.ps;
+    T *vx = P + L.ps;
     const long a = L.id(1 - g, k, l), b = L.id(L.nx + g, k, l);
-    vx[a] = fmin2<double>(vx[a], 0.0);
-    vx[b] = fmax2<double>(vx[b], 0.0);
+    vx[a] = fmin2<T>(vx[a], T(0));
+    vx[b] = fmax2<T>(vx[b], T(0));
 }
 
 // ... and through the z ghost zones
-static __global__ __launch_bounds__(256) void k_custom_boundary_z(Lay L, Fields<double> F)
+template <typename T>
+static __global__ __launch_bounds__(256) void k_custom_boundary_z(Lay L, T *P)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ps = L.ps;
@@ -1106,10 +1212,10 @@ static __global__ __launch_bounds__(256) void k_custom_boundary_z(Lay L, Fields<
     const int g = (int)(t / nxy) + 1;
     const long r = t % nxy;
     const int j = (int)(r % (L.nx + 6)) - 2, k = (int)(r / (L.nx + 6)) - 2;
-    double *vz = F.P + 3 * ps;
+    T *vz = P + 3 * ps;
     long a = L.id(j, k, 1 - g), b = L.id(j, k, L.nz + g);
-    vz[a] = fmin2<double>(vz[a], 0.0);
-    vz[b] = fmax2<double>(vz[b], 0.0);
+    vz[a] = fmin2<T>(vz[a], T(0));
+    vz[b] = fmax2<T>(vz[b], T(0));
 }
 
 // ------------------------------------------------------------------------------------------
