@@ -386,6 +386,17 @@ def main():
              "state_finite": finite, "tuning": tuning}
         if copy_gbps:
             r["roofline"]["frac_of_copy_probe"] = r["roofline"]["achieved"] / copy_gbps
+        if form == "fp64" and r["roofline"]["kernel"] == "tail":
+            # the tail's access pattern alone (stage 1: 26 planes read, stages 2-3: 34; 20 written),
+            # same layout and column tiles, no arithmetic: the bandwidth this kernel could reach
+            cells = (cfg.nx + 1) * (cfg.ny + 1) * (cfg.nz + 1)
+            g1, g23 = s.pattern_probe(26, 20), s.pattern_probe(34, 20)
+            t_stage = (46 * 8 * cells / (g1 * 1e9) + 2 * 54 * 8 * cells / (g23 * 1e9)) / 3
+            ceiling = KERNEL_BYTES["fp64"]["tail"] * cells / t_stage / 1e9
+            r["roofline"]["pattern_ceiling_GBps"] = ceiling
+            r["roofline"]["frac_of_pattern_ceiling"] = r["roofline"]["achieved"] / ceiling
+            log(f"[bench] tail access pattern without arithmetic: {g1:.0f} GB/s (26 read + 20 written planes), "
+                f"{g23:.0f} GB/s (34 + 20); the tail reaches {r['roofline']['achieved'] / ceiling:.2f} of it")
         s.close()
         if form == "fp64":
             ref_dts = dts

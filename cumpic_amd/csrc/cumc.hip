@@ -690,6 +690,68 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double2 *__restrict__ 
     if (i < n) b[i] = a[i];
 }
 
+// The stage tail's memory access pattern with no arithmetic: NR FP64 planes read and NW written
+// at each cell, one (j,k) column per thread over a chunk of planes, with the tail's column tiles
+// and XCD tile order.  Its bandwidth is the ceiling the tail itself can reach on this layout.
+template <int NR, int NW>
+__global__ __launch_bounds__(256) void k_pattern_probe(Lay L, const double *__restrict__ in, double *__restrict__ out,
+                                                       int gx, int gy, int zc, int tw)
+{
+    int bx, by;
+    if (!xcd_tile(gx, gy, bx, by)) return;
+    int j, k;
+    col_of(L, bx, tw, j, k);
+    const int l0 = by * zc;
+    if (k > L.ny || j > L.nx || l0 > L.nz) return;
+    const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
+    const long ps = L.ps;
+    unsigned g = (unsigned)L.id(j, k, l0);
+    for (int l = l0; l <= l1; l++, g += (unsigned)L.py) {
+        double a = 0.0;
+#pragma unroll
+        for (int c = 0; c < NR; c++) a += ldi(in + c * ps, g);
+#pragma unroll
+        for (int c = 0; c < NW; c++) sti(out + c * ps, g, a + (double)c);
+        if (NW == 0 && a == -1.0) sti(out, g, a);  // keeps the loads of a read-only pattern
+    }
+}
+
+extern "C" int cumc_pattern_probe(cumc_ctx *ctx, int nread, int nwrite, int reps, double *gbps)
+{
+    if (!ctx || reps < 1 || !gbps) return CUMC_EINVAL;
+    const Lay &L = ctx->L;
+    const int zc = ctx->zchunk, tw = ctx->tile_w;
+    const int gx = col_tiles(L, tw), gy = (L.nz + zc) / zc;
+    const dim3 grid(xcd_blocks(gx, gy));
+    void (*kern)(Lay, const double *, double *, int, int, int, int) = nullptr;
+    if (nread == 26 && nwrite == 20) kern = k_pattern_probe<26, 20>;
+    else if (nread == 34 && nwrite == 20) kern = k_pattern_probe<34, 20>;
+    else if (nread == 34 && nwrite == 0) kern = k_pattern_probe<34, 0>;
+    else if (nread == 0 && nwrite == 20) kern = k_pattern_probe<0, 20>;
+    else return fail(ctx, CUMC_EINVAL, "pattern probe: supported (reads, writes) are (26,20) (34,20) (34,0) (0,20)");
+    double *in = nullptr, *out = nullptr;
+    const size_t pb = (size_t)L.ps * sizeof(double);
+    HIPCHK(ctx, hipMalloc(&in, (nread > 0 ? nread : 1) * pb));
+    if (hipMalloc(&out, (nwrite > 0 ? nwrite : 1) * pb) != hipSuccess) {
+        hipFree(in);
+        return fail(ctx, CUMC_ENOMEM, "pattern probe buffers");
+    }
+    hipMemsetAsync(in, 0, (nread > 0 ? nread : 1) * pb, ctx->stream);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, L, in, out, gx, gy, zc, tw);
+    hipEventRecord(ctx->ev0, ctx->stream);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, L, in, out, gx, gy, zc, tw);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    hipFree(in);
+    hipFree(out);
+    HIPCHK(ctx, e);
+    const double cells = (double)(L.nx + 1) * (L.ny + 1) * (L.nz + 1);
+    *gbps = (nread + nwrite) * 8.0 * cells * reps / (ms * 1e-3) / 1e9;
+    return CUMC_OK;
+}
+
 extern "C" int cumc_copy_probe(long bytes, int reps, double *gbps)
 {
     if (bytes < 1024 || reps < 1 || bytes / 16 / 256 >= 0x7fffffffL) return CUMC_EINVAL;

@@ -517,7 +517,9 @@ __device__ inline double block_min(double v)
 }
 
 // Column tile (bx, by) as in ecorn_tile.  Every thread of the block reaches the stage-3 block
-// reduction at the end, so callers may loop over tiles with block-uniform trip counts.
+// reduction at the end, so callers may loop over tiles with block-uniform trip counts.  In the
+// mixed form (R = float) the stage-input primitives come from, and the new ones go to, the FP32
+// planes P32 / EPS32 / BC32.
 template <typename R, int STAGE, int COORD>
 __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, const double *dtp,
                                  const double *__restrict__ Uin, double *__restrict__ Uout,
@@ -541,8 +543,10 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
         auto e = [&](int c, unsigned i) { return (double)ldi(F.E + c * ps, i); };
         auto u0 = [&](int c, unsigned i) { return ldi(U0 + c * ps, i); };
         auto ui = [&](int c, unsigned i) { return STAGE == 1 ? 0.0 : ldi(Uin + c * ps, i); };
-        auto pp = [&](int c, unsigned i) { return ldi(F.P + c * ps, i); };
-        auto bcl = [&](int c, unsigned i) { return ldi(F.BC + c * ps, i); };
+        // stage-input primitives: the FP64 planes, or the FP32 ones of the mixed form
+        constexpr bool f32p = sizeof(R) == 4;
+        auto pp = [&](int c, unsigned i) { return f32p ? (double)ldi(F.P32 + c * ps, i) : ldi(F.P + c * ps, i); };
+        auto bcl = [&](int c, unsigned i) { return f32p ? (double)ldi(F.BC32 + c * ps, i) : ldi(F.BC + c * ps, i); };
         const double gm1 = M.ggas - 1.0;
         // flux-divergence geometry of the column (coords.f90:255-282)
         const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k);
@@ -593,7 +597,11 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             if (interior) {
                 double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
                 if (COORD == 2 || COORD == 3 || M.model != 0) {
-                    const double rho = pp(0, g), vx = pp(1, g), vy = pp(2, g), vz = pp(3, g);
+                    // (the mixed form takes the density from the stage-input U, which holds the
+                    // FP64 value, so that the density-floor switch of the model gravity sees
+                    // what the FP64 form sees)
+                    const double rho = f32p ? (STAGE == 1 ? u0(0, g) : ui(0, g)) : pp(0, g);
+                    const double vx = pp(1, g), vy = pp(2, g), vz = pp(3, g);
                     const double p = pp(4, g);
                     const double bx = bcl(0, g), by = bcl(1, g), bz = bcl(2, g);
                     if (COORD == 2) {
@@ -665,9 +673,15 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
                 bcx = 0.5 * (bxn + bxm);
                 bcy = 0.5 * (byn + bym);
                 bcz = 0.5 * (bzn + bzL);
-                sti(F.BC, g, bcx);
-                sti(F.BC + ps, g, bcy);
-                sti(F.BC + 2 * ps, g, bcz);
+                if (f32p) {
+                    sti(F.BC32, g, (float)bcx);
+                    sti(F.BC32 + ps, g, (float)bcy);
+                    sti(F.BC32 + 2 * ps, g, (float)bcz);
+                } else {
+                    sti(F.BC, g, bcx);
+                    sti(F.BC + ps, g, bcy);
+                    sti(F.BC + 2 * ps, g, bcz);
+                }
             } else {
                 bcx = bcl(0, g);
                 bcy = bcl(1, g);
@@ -698,15 +712,27 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             sti(Uout + 5 * ps, g, bxn);
             sti(Uout + 6 * ps, g, byn);
             sti(Uout + 7 * ps, g, bzn);
-            sti(F.P, g, rho);
-            sti(F.P + ps, g, vx);
-            sti(F.P + 2 * ps, g, vy);
-            sti(F.P + 3 * ps, g, vz);
-            sti(F.P + 4 * ps, g, p);
-            sti(F.P + 5 * ps, g, bxn);
-            sti(F.P + 6 * ps, g, byn);
-            sti(F.P + 7 * ps, g, bzn);
-            sti(F.EPS, g, eps);
+            if (f32p) {
+                sti(F.P32, g, (float)rho);
+                sti(F.P32 + ps, g, (float)vx);
+                sti(F.P32 + 2 * ps, g, (float)vy);
+                sti(F.P32 + 3 * ps, g, (float)vz);
+                sti(F.P32 + 4 * ps, g, (float)p);
+                sti(F.P32 + 5 * ps, g, (float)bxn);
+                sti(F.P32 + 6 * ps, g, (float)byn);
+                sti(F.P32 + 7 * ps, g, (float)bzn);
+                sti(F.EPS32, g, (float)eps);
+            } else {
+                sti(F.P, g, rho);
+                sti(F.P + ps, g, vx);
+                sti(F.P + 2 * ps, g, vy);
+                sti(F.P + 3 * ps, g, vz);
+                sti(F.P + 4 * ps, g, p);
+                sti(F.P + 5 * ps, g, bxn);
+                sti(F.P + 6 * ps, g, byn);
+                sti(F.P + 7 * ps, g, bzn);
+                sti(F.EPS, g, eps);
+            }
             if (STAGE == 3 && interior)
                 dtmin = fmin2<double>(cell_dt<COORD>(tb, M, j, k, l, rho, vx, vy, vz, p, bcx, bcy, bcz), dtmin);
             // carries
@@ -724,11 +750,11 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
         if (threadIdx.x == 0) atomicMin(dtnext, (unsigned long long)__double_as_longlong(m));
     }
 }
-// The stage tail of the mixed form: the same arithmetic in FP64 with 64-bit per-lane addresses
-// (measured faster than the 32-bit-offset form when the flux and EMF planes are FP32).  The new
-// primitive state goes to the FP32 planes P32 / EPS32 / BC32 that the next sweeps read; the
-// FP64 P / EPS / BC are rebuilt from the conserved state only when the host needs them, which
-// takes 48 of the 96 bytes per cell this kernel would otherwise write.
+// The stage tail of the mixed form's stages 2 and 3: the arithmetic of tail_tile with 64-bit
+// per-lane addresses, which measured faster there.  As in tail_tile's mixed instantiation the
+// new primitive state goes to the FP32 planes P32 / EPS32 / BC32 that the next sweeps read (48
+// bytes per cell instead of 96); the FP64 P / EPS / BC are rebuilt from the conserved state only
+// when the host needs them.
 template <typename R, int STAGE, int COORD>
 __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, const double *dtp,
                                  const double *__restrict__ Uin, double *__restrict__ Uout,
@@ -944,8 +970,12 @@ __global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model
 {
     int bx, by;
     if (!xcd_tile(gx, gy, bx, by)) return;
-    if constexpr (sizeof(R) == 4) tail_tile_wide<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
-    else tail_tile<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
+    // mixed form, stages 2-3: the 64-bit-address form (1.30 against 1.34 ms per launch on the
+    // 256^3 box); stage 1 and the FP64 form: 32-bit offsets (mixed stage 1: 1.11 against 1.73 ms)
+    if constexpr (sizeof(R) == 4 && STAGE != 1)
+        tail_tile_wide<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
+    else
+        tail_tile<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
 }
 
 // initial_update's FROMRVETOU + CUSTOM_CHECKRHO + EOS on [0,n]^3 (initial.f90:103-126)

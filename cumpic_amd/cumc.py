@@ -77,6 +77,7 @@ def lib():
             "cumc_profile": [_CTX, ctypes.c_int],
             "cumc_kernel_times": [_CTX, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_long),
                                   ctypes.POINTER(ctypes.c_char_p)],
+            "cumc_pattern_probe": [_CTX, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D],
         }
         T = ctypes.POINTER(Tuning)
         sig.update({
@@ -240,6 +241,14 @@ class Solver:
         t = Tuning()
         self._check(lib().cumc_autotune(self.h, ctypes.byref(t)), "cumc_autotune")
         return t.as_dict()
+
+    def pattern_probe(self, nread: int, nwrite: int, reps: int = 5) -> float:
+        """GB/s of the stage tail's access pattern (nread FP64 planes read, nwrite written per
+        cell, no arithmetic) on this context's layout and column tiles."""
+        g = ctypes.c_double()
+        self._check(lib().cumc_pattern_probe(self.h, int(nread), int(nwrite), int(reps), ctypes.byref(g)),
+                    "cumc_pattern_probe")
+        return g.value
 
     def tape_run(self, nsteps: int, every: int = 8, dts=None):
         """initial_update + nsteps steps from the uploaded state, checkpointed every `every` steps

@@ -107,6 +107,14 @@ int cumc_kernel_times(cumc_ctx *ctx, int n, double *total_ms, long *launches, co
  * known-size kernel against which HBM traffic counters are calibrated. */
 int cumc_copy_probe(long bytes, int reps, double *gbps);
 
+/* Diagnostic, no reference counterpart: the stage tail's memory access pattern on this context's
+ * layout and tuned column tiles, with no arithmetic -- `nread` FP64 planes read and `nwrite`
+ * written per cell of [0,n]^3, run `reps` times; reports (nread + nwrite) * 8 bytes per cell per
+ * second in GB/s, the bandwidth ceiling of a kernel with that access pattern.  Supported
+ * (nread, nwrite): (26, 20) and (34, 20) (the FP64 tail of stage 1 and of stages 2-3), (34, 0)
+ * and (0, 20). */
+int cumc_pattern_probe(cumc_ctx *ctx, int nread, int nwrite, int reps, double *gbps);
+
 const char *cumc_error_string(int code);
 const char *cumc_last_error(cumc_ctx *ctx);
 
