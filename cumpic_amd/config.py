@@ -18,6 +18,8 @@ IRHO, IVX, IVY, IVZ, ITAU, IBX, IBY, IBZ = range(8)
 
 CARTESIAN, CYLINDRICAL, SPHERICAL = 1, 2, 3  # definition.f90:65-67
 TVDMM, PPMC = 1, 4  # definition.f90:158,167
+TVDMC, TVDVL = 2, 3  # definition.f90:161,164: declared there, implemented here as extensions
+LF, HLL, HLLD = 1, 2, 4  # definition.f90:140,143,149: HLL is the reference's; LF and HLLD extensions
 MODEL_RIEMANN, MODEL_TORUS_CYN, MODEL_TORUS_SPH = 0, 1, 2
 FP64, MIXED = 0, 1
 
@@ -38,6 +40,7 @@ class Config:
     eps_floor: float = 0.0
     r_sh: float = 1.0
     precision: int = FP64
+    solver: int = HLL
 
     @property
     def shape(self):
@@ -71,6 +74,7 @@ class CConfig(ctypes.Structure):
         ("rho_floor", ctypes.c_double),
         ("eps_floor", ctypes.c_double),
         ("r_sh", ctypes.c_double),
+        ("solver", ctypes.c_int),
     ]
 
 
@@ -88,6 +92,7 @@ def to_cconfig(cfg: Config) -> CConfig:
     c.rho_floor = cfg.rho_floor
     c.eps_floor = cfg.eps_floor
     c.r_sh = cfg.r_sh
+    c.solver = cfg.solver
     return c
 
 

@@ -146,20 +146,18 @@ void batch_after_run(cumc_batch *b, int nsteps);
 void launch_ghost(cumc_ctx *ctx);
 void launch_ghost32(cumc_ctx *ctx);
 
-template <typename R>
-void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
+template <typename R, int RECON, int SOLVER>
+void launch_fluxes_t(cumc_ctx *ctx, const Fields<R> &F, int only_dir)
 {
     const Lay &L = ctx->L;
     hipStream_t s = ctx->stream;
     Tabs tb = tabs_of(ctx);
     const Model &M = ctx->M;
-    const bool ppm = M.recon == CUMC_PPMC;
     if (!only_dir || only_dir == 1) {
         mark(ctx, KC_FLUX_X);
         const long waves = ((long)(L.ny + 2) * L.px + XS_FACES - 1) / XS_FACES;
         dim3 grid(nblk(waves * 64, 256), L.nz + 2);
-        if (ppm) hipLaunchKernelGGL((k_flux_x<R, 4>), grid, dim3(256), 0, s, L, F, tb, M);
-        else hipLaunchKernelGGL((k_flux_x<R, 1>), grid, dim3(256), 0, s, L, F, tb, M);
+        hipLaunchKernelGGL((k_flux_x<R, RECON, SOLVER>), grid, dim3(256), 0, s, L, F, tb, M);
         mark(ctx, -1);
     }
     const int seg = ctx->seg;
@@ -168,8 +166,7 @@ void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
         const int nseg = (L.ny + 1 + seg - 1) / seg;
         const int sl = (L.ny + 1 + nseg - 1) / nseg;
         dim3 grid(nblk((long)L.px * (L.nz + 2), 128), nseg);
-        if (ppm) hipLaunchKernelGGL((k_flux_walk<R, 2, 4>), grid, dim3(128), 0, s, L, F, tb, M, sl);
-        else hipLaunchKernelGGL((k_flux_walk<R, 2, 1>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+        hipLaunchKernelGGL((k_flux_walk<R, 2, RECON, SOLVER>), grid, dim3(128), 0, s, L, F, tb, M, sl);
         mark(ctx, -1);
     }
     if (!only_dir || only_dir == 3) {
@@ -177,9 +174,30 @@ void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
         const int nseg = (L.nz + 1 + seg - 1) / seg;
         const int sl = (L.nz + 1 + nseg - 1) / nseg;
         dim3 grid(nblk((long)L.px * (L.ny + 2), 128), nseg);
-        if (ppm) hipLaunchKernelGGL((k_flux_walk<R, 3, 4>), grid, dim3(128), 0, s, L, F, tb, M, sl);
-        else hipLaunchKernelGGL((k_flux_walk<R, 3, 1>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+        hipLaunchKernelGGL((k_flux_walk<R, 3, RECON, SOLVER>), grid, dim3(128), 0, s, L, F, tb, M, sl);
         mark(ctx, -1);
+    }
+}
+
+template <typename R, int SOLVER>
+void launch_fluxes_s(cumc_ctx *ctx, const Fields<R> &F, int only_dir)
+{
+    switch (ctx->M.recon) {
+    case CUMC_TVDMM: launch_fluxes_t<R, 1, SOLVER>(ctx, F, only_dir); break;
+    case CUMC_TVDMC: launch_fluxes_t<R, 2, SOLVER>(ctx, F, only_dir); break;
+    case CUMC_TVDVL: launch_fluxes_t<R, 3, SOLVER>(ctx, F, only_dir); break;
+    default: launch_fluxes_t<R, 4, SOLVER>(ctx, F, only_dir); break;
+    }
+}
+
+// the three sweeps (or one, only_dir = 1..3) with the configured reconstruction and solver
+template <typename R>
+void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
+{
+    switch (ctx->M.solver) {
+    case CUMC_LF: launch_fluxes_s<R, 1>(ctx, F, only_dir); break;
+    case CUMC_HLLD: launch_fluxes_s<R, 4>(ctx, F, only_dir); break;
+    default: launch_fluxes_s<R, 2>(ctx, F, only_dir); break;
     }
 }
 

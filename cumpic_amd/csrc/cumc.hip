@@ -61,7 +61,9 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     if (c.nx < 1 || c.ny < 1 || c.nz < 1) return CUMC_EINVAL;
     if (c.coordinate != CUMC_CARTESIAN && c.coordinate != CUMC_CYLINDRICAL && c.coordinate != CUMC_SPHERICAL)
         return CUMC_EINVAL;
-    if (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM) return CUMC_EINVAL;
+    if (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM && c.recon != CUMC_TVDMC && c.recon != CUMC_TVDVL)
+        return CUMC_EINVAL;
+    if (c.solver != 0 && c.solver != CUMC_HLL && c.solver != CUMC_LF && c.solver != CUMC_HLLD) return CUMC_EINVAL;
     if (c.model != CUMC_MODEL_RIEMANN && c.model != CUMC_MODEL_TORUS_CYN && c.model != CUMC_MODEL_TORUS_SPH)
         return CUMC_EINVAL;
     if (c.precision != CUMC_FP64 && c.precision != CUMC_MIXED) return CUMC_EINVAL;
@@ -92,6 +94,7 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     Model &M = ctx->M;
     M.coordinate = c.coordinate;
     M.recon = c.recon;
+    M.solver = c.solver == 0 ? CUMC_HLL : c.solver;
     M.model = c.model;
     M.geom_repeat = c.geom_source_repeat;
     for (int s = 0; s < 6; s++) M.bflag[s] = c.boundary_flag[s];

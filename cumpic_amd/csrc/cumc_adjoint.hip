@@ -1073,6 +1073,10 @@ static bool adjoint_supported(const cumc_ctx *ctx, std::string &why)
     if (c.precision != CUMC_FP64) { why = "reverse mode needs the FP64 form"; return false; }
     if (c.coordinate != CUMC_CARTESIAN) { why = "reverse mode covers Cartesian grids"; return false; }
     if (c.model != CUMC_MODEL_RIEMANN) { why = "reverse mode covers the riemann model hooks"; return false; }
+    if (ctx->M.solver != CUMC_HLL || (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM)) {
+        why = "reverse mode covers the reference's HLL solver with PPM or TVD minmod";
+        return false;
+    }
     if (c.boundary_flag[2] != 0 && c.boundary_flag[3] == 0) {
         why = "y-outer periodic with y-inner non-periodic leaves ghost eps stale (BOUNDARY1D); not differentiated";
         return false;

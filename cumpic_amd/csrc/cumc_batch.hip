@@ -463,6 +463,8 @@ extern "C" int cumc_batch_create(cumc_ctx *const *boxes, int nbox, cumc_batch **
         if (!c || !c->grid_set) return CUMC_ESTATE;
         if (c->cfg.precision != CUMC_FP64 || c->cfg.model != CUMC_MODEL_RIEMANN) return CUMC_EINVAL;
         if (c->cfg.coordinate != boxes[0]->cfg.coordinate || c->cfg.recon != boxes[0]->cfg.recon) return CUMC_EINVAL;
+        // the batched sweeps carry the reference's options only: HLL with PPM or TVD minmod
+        if (c->M.solver != CUMC_HLL || (c->cfg.recon != CUMC_PPMC && c->cfg.recon != CUMC_TVDMM)) return CUMC_EINVAL;
         for (int j = 0; j < i; j++)
             if (boxes[j] == c) return CUMC_EINVAL;
     }

@@ -61,6 +61,7 @@ struct Tabs {
 
 struct Model {
     int coordinate, recon, model, geom_repeat;
+    int solver;  // CUMC_HLL, CUMC_LF or CUMC_HLLD
     int bflag[6];
     int bfac[6][8];  // x-in, x-out, y-in, y-out, z-in, z-out
     double ggas, cfl, rho_floor, eps_floor, r_sh;
@@ -113,13 +114,13 @@ __device__ inline void recon_lane(const Tabs &tb, int ax, int jc, R c, R &vm, R 
                      tb.atr<R>(ax, TB_A3M, jc), tb.atr<R>(ax, TB_A0P, jc), tb.atr<R>(ax, TB_A1P, jc),
                      tb.atr<R>(ax, TB_A2P, jc), tb.atr<R>(ax, TB_A3P, jc), cm1, c, cp1, dmm1, dm, dmp1, vm, vp);
     } else {
-        tvd_edges<R>(tb.atr<R>(ax, TB_CF, jc), tb.atr<R>(ax, TB_CB, jc), cm1, c, cp1, vm, vp);
+        tvd_any<R, RECON>(tb.atr<R>(ax, TB_CF, jc), tb.atr<R>(ax, TB_CB, jc), cm1, c, cp1, vm, vp);
     }
 }
 
 // One 256-thread tile of the x sweep: tile bx of plane l.  Tiles per plane:
 // ceil(ceil((ny+2)*px / XS_FACES) * 64 / 256), planes l in [0, nz+1].
-template <typename R, int RECON>
+template <typename R, int RECON, int SOLVER = 2>
 __device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, int bx, int l)
 {
     const int lane = threadIdx.x & 63;
@@ -158,7 +159,7 @@ __device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs 
     FaceState<R> sl{vl[0], vl[1], vl[2], vl[3], vl[4], bn, vl[5], vl[6]};
     FaceState<R> sr{vr[0], vr[1], vr[2], vr[3], vr[4], bn, vr[5], vr[6]};
     R fl[7];
-    hll_face<1, R>(R(M.ggas), sl, sr, fl);
+    riemann_face<1, R, SOLVER>(R(M.ggas), sl, sr, fl);
     if (active) {
         R *o = F.F[0];
 #pragma unroll
@@ -166,10 +167,10 @@ __device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs 
     }
 }
 
-template <typename R, int RECON>
+template <typename R, int RECON, int SOLVER>
 __global__ __launch_bounds__(256) void k_flux_x(Lay L, Fields<R> F, Tabs tb, Model M)
 {
-    flux_x_tile<R, RECON>(L, F, tb, M, blockIdx.x, blockIdx.y);
+    flux_x_tile<R, RECON, SOLVER>(L, F, tb, M, blockIdx.x, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_flux_x(Lay L, Fields<R> F, Tabs tb, Mod
 // (solver/spatial.f90:474-511 and :552-589)
 // ------------------------------------------------------------------------------------------
 // One 128-thread tile: bx covers px*(nt+2) pencils, segment by of seglen faces along DIR.
-template <typename R, int DIR, int RECON>
+template <typename R, int DIR, int RECON, int SOLVER = 2>
 __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, int seglen,
                                       int bx, int by)
 {
@@ -251,7 +252,7 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
         } else {
             R cf = tb.atr<R>(ax, TB_CF, c), cb = tb.atr<R>(ax, TB_CB, c);
 #pragma unroll
-            for (int v = 0; v < 7; v++) tvd_edges<R>(cf, cb, w0[v], w1[v], w2[v], vm[v], vp[v]);
+            for (int v = 0; v < 7; v++) tvd_any<R, RECON>(cf, cb, w0[v], w1[v], w2[v], vm[v], vp[v]);
         }
         if (c > s0) {
             // face c-1: left state = vp(c-1), right state = vm(c)
@@ -262,7 +263,7 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
                 // rotated frame n=y, t1=z, t2=x ; reconstructed cell B order (x, z)
                 FaceState<R> sl{vpp[0], vpp[2], vpp[3], vpp[1], vpp[4], bn, vpp[6], vpp[5]};
                 FaceState<R> sr{vm[0], vm[2], vm[3], vm[1], vm[4], bn, vm[6], vm[5]};
-                hll_face<2, R>(R(M.ggas), sl, sr, fl);
+                riemann_face<2, R, SOLVER>(R(M.ggas), sl, sr, fl);
                 R *o = F.F[1];
                 sti(o, gf, fl[0]);
                 sti(o + 1 * ps, gf, fl[3]);
@@ -275,7 +276,7 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
                 // rotated frame n=z, t1=x, t2=y ; reconstructed cell B order (x, y)
                 FaceState<R> sl{vpp[0], vpp[3], vpp[1], vpp[2], vpp[4], bn, vpp[5], vpp[6]};
                 FaceState<R> sr{vm[0], vm[3], vm[1], vm[2], vm[4], bn, vm[5], vm[6]};
-                hll_face<3, R>(R(M.ggas), sl, sr, fl);
+                riemann_face<3, R, SOLVER>(R(M.ggas), sl, sr, fl);
                 R *o = F.F[2];
                 sti(o, gf, fl[0]);
                 sti(o + 1 * ps, gf, fl[2]);
@@ -296,10 +297,10 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
     }
 }
 
-template <typename R, int DIR, int RECON>
+template <typename R, int DIR, int RECON, int SOLVER>
 __global__ __launch_bounds__(128) void k_flux_walk(Lay L, Fields<R> F, Tabs tb, Model M, int seglen)
 {
-    flux_walk_tile<R, DIR, RECON>(L, F, tb, M, seglen, blockIdx.x, blockIdx.y);
+    flux_walk_tile<R, DIR, RECON, SOLVER>(L, F, tb, M, seglen, blockIdx.x, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------------

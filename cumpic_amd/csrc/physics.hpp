@@ -136,6 +136,34 @@ __host__ __device__ inline void tvd_edges(R cf, R cb, R vm1, R vc, R vp1, R &vm_
     vm_out = vc - R(0.5) * slope;
 }
 
+// TVD with the MC (LIM 2: minmod(cf dq+, cb dq-, (dq+ + dq-)/2)) or van Leer (LIM 3: harmonic
+// mean) limiter on the width-normalised differences of TVD_MM (reconstruct.f90:115-124): the
+// TVDMC / TVDVL options that definition.f90:160-164 declares and the reference leaves
+// unimplemented (extensions; restated as orc_tvd).
+template <typename R, int LIM>
+__host__ __device__ inline void tvd_lim_edges(R cf, R cb, R vm1, R vc, R vp1, R &vm_out, R &vp_out)
+{
+    R dqf = R(2.0) * (vp1 - vc) / cf;
+    R dqb = R(2.0) * (vc - vm1) / cb;
+    R slope;
+    if (LIM == 2) {
+        R m = fmin2<R>(fmin2<R>(cf * fabs_<R>(dqf), cb * fabs_<R>(dqb)), R(0.5) * fabs_<R>(dqf + dqb));
+        slope = R(0.5) * (sign1<R>(dqf) + sign1<R>(dqb)) * m;
+    } else {
+        slope = (dqf * dqb > R(0)) ? R(2.0) * dqf * dqb / (dqf + dqb) : R(0);
+    }
+    vp_out = vc + R(0.5) * slope;
+    vm_out = vc - R(0.5) * slope;
+}
+
+// edge values of cell c by a TVD option: 1 minmod (the reference's TVD_MM), 2 MC, 3 van Leer
+template <typename R, int RECON>
+__host__ __device__ inline void tvd_any(R cf, R cb, R vm1, R vc, R vp1, R &vm_out, R &vp_out)
+{
+    if (RECON == 1) tvd_edges<R>(cf, cb, vm1, vc, vp1, vm_out, vp_out);
+    else tvd_lim_edges<R, RECON>(cf, cb, vm1, vc, vp1, vm_out, vp_out);
+}
+
 // One side of a face in the rotated frame of the sweep: n = normal, t1/t2 = transverse in the
 // cyclic order of flux.f90:182-206 (x: n=x,t1=y,t2=z ; y: n=y,t1=z,t2=x ; z: n=z,t1=x,t2=y).
 template <typename R>
@@ -212,6 +240,155 @@ __host__ __device__ inline void hll_face(R g, const FaceState<R> &L, const FaceS
     } else {
         for (int i = 0; i < 7; i++) f[i] = R(NAN);
     }
+}
+
+// One side of a face with its conserved vector and physical flux in the rotated order rho, m_n,
+// m_t1, m_t2, E, B_t1, B_t2 (the expressions of hll_face), total pressure and fast speed: the
+// common front of the LF and HLLD solvers.
+template <typename R>
+struct Side7 {
+    R rho, vn, vt1, vt2, p, bt1, bt2, pt, cf;
+    R u[7], f[7];
+};
+
+template <int DIR, typename R>
+__host__ __device__ inline void face_side(R g, const FaceState<R> &S, Side7<R> &o)
+{
+    const R gm1 = g - R(1.0);
+    R eps = S.p / S.rho / gm1;
+    R cs = rsqrt_<R>(g * S.p / S.rho);
+    R vsq = dot_xyz<DIR, R>(S.vn, S.vt1, S.vt2, S.vn, S.vt1, S.vt2);
+    R bsq = dot_xyz<DIR, R>(S.bn, S.bt1, S.bt2, S.bn, S.bt1, S.bt2);
+    R vdb = dot_xyz<DIR, R>(S.bn, S.bt1, S.bt2, S.vn, S.vt1, S.vt2);
+    o.rho = S.rho; o.vn = S.vn; o.vt1 = S.vt1; o.vt2 = S.vt2; o.p = S.p; o.bt1 = S.bt1; o.bt2 = S.bt2;
+    o.u[0] = S.rho;
+    o.u[1] = S.vn * S.rho;
+    o.u[2] = S.vt1 * S.rho;
+    o.u[3] = S.vt2 * S.rho;
+    o.u[4] = S.rho * (eps + R(0.5) * vsq) + R(0.5) * bsq;
+    o.u[5] = S.bt1;
+    o.u[6] = S.bt2;
+    o.f[0] = o.u[0] * S.vn;
+    o.f[1] = o.u[1] * S.vn + S.p + R(0.5) * bsq - S.bn * S.bn;
+    o.f[2] = o.u[2] * S.vn - S.bn * S.bt1;
+    o.f[3] = o.u[3] * S.vn - S.bn * S.bt2;
+    o.f[4] = o.u[4] * S.vn + (S.p + R(0.5) * bsq) * S.vn - S.bn * vdb;
+    o.f[5] = (S.vn * S.bt1 - S.vt1 * S.bn);
+    o.f[6] = (S.vn * S.bt2 - S.vt2 * S.bn);
+    o.pt = S.p + R(0.5) * bsq;
+    o.cf = signalspeed<R>(cs, S.bn, S.bt1, S.bt2, S.rho);
+}
+
+// Local Lax-Friedrichs (Rusanov) flux, the LF option of definition.f90:139-140 (an extension;
+// restated as orc_lf): F = (F_L + F_R)/2 - s (U_R - U_L)/2, s = max(|v_nL| + c_fL, |v_nR| + c_fR).
+template <int DIR, typename R>
+__host__ __device__ inline void lf_face(R g, const FaceState<R> &L, const FaceState<R> &Rr, R f[7])
+{
+    Side7<R> a, b;
+    face_side<DIR, R>(g, L, a);
+    face_side<DIR, R>(g, Rr, b);
+    const R smax = fmax2<R>(fabs_<R>(a.vn) + a.cf, fabs_<R>(b.vn) + b.cf);
+    for (int i = 0; i < 7; i++) f[i] = R(0.5) * (a.f[i] + b.f[i]) - R(0.5) * smax * (b.u[i] - a.u[i]);
+}
+
+// HLLD flux of Miyoshi & Kusano (2005, J. Comput. Phys. 208, 315), the HLLD option of
+// definition.f90:148-149 (an extension; restated as orc_hlld with the same operation order):
+// outer speeds min/max(v_nL, v_nR) -/+ max(c_fL, c_fR) (eq. 67); contact speed S_M and p_T*
+// (eqs. 38, 41); single-star states (eqs. 43-48, degenerate denominators keep the transverse v
+// and B); Alfven speeds S_M -/+ |B_n|/sqrt(rho*) (eq. 51); double-star states (eqs. 59-63).
+template <int DIR, typename R>
+__host__ __device__ inline void hlld_face(R g, const FaceState<R> &L, const FaceState<R> &Rr, R f[7])
+{
+    Side7<R> s[2];
+    face_side<DIR, R>(g, L, s[0]);
+    face_side<DIR, R>(g, Rr, s[1]);
+    const R bn = L.bn;
+    const R cmax = fmax2<R>(s[0].cf, s[1].cf);
+    const R sL = fmin2<R>(s[0].vn, s[1].vn) - cmax;
+    const R sR = fmax2<R>(s[0].vn, s[1].vn) + cmax;
+    if (sL >= R(0)) {
+        for (int i = 0; i < 7; i++) f[i] = s[0].f[i];
+        return;
+    }
+    if (sR <= R(0)) {
+        for (int i = 0; i < 7; i++) f[i] = s[1].f[i];
+        return;
+    }
+    if (!(sL < R(0) && sR > R(0))) {
+        for (int i = 0; i < 7; i++) f[i] = R(NAN);
+        return;
+    }
+    const R sdl = sL - s[0].vn, sdr = sR - s[1].vn;
+    const R den = sdr * s[1].rho - sdl * s[0].rho;
+    const R sm = (sdr * s[1].rho * s[1].vn - sdl * s[0].rho * s[0].vn - s[1].pt + s[0].pt) / den;
+    const R pts = (sdr * s[1].rho * s[0].pt - sdl * s[0].rho * s[1].pt + s[0].rho * s[1].rho * sdr * sdl * (s[1].vn - s[0].vn)) / den;
+    R us[2][7], vts[2][2], bts[2][2], sq[2];
+    const R sa[2] = {sL, sR}, sda[2] = {sdl, sdr};
+    for (int a = 0; a < 2; a++) {
+        const Side7<R> &q = s[a];
+        const R sdm = sa[a] - sm;
+        const R rhos = q.rho * sda[a] / sdm;
+        const R tmp = q.rho * sda[a] * sdm - bn * bn;
+        if (fabs_<R>(tmp) < R(1.0e-8) * fabs_<R>(pts)) {
+            vts[a][0] = q.vt1;
+            vts[a][1] = q.vt2;
+            bts[a][0] = q.bt1;
+            bts[a][1] = q.bt2;
+        } else {
+            const R fac = (q.rho * sda[a] * sda[a] - bn * bn) / tmp;
+            vts[a][0] = q.vt1 - bn * q.bt1 * (sm - q.vn) / tmp;
+            vts[a][1] = q.vt2 - bn * q.bt2 * (sm - q.vn) / tmp;
+            bts[a][0] = q.bt1 * fac;
+            bts[a][1] = q.bt2 * fac;
+        }
+        const R vdb = q.vn * bn + q.vt1 * q.bt1 + q.vt2 * q.bt2;
+        const R vdbs = sm * bn + vts[a][0] * bts[a][0] + vts[a][1] * bts[a][1];
+        const R es = (sda[a] * q.u[4] - q.pt * q.vn + pts * sm + bn * (vdb - vdbs)) / sdm;
+        us[a][0] = rhos;
+        us[a][1] = rhos * sm;
+        us[a][2] = rhos * vts[a][0];
+        us[a][3] = rhos * vts[a][1];
+        us[a][4] = es;
+        us[a][5] = bts[a][0];
+        us[a][6] = bts[a][1];
+        sq[a] = rsqrt_<R>(rhos);
+    }
+    const R ssL = sm - fabs_<R>(bn) / sq[0];
+    const R ssR = sm + fabs_<R>(bn) / sq[1];
+    if (ssL >= R(0)) {
+        for (int i = 0; i < 7; i++) f[i] = s[0].f[i] + sL * (us[0][i] - s[0].u[i]);
+        return;
+    }
+    if (ssR <= R(0)) {
+        for (int i = 0; i < 7; i++) f[i] = s[1].f[i] + sR * (us[1][i] - s[1].u[i]);
+        return;
+    }
+    const R sgn = sign1<R>(bn);
+    const R sqs = sq[0] + sq[1];
+    const R vss1 = (sq[0] * vts[0][0] + sq[1] * vts[1][0] + (bts[1][0] - bts[0][0]) * sgn) / sqs;
+    const R vss2 = (sq[0] * vts[0][1] + sq[1] * vts[1][1] + (bts[1][1] - bts[0][1]) * sgn) / sqs;
+    const R bss1 = (sq[0] * bts[1][0] + sq[1] * bts[0][0] + sq[0] * sq[1] * (vts[1][0] - vts[0][0]) * sgn) / sqs;
+    const R bss2 = (sq[0] * bts[1][1] + sq[1] * bts[0][1] + sq[0] * sq[1] * (vts[1][1] - vts[0][1]) * sgn) / sqs;
+    const R vdbss = sm * bn + vss1 * bss1 + vss2 * bss2;
+    const int a = sm >= R(0) ? 0 : 1;
+    const Side7<R> &q = s[a];
+    const R vdbs = sm * bn + vts[a][0] * bts[a][0] + vts[a][1] * bts[a][1];
+    const R ess = a == 0 ? us[0][4] - sq[0] * (vdbs - vdbss) * sgn : us[1][4] + sq[1] * (vdbs - vdbss) * sgn;
+    const R uss[7] = {us[a][0], us[a][1], us[a][0] * vss1, us[a][0] * vss2, ess, bss1, bss2};
+    const R ss = a == 0 ? ssL : ssR;
+    for (int i = 0; i < 7; i++) {
+        const R fs = q.f[i] + sa[a] * (us[a][i] - q.u[i]);
+        f[i] = fs + ss * (uss[i] - us[a][i]);
+    }
+}
+
+// the face solver of the configuration: CUMC_LF (1), CUMC_HLL (2, the reference's), CUMC_HLLD (4)
+template <int DIR, typename R, int SOLVER>
+__host__ __device__ inline void riemann_face(R g, const FaceState<R> &L, const FaceState<R> &Rr, R f[7])
+{
+    if (SOLVER == 1) lf_face<DIR, R>(g, L, Rr, f);
+    else if (SOLVER == 4) hlld_face<DIR, R>(g, L, Rr, f);
+    else hll_face<DIR, R>(g, L, Rr, f);
 }
 
 }  // namespace cumc

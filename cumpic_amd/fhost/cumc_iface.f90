@@ -21,6 +21,7 @@ TYPE, BIND(C) :: cumc_config
   REAL(C_DOUBLE) :: ggas, cfl
   REAL(C_DOUBLE) :: rho_floor, eps_floor
   REAL(C_DOUBLE) :: r_sh
+  INTEGER(C_INT) :: solver
   INTEGER(C_INT) :: precision
 END TYPE
 

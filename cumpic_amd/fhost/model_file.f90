@@ -6,7 +6,7 @@
 ! the GPU and are selected by cfg%model.
 !
 ! Run file (little-endian stream): 16 int32  nx ny nz coordinate bflag(6) recon model
-!   geom_source_repeat precision max_steps pad ; 7 float64 ggas cfl rho_floor eps_floor r_sh
+!   geom_source_repeat precision max_steps solver (0: HLL) ; 7 float64 ggas cfl rho_floor eps_floor r_sh
 !   total_time output_profiletime ; xf(nx+7) yf(ny+7) zf(nz+7) ; prim(8,nx+6,ny+6,nz+6) ;
 !   eps(nx+6,ny+6,nz+6)
 !-----------------------------------------------------------------------------------------------
@@ -40,6 +40,7 @@ SUBROUTINE GET_MODEL(path, cfg, xf, yf, zf, prim, eps, max_steps, total_time, ou
   cfg%geom_source_repeat = hdr(13)
   cfg%precision = hdr(14)
   max_steps = hdr(15)
+  cfg%solver = hdr(16)
   cfg%ggas = rh(1); cfg%cfl = rh(2)
   cfg%rho_floor = rh(3); cfg%eps_floor = rh(4); cfg%r_sh = rh(5)
   total_time = rh(6)

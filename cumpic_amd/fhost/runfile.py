@@ -9,7 +9,7 @@ def write_runfile(path, prob, max_steps: int, total_time: float, output_time: fl
     definition.f90:132, 100.0 there); profiles are written only when the host gets a directory."""
     c = prob.cfg
     hdr = np.array([c.nx, c.ny, c.nz, c.coordinate, *c.boundary_flag, c.recon, c.model, c.geom_source_repeat,
-                    c.precision, max_steps, 0], dtype="<i4")
+                    c.precision, max_steps, c.solver], dtype="<i4")
     rh = np.array([c.ggas, c.cfl, c.rho_floor, c.eps_floor, c.r_sh, total_time, output_time], dtype="<f8")
     with open(path, "wb") as f:
         for a in (hdr, rh, prob.xf, prob.yf, prob.zf, prob.prim, prob.eps):

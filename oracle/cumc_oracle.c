@@ -181,6 +181,35 @@ void orc_tvdmm(const double d[3], const double v[3], double *vm_out, double *vp_
     *vm_out = vc - 0.5 * slope;
 }
 
+/* TVD with the MC (lim 2) or van Leer (lim 3) limiter: the reconstruction options that
+ * definition.f90:160-164 declares (TVDMC, TVDVL) and reconstruct.f90 leaves unimplemented.  An
+ * extension, not a restatement: the limiter acts on the same width-normalised one-sided
+ * differences dqf, dqb as TVD_MM (reconstruct.f90:115-124), with the grid factors cf, cb in
+ * place of MC's 2 (so that the face values stay between the neighbouring averages on any grid):
+ *   MC: minmod(cf dq+, cb dq-, (dq+ + dq-)/2)      (van Leer 1977, monotonised central)
+ *   VL: 2 dq+ dq- / (dq+ + dq-) when dq+ dq- > 0, else 0   (van Leer 1974, harmonic)
+ * On a uniform grid cf = cb = 2 and these are the textbook limiters; lim 1 is orc_tvdmm. */
+void orc_tvd(int lim, const double d[3], const double v[3], double *vm_out, double *vp_out)
+{
+    const double dh_m1 = d[0], dh_c = d[1], dh_p1 = d[2];
+    const double vm1 = v[0], vc = v[1], vp1 = v[2];
+    double cf = (dh_c + dh_p1) / dh_c;
+    double cb = (dh_c + dh_m1) / dh_c;
+    double dqf = 2.0 * (vp1 - vc) / cf;
+    double dqb = 2.0 * (vc - vm1) / cb;
+    double slope;
+    if (lim == 2) {
+        double m = fmin2(fmin2(cf * fabs(dqf), cb * fabs(dqb)), 0.5 * fabs(dqf + dqb));
+        slope = 0.5 * (fsign1(dqf) + fsign1(dqb)) * m;
+    } else if (lim == 3) {
+        slope = (dqf * dqb > 0.0) ? 2.0 * dqf * dqb / (dqf + dqb) : 0.0;
+    } else {
+        slope = 0.5 * (fsign1(dqf) + fsign1(dqb)) * fmin2(fabs(dqf), fabs(dqb));
+    }
+    *vp_out = vc + 0.5 * slope;
+    *vm_out = vc - 0.5 * slope;
+}
+
 /* coords.f90:77-97 */
 static inline double cdx(const orc *o, int j) { return o->xf[j] - o->xf[j - 1]; }
 static inline double cdy(const orc *o, int k) { return o->yf[k] - o->yf[k - 1]; }
@@ -207,6 +236,9 @@ static void interpolate(const orc *o, int dir, int c, const double v[5], double 
     if (o->c.recon == 1) {
         double d[3] = {w(o, c - 1), w(o, c), w(o, c + 1)};
         orc_tvdmm(d, v + 1, vm, vp);
+    } else if (o->c.recon == 2 || o->c.recon == 3) {
+        double d[3] = {w(o, c - 1), w(o, c), w(o, c + 1)};
+        orc_tvd(o->c.recon, d, v + 1, vm, vp);
     } else {
         double d[5] = {w(o, c - 2), w(o, c - 1), w(o, c), w(o, c + 1), w(o, c + 2)};
         orc_ppm(d, v, vm, vp);
@@ -319,6 +351,178 @@ int orc_hll(int dir, double g, const double *pl, const double *pr, double *flux)
     /* NaN speeds: the reference leaves flux() untouched; this restatement writes NaN */
     for (int i = 0; i < NV; i++) flux[i] = NAN;
     return 3;
+}
+
+/* Rotated face state of one side (n = normal, t1/t2 = transverse as in flux.f90:182-206) with its
+ * conserved vector U and physical flux F in the order rho, m_n, m_t1, m_t2, E, B_t1, B_t2. */
+typedef struct {
+    double rho, vn, vt1, vt2, p, bt1, bt2, pt, cf;
+    double u[7], f[7];
+} side7;
+
+static void side_of(int dir, double g, const double *pw, side7 *s)
+{
+    int ivn, ivt1, ivt2, ibn, ibt1, ibt2;
+    dir_indices(dir, &ivn, &ivt1, &ivt2, &ibn, &ibt1, &ibt2);
+    double eps = eos_epsilon(g, pw[ITAU], pw[IRHO]);
+    double cs = eos_soundspeed(g, pw[ITAU], pw[IRHO]);
+    double u8[NV], f8[NV];
+    orc_ptou_face(pw, eps, u8);
+    orc_ptof(dir, pw, u8, f8);
+    double bsquare = dotp(pw[IBX], pw[IBY], pw[IBZ], pw[IBX], pw[IBY], pw[IBZ]);
+    s->rho = pw[IRHO];
+    s->vn = pw[ivn];
+    s->vt1 = pw[ivt1];
+    s->vt2 = pw[ivt2];
+    s->p = pw[ITAU];
+    s->bt1 = pw[ibt1];
+    s->bt2 = pw[ibt2];
+    s->pt = pw[ITAU] + 0.5 * bsquare;
+    s->cf = orc_signalspeed(cs, pw[ibn], pw[ibt1], pw[ibt2], pw[IRHO]);
+    const int iu[7] = {IRHO, ivn, ivt1, ivt2, ITAU, ibt1, ibt2};
+    for (int i = 0; i < 7; i++) {
+        s->u[i] = u8[iu[i]];
+        s->f[i] = f8[iu[i]];
+    }
+}
+
+static void unrotate7(int dir, const double f7[7], double *flux)
+{
+    int ivn, ivt1, ivt2, ibn, ibt1, ibt2;
+    dir_indices(dir, &ivn, &ivt1, &ivt2, &ibn, &ibt1, &ibt2);
+    flux[IRHO] = f7[0];
+    flux[ivn] = f7[1];
+    flux[ivt1] = f7[2];
+    flux[ivt2] = f7[3];
+    flux[ITAU] = f7[4];
+    flux[ibn] = 0.0;
+    flux[ibt1] = f7[5];
+    flux[ibt2] = f7[6];
+}
+
+/* Local Lax-Friedrichs (Rusanov) flux, the LF option definition.f90:139-140 declares and
+ * rsolver.f90 leaves unimplemented (an extension):
+ *   F = (F_L + F_R)/2 - s (U_R - U_L)/2,  s = max(|v_nL| + c_fL, |v_nR| + c_fR)
+ * with the fast speeds of compute_signalspeed (definition.f90:394-410). */
+int orc_lf(int dir, double g, const double *pl, const double *pr, double *flux)
+{
+    side7 L, R;
+    side_of(dir, g, pl, &L);
+    side_of(dir, g, pr, &R);
+    double smax = fmax2(fabs(L.vn) + L.cf, fabs(R.vn) + R.cf);
+    double f[7];
+    for (int i = 0; i < 7; i++) f[i] = 0.5 * (L.f[i] + R.f[i]) - 0.5 * smax * (R.u[i] - L.u[i]);
+    unrotate7(dir, f, flux);
+    return 1;
+}
+
+/* HLLD flux of Miyoshi & Kusano (2005, J. Comput. Phys. 208, 315), the HLLD option
+ * definition.f90:148-149 declares and rsolver.f90 leaves unimplemented (an extension), written
+ * from the paper: outer speeds S_L,R = min/max(v_nL, v_nR) -/+ max(c_fL, c_fR) (eq. 67), contact
+ * speed S_M and total pressure p_T* (eqs. 38, 41), single-star states (eqs. 43-48; the
+ * degenerate case rho (S - v_n)(S - S_M) - B_n^2 -> 0 keeps the transverse v and B, as the paper
+ * prescribes), Alfven speeds S*_L,R = S_M -/+ |B_n| / sqrt(rho*) (eq. 51) and double-star states
+ * (eqs. 59-63).  Returns the region: 0 F_L, 1 F*_L, 2 F**_L, 3 F**_R, 4 F*_R, 5 F_R, 6 NaN. */
+int orc_hlld(int dir, double g, const double *pl, const double *pr, double *flux)
+{
+    int ivn, ivt1, ivt2, ibn, ibt1, ibt2;
+    dir_indices(dir, &ivn, &ivt1, &ivt2, &ibn, &ibt1, &ibt2);
+    side7 L, R;
+    side_of(dir, g, pl, &L);
+    side_of(dir, g, pr, &R);
+    const double bn = pl[ibn];
+    double f[7];
+    const double cmax = fmax2(L.cf, R.cf);
+    const double sL = fmin2(L.vn, R.vn) - cmax;
+    const double sR = fmax2(L.vn, R.vn) + cmax;
+    int region;
+    if (sL >= 0.0) {
+        for (int i = 0; i < 7; i++) f[i] = L.f[i];
+        region = 0;
+    } else if (sR <= 0.0) {
+        for (int i = 0; i < 7; i++) f[i] = R.f[i];
+        region = 5;
+    } else if (sL < 0.0 && sR > 0.0) {
+        const double sdl = sL - L.vn, sdr = sR - R.vn;
+        const double den = sdr * R.rho - sdl * L.rho;
+        const double sm = (sdr * R.rho * R.vn - sdl * L.rho * L.vn - R.pt + L.pt) / den;
+        const double pts = (sdr * R.rho * L.pt - sdl * L.rho * R.pt + L.rho * R.rho * sdr * sdl * (R.vn - L.vn)) / den;
+        /* single-star states of both sides: us[a][0..6], with v_t*, B_t* */
+        double us[2][7], vts[2][2], bts[2][2], sq[2];
+        const side7 *S[2] = {&L, &R};
+        const double sa[2] = {sL, sR}, sda[2] = {sdl, sdr};
+        for (int a = 0; a < 2; a++) {
+            const side7 *q = S[a];
+            const double sdm = sa[a] - sm;
+            const double rhos = q->rho * sda[a] / sdm;
+            const double tmp = q->rho * sda[a] * sdm - bn * bn;
+            if (fabs(tmp) < 1.0e-8 * fabs(pts)) {
+                vts[a][0] = q->vt1;
+                vts[a][1] = q->vt2;
+                bts[a][0] = q->bt1;
+                bts[a][1] = q->bt2;
+            } else {
+                const double fac = (q->rho * sda[a] * sda[a] - bn * bn) / tmp;
+                vts[a][0] = q->vt1 - bn * q->bt1 * (sm - q->vn) / tmp;
+                vts[a][1] = q->vt2 - bn * q->bt2 * (sm - q->vn) / tmp;
+                bts[a][0] = q->bt1 * fac;
+                bts[a][1] = q->bt2 * fac;
+            }
+            const double vdb = q->vn * bn + q->vt1 * q->bt1 + q->vt2 * q->bt2;
+            const double vdbs = sm * bn + vts[a][0] * bts[a][0] + vts[a][1] * bts[a][1];
+            const double es = (sda[a] * q->u[4] - q->pt * q->vn + pts * sm + bn * (vdb - vdbs)) / sdm;
+            us[a][0] = rhos;
+            us[a][1] = rhos * sm;
+            us[a][2] = rhos * vts[a][0];
+            us[a][3] = rhos * vts[a][1];
+            us[a][4] = es;
+            us[a][5] = bts[a][0];
+            us[a][6] = bts[a][1];
+            sq[a] = sqrt(rhos);
+        }
+        const double ssL = sm - fabs(bn) / sq[0];
+        const double ssR = sm + fabs(bn) / sq[1];
+        if (ssL >= 0.0) {
+            for (int i = 0; i < 7; i++) f[i] = L.f[i] + sL * (us[0][i] - L.u[i]);
+            region = 1;
+        } else if (ssR <= 0.0) {
+            for (int i = 0; i < 7; i++) f[i] = R.f[i] + sR * (us[1][i] - R.u[i]);
+            region = 4;
+        } else {
+            /* double-star states (eqs. 59-63) */
+            const double sgn = fsign1(bn);
+            const double sqs = sq[0] + sq[1];
+            const double vss1 = (sq[0] * vts[0][0] + sq[1] * vts[1][0] + (bts[1][0] - bts[0][0]) * sgn) / sqs;
+            const double vss2 = (sq[0] * vts[0][1] + sq[1] * vts[1][1] + (bts[1][1] - bts[0][1]) * sgn) / sqs;
+            const double bss1 = (sq[0] * bts[1][0] + sq[1] * bts[0][0] + sq[0] * sq[1] * (vts[1][0] - vts[0][0]) * sgn) / sqs;
+            const double bss2 = (sq[0] * bts[1][1] + sq[1] * bts[0][1] + sq[0] * sq[1] * (vts[1][1] - vts[0][1]) * sgn) / sqs;
+            const double vdbss = sm * bn + vss1 * bss1 + vss2 * bss2;
+            const int a = sm >= 0.0 ? 0 : 1;
+            const side7 *q = S[a];
+            const double vdbs = sm * bn + vts[a][0] * bts[a][0] + vts[a][1] * bts[a][1];
+            const double ess = a == 0 ? us[0][4] - sq[0] * (vdbs - vdbss) * sgn : us[1][4] + sq[1] * (vdbs - vdbss) * sgn;
+            const double uss[7] = {us[a][0], us[a][1], us[a][0] * vss1, us[a][0] * vss2, ess, bss1, bss2};
+            const double ss = a == 0 ? ssL : ssR;
+            for (int i = 0; i < 7; i++) {
+                const double fs = q->f[i] + sa[a] * (us[a][i] - q->u[i]);
+                f[i] = fs + ss * (uss[i] - us[a][i]);
+            }
+            region = a == 0 ? 2 : 3;
+        }
+    } else {
+        for (int i = 0; i < 7; i++) f[i] = NAN;
+        region = 6;
+    }
+    unrotate7(dir, f, flux);
+    return region;
+}
+
+/* the face solver of the configuration: 1 LF, 2 (or 0) HLL, 4 HLLD */
+static int riemann(int solver, int dir, double g, const double *pl, const double *pr, double *flux)
+{
+    if (solver == 1) return orc_lf(dir, g, pl, pr, flux);
+    if (solver == 4) return orc_hlld(dir, g, pl, pr, flux);
+    return orc_hll(dir, g, pl, pr, flux);
 }
 
 /* ---------------------------------------------------------------------------------------------- */
@@ -785,7 +989,7 @@ static void sweep(orc *o, int dir)
         for (int k = 0; k <= ny + 1 - dk; k++)
             for (int j = 0; j <= nx + 1 - dj; j++) {
                 long c = ID(j, k, l);
-                orc_hll(dir, g, &o->primL[c * NV], &o->primR[c * NV], &o->flux[c * NV]);
+                riemann(o->c.solver, dir, g, &o->primL[c * NV], &o->primR[c * NV], &o->flux[c * NV]);
             }
     /* (iii) flux difference on [1,n]^3 */
 #pragma omp parallel for collapse(2) schedule(static)

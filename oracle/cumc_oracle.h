@@ -25,12 +25,13 @@ typedef struct orc_config {
     int nx, ny, nz;            /* interior cells per axis (definition.f90:100-112) */
     int coordinate;            /* 1 cartesian, 2 cylindrical, 3 spherical (definition.f90:65-72) */
     int boundary_flag[6];      /* 0 periodic, 1 outflow, >=2 parity (definition.f90:87) */
-    int recon;                 /* 1 TVDMM, 4 PPMC (definition.f90:157-173) */
+    int recon;                 /* 1 TVDMM, 4 PPMC (definition.f90:157-173); 2 TVDMC, 3 TVDVL (extensions) */
     int model;                 /* 0 riemann plugin, 1 torus_cyn plugin */
     int geom_source_repeat;    /* 8 reproduces spatial.f90:86-97 (source added per i) */
     double ggas;               /* adiabatic index (model defs.f90:14) */
     double cfl;                /* definition.f90:116 */
     double rho_floor, eps_floor, r_sh; /* torus_cyn/src/defs.f90, model.f90:246-247 */
+    int solver;                /* 2 (or 0) HLL (definition.f90:143); 1 LF, 4 HLLD (extensions) */
 } orc_config;
 
 typedef struct orc orc;
@@ -55,9 +56,14 @@ int orc_run(orc *o, int max_steps, double total_time, double *dts, double *t_out
 /* cell-level entry points used by the closed-form tests */
 void orc_ppm(const double d[5], const double v[5], double *vm_out, double *vp_out);
 void orc_tvdmm(const double d[3], const double v[3], double *vm_out, double *vp_out);
+/* lim 1 minmod (= orc_tvdmm), 2 MC, 3 van Leer */
+void orc_tvd(int lim, const double d[3], const double v[3], double *vm_out, double *vp_out);
 double orc_signalspeed(double cs, double bn, double bt1, double bt2, double rho);
 /* dir 1,2,3; pl/pr: 8 face primitives; flux: 8 outputs; returns branch 0 L, 1 HLL, 2 R */
 int orc_hll(int dir, double ggas, const double *pl, const double *pr, double *flux);
+/* local Lax-Friedrichs; HLLD (returns the wave region 0..5, 6 for NaN speeds) */
+int orc_lf(int dir, double ggas, const double *pl, const double *pr, double *flux);
+int orc_hlld(int dir, double ggas, const double *pl, const double *pr, double *flux);
 void orc_ptof(int dir, const double *p, const double *u, double *f);
 void orc_ptou_face(const double *p, double eps, double *u);
 void orc_ptou_cell(const double *p, double eps, const double *bc, double *u);

@@ -25,6 +25,7 @@ class OrcConfig(ctypes.Structure):
         ("recon", ctypes.c_int), ("model", ctypes.c_int), ("geom_source_repeat", ctypes.c_int),
         ("ggas", ctypes.c_double), ("cfl", ctypes.c_double),
         ("rho_floor", ctypes.c_double), ("eps_floor", ctypes.c_double), ("r_sh", ctypes.c_double),
+        ("solver", ctypes.c_int),
     ]
 
 
@@ -60,8 +61,10 @@ def lib():
         L.orc_tvdmm.argtypes = [_D, _D, _D, _D]
         L.orc_signalspeed.restype = ctypes.c_double
         L.orc_signalspeed.argtypes = [ctypes.c_double] * 5
-        L.orc_hll.restype = ctypes.c_int
-        L.orc_hll.argtypes = [ctypes.c_int, ctypes.c_double, _D, _D, _D]
+        L.orc_tvd.argtypes = [ctypes.c_int, _D, _D, _D, _D]
+        for name in ("orc_hll", "orc_lf", "orc_hlld"):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_double, _D, _D, _D]
         L.orc_ptof.argtypes = [ctypes.c_int, _D, _D, _D]
         L.orc_ptou_face.argtypes = [_D, ctypes.c_double, _D]
         L.orc_ptou_cell.argtypes = [_D, ctypes.c_double, _D, _D]
@@ -84,6 +87,7 @@ def _cfg(cfg) -> OrcConfig:
     c.recon, c.model, c.geom_source_repeat = cfg.recon, cfg.model, cfg.geom_source_repeat
     c.ggas, c.cfl = cfg.ggas, cfg.cfl
     c.rho_floor, c.eps_floor, c.r_sh = cfg.rho_floor, cfg.eps_floor, cfg.r_sh
+    c.solver = getattr(cfg, "solver", 2)
     return c
 
 
@@ -187,6 +191,15 @@ def tvdmm(d, v):
     return vm.value, vp.value
 
 
+def tvd(lim, d, v):
+    """TVD edge values with limiter lim (1 minmod, 2 MC, 3 van Leer)."""
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    vm, vp = ctypes.c_double(), ctypes.c_double()
+    lib().orc_tvd(int(lim), _p(d), _p(v), ctypes.byref(vm), ctypes.byref(vp))
+    return vm.value, vp.value
+
+
 def signalspeed(cs, bn, bt1, bt2, rho):
     return lib().orc_signalspeed(cs, bn, bt1, bt2, rho)
 
@@ -196,6 +209,24 @@ def hll(direction, gamma, pl, pr):
     pr = np.ascontiguousarray(pr, dtype=np.float64)
     f = np.zeros(8)
     br = lib().orc_hll(int(direction), float(gamma), _p(pl), _p(pr), _p(f))
+    return f, br
+
+
+def lf(direction, gamma, pl, pr):
+    pl = np.ascontiguousarray(pl, dtype=np.float64)
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    f = np.zeros(8)
+    br = lib().orc_lf(int(direction), float(gamma), _p(pl), _p(pr), _p(f))
+    return f, br
+
+
+def hlld(direction, gamma, pl, pr):
+    """HLLD flux (8 unrotated components) and the wave region: 0 F_L, 1 F*_L, 2 F**_L,
+    3 F**_R, 4 F*_R, 5 F_R."""
+    pl = np.ascontiguousarray(pl, dtype=np.float64)
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    f = np.zeros(8)
+    br = lib().orc_hlld(int(direction), float(gamma), _p(pl), _p(pr), _p(f))
     return f, br
 
 

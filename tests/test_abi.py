@@ -36,10 +36,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_config_struct_layout_matches_header():
     from cumpic_amd.cumc import CConfigP
-    # 3+1+6+3 ints (52 B) + pad + 5 doubles + 1 int, padded to 8
+    # 3+1+6+3 ints (52 B) + pad + 5 doubles + solver + precision
     assert ctypes.sizeof(CConfigP) == 104
-    assert CConfigP.ggas.offset == 56 and CConfigP.precision.offset == 96
-    assert ctypes.sizeof(CConfig) == 96
+    assert CConfigP.ggas.offset == 56 and CConfigP.solver.offset == 96 and CConfigP.precision.offset == 100
+    assert ctypes.sizeof(CConfig) == 104  # padded to 8 after solver
 
 
 def test_error_strings_without_gpu():
