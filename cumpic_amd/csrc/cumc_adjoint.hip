@@ -1428,6 +1428,8 @@ struct batch_tape {
     int total[Q_N] = {0};
     std::vector<double> dts;  // nsteps x nbox
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<hipEvent_t> seg_ev;  // start / end of each segment's forward pass in the backward
+    double recompute_ms = 0.0;       // their total in the last backward pass
 };
 
 void batch_tape_release(cumc_batch *b)
@@ -1438,6 +1440,7 @@ void batch_tape_release(cumc_batch *b)
     hipFree(bt->pre);
     if (bt->e0) hipEventDestroy(bt->e0);
     if (bt->e1) hipEventDestroy(bt->e1);
+    for (hipEvent_t e : bt->seg_ev) hipEventDestroy(e);
     delete bt;
     bt = nullptr;
 }
@@ -1675,10 +1678,17 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
     if ((r = batch_reserve_hist(b, N > 0 ? N : 1))) return r;
     if (N > 0)
         BTCHK(b, hipMemcpyAsync(batch_hist(b), bt->dts.data(), sizeof(double) * bt->dts.size(), hipMemcpyHostToDevice, s));
+    const int nseg = N > 0 ? (N - 1) / k + 1 : 0;
+    while ((int)bt->seg_ev.size() < 2 * nseg) {
+        hipEvent_t e;
+        BTCHK(b, hipEventCreate(&e));
+        bt->seg_ev.push_back(e);
+    }
     BTCHK(b, hipEventRecord(bt->e0, s));
-    int q = 0;
-    for (int s0 = ((N - 1) / k) * k; N > 0 && s0 >= 0; s0 -= k) {
+    int q = 0, seg = 0;
+    for (int s0 = ((N - 1) / k) * k; N > 0 && s0 >= 0; s0 -= k, seg++) {
         const int s1 = s0 + k < N ? s0 + k : N;
+        BTCHK(b, hipEventRecord(bt->seg_ev[2 * seg], s));
         bt_copy(b, bt, CP_LOAD_CK, s0 / k, 0);
         bt_copy(b, bt, CP_SAVE_SEG, 0, 0);
         // segment forward: every stage's output state is stored, so the reverse below runs
@@ -1693,6 +1703,7 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
             batch_stage(b, 3, par);
             bt_copy(b, bt, CP_SAVE_SEG, i + 1, 1 - par);
         }
+        BTCHK(b, hipEventRecord(bt->seg_ev[2 * seg + 1], s));
         for (int m = s1 - 1; m >= s0; m--) {
             const int i = m - s0;
             batch_set_dt(b, m);
@@ -1714,6 +1725,11 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
     float ms = 0.f;
     hipEventElapsedTime(&ms, bt->e0, bt->e1);
     batch_last_ms(b) = ms;
+    bt->recompute_ms = 0.0;
+    for (int g = 0; g < nseg; g++) {
+        float m = 0.f;
+        if (hipEventElapsedTime(&m, bt->seg_ev[2 * g], bt->seg_ev[2 * g + 1]) == hipSuccess) bt->recompute_ms += m;
+    }
     for (int i = 0; i < nb; i++)
         if (grad_prim && grad_prim[i] && (r = soa_out(batch_box(b, i), batch_box(b, i)->tape->A.P, 8, grad_prim[i]))) {
             batch_err(b) = "box " + std::to_string(i) + ": " + batch_box(b, i)->err;
@@ -1723,6 +1739,15 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
     bt_copy(b, bt, CP_LOAD_CK, bt->nck - 1, 0);
     BTCHK(b, hipStreamSynchronize(s));
     for (int i = 0; i < nb; i++) batch_box(b, i)->dt_valid = false;
+    return CUMC_OK;
+}
+
+extern "C" int cumc_batch_tape_recompute_ms(cumc_batch *b, double *ms)
+{
+    if (!b || !ms) return CUMC_EINVAL;
+    batch_tape *bt = batch_adj(b);
+    if (!bt) return CUMC_ESTATE;
+    *ms = bt->recompute_ms;
     return CUMC_OK;
 }
 

@@ -97,6 +97,7 @@ def lib():
             "cumc_batch_tape_run": [_CTX, ctypes.c_int, ctypes.c_int, _D, _D],
             "cumc_batch_tape_backward": [_CTX, ctypes.POINTER(_D), ctypes.POINTER(_D)],
             "cumc_batch_tape_free": [_CTX],
+            "cumc_batch_tape_recompute_ms": [_CTX, _D],
         })
         for name, args in sig.items():
             f = getattr(L, name)
@@ -389,3 +390,9 @@ class Batch:
 
     def tape_free(self):
         self._check(lib().cumc_batch_tape_free(self.h), "cumc_batch_tape_free")
+
+    def tape_recompute_ms(self) -> float:
+        """Device ms of the last backward pass spent re-running checkpoint segments forward."""
+        d = ctypes.c_double()
+        self._check(lib().cumc_batch_tape_recompute_ms(self.h, ctypes.byref(d)), "cumc_batch_tape_recompute_ms")
+        return d.value

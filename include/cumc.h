@@ -202,6 +202,10 @@ int cumc_tape_free(cumc_ctx *ctx);
 int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const double *dt_in, double *dt_out);
 int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed_bcell, double *const *grad_prim);
 int cumc_batch_tape_free(cumc_batch *b);
+/* Diagnostic, no reference counterpart: device time of the last backward pass spent re-running
+ * each checkpoint segment forward (storing every stage's state) before reversing it; the rest
+ * of cumc_batch_last_ms is the reverse sweep proper. */
+int cumc_batch_tape_recompute_ms(cumc_batch *b, double *ms);
 
 #ifdef __cplusplus
 }
