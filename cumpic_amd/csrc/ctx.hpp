@@ -186,6 +186,7 @@ void launch_fluxes_s(cumc_ctx *ctx, const Fields<R> &F, int only_dir)
     case CUMC_TVDMM: launch_fluxes_t<R, 1, SOLVER>(ctx, F, only_dir); break;
     case CUMC_TVDMC: launch_fluxes_t<R, 2, SOLVER>(ctx, F, only_dir); break;
     case CUMC_TVDVL: launch_fluxes_t<R, 3, SOLVER>(ctx, F, only_dir); break;
+    case CUMC_WENO: launch_fluxes_t<R, 5, SOLVER>(ctx, F, only_dir); break;
     default: launch_fluxes_t<R, 4, SOLVER>(ctx, F, only_dir); break;
     }
 }

@@ -61,7 +61,8 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     if (c.nx < 1 || c.ny < 1 || c.nz < 1) return CUMC_EINVAL;
     if (c.coordinate != CUMC_CARTESIAN && c.coordinate != CUMC_CYLINDRICAL && c.coordinate != CUMC_SPHERICAL)
         return CUMC_EINVAL;
-    if (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM && c.recon != CUMC_TVDMC && c.recon != CUMC_TVDVL)
+    if (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM && c.recon != CUMC_TVDMC && c.recon != CUMC_TVDVL &&
+        c.recon != CUMC_WENO)
         return CUMC_EINVAL;
     if (c.solver != 0 && c.solver != CUMC_HLL && c.solver != CUMC_LF && c.solver != CUMC_HLLD) return CUMC_EINVAL;
     if (c.model != CUMC_MODEL_RIEMANN && c.model != CUMC_MODEL_TORUS_CYN && c.model != CUMC_MODEL_TORUS_SPH)

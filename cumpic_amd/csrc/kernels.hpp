@@ -106,7 +106,9 @@ __device__ inline void recon_lane(const Tabs &tb, int ax, int jc, R c, R &vm, R 
 {
     R cm1 = __shfl_up(c, 1, 64);
     R cp1 = __shfl_down(c, 1, 64);
-    if (RECON == 4) {
+    if (RECON == 5) {
+        weno5_edges<R>(__shfl_up(c, 2, 64), cm1, c, cp1, __shfl_down(c, 2, 64), vm, vp);
+    } else if (RECON == 4) {
         R dm = ppm_slope<R>(tb.atr<R>(ax, TB_B0C, jc), tb.atr<R>(ax, TB_B1C, jc), cm1, c, cp1);
         R dmm1 = __shfl_up(dm, 1, 64);
         R dmp1 = __shfl_down(dm, 1, 64);
@@ -217,12 +219,17 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
 
     // window: w0 = v(c-1), w1 = v(c), w2 = v(c+1), w3 = v(c+2) ; slopes d0 = dm(c-1), d1 = dm(c), d2 = dm(c+1)
     R w0[7], w1[7], w2[7], w3[7], d0[7], d1[7], vpp[7];
+    R wm[RECON == 5 ? 7 : 1];  // WENO5: the value two cells back
     {
         R a[7];
         load7(s0 - 2, a);
         load7(s0 - 1, w0);
         load7(s0, w1);
         load7(s0 + 1, w2);
+        if (RECON == 5) {
+#pragma unroll
+            for (int v = 0; v < 7; v++) wm[v] = a[v];
+        }
         if (RECON == 4) {
             R b0 = tb.atr<R>(ax, TB_B0C, s0 - 1), b1 = tb.atr<R>(ax, TB_B1C, s0 - 1);
             R c0 = tb.atr<R>(ax, TB_B0C, s0), c1 = tb.atr<R>(ax, TB_B1C, s0);
@@ -236,7 +243,10 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
     for (int c = s0; c <= s1 + 1; c++) {
         load7(c + 2, w3);
         R vm[7], vp[7];
-        if (RECON == 4) {
+        if (RECON == 5) {
+#pragma unroll
+            for (int v = 0; v < 7; v++) weno5_edges<R>(wm[v], w0[v], w1[v], w2[v], w3[v], vm[v], vp[v]);
+        } else if (RECON == 4) {
             R b0 = tb.atr<R>(ax, TB_B0C, c + 1), b1 = tb.atr<R>(ax, TB_B1C, c + 1);
             R a0m = tb.atr<R>(ax, TB_A0M, c), a1m = tb.atr<R>(ax, TB_A1M, c), a2m = tb.atr<R>(ax, TB_A2M, c);
             R a3m = tb.atr<R>(ax, TB_A3M, c), a0p = tb.atr<R>(ax, TB_A0P, c), a1p = tb.atr<R>(ax, TB_A1P, c);
@@ -289,6 +299,7 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
         }
 #pragma unroll
         for (int v = 0; v < 7; v++) {
+            if (RECON == 5) wm[v] = w0[v];
             vpp[v] = vp[v];
             w0[v] = w1[v];
             w1[v] = w2[v];

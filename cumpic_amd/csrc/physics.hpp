@@ -156,6 +156,36 @@ __host__ __device__ inline void tvd_lim_edges(R cf, R cb, R vm1, R vc, R vp1, R 
     vm_out = vc - R(0.5) * slope;
 }
 
+// WENO5 face value at the right face of the middle cell (Jiang & Shu 1996): the WENO option
+// that definition.f90:169-170 declares and the reference leaves unimplemented (an extension;
+// restated as orc_weno5 with the same operation order).  Uniform-grid coefficients on every
+// grid, linear weights (1, 6, 3)/10, eps = 1e-6.
+template <typename R>
+__host__ __device__ inline R weno5_face(R vm2, R vm1, R vc, R vp1, R vp2)
+{
+    const R p0 = (R(2.0) * vm2 - R(7.0) * vm1 + R(11.0) * vc) / R(6.0);
+    const R p1 = (-vm1 + R(5.0) * vc + R(2.0) * vp1) / R(6.0);
+    const R p2 = (R(2.0) * vc + R(5.0) * vp1 - vp2) / R(6.0);
+    const R c13 = R(13.0 / 12.0);
+    const R t0 = vm2 - R(2.0) * vm1 + vc, s0 = vm2 - R(4.0) * vm1 + R(3.0) * vc;
+    const R t1 = vm1 - R(2.0) * vc + vp1, s1 = vm1 - vp1;
+    const R t2 = vc - R(2.0) * vp1 + vp2, s2 = R(3.0) * vc - R(4.0) * vp1 + vp2;
+    const R b0 = c13 * t0 * t0 + R(0.25) * s0 * s0;
+    const R b1 = c13 * t1 * t1 + R(0.25) * s1 * s1;
+    const R b2 = c13 * t2 * t2 + R(0.25) * s2 * s2;
+    const R e0 = R(1.0e-6) + b0, e1 = R(1.0e-6) + b1, e2 = R(1.0e-6) + b2;
+    const R a0 = R(0.1) / (e0 * e0), a1 = R(0.6) / (e1 * e1), a2 = R(0.3) / (e2 * e2);
+    return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+}
+
+// WENO5 edge values of cell c from its 5-cell stencil (the left one on the mirrored stencil)
+template <typename R>
+__host__ __device__ inline void weno5_edges(R vm2, R vm1, R vc, R vp1, R vp2, R &vm_out, R &vp_out)
+{
+    vp_out = weno5_face<R>(vm2, vm1, vc, vp1, vp2);
+    vm_out = weno5_face<R>(vp2, vp1, vc, vm1, vm2);
+}
+
 // edge values of cell c by a TVD option: 1 minmod (the reference's TVD_MM), 2 MC, 3 van Leer
 template <typename R, int RECON>
 __host__ __device__ inline void tvd_any(R cf, R cb, R vm1, R vc, R vp1, R &vm_out, R &vp_out)

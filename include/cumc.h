@@ -31,8 +31,8 @@ enum {
 
 enum { CUMC_CARTESIAN = 1, CUMC_CYLINDRICAL = 2, CUMC_SPHERICAL = 3 };  /* solver/definition.f90:65-67 */
 enum { CUMC_TVDMM = 1, CUMC_PPMC = 4 };                  /* solver/definition.f90:158,167 */
-/* declared by solver/definition.f90:161,164 but not implemented by the reference; extensions */
-enum { CUMC_TVDMC = 2, CUMC_TVDVL = 3 };
+/* declared by solver/definition.f90:161,164,170 but not implemented by the reference; extensions */
+enum { CUMC_TVDMC = 2, CUMC_TVDVL = 3, CUMC_WENO = 5 };
 /* Riemann solvers: HLL is the reference's (rsolver.f90:43-130); LF and HLLD are declared by
  * definition.f90:140,149 without an implementation there, and are extensions here */
 enum { CUMC_LF = 1, CUMC_HLL = 2, CUMC_HLLD = 4 };
@@ -45,7 +45,7 @@ typedef struct cumc_config {
     int nx, ny, nz;               /* interior cells of this subdomain (definition.f90:110-112) */
     int coordinate;               /* CUMC_CARTESIAN | CUMC_CYLINDRICAL | CUMC_SPHERICAL (definition.f90:72) */
     int boundary_flag[6];         /* x-in, x-out, y-in, y-out, z-in, z-out (definition.f90:87) */
-    int recon;                    /* CUMC_PPMC | CUMC_TVDMM (definition.f90:173) | CUMC_TVDMC | CUMC_TVDVL */
+    int recon;                    /* CUMC_PPMC | CUMC_TVDMM (definition.f90:173) | CUMC_TVDMC | CUMC_TVDVL | CUMC_WENO */
     int model;                    /* which model plugin's in-loop hooks to run: CUMC_MODEL_* */
     int geom_source_repeat;       /* 8 reproduces spatial.f90:86-97; 1 adds the source once */
     double ggas;                  /* adiabatic index */

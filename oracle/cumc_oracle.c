@@ -210,6 +210,36 @@ void orc_tvd(int lim, const double d[3], const double v[3], double *vm_out, doub
     *vm_out = vc - 0.5 * slope;
 }
 
+/* WENO5 face value at the right face of cell c from v = (c-2 .. c+2): Jiang & Shu (1996), the
+ * "WENO (Shu 1997)" option definition.f90:169-170 declares and reconstruct.f90 leaves
+ * unimplemented (an extension).  Candidate quadratics p0..p2, smoothness indicators b0..b2,
+ * linear weights (1, 6, 3)/10 and eps = 1e-6, with the uniform-grid (index-space) coefficients
+ * on every grid. */
+static double weno5_face(double vm2, double vm1, double vc, double vp1, double vp2)
+{
+    const double p0 = (2.0 * vm2 - 7.0 * vm1 + 11.0 * vc) / 6.0;
+    const double p1 = (-vm1 + 5.0 * vc + 2.0 * vp1) / 6.0;
+    const double p2 = (2.0 * vc + 5.0 * vp1 - vp2) / 6.0;
+    const double c13 = 13.0 / 12.0;
+    const double t0 = vm2 - 2.0 * vm1 + vc, s0 = vm2 - 4.0 * vm1 + 3.0 * vc;
+    const double t1 = vm1 - 2.0 * vc + vp1, s1 = vm1 - vp1;
+    const double t2 = vc - 2.0 * vp1 + vp2, s2 = 3.0 * vc - 4.0 * vp1 + vp2;
+    const double b0 = c13 * t0 * t0 + 0.25 * s0 * s0;
+    const double b1 = c13 * t1 * t1 + 0.25 * s1 * s1;
+    const double b2 = c13 * t2 * t2 + 0.25 * s2 * s2;
+    const double e0 = 1.0e-6 + b0, e1 = 1.0e-6 + b1, e2 = 1.0e-6 + b2;
+    const double a0 = 0.1 / (e0 * e0), a1 = 0.6 / (e1 * e1), a2 = 0.3 / (e2 * e2);
+    return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+}
+
+/* WENO5 edge values of the middle cell of v[5]: vp at its right face, vm at its left face (the
+ * same formula on the mirrored stencil) */
+void orc_weno5(const double v[5], double *vm_out, double *vp_out)
+{
+    *vp_out = weno5_face(v[0], v[1], v[2], v[3], v[4]);
+    *vm_out = weno5_face(v[4], v[3], v[2], v[1], v[0]);
+}
+
 /* coords.f90:77-97 */
 static inline double cdx(const orc *o, int j) { return o->xf[j] - o->xf[j - 1]; }
 static inline double cdy(const orc *o, int k) { return o->yf[k] - o->yf[k - 1]; }
@@ -239,6 +269,8 @@ static void interpolate(const orc *o, int dir, int c, const double v[5], double 
     } else if (o->c.recon == 2 || o->c.recon == 3) {
         double d[3] = {w(o, c - 1), w(o, c), w(o, c + 1)};
         orc_tvd(o->c.recon, d, v + 1, vm, vp);
+    } else if (o->c.recon == 5) {
+        orc_weno5(v, vm, vp);
     } else {
         double d[5] = {w(o, c - 2), w(o, c - 1), w(o, c), w(o, c + 1), w(o, c + 2)};
         orc_ppm(d, v, vm, vp);

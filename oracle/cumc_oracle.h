@@ -25,7 +25,7 @@ typedef struct orc_config {
     int nx, ny, nz;            /* interior cells per axis (definition.f90:100-112) */
     int coordinate;            /* 1 cartesian, 2 cylindrical, 3 spherical (definition.f90:65-72) */
     int boundary_flag[6];      /* 0 periodic, 1 outflow, >=2 parity (definition.f90:87) */
-    int recon;                 /* 1 TVDMM, 4 PPMC (definition.f90:157-173); 2 TVDMC, 3 TVDVL (extensions) */
+    int recon;                 /* 1 TVDMM, 4 PPMC (definition.f90:157-173); 2 TVDMC, 3 TVDVL, 5 WENO (extensions) */
     int model;                 /* 0 riemann plugin, 1 torus_cyn plugin */
     int geom_source_repeat;    /* 8 reproduces spatial.f90:86-97 (source added per i) */
     double ggas;               /* adiabatic index (model defs.f90:14) */
@@ -56,6 +56,8 @@ int orc_run(orc *o, int max_steps, double total_time, double *dts, double *t_out
 /* cell-level entry points used by the closed-form tests */
 void orc_ppm(const double d[5], const double v[5], double *vm_out, double *vp_out);
 void orc_tvdmm(const double d[3], const double v[3], double *vm_out, double *vp_out);
+/* WENO5 (Jiang-Shu) edge values of the middle cell of v[5] (extension) */
+void orc_weno5(const double v[5], double *vm_out, double *vp_out);
 /* lim 1 minmod (= orc_tvdmm), 2 MC, 3 van Leer */
 void orc_tvd(int lim, const double d[3], const double v[3], double *vm_out, double *vp_out);
 double orc_signalspeed(double cs, double bn, double bt1, double bt2, double rho);

@@ -62,6 +62,7 @@ def lib():
         L.orc_signalspeed.restype = ctypes.c_double
         L.orc_signalspeed.argtypes = [ctypes.c_double] * 5
         L.orc_tvd.argtypes = [ctypes.c_int, _D, _D, _D, _D]
+        L.orc_weno5.argtypes = [_D, _D, _D]
         for name in ("orc_hll", "orc_lf", "orc_hlld"):
             getattr(L, name).restype = ctypes.c_int
             getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_double, _D, _D, _D]
@@ -197,6 +198,14 @@ def tvd(lim, d, v):
     v = np.ascontiguousarray(v, dtype=np.float64)
     vm, vp = ctypes.c_double(), ctypes.c_double()
     lib().orc_tvd(int(lim), _p(d), _p(v), ctypes.byref(vm), ctypes.byref(vp))
+    return vm.value, vp.value
+
+
+def weno5(v):
+    """WENO5 edge values (vm, vp) of the middle cell of the 5-cell stencil v."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    vm, vp = ctypes.c_double(), ctypes.c_double()
+    lib().orc_weno5(_p(v), ctypes.byref(vm), ctypes.byref(vp))
     return vm.value, vp.value
 
 

@@ -1,14 +1,13 @@
 """HIP path vs the CPU restatement for the extension options (TVD with the MC / van Leer
-limiters, the local Lax-Friedrichs and HLLD solvers): bit for bit on the same seeded inputs, as
-for the reference's own options, in the three sweep directions, whole steps with several
-boundary sets and geometries, and the mixed form against FP64 within its tolerance."""
+limiters, WENO5, the local Lax-Friedrichs and HLLD solvers): bit for bit on the same seeded
+inputs, as for the reference's own options, in the three sweep directions, whole steps with
+several boundary sets and geometries, and the mixed form against FP64 within its tolerance."""
 from __future__ import annotations
 
-import numpy as np
 import pytest
 
 from cumpic_amd import ics
-from cumpic_amd.config import FP64, HLL, HLLD, LF, MIXED, PPMC, TVDMC, TVDMM, TVDVL
+from cumpic_amd.config import FP64, HLL, HLLD, LF, MIXED, PPMC, TVDMC, TVDMM, TVDVL, WENO
 from oracle.oracle import Oracle
 from tests.helpers import assert_bitwise, interior, rel_l1
 
@@ -28,7 +27,7 @@ def _states_equal(g, o, what):
 
 
 @pytest.mark.parametrize("direction", [1, 2, 3])
-@pytest.mark.parametrize("recon,solver", [(PPMC, HLLD), (TVDVL, LF), (TVDMC, HLL)])
+@pytest.mark.parametrize("recon,solver", [(PPMC, HLLD), (TVDVL, LF), (TVDMC, HLL), (WENO, HLL)])
 def test_extension_sweep_flux_bitwise(direction, recon, solver):
     from cumpic_amd.cumc import Solver
     print(f"[case] sweep_flux dir={direction} recon={recon} solver={solver} n=12", flush=True)
@@ -55,6 +54,8 @@ def test_extension_sweep_flux_bitwise(direction, recon, solver):
     (TVDMC, HLLD, (2, 1, 3, 3, 0, 0)),
     (PPMC, LF, (0, 0, 0, 0, 0, 0)),
     (TVDVL, LF, (1, 0, 0, 1, 2, 0)),
+    (WENO, HLL, (0, 0, 0, 0, 0, 0)),
+    (WENO, HLLD, (1, 1, 2, 2, 1, 1)),
 ])
 def test_extension_steps_bitwise(recon, solver, flags):
     from cumpic_amd.cumc import Solver

@@ -1,7 +1,8 @@
 """Pins of the CPU restatement's extensions: the options solver/definition.f90 declares without
 an implementation in the reference -- TVD with the MC and van Leer limiters (TVDMC, TVDVL,
-definition.f90:160-164), the local Lax-Friedrichs flux (LF, :139-140) and the HLLD solver of
-Miyoshi & Kusano (2005) (HLLD, :148-149).  With no reference code to restate, each is written
+definition.f90:160-164), WENO5 of Jiang & Shu (1996) (WENO, :169-170), the local
+Lax-Friedrichs flux (LF, :139-140) and the HLLD solver of Miyoshi & Kusano (2005) (HLLD,
+:148-149).  With no reference code to restate, each is written
 from its published definition and pinned by facts the mathematics fixes: textbook limiter
 functions, exactness on linear data, consistency F(U, U) = F(U), exact resolution of isolated
 contact and rotational discontinuities (the property that defines HLLD), mirror symmetry, and
@@ -15,7 +16,7 @@ import numpy as np
 import pytest
 
 from cumpic_amd import ics
-from cumpic_amd.config import HLLD, LF, TVDMC, TVDVL
+from cumpic_amd.config import HLL, HLLD, LF, TVDMC, TVDVL, WENO
 from oracle import oracle as orc
 from oracle.oracle import Oracle
 
@@ -255,7 +256,7 @@ def test_hlld_rotation_covariance(d):
 # Whole steps with the extensions
 # --------------------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("recon,solver", [(TVDMC, HLLD), (TVDVL, LF), (4, HLLD), (1, LF)])
+@pytest.mark.parametrize("recon,solver", [(TVDMC, HLLD), (TVDVL, LF), (4, HLLD), (1, LF), (WENO, HLL), (WENO, HLLD)])
 def test_whole_steps_conserve_and_keep_divb(recon, solver):
     """Periodic box: mass, momentum and energy conserved to round-off and div B at round-off
     after a few steps with the extension options."""
@@ -276,3 +277,59 @@ def test_whole_steps_conserve_and_keep_divb(recon, solver):
     _, dabs = o.find_divb()
     assert dabs < 1e-12
     assert np.isfinite(o.get_state()["prim"]).all()
+
+
+# --------------------------------------------------------------------------------------------
+# WENO5 (orc_weno5)
+# --------------------------------------------------------------------------------------------
+
+def test_weno5_exact_for_quadratics_and_constants():
+    """Cell averages of any quadratic: every candidate stencil is exact, so both face values are
+    exact whatever the nonlinear weights.  Catches a wrong candidate-stencil coefficient."""
+    for _ in range(3000):
+        a, b, c = RNG.normal(size=3)
+        x = np.arange(-2, 3, dtype=float)  # cell centres, unit widths; faces at +-0.5
+        # average of a + b x + c x^2 over [x-1/2, x+1/2] = a + b x + c (x^2 + 1/12)
+        avg = a + b * x + c * (x * x + 1.0 / 12.0)
+        vm, vp = orc.weno5(avg)
+        assert abs(vp - (a + 0.5 * b + 0.25 * c)) < 1e-12 and abs(vm - (a - 0.5 * b + 0.25 * c)) < 1e-12
+    k = RNG.normal()
+    assert orc.weno5([k] * 5) == (k, k)
+
+
+def test_weno5_fifth_order_on_smooth_data():
+    """Face values of sin from its cell averages: the error falls ~32x per halving of the cell
+    width (fifth order), which the linear weights (1, 6, 3)/10 are what give; third-order
+    weights would show ~8x."""
+    errs = []
+    for n in (16, 32, 64):
+        h = 2 * math.pi / n
+        e = 0.0
+        for i in range(n):
+            xc = (i + 0.5) * h
+            avg = [(math.cos(xc + (j - 0.5) * h) - math.cos(xc + (j + 0.5) * h)) / h for j in range(-2, 3)]
+            vm, vp = orc.weno5(avg)
+            e = max(e, abs(vp - math.sin(xc + 0.5 * h)), abs(vm - math.sin(xc - 0.5 * h)))
+        errs.append(e)
+    assert errs[0] / errs[1] > 20 and errs[1] / errs[2] > 25, errs
+
+
+def test_weno5_essentially_non_oscillatory_at_a_step():
+    """Across a jump the stencils that straddle it get weights ~ (eps / beta)^2, so the face
+    values stay inside the data range up to that size, for every position of the jump.  (eps is
+    absolute, as Jiang & Shu define it, so this holds for jumps well above sqrt(eps) = 1e-3; for
+    smaller ones the weights approach the linear ones.)"""
+    for jump in range(1, 5):
+        for hi in (1.0, 10.0, 0.1):
+            v = np.where(np.arange(5) >= jump, hi, 0.0)
+            vm, vp = orc.weno5(v)
+            tol = 1e-6 * hi
+            assert -tol <= vm <= hi + tol and -tol <= vp <= hi + tol, (jump, hi, vm, vp)
+
+
+def test_weno5_mirror_symmetry():
+    for _ in range(2000):
+        v = RNG.normal(size=5)
+        vm, vp = orc.weno5(v)
+        vm2, vp2 = orc.weno5(v[::-1].copy())
+        assert vm == vp2 and vp == vm2
