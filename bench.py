@@ -320,6 +320,10 @@ def main():
                     help="plain device copy of this many MiB: the achievable HBM rate, and the counter "
                          "calibration (0 skips it)")
     ap.add_argument("--no-tune", action="store_true", help="time the default launch shapes (no autotune)")
+    ap.add_argument("--recon", choices=["default", "ppm", "tvdmm", "tvdmc", "tvdvl", "weno"], default="default",
+                    help="reconstruction (ppm and tvdmm are the reference's; the others are extensions)")
+    ap.add_argument("--solver", choices=["hll", "lf", "hlld"], default="hll",
+                    help="Riemann solver (hll is the reference's; lf and hlld are extensions)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -369,6 +373,12 @@ def main():
         prob = ics.spectral_box(n=n, seed=1001, steps=steps)
         workload = f"3D periodic spectral MHD box {n}^3, PPM+HLL+upwind CT+SSP-RK3"
     cfg = prob.cfg
+    from cumpic_amd import config as C
+    if args.recon != "default":
+        cfg.recon = {"ppm": C.PPMC, "tvdmm": C.TVDMM, "tvdmc": C.TVDMC, "tvdvl": C.TVDVL, "weno": C.WENO}[args.recon]
+    cfg.solver = {"hll": C.HLL, "lf": C.LF, "hlld": C.HLLD}[args.solver]
+    if args.recon != "default" or args.solver != "hll":
+        workload += f" (options: recon {args.recon}, solver {args.solver})"
 
     cpu, parity = None, None
     if not args.no_cpu and "fp64" in forms:
@@ -394,7 +404,8 @@ def main():
         dsig, dabs = s.find_divb()
         st = s.download()
         finite = bool(np.isfinite(st["prim"]).all())
-        profiled = args.workload == "box" and args.n == 256
+        profiled = (args.workload == "box" and args.n == 256 and args.recon in ("default", "ppm")
+                    and args.solver == "hll")
         r = {"value": cfg.zones * steps / wall, "ms_per_step": wall / steps * 1e3,
              "roofline": roofline(kt, cfg, form, profiled=profiled),
              "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()}, "divb_max_abs": dabs,

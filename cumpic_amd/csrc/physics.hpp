@@ -163,9 +163,10 @@ __host__ __device__ inline void tvd_lim_edges(R cf, R cb, R vm1, R vc, R vp1, R 
 template <typename R>
 __host__ __device__ inline R weno5_face(R vm2, R vm1, R vc, R vp1, R vp2)
 {
-    const R p0 = (R(2.0) * vm2 - R(7.0) * vm1 + R(11.0) * vc) / R(6.0);
-    const R p1 = (-vm1 + R(5.0) * vc + R(2.0) * vp1) / R(6.0);
-    const R p2 = (R(2.0) * vc + R(5.0) * vp1 - vp2) / R(6.0);
+    // the candidate values times 6; the 1/6 goes into the final normalisation
+    const R p0 = R(2.0) * vm2 - R(7.0) * vm1 + R(11.0) * vc;
+    const R p1 = -vm1 + R(5.0) * vc + R(2.0) * vp1;
+    const R p2 = R(2.0) * vc + R(5.0) * vp1 - vp2;
     const R c13 = R(13.0 / 12.0);
     const R t0 = vm2 - R(2.0) * vm1 + vc, s0 = vm2 - R(4.0) * vm1 + R(3.0) * vc;
     const R t1 = vm1 - R(2.0) * vc + vp1, s1 = vm1 - vp1;
@@ -175,7 +176,7 @@ __host__ __device__ inline R weno5_face(R vm2, R vm1, R vc, R vp1, R vp2)
     const R b2 = c13 * t2 * t2 + R(0.25) * s2 * s2;
     const R e0 = R(1.0e-6) + b0, e1 = R(1.0e-6) + b1, e2 = R(1.0e-6) + b2;
     const R a0 = R(0.1) / (e0 * e0), a1 = R(0.6) / (e1 * e1), a2 = R(0.3) / (e2 * e2);
-    return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+    return (a0 * p0 + a1 * p1 + a2 * p2) / (R(6.0) * (a0 + a1 + a2));
 }
 
 // WENO5 edge values of cell c from its 5-cell stencil (the left one on the mirrored stencil)
@@ -349,15 +350,18 @@ __host__ __device__ inline void hlld_face(R g, const FaceState<R> &L, const Face
         return;
     }
     const R sdl = sL - s[0].vn, sdr = sR - s[1].vn;
-    const R den = sdr * s[1].rho - sdl * s[0].rho;
-    const R sm = (sdr * s[1].rho * s[1].vn - sdl * s[0].rho * s[0].vn - s[1].pt + s[0].pt) / den;
-    const R pts = (sdr * s[1].rho * s[0].pt - sdl * s[0].rho * s[1].pt + s[0].rho * s[1].rho * sdr * sdl * (s[1].vn - s[0].vn)) / den;
+    const R iden = R(1.0) / (sdr * s[1].rho - sdl * s[0].rho);
+    const R sm = (sdr * s[1].rho * s[1].vn - sdl * s[0].rho * s[0].vn - s[1].pt + s[0].pt) * iden;
+    const R pts = (sdr * s[1].rho * s[0].pt - sdl * s[0].rho * s[1].pt + s[0].rho * s[1].rho * sdr * sdl * (s[1].vn - s[0].vn)) * iden;
+    // (every array below is indexed by compile-time constants only, so it stays in registers)
     R us[2][7], vts[2][2], bts[2][2], sq[2];
     const R sa[2] = {sL, sR}, sda[2] = {sdl, sdr};
+#pragma unroll
     for (int a = 0; a < 2; a++) {
         const Side7<R> &q = s[a];
         const R sdm = sa[a] - sm;
-        const R rhos = q.rho * sda[a] / sdm;
+        const R isdm = R(1.0) / sdm;
+        const R rhos = q.rho * sda[a] * isdm;
         const R tmp = q.rho * sda[a] * sdm - bn * bn;
         if (fabs_<R>(tmp) < R(1.0e-8) * fabs_<R>(pts)) {
             vts[a][0] = q.vt1;
@@ -366,14 +370,15 @@ __host__ __device__ inline void hlld_face(R g, const FaceState<R> &L, const Face
             bts[a][1] = q.bt2;
         } else {
             const R fac = (q.rho * sda[a] * sda[a] - bn * bn) / tmp;
-            vts[a][0] = q.vt1 - bn * q.bt1 * (sm - q.vn) / tmp;
-            vts[a][1] = q.vt2 - bn * q.bt2 * (sm - q.vn) / tmp;
+            const R fv = bn * (sm - q.vn) / tmp;
+            vts[a][0] = q.vt1 - q.bt1 * fv;
+            vts[a][1] = q.vt2 - q.bt2 * fv;
             bts[a][0] = q.bt1 * fac;
             bts[a][1] = q.bt2 * fac;
         }
         const R vdb = q.vn * bn + q.vt1 * q.bt1 + q.vt2 * q.bt2;
         const R vdbs = sm * bn + vts[a][0] * bts[a][0] + vts[a][1] * bts[a][1];
-        const R es = (sda[a] * q.u[4] - q.pt * q.vn + pts * sm + bn * (vdb - vdbs)) / sdm;
+        const R es = (sda[a] * q.u[4] - q.pt * q.vn + pts * sm + bn * (vdb - vdbs)) * isdm;
         us[a][0] = rhos;
         us[a][1] = rhos * sm;
         us[a][2] = rhos * vts[a][0];
@@ -394,21 +399,24 @@ __host__ __device__ inline void hlld_face(R g, const FaceState<R> &L, const Face
         return;
     }
     const R sgn = sign1<R>(bn);
-    const R sqs = sq[0] + sq[1];
-    const R vss1 = (sq[0] * vts[0][0] + sq[1] * vts[1][0] + (bts[1][0] - bts[0][0]) * sgn) / sqs;
-    const R vss2 = (sq[0] * vts[0][1] + sq[1] * vts[1][1] + (bts[1][1] - bts[0][1]) * sgn) / sqs;
-    const R bss1 = (sq[0] * bts[1][0] + sq[1] * bts[0][0] + sq[0] * sq[1] * (vts[1][0] - vts[0][0]) * sgn) / sqs;
-    const R bss2 = (sq[0] * bts[1][1] + sq[1] * bts[0][1] + sq[0] * sq[1] * (vts[1][1] - vts[0][1]) * sgn) / sqs;
+    const R isq = R(1.0) / (sq[0] + sq[1]);
+    const R vss1 = (sq[0] * vts[0][0] + sq[1] * vts[1][0] + (bts[1][0] - bts[0][0]) * sgn) * isq;
+    const R vss2 = (sq[0] * vts[0][1] + sq[1] * vts[1][1] + (bts[1][1] - bts[0][1]) * sgn) * isq;
+    const R bss1 = (sq[0] * bts[1][0] + sq[1] * bts[0][0] + sq[0] * sq[1] * (vts[1][0] - vts[0][0]) * sgn) * isq;
+    const R bss2 = (sq[0] * bts[1][1] + sq[1] * bts[0][1] + sq[0] * sq[1] * (vts[1][1] - vts[0][1]) * sgn) * isq;
     const R vdbss = sm * bn + vss1 * bss1 + vss2 * bss2;
-    const int a = sm >= R(0) ? 0 : 1;
-    const Side7<R> &q = s[a];
-    const R vdbs = sm * bn + vts[a][0] * bts[a][0] + vts[a][1] * bts[a][1];
-    const R ess = a == 0 ? us[0][4] - sq[0] * (vdbs - vdbss) * sgn : us[1][4] + sq[1] * (vdbs - vdbss) * sgn;
-    const R uss[7] = {us[a][0], us[a][1], us[a][0] * vss1, us[a][0] * vss2, ess, bss1, bss2};
-    const R ss = a == 0 ? ssL : ssR;
+    const bool left = sm >= R(0);
+    const R vdbsL = sm * bn + vts[0][0] * bts[0][0] + vts[0][1] * bts[0][1];
+    const R vdbsR = sm * bn + vts[1][0] * bts[1][0] + vts[1][1] * bts[1][1];
+    const R ess = left ? us[0][4] - sq[0] * (vdbsL - vdbss) * sgn : us[1][4] + sq[1] * (vdbsR - vdbss) * sgn;
+    const R rs = left ? us[0][0] : us[1][0];
+    const R uss[7] = {rs, left ? us[0][1] : us[1][1], rs * vss1, rs * vss2, ess, bss1, bss2};
+    const R sa_ = left ? sL : sR, ss = left ? ssL : ssR;
+#pragma unroll
     for (int i = 0; i < 7; i++) {
-        const R fs = q.f[i] + sa[a] * (us[a][i] - q.u[i]);
-        f[i] = fs + ss * (uss[i] - us[a][i]);
+        const R usi = left ? us[0][i] : us[1][i];
+        const R fs = (left ? s[0].f[i] : s[1].f[i]) + sa_ * (usi - (left ? s[0].u[i] : s[1].u[i]));
+        f[i] = fs + ss * (uss[i] - usi);
     }
 }
 

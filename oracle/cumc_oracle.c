@@ -217,9 +217,10 @@ void orc_tvd(int lim, const double d[3], const double v[3], double *vm_out, doub
  * on every grid. */
 static double weno5_face(double vm2, double vm1, double vc, double vp1, double vp2)
 {
-    const double p0 = (2.0 * vm2 - 7.0 * vm1 + 11.0 * vc) / 6.0;
-    const double p1 = (-vm1 + 5.0 * vc + 2.0 * vp1) / 6.0;
-    const double p2 = (2.0 * vc + 5.0 * vp1 - vp2) / 6.0;
+    /* the candidate values times 6; the 1/6 goes into the final normalisation */
+    const double p0 = 2.0 * vm2 - 7.0 * vm1 + 11.0 * vc;
+    const double p1 = -vm1 + 5.0 * vc + 2.0 * vp1;
+    const double p2 = 2.0 * vc + 5.0 * vp1 - vp2;
     const double c13 = 13.0 / 12.0;
     const double t0 = vm2 - 2.0 * vm1 + vc, s0 = vm2 - 4.0 * vm1 + 3.0 * vc;
     const double t1 = vm1 - 2.0 * vc + vp1, s1 = vm1 - vp1;
@@ -229,7 +230,7 @@ static double weno5_face(double vm2, double vm1, double vc, double vp1, double v
     const double b2 = c13 * t2 * t2 + 0.25 * s2 * s2;
     const double e0 = 1.0e-6 + b0, e1 = 1.0e-6 + b1, e2 = 1.0e-6 + b2;
     const double a0 = 0.1 / (e0 * e0), a1 = 0.6 / (e1 * e1), a2 = 0.3 / (e2 * e2);
-    return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+    return (a0 * p0 + a1 * p1 + a2 * p2) / (6.0 * (a0 + a1 + a2));
 }
 
 /* WENO5 edge values of the middle cell of v[5]: vp at its right face, vm at its left face (the
@@ -476,9 +477,9 @@ int orc_hlld(int dir, double g, const double *pl, const double *pr, double *flux
         region = 5;
     } else if (sL < 0.0 && sR > 0.0) {
         const double sdl = sL - L.vn, sdr = sR - R.vn;
-        const double den = sdr * R.rho - sdl * L.rho;
-        const double sm = (sdr * R.rho * R.vn - sdl * L.rho * L.vn - R.pt + L.pt) / den;
-        const double pts = (sdr * R.rho * L.pt - sdl * L.rho * R.pt + L.rho * R.rho * sdr * sdl * (R.vn - L.vn)) / den;
+        const double iden = 1.0 / (sdr * R.rho - sdl * L.rho);
+        const double sm = (sdr * R.rho * R.vn - sdl * L.rho * L.vn - R.pt + L.pt) * iden;
+        const double pts = (sdr * R.rho * L.pt - sdl * L.rho * R.pt + L.rho * R.rho * sdr * sdl * (R.vn - L.vn)) * iden;
         /* single-star states of both sides: us[a][0..6], with v_t*, B_t* */
         double us[2][7], vts[2][2], bts[2][2], sq[2];
         const side7 *S[2] = {&L, &R};
@@ -486,7 +487,8 @@ int orc_hlld(int dir, double g, const double *pl, const double *pr, double *flux
         for (int a = 0; a < 2; a++) {
             const side7 *q = S[a];
             const double sdm = sa[a] - sm;
-            const double rhos = q->rho * sda[a] / sdm;
+            const double isdm = 1.0 / sdm;
+            const double rhos = q->rho * sda[a] * isdm;
             const double tmp = q->rho * sda[a] * sdm - bn * bn;
             if (fabs(tmp) < 1.0e-8 * fabs(pts)) {
                 vts[a][0] = q->vt1;
@@ -495,14 +497,15 @@ int orc_hlld(int dir, double g, const double *pl, const double *pr, double *flux
                 bts[a][1] = q->bt2;
             } else {
                 const double fac = (q->rho * sda[a] * sda[a] - bn * bn) / tmp;
-                vts[a][0] = q->vt1 - bn * q->bt1 * (sm - q->vn) / tmp;
-                vts[a][1] = q->vt2 - bn * q->bt2 * (sm - q->vn) / tmp;
+                const double fv = bn * (sm - q->vn) / tmp;
+                vts[a][0] = q->vt1 - q->bt1 * fv;
+                vts[a][1] = q->vt2 - q->bt2 * fv;
                 bts[a][0] = q->bt1 * fac;
                 bts[a][1] = q->bt2 * fac;
             }
             const double vdb = q->vn * bn + q->vt1 * q->bt1 + q->vt2 * q->bt2;
             const double vdbs = sm * bn + vts[a][0] * bts[a][0] + vts[a][1] * bts[a][1];
-            const double es = (sda[a] * q->u[4] - q->pt * q->vn + pts * sm + bn * (vdb - vdbs)) / sdm;
+            const double es = (sda[a] * q->u[4] - q->pt * q->vn + pts * sm + bn * (vdb - vdbs)) * isdm;
             us[a][0] = rhos;
             us[a][1] = rhos * sm;
             us[a][2] = rhos * vts[a][0];
@@ -523,11 +526,11 @@ int orc_hlld(int dir, double g, const double *pl, const double *pr, double *flux
         } else {
             /* double-star states (eqs. 59-63) */
             const double sgn = fsign1(bn);
-            const double sqs = sq[0] + sq[1];
-            const double vss1 = (sq[0] * vts[0][0] + sq[1] * vts[1][0] + (bts[1][0] - bts[0][0]) * sgn) / sqs;
-            const double vss2 = (sq[0] * vts[0][1] + sq[1] * vts[1][1] + (bts[1][1] - bts[0][1]) * sgn) / sqs;
-            const double bss1 = (sq[0] * bts[1][0] + sq[1] * bts[0][0] + sq[0] * sq[1] * (vts[1][0] - vts[0][0]) * sgn) / sqs;
-            const double bss2 = (sq[0] * bts[1][1] + sq[1] * bts[0][1] + sq[0] * sq[1] * (vts[1][1] - vts[0][1]) * sgn) / sqs;
+            const double isq = 1.0 / (sq[0] + sq[1]);
+            const double vss1 = (sq[0] * vts[0][0] + sq[1] * vts[1][0] + (bts[1][0] - bts[0][0]) * sgn) * isq;
+            const double vss2 = (sq[0] * vts[0][1] + sq[1] * vts[1][1] + (bts[1][1] - bts[0][1]) * sgn) * isq;
+            const double bss1 = (sq[0] * bts[1][0] + sq[1] * bts[0][0] + sq[0] * sq[1] * (vts[1][0] - vts[0][0]) * sgn) * isq;
+            const double bss2 = (sq[0] * bts[1][1] + sq[1] * bts[0][1] + sq[0] * sq[1] * (vts[1][1] - vts[0][1]) * sgn) * isq;
             const double vdbss = sm * bn + vss1 * bss1 + vss2 * bss2;
             const int a = sm >= 0.0 ? 0 : 1;
             const side7 *q = S[a];
