@@ -151,12 +151,21 @@ __device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs 
     const R bn = ldi(SP + 5 * ps, gu);  // Bx on face j
 
     R vl[7], vr[7];
+    if (RECON == 5) {
+        R vm[7];
 #pragma unroll
-    for (int v = 0; v < 7; v++) {
-        R vm, vp;
-        recon_lane<R, RECON>(tb, 0, jc, q[v], vm, vp);
-        vl[v] = vp;
-        vr[v] = __shfl_down(vm, 1, 64);
+        for (int v = 0; v < 7; v++) recon_lane<R, RECON>(tb, 0, jc, q[v], vm[v], vl[v]);
+        weno_positivity<R>(q, vm, vl);
+#pragma unroll
+        for (int v = 0; v < 7; v++) vr[v] = __shfl_down(vm[v], 1, 64);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 7; v++) {
+            R vm, vp;
+            recon_lane<R, RECON>(tb, 0, jc, q[v], vm, vp);
+            vl[v] = vp;
+            vr[v] = __shfl_down(vm, 1, 64);
+        }
     }
     FaceState<R> sl{vl[0], vl[1], vl[2], vl[3], vl[4], bn, vl[5], vl[6]};
     FaceState<R> sr{vr[0], vr[1], vr[2], vr[3], vr[4], bn, vr[5], vr[6]};
@@ -246,6 +255,7 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
         if (RECON == 5) {
 #pragma unroll
             for (int v = 0; v < 7; v++) weno5_edges<R>(wm[v], w0[v], w1[v], w2[v], w3[v], vm[v], vp[v]);
+            weno_positivity<R>(w1, vm, vp);
         } else if (RECON == 4) {
             R b0 = tb.atr<R>(ax, TB_B0C, c + 1), b1 = tb.atr<R>(ax, TB_B1C, c + 1);
             R a0m = tb.atr<R>(ax, TB_A0M, c), a1m = tb.atr<R>(ax, TB_A1M, c), a2m = tb.atr<R>(ax, TB_A2M, c);

@@ -137,7 +137,8 @@ __host__ __device__ inline void tvd_edges(R cf, R cb, R vm1, R vc, R vp1, R &vm_
 }
 
 // TVD with the MC (LIM 2: minmod(cf dq+, cb dq-, (dq+ + dq-)/2)) or van Leer (LIM 3: harmonic
-// mean) limiter on the width-normalised differences of TVD_MM (reconstruct.f90:115-124): the
+// mean, capped at min(cf |dq+|, cb |dq-|)) limiter on the width-normalised differences of TVD_MM
+// (reconstruct.f90:115-124): the
 // TVDMC / TVDVL options that definition.f90:160-164 declares and the reference leaves
 // unimplemented (extensions; restated as orc_tvd).
 template <typename R, int LIM>
@@ -150,7 +151,12 @@ __host__ __device__ inline void tvd_lim_edges(R cf, R cb, R vm1, R vc, R vp1, R 
         R m = fmin2<R>(fmin2<R>(cf * fabs_<R>(dqf), cb * fabs_<R>(dqb)), R(0.5) * fabs_<R>(dqf + dqb));
         slope = R(0.5) * (sign1<R>(dqf) + sign1<R>(dqb)) * m;
     } else {
-        slope = (dqf * dqb > R(0)) ? R(2.0) * dqf * dqb / (dqf + dqb) : R(0);
+        if (dqf * dqb > R(0)) {
+            R h = R(2.0) * dqf * dqb / (dqf + dqb);
+            slope = sign1<R>(h) * fmin2<R>(fmin2<R>(fabs_<R>(h), cf * fabs_<R>(dqf)), cb * fabs_<R>(dqb));
+        } else {
+            slope = R(0);
+        }
     }
     vp_out = vc + R(0.5) * slope;
     vm_out = vc - R(0.5) * slope;
@@ -185,6 +191,20 @@ __host__ __device__ inline void weno5_edges(R vm2, R vm1, R vc, R vp1, R vp2, R 
 {
     vp_out = weno5_face<R>(vm2, vm1, vc, vp1, vp2);
     vm_out = weno5_face<R>(vp2, vp1, vc, vm1, vm2);
+}
+
+// WENO5 is not positivity-preserving: a cell whose face density (variable 0) or pressure
+// (variable 4) is not positive gets the piecewise-constant profile in every variable (the same
+// fallback as the restatement's sweep)
+template <typename R>
+__host__ __device__ inline void weno_positivity(const R *c, R *vm, R *vp)
+{
+    if (vm[0] <= R(0) || vp[0] <= R(0) || vm[4] <= R(0) || vp[4] <= R(0)) {
+        for (int v = 0; v < 7; v++) {
+            vm[v] = c[v];
+            vp[v] = c[v];
+        }
+    }
 }
 
 // edge values of cell c by a TVD option: 1 minmod (the reference's TVD_MM), 2 MC, 3 van Leer

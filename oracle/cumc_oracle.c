@@ -187,8 +187,10 @@ void orc_tvdmm(const double d[3], const double v[3], double *vm_out, double *vp_
  * differences dqf, dqb as TVD_MM (reconstruct.f90:115-124), with the grid factors cf, cb in
  * place of MC's 2 (so that the face values stay between the neighbouring averages on any grid):
  *   MC: minmod(cf dq+, cb dq-, (dq+ + dq-)/2)      (van Leer 1977, monotonised central)
- *   VL: 2 dq+ dq- / (dq+ + dq-) when dq+ dq- > 0, else 0   (van Leer 1974, harmonic)
- * On a uniform grid cf = cb = 2 and these are the textbook limiters; lim 1 is orc_tvdmm. */
+ *   VL: 2 dq+ dq- / (dq+ + dq-) when dq+ dq- > 0, else 0   (van Leer 1974, harmonic), its size
+ *       capped at min(cf |dq+|, cb |dq-|) for the same bound
+ * On a uniform grid cf = cb = 2, the cap never binds, and these are the textbook limiters; lim 1
+ * is orc_tvdmm. */
 void orc_tvd(int lim, const double d[3], const double v[3], double *vm_out, double *vp_out)
 {
     const double dh_m1 = d[0], dh_c = d[1], dh_p1 = d[2];
@@ -202,7 +204,12 @@ void orc_tvd(int lim, const double d[3], const double v[3], double *vm_out, doub
         double m = fmin2(fmin2(cf * fabs(dqf), cb * fabs(dqb)), 0.5 * fabs(dqf + dqb));
         slope = 0.5 * (fsign1(dqf) + fsign1(dqb)) * m;
     } else if (lim == 3) {
-        slope = (dqf * dqb > 0.0) ? 2.0 * dqf * dqb / (dqf + dqb) : 0.0;
+        if (dqf * dqb > 0.0) {
+            double h = 2.0 * dqf * dqb / (dqf + dqb);
+            slope = fsign1(h) * fmin2(fmin2(fabs(h), cf * fabs(dqf)), cb * fabs(dqb));
+        } else {
+            slope = 0.0;
+        }
     } else {
         slope = 0.5 * (fsign1(dqf) + fsign1(dqb)) * fmin2(fabs(dqf), fabs(dqb));
     }
@@ -214,7 +221,8 @@ void orc_tvd(int lim, const double d[3], const double v[3], double *vm_out, doub
  * "WENO (Shu 1997)" option definition.f90:169-170 declares and reconstruct.f90 leaves
  * unimplemented (an extension).  Candidate quadratics p0..p2, smoothness indicators b0..b2,
  * linear weights (1, 6, 3)/10 and eps = 1e-6, with the uniform-grid (index-space) coefficients
- * on every grid. */
+ * on every grid.  (The sweep falls back to the piecewise-constant profile for a cell whose WENO
+ * face density or pressure is not positive: WENO is not positivity-preserving.) */
 static double weno5_face(double vm2, double vm1, double vc, double vp1, double vp2)
 {
     /* the candidate values times 6; the 1/6 goes into the final normalisation */
@@ -1014,6 +1022,20 @@ static void sweep(orc *o, int dir)
                     interpolate(o, dir, c, v, &vm, &vp);
                     PRR(IBX + bc, j - dj, k - dk, l - dl) = vm;
                     PL(IBX + bc, j, k, l) = vp;
+                }
+                if (o->c.recon == 5 && (PRR(IRHO, j - dj, k - dk, l - dl) <= 0.0 || PL(IRHO, j, k, l) <= 0.0 ||
+                                        PRR(ITAU, j - dj, k - dk, l - dl) <= 0.0 || PL(ITAU, j, k, l) <= 0.0)) {
+                    /* WENO positivity fallback: a cell whose face density or pressure is not
+                     * positive gets the piecewise-constant profile in every variable */
+                    for (int i = IRHO; i <= ITAU; i++) {
+                        PRR(i, j - dj, k - dk, l - dl) = PR(i, j, k, l);
+                        PL(i, j, k, l) = PR(i, j, k, l);
+                    }
+                    for (int t = 0; t < 2; t++) {
+                        int bc = t ? bt_b : bt_a;
+                        PRR(IBX + bc, j - dj, k - dk, l - dl) = BC(bc, j, k, l);
+                        PL(IBX + bc, j, k, l) = BC(bc, j, k, l);
+                    }
                 }
                 PRR(ibn, j - dj, k - dk, l - dl) = PR(ibn, j - dj, k - dk, l - dl);
                 PL(ibn, j, k, l) = PR(ibn, j, k, l);
