@@ -322,8 +322,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="time the default launch shapes (no autotune)")
     ap.add_argument("--recon", choices=["default", "ppm", "tvdmm", "tvdmc", "tvdvl", "weno"], default="default",
                     help="reconstruction (ppm and tvdmm are the reference's; the others are extensions)")
-    ap.add_argument("--solver", choices=["hll", "lf", "hlld"], default="hll",
-                    help="Riemann solver (hll is the reference's; lf and hlld are extensions)")
+    ap.add_argument("--solver", choices=["hll", "lf", "hllc", "hlld"], default="hll",
+                    help="Riemann solver (hll is the reference's; lf, hllc and hlld are extensions)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -376,7 +376,7 @@ def main():
     from cumpic_amd import config as C
     if args.recon != "default":
         cfg.recon = {"ppm": C.PPMC, "tvdmm": C.TVDMM, "tvdmc": C.TVDMC, "tvdvl": C.TVDVL, "weno": C.WENO}[args.recon]
-    cfg.solver = {"hll": C.HLL, "lf": C.LF, "hlld": C.HLLD}[args.solver]
+    cfg.solver = {"hll": C.HLL, "lf": C.LF, "hllc": C.HLLC, "hlld": C.HLLD}[args.solver]
     if args.recon != "default" or args.solver != "hll":
         workload += f" (options: recon {args.recon}, solver {args.solver})"
 

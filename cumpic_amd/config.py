@@ -19,7 +19,7 @@ IRHO, IVX, IVY, IVZ, ITAU, IBX, IBY, IBZ = range(8)
 CARTESIAN, CYLINDRICAL, SPHERICAL = 1, 2, 3  # definition.f90:65-67
 TVDMM, PPMC = 1, 4  # definition.f90:158,167
 TVDMC, TVDVL, WENO = 2, 3, 5  # definition.f90:161,164,170: declared there, implemented here as extensions
-LF, HLL, HLLD = 1, 2, 4  # definition.f90:140,143,149: HLL is the reference's; LF and HLLD extensions
+LF, HLL, HLLC, HLLD = 1, 2, 3, 4  # definition.f90:140-149: HLL is the reference's; the others extensions
 MODEL_RIEMANN, MODEL_TORUS_CYN, MODEL_TORUS_SPH = 0, 1, 2
 FP64, MIXED = 0, 1
 

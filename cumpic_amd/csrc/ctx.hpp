@@ -197,6 +197,7 @@ void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
 {
     switch (ctx->M.solver) {
     case CUMC_LF: launch_fluxes_s<R, 1>(ctx, F, only_dir); break;
+    case CUMC_HLLC: launch_fluxes_s<R, 3>(ctx, F, only_dir); break;
     case CUMC_HLLD: launch_fluxes_s<R, 4>(ctx, F, only_dir); break;
     default: launch_fluxes_s<R, 2>(ctx, F, only_dir); break;
     }

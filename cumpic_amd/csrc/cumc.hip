@@ -64,7 +64,8 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     if (c.recon != CUMC_PPMC && c.recon != CUMC_TVDMM && c.recon != CUMC_TVDMC && c.recon != CUMC_TVDVL &&
         c.recon != CUMC_WENO)
         return CUMC_EINVAL;
-    if (c.solver != 0 && c.solver != CUMC_HLL && c.solver != CUMC_LF && c.solver != CUMC_HLLD) return CUMC_EINVAL;
+    if (c.solver != 0 && c.solver != CUMC_HLL && c.solver != CUMC_LF && c.solver != CUMC_HLLC && c.solver != CUMC_HLLD)
+        return CUMC_EINVAL;
     if (c.model != CUMC_MODEL_RIEMANN && c.model != CUMC_MODEL_TORUS_CYN && c.model != CUMC_MODEL_TORUS_SPH)
         return CUMC_EINVAL;
     if (c.precision != CUMC_FP64 && c.precision != CUMC_MIXED) return CUMC_EINVAL;

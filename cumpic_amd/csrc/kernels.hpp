@@ -61,7 +61,7 @@ struct Tabs {
 
 struct Model {
     int coordinate, recon, model, geom_repeat;
-    int solver;  // CUMC_HLL, CUMC_LF or CUMC_HLLD
+    int solver;  // CUMC_HLL, CUMC_LF, CUMC_HLLC or CUMC_HLLD
     int bflag[6];
     int bfac[6][8];  // x-in, x-out, y-in, y-out, z-in, z-out
     double ggas, cfl, rho_floor, eps_floor, r_sh;

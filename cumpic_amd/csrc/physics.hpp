@@ -440,11 +440,68 @@ __host__ __device__ inline void hlld_face(R g, const FaceState<R> &L, const Face
     }
 }
 
-// the face solver of the configuration: CUMC_LF (1), CUMC_HLL (2, the reference's), CUMC_HLLD (4)
+// HLLC flux for MHD in the form of Li (2005, J. Comput. Phys. 203, 344), the HLLC option of
+// definition.f90:145-146 (an extension; restated as orc_hllc with the same operation order): HLL
+// state for the star-region field B* and velocity (normal component S_M), p_T* from the left
+// state, single-star states on the side of S_M.
+template <int DIR, typename R>
+__host__ __device__ inline void hllc_face(R g, const FaceState<R> &L, const FaceState<R> &Rr, R f[7])
+{
+    Side7<R> a, b;
+    face_side<DIR, R>(g, L, a);
+    face_side<DIR, R>(g, Rr, b);
+    const R bn = L.bn;
+    const R cmax = fmax2<R>(a.cf, b.cf);
+    const R sL = fmin2<R>(a.vn, b.vn) - cmax;
+    const R sR = fmax2<R>(a.vn, b.vn) + cmax;
+    if (sL >= R(0)) {
+        for (int i = 0; i < 7; i++) f[i] = a.f[i];
+        return;
+    }
+    if (sR <= R(0)) {
+        for (int i = 0; i < 7; i++) f[i] = b.f[i];
+        return;
+    }
+    if (!(sL < R(0) && sR > R(0))) {
+        for (int i = 0; i < 7; i++) f[i] = R(NAN);
+        return;
+    }
+    const R ids = R(1.0) / (sR - sL);
+    R uh[7];
+#pragma unroll
+    for (int i = 0; i < 7; i++) uh[i] = (sR * b.u[i] - sL * a.u[i] - b.f[i] + a.f[i]) * ids;
+    const R irh = R(1.0) / uh[0];
+    const R sm = uh[1] * irh, vt1 = uh[2] * irh, vt2 = uh[3] * irh;
+    const R bt1 = uh[5], bt2 = uh[6];
+    const R pts = a.rho * (sL - a.vn) * (sm - a.vn) + a.pt;
+    const R vdbs = sm * bn + vt1 * bt1 + vt2 * bt2;
+    const bool left = sm >= R(0);
+    // the side of S_M, selected element by element (no runtime array index)
+    const R s = left ? sL : sR;
+    const R qrho = left ? a.rho : b.rho, qvn = left ? a.vn : b.vn, qvt1 = left ? a.vt1 : b.vt1;
+    const R qvt2 = left ? a.vt2 : b.vt2, qbt1 = left ? a.bt1 : b.bt1, qbt2 = left ? a.bt2 : b.bt2;
+    const R qpt = left ? a.pt : b.pt, qe = left ? a.u[4] : b.u[4];
+    const R sd = s - qvn, isdm = R(1.0) / (s - sm);
+    const R vdb = qvn * bn + qvt1 * qbt1 + qvt2 * qbt2;
+    R us[7];
+    us[0] = qrho * sd * isdm;
+    us[1] = us[0] * sm;
+    us[2] = (qrho * qvt1 * sd - (bn * bt1 - bn * qbt1)) * isdm;
+    us[3] = (qrho * qvt2 * sd - (bn * bt2 - bn * qbt2)) * isdm;
+    us[4] = (qe * sd - qpt * qvn + pts * sm + bn * (vdb - vdbs)) * isdm;
+    us[5] = bt1;
+    us[6] = bt2;
+#pragma unroll
+    for (int i = 0; i < 7; i++) f[i] = (left ? a.f[i] : b.f[i]) + s * (us[i] - (left ? a.u[i] : b.u[i]));
+}
+
+// the face solver of the configuration: CUMC_LF (1), CUMC_HLL (2, the reference's), CUMC_HLLC (3),
+// CUMC_HLLD (4)
 template <int DIR, typename R, int SOLVER>
 __host__ __device__ inline void riemann_face(R g, const FaceState<R> &L, const FaceState<R> &Rr, R f[7])
 {
     if (SOLVER == 1) lf_face<DIR, R>(g, L, Rr, f);
+    else if (SOLVER == 3) hllc_face<DIR, R>(g, L, Rr, f);
     else if (SOLVER == 4) hlld_face<DIR, R>(g, L, Rr, f);
     else hll_face<DIR, R>(g, L, Rr, f);
 }

@@ -33,9 +33,9 @@ enum { CUMC_CARTESIAN = 1, CUMC_CYLINDRICAL = 2, CUMC_SPHERICAL = 3 };  /* solve
 enum { CUMC_TVDMM = 1, CUMC_PPMC = 4 };                  /* solver/definition.f90:158,167 */
 /* declared by solver/definition.f90:161,164,170 but not implemented by the reference; extensions */
 enum { CUMC_TVDMC = 2, CUMC_TVDVL = 3, CUMC_WENO = 5 };
-/* Riemann solvers: HLL is the reference's (rsolver.f90:43-130); LF and HLLD are declared by
- * definition.f90:140,149 without an implementation there, and are extensions here */
-enum { CUMC_LF = 1, CUMC_HLL = 2, CUMC_HLLD = 4 };
+/* Riemann solvers: HLL is the reference's (rsolver.f90:43-130); LF, HLLC and HLLD are declared by
+ * definition.f90:140,146,149 without an implementation there, and are extensions here */
+enum { CUMC_LF = 1, CUMC_HLL = 2, CUMC_HLLC = 3, CUMC_HLLD = 4 };
 enum { CUMC_MODEL_RIEMANN = 0, CUMC_MODEL_TORUS_CYN = 1, CUMC_MODEL_TORUS_SPH = 2 };
 enum { CUMC_FP64 = 0, CUMC_MIXED = 1 };
 
@@ -52,7 +52,7 @@ typedef struct cumc_config {
     double cfl;                   /* definition.f90:116 */
     double rho_floor, eps_floor;  /* torus_cyn floors (model.f90:246-247) */
     double r_sh;                  /* torus_cyn Schwarzschild radius (defs.f90:23) */
-    int solver;                   /* CUMC_HLL (or 0) | CUMC_LF | CUMC_HLLD (definition.f90:152) */
+    int solver;                   /* CUMC_HLL (or 0) | CUMC_LF | CUMC_HLLC | CUMC_HLLD (definition.f90:152) */
     int precision;                /* CUMC_FP64 | CUMC_MIXED */
 } cumc_config;
 

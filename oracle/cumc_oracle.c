@@ -560,10 +560,71 @@ int orc_hlld(int dir, double g, const double *pl, const double *pr, double *flux
     return region;
 }
 
-/* the face solver of the configuration: 1 LF, 2 (or 0) HLL, 4 HLLD */
+/* HLLC flux for MHD in the form of Li (2005, J. Comput. Phys. 203, 344), the HLLC option
+ * definition.f90:145-146 declares and rsolver.f90 leaves unimplemented (an extension): outer
+ * speeds as HLLD's (Miyoshi & Kusano eq. 67); the HLL state U_hll = (S_R U_R - S_L U_L - F_R +
+ * F_L) / (S_R - S_L) gives the star-region field B* = B_hll and velocity v* = m_hll / rho_hll
+ * with normal component S_M = m_n,hll / rho_hll; total pressure
+ * p_T* = rho_L (S_L - v_nL)(S_M - v_nL) + p_TL (B_n continuous); on each side
+ * rho* = rho (S - v_n) / (S - S_M), m_t* = (rho v_t (S - v_n) - (B_n B_t* - B_n B_t)) / (S - S_M),
+ * E* = (E (S - v_n) - p_T v_n + p_T* S_M + B_n (v.B - v*.B*)) / (S - S_M), and
+ * F* = F + S (U* - U).  Returns the region: 0 F_L, 1 F*_L, 2 F*_R, 3 F_R, 4 NaN. */
+int orc_hllc(int dir, double g, const double *pl, const double *pr, double *flux)
+{
+    int ivn, ivt1, ivt2, ibn, ibt1, ibt2;
+    dir_indices(dir, &ivn, &ivt1, &ivt2, &ibn, &ibt1, &ibt2);
+    side7 L, R;
+    side_of(dir, g, pl, &L);
+    side_of(dir, g, pr, &R);
+    const double bn = pl[ibn];
+    double f[7];
+    const double cmax = fmax2(L.cf, R.cf);
+    const double sL = fmin2(L.vn, R.vn) - cmax;
+    const double sR = fmax2(L.vn, R.vn) + cmax;
+    int region;
+    if (sL >= 0.0) {
+        for (int i = 0; i < 7; i++) f[i] = L.f[i];
+        region = 0;
+    } else if (sR <= 0.0) {
+        for (int i = 0; i < 7; i++) f[i] = R.f[i];
+        region = 3;
+    } else if (sL < 0.0 && sR > 0.0) {
+        const double ids = 1.0 / (sR - sL);
+        double uh[7];
+        for (int i = 0; i < 7; i++) uh[i] = (sR * R.u[i] - sL * L.u[i] - R.f[i] + L.f[i]) * ids;
+        const double irh = 1.0 / uh[0];
+        const double sm = uh[1] * irh, vt1 = uh[2] * irh, vt2 = uh[3] * irh;
+        const double bt1 = uh[5], bt2 = uh[6];
+        const double pts = L.rho * (sL - L.vn) * (sm - L.vn) + L.pt;
+        const double vdbs = sm * bn + vt1 * bt1 + vt2 * bt2;
+        const int a = sm >= 0.0 ? 0 : 1;
+        const side7 *q = a == 0 ? &L : &R;
+        const double s = a == 0 ? sL : sR;
+        const double sd = s - q->vn, isdm = 1.0 / (s - sm);
+        const double vdb = q->vn * bn + q->vt1 * q->bt1 + q->vt2 * q->bt2;
+        double us[7];
+        us[0] = q->rho * sd * isdm;
+        us[1] = us[0] * sm;
+        us[2] = (q->rho * q->vt1 * sd - (bn * bt1 - bn * q->bt1)) * isdm;
+        us[3] = (q->rho * q->vt2 * sd - (bn * bt2 - bn * q->bt2)) * isdm;
+        us[4] = (q->u[4] * sd - q->pt * q->vn + pts * sm + bn * (vdb - vdbs)) * isdm;
+        us[5] = bt1;
+        us[6] = bt2;
+        for (int i = 0; i < 7; i++) f[i] = q->f[i] + s * (us[i] - q->u[i]);
+        region = a == 0 ? 1 : 2;
+    } else {
+        for (int i = 0; i < 7; i++) f[i] = NAN;
+        region = 4;
+    }
+    unrotate7(dir, f, flux);
+    return region;
+}
+
+/* the face solver of the configuration: 1 LF, 2 (or 0) HLL, 3 HLLC, 4 HLLD */
 static int riemann(int solver, int dir, double g, const double *pl, const double *pr, double *flux)
 {
     if (solver == 1) return orc_lf(dir, g, pl, pr, flux);
+    if (solver == 3) return orc_hllc(dir, g, pl, pr, flux);
     if (solver == 4) return orc_hlld(dir, g, pl, pr, flux);
     return orc_hll(dir, g, pl, pr, flux);
 }

@@ -31,7 +31,7 @@ typedef struct orc_config {
     double ggas;               /* adiabatic index (model defs.f90:14) */
     double cfl;                /* definition.f90:116 */
     double rho_floor, eps_floor, r_sh; /* torus_cyn/src/defs.f90, model.f90:246-247 */
-    int solver;                /* 2 (or 0) HLL (definition.f90:143); 1 LF, 4 HLLD (extensions) */
+    int solver;                /* 2 (or 0) HLL (definition.f90:143); 1 LF, 3 HLLC, 4 HLLD (extensions) */
 } orc_config;
 
 typedef struct orc orc;
@@ -66,6 +66,8 @@ int orc_hll(int dir, double ggas, const double *pl, const double *pr, double *fl
 /* local Lax-Friedrichs; HLLD (returns the wave region 0..5, 6 for NaN speeds) */
 int orc_lf(int dir, double ggas, const double *pl, const double *pr, double *flux);
 int orc_hlld(int dir, double ggas, const double *pl, const double *pr, double *flux);
+/* HLLC for MHD (Li 2005; returns the region 0 F_L, 1 F*_L, 2 F*_R, 3 F_R, 4 NaN) */
+int orc_hllc(int dir, double ggas, const double *pl, const double *pr, double *flux);
 void orc_ptof(int dir, const double *p, const double *u, double *f);
 void orc_ptou_face(const double *p, double eps, double *u);
 void orc_ptou_cell(const double *p, double eps, const double *bc, double *u);

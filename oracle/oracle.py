@@ -63,7 +63,7 @@ def lib():
         L.orc_signalspeed.argtypes = [ctypes.c_double] * 5
         L.orc_tvd.argtypes = [ctypes.c_int, _D, _D, _D, _D]
         L.orc_weno5.argtypes = [_D, _D, _D]
-        for name in ("orc_hll", "orc_lf", "orc_hlld"):
+        for name in ("orc_hll", "orc_lf", "orc_hllc", "orc_hlld"):
             getattr(L, name).restype = ctypes.c_int
             getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_double, _D, _D, _D]
         L.orc_ptof.argtypes = [ctypes.c_int, _D, _D, _D]
@@ -226,6 +226,15 @@ def lf(direction, gamma, pl, pr):
     pr = np.ascontiguousarray(pr, dtype=np.float64)
     f = np.zeros(8)
     br = lib().orc_lf(int(direction), float(gamma), _p(pl), _p(pr), _p(f))
+    return f, br
+
+
+def hllc(direction, gamma, pl, pr):
+    """HLLC flux (8 unrotated components) and the wave region: 0 F_L, 1 F*_L, 2 F*_R, 3 F_R."""
+    pl = np.ascontiguousarray(pl, dtype=np.float64)
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    f = np.zeros(8)
+    br = lib().orc_hllc(int(direction), float(gamma), _p(pl), _p(pr), _p(f))
     return f, br
 
 

@@ -7,7 +7,7 @@ from __future__ import annotations
 import pytest
 
 from cumpic_amd import ics
-from cumpic_amd.config import FP64, HLL, HLLD, LF, MIXED, PPMC, TVDMC, TVDMM, TVDVL, WENO
+from cumpic_amd.config import FP64, HLL, HLLC, HLLD, LF, MIXED, PPMC, TVDMC, TVDMM, TVDVL, WENO
 from oracle.oracle import Oracle
 from tests.helpers import assert_bitwise, interior, rel_l1
 
@@ -27,7 +27,7 @@ def _states_equal(g, o, what):
 
 
 @pytest.mark.parametrize("direction", [1, 2, 3])
-@pytest.mark.parametrize("recon,solver", [(PPMC, HLLD), (TVDVL, LF), (TVDMC, HLL), (WENO, HLL)])
+@pytest.mark.parametrize("recon,solver", [(PPMC, HLLD), (TVDVL, LF), (TVDMC, HLL), (WENO, HLL), (PPMC, HLLC)])
 def test_extension_sweep_flux_bitwise(direction, recon, solver):
     from cumpic_amd.cumc import Solver
     print(f"[case] sweep_flux dir={direction} recon={recon} solver={solver} n=12", flush=True)
@@ -56,6 +56,8 @@ def test_extension_sweep_flux_bitwise(direction, recon, solver):
     (TVDVL, LF, (1, 0, 0, 1, 2, 0)),
     (WENO, HLL, (0, 0, 0, 0, 0, 0)),
     (WENO, HLLD, (1, 1, 2, 2, 1, 1)),
+    (PPMC, HLLC, (0, 0, 0, 0, 0, 0)),
+    (TVDMC, HLLC, (1, 1, 2, 2, 1, 1)),
 ])
 def test_extension_steps_bitwise(recon, solver, flags):
     from cumpic_amd.cumc import Solver

@@ -9,14 +9,14 @@ import numpy as np
 import pytest
 
 from cumpic_amd import ics
-from cumpic_amd.config import HLL, HLLD, LF, PPMC, TVDMC, TVDMM, TVDVL, WENO
+from cumpic_amd.config import HLL, HLLC, HLLD, LF, PPMC, TVDMC, TVDMM, TVDVL, WENO
 from oracle.oracle import Oracle
 from tests.helpers import assert_bitwise
 
 pytestmark = pytest.mark.gpu
 
 RECONS = [TVDMM, TVDMC, TVDVL, PPMC, WENO]
-SOLVERS = [HLL, LF, HLLD]
+SOLVERS = [HLL, LF, HLLC, HLLD]
 
 
 def _config(i):

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from cumpic_amd import ics
-from cumpic_amd.config import HLL, HLLD, LF, TVDMC, TVDVL, WENO
+from cumpic_amd.config import HLL, HLLC, HLLD, LF, TVDMC, TVDVL, WENO
 from oracle import oracle as orc
 from oracle.oracle import Oracle
 
@@ -220,12 +220,12 @@ def test_hlld_resolves_an_isolated_rotational_discontinuity_exactly():
         assert not np.allclose(fh, mhd_flux(1, sl), rtol=1e-6, atol=1e-6)
 
 
-@pytest.mark.parametrize("solver", ["lf", "hlld"])
+@pytest.mark.parametrize("solver", ["lf", "hllc", "hlld"])
 def test_mirror_symmetry(solver):
     """x -> -x maps (v_n, B_n) -> (-v_n, -B_n) and swaps the sides; the flux of rho, m_t, E and
     B_t changes sign and the flux of m_n does not.  Catches swapped outer speeds or a sign error
     in the double-star states."""
-    fn = orc.lf if solver == "lf" else orc.hlld
+    fn = {"lf": orc.lf, "hllc": orc.hllc, "hlld": orc.hlld}[solver]
     flip = np.array([-1, 1, -1, -1, -1, 1, -1, -1], dtype=float)
     for _ in range(600):
         sl, sr = rand_state(), rand_state()
@@ -253,10 +253,64 @@ def test_hlld_rotation_covariance(d):
 
 
 # --------------------------------------------------------------------------------------------
+# HLLC (orc_hllc)
+# --------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_hllc_consistency_and_upwinding(d):
+    """F_HLLC(U, U) = F(U) with and without a normal field, and supersonic flow takes the
+    upwind physical flux."""
+    for k in range(400):
+        s = rand_state()
+        if k % 3 == 0:
+            s[5 + d - 1] = 0.0
+        f, _ = orc.hllc(d, G, s, s)
+        assert np.allclose(f, mhd_flux(d, s), rtol=1e-12, atol=1e-12), (d, s, f)
+    for _ in range(200):
+        sl, sr = rand_state(), rand_state()
+        sr[5 + d - 1] = sl[5 + d - 1]
+        sign = RNG.choice([-1.0, 1.0])
+        sl[d] = sr[d] = sign * 30.0
+        f, region = orc.hllc(d, G, sl, sr)
+        assert region == (0 if sign > 0 else 3)
+        assert np.allclose(f, mhd_flux(d, sl if sign > 0 else sr), rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("bn", [0.0, 0.7, -1.3])
+def test_hllc_resolves_an_isolated_contact_exactly(bn):
+    """The property that defines HLLC: a contact moving at v_n with equal total pressure,
+    velocity and field on both sides gets the upwind state's physical flux (HLL smears it)."""
+    for _ in range(300):
+        base = rand_state()
+        base[5] = bn
+        u = RNG.uniform(0.05, 0.5) * RNG.choice([-1.0, 1.0])
+        base[1] = u
+        sl, sr = base.copy(), base.copy()
+        sl[0], sr[0] = RNG.uniform(0.2, 2.0), RNG.uniform(0.2, 2.0)
+        f, region = orc.hllc(1, G, sl, sr)
+        assert region == (1 if u > 0 else 2)
+        assert np.allclose(f, mhd_flux(1, sl if u > 0 else sr), rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_hllc_rotation_covariance(d):
+    perm = {2: [0, 2, 3, 1, 4, 6, 7, 5], 3: [0, 3, 1, 2, 4, 7, 5, 6]}[d]
+    for _ in range(300):
+        sl, sr = rand_state(), rand_state()
+        sr[5] = sl[5]
+        f1, _ = orc.hllc(1, G, sl, sr)
+        pl, pr = np.zeros(8), np.zeros(8)
+        pl[perm], pr[perm] = sl, sr
+        fd, _ = orc.hllc(d, G, pl, pr)
+        assert np.allclose(fd[perm], f1, rtol=1e-12, atol=1e-13)
+
+
+# --------------------------------------------------------------------------------------------
 # Whole steps with the extensions
 # --------------------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("recon,solver", [(TVDMC, HLLD), (TVDVL, LF), (4, HLLD), (1, LF), (WENO, HLL), (WENO, HLLD)])
+@pytest.mark.parametrize("recon,solver", [(TVDMC, HLLD), (TVDVL, LF), (4, HLLD), (1, LF), (WENO, HLL), (WENO, HLLD),
+                                          (4, HLLC), (TVDMC, HLLC)])
 def test_whole_steps_conserve_and_keep_divb(recon, solver):
     """Periodic box: mass, momentum and energy conserved to round-off and div B at round-off
     after a few steps with the extension options."""
