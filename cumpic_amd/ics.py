@@ -91,6 +91,79 @@ def riemann_1d(n: int = 800, seed: int = 1000, gamma: float = 2.0, cfl: float = 
     return Problem(cfg, xf, yf, zf, prim, eps, steps=10 ** 9, total_time=t_end, name="riemann1d")
 
 
+# The riemann plugin's named tests (model/riemann/src/defs.f90:20-22) and its default
+# adiabatic index (defs.f90:14).
+TORO_1, BRIO_WU, MHD_ROTOR = 1, 2, 3
+RIEMANN_GGAS = 5.0 / 3.0
+
+
+def _cell_centres(f: np.ndarray, n: int) -> np.ndarray:
+    """GET_COORD's centre 0.5*(xF(j) + xF(j-1)) of cells j = 0..n (solver/coords.f90:63-65)."""
+    return 0.5 * (f[ix(0) + 1:ix(n) + 2] + f[ix(0):ix(n) + 1])
+
+
+def riemann_test(test: int, n: int = 400, gamma: float = RIEMANN_GGAS, cfl: float = 0.7,
+                 recon: int = PPMC, t_end: float | None = None) -> Problem:
+    """One of the riemann plugin's three named problems on the unit interval / square, filled
+    on [0,n]^3 as GET_MODEL does (model/riemann/src/model.f90:15-21, RIEMANN_PROBLEM :32-57),
+    outflow on every side, zero for every primitive a problem leaves unset.
+
+    * TORO_1 (riemann1d.f90:15-47): rho, p = 1, 1 for x <= 0.5, else 0.125, 0.1; no field;
+      n x 1 x 1 cells; stop time 0.2.
+    * BRIO_WU (riemann1d.f90:54-89): TORO_1's gas with By = +1 / -1 on the two sides and a
+      uniform Bx = 0.75; n x 1 x 1 cells; stop time 0.1.
+    * MHD_ROTOR (riemann2d.f90:15-63): a disc r <= r0 = 0.1 about (0.5, 0.5) with rho = 10
+      rotating rigidly at v = (-(y-0.5), x-0.5)/r0, a linear taper fr = (r1-r)/(r1-r0) out to
+      r1 = 0.115 (rho = 1 + 9 fr, v scaled by fr r0/r), rho = 1 at rest beyond; p = 0.5 and
+      Bx = 2.5/sqrt(4 pi) everywhere; n x n x 1 cells; stop time 0.15.
+
+    eps = p/rho/(gamma-1) in every case.  Cell coordinates are GET_COORD's centres (ghost
+    faces included, as for cell 0).
+    """
+    if test not in (TORO_1, BRIO_WU, MHD_ROTOR):
+        raise ValueError(f"unknown riemann test {test}")
+    two_d = test == MHD_ROTOR
+    ny = n if two_d else 1
+    cfg = Config(nx=n, ny=ny, nz=1, coordinate=CARTESIAN, boundary_flag=(1, 1, 1, 1, 1, 1), recon=recon,
+                 model=MODEL_RIEMANN, ggas=gamma, cfl=cfl)
+    xf = uniform_faces(n, 0.0, 1.0)
+    yf = uniform_faces(ny, 0.0, 1.0)
+    zf = uniform_faces(1, 0.0, 1.0)
+    prim, eps = _alloc(cfg)
+    x = _cell_centres(xf, n)[None, :]
+    y = _cell_centres(yf, ny)[:, None]
+    shape = (ny + 1, n + 1)
+    vx, vy, bx, by = (np.zeros(shape) for _ in range(4))
+    if test == MHD_ROTOR:
+        r0, r1 = 0.1, 0.115
+        r = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2)
+        fr = (r1 - r) / (r1 - r0)
+        inner, outer = r <= r0, r >= r1
+        taper = ~(inner | outer)
+        rs = np.where(taper, r, 1.0)
+        rho = np.where(inner, 10.0, np.where(outer, 1.0, 1.0 + 9.0 * fr))
+        vx = np.where(inner, -1.0 * (y - 0.5) / r0, np.where(outer, 0.0, -(fr * 1.0 * (y - 0.5) / rs)))
+        vy = np.where(inner, 1.0 * (x - 0.5) / r0, np.where(outer, 0.0, fr * 1.0 * (x - 0.5) / rs))
+        p = np.full(shape, 0.5)
+        bx[:] = 2.5 / math.sqrt(4.0 * (4.0 * math.atan(1.0)))
+        t_def = 0.15
+    else:
+        left = np.broadcast_to(x <= 0.5, shape)
+        rho = np.where(left, 1.0, 0.125)
+        p = np.where(left, 1.0, 0.1)
+        if test == BRIO_WU:
+            by = np.where(left, 1.0, -1.0)
+            bx[:] = 0.75
+        t_def = 0.2 if test == TORO_1 else 0.1
+    sl = (slice(ix(0), ix(1) + 1), slice(ix(0), ix(ny) + 1), slice(ix(0), ix(n) + 1))
+    for comp, val in ((IRHO, rho), (IVX, vx), (IVY, vy), (ITAU, p), (IBX, bx), (IBY, by)):
+        prim[sl + (comp,)] = val
+    eps[sl] = p / rho / (gamma - 1.0)
+    name = {TORO_1: "toro1", BRIO_WU: "briowu", MHD_ROTOR: "mhdrotor"}[test]
+    return Problem(cfg, xf, yf, zf, prim, eps, steps=10 ** 9,
+                   total_time=t_def if t_end is None else t_end, name=name)
+
+
 def _spectral_field(shape, kmax: float, slope: float, rng, kz_zero: bool = False) -> np.ndarray:
     """Real random field with Gaussian random-phase modes of amplitude |k|^slope on integer
     wavevectors 1 <= |k| <= kmax (zero elsewhere), normalised to unit rms.  shape = (nz, ny, nx);

@@ -1,5 +1,5 @@
 """Regenerate tests/golden/oracle_runs.json: SHA-256 digests (and a few summary values) of the
-CPU restatement's final state for four canonical runs.  The reference ships no tests, fixtures or
+CPU restatement's final state for seven canonical runs.  The reference ships no tests, fixtures or
 golden data and its Fortran build needs an HDF5 module this image lacks, so these are not
 reference outputs: they freeze the restatement (pinned by tests/test_oracle_pins.py) so that any
 later change of its arithmetic is deliberate.  Run from the repository root:
@@ -32,6 +32,9 @@ def cases():
         "torus_cyn_24x12x24_3steps": (ics.torus_cyn(24, 12, 24), 3),
         "spherical_shell_14x10x8_3steps": (ics.spherical_shell(14, 10, 8, seed=1500, noise=0.2,
                                                                boundary_flag=(1, 1, 1, 1, 0, 0)), 3),
+        "toro1_200_t0.2": (ics.riemann_test(ics.TORO_1, n=200), None),
+        "briowu_200_t0.1": (ics.riemann_test(ics.BRIO_WU, n=200), None),
+        "mhdrotor_64sq_t0.15": (ics.riemann_test(ics.MHD_ROTOR, n=64), None),
     }
 
 
