@@ -84,10 +84,14 @@ def read_traffic(kernel, tag):
     return None
 
 
-def cpu_baseline(prob, steps=1):
-    """The CPU restatement on the host cores: same initial state, `steps` RK3 steps."""
+def cpu_baseline(prob, steps=1, one_core_n=128):
+    """The CPU restatement on the host cores: same initial state, `steps` RK3 steps; beside it a
+    one-thread figure on a bounded sample (the same workload builder at one_core_n^3, one step)."""
+    from cumpic_amd import ics
+    from oracle import oracle as orc
     from oracle.oracle import Oracle
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc.set_threads(cores)
     o = Oracle.from_problem(prob)
     o.initial_update()
     t0 = time.perf_counter()
@@ -96,14 +100,29 @@ def cpu_baseline(prob, steps=1):
     state = o.get_state()
     o.close()
     c = prob.cfg
-    return {
+    out = {
         "value": c.zones * steps / wall,
         "unit": "zone-cycles/s",
         "cores": cores,
         "kind": "port",
         "sample": f"{c.nx}x{c.ny}x{c.nz} {prob.name}, {steps} RK3 step(s) after initial_update, "
                   f"{wall:.1f} s wall on {cores} OpenMP threads",
-    }, state
+    }
+    if one_core_n:
+        small = ics.spectral_box(n=one_core_n, seed=1001, steps=1)
+        orc.set_threads(1)
+        try:
+            o1 = Oracle.from_problem(small)
+            o1.initial_update()
+            t0 = time.perf_counter()
+            o1.run(1)
+            w1 = time.perf_counter() - t0
+            o1.close()
+        finally:
+            orc.set_threads(cores)
+        out["value_1core"] = small.cfg.zones / w1
+        out["sample_1core"] = f"{one_core_n}^3 periodic spectral box (seed 1001), 1 RK3 step, {w1:.1f} s on 1 thread"
+    return out, state
 
 
 def timed_run(Solver, prob, prec, steps, warmup, tune=True):
@@ -383,7 +402,7 @@ def main():
     cpu, parity = None, None
     if not args.no_cpu and "fp64" in forms:
         log("[bench] CPU restatement: 1 step on the host cores")
-        cpu, ostate = cpu_baseline(prob, steps=1)
+        cpu, ostate = cpu_baseline(prob, steps=1, one_core_n=128 if args.workload == "box" else 0)
         log(f"[bench] cpu {cpu['value']:.3e} zone-cycles/s")
         g = Solver.from_problem(prob, precision=FP64)
         g.initial_update()

@@ -16,6 +16,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 enum { IRHO = 0, IVX = 1, IVY = 2, IVZ = 3, ITAU = 4, IBX = 5, IBY = 6, IBZ = 7, NV = 8 };
 enum { IEX = 0, IEY = 1, IEZ = 2 };
@@ -705,6 +708,17 @@ void orc_destroy(orc *o)
 }
 
 long orc_ncell(const orc *o) { return o->ncell; }
+
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 void orc_set_state(orc *o, const double *prim, const double *eps)
 {

@@ -39,6 +39,8 @@ typedef struct orc orc;
 orc *orc_create(const orc_config *cfg, const double *xf, const double *yf, const double *zf);
 void orc_destroy(orc *o);
 long orc_ncell(const orc *o);
+/* OpenMP threads of the loops (n <= 0 leaves them); returns the count now in force. */
+int orc_set_threads(int n);
 /* full ghost-inclusive arrays in the layout above */
 void orc_set_state(orc *o, const double *prim, const double *eps);
 void orc_get_state(const orc *o, double *prim, double *eps, double *bcell, double *cons);

@@ -43,6 +43,8 @@ def lib():
         L.orc_create.argtypes = [ctypes.POINTER(OrcConfig), _D, _D, _D]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_ncell.restype = ctypes.c_long
+        L.orc_set_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_ncell.argtypes = [ctypes.c_void_p]
         L.orc_set_state.argtypes = [ctypes.c_void_p, _D, _D]
         L.orc_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
@@ -175,6 +177,11 @@ class Oracle:
 
 
 # ---- point-level functions --------------------------------------------------------------------------
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the restatement's loops (n <= 0 leaves them); returns the count in force."""
+    return int(lib().orc_set_threads(int(n)))
+
 
 def ppm(d, v):
     d = np.ascontiguousarray(d, dtype=np.float64)

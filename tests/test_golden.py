@@ -1,4 +1,4 @@
-"""Frozen digests of four canonical runs (tests/golden/oracle_runs.json, written by
+"""Frozen digests of seven canonical runs (tests/golden/oracle_runs.json, written by
 tests/golden/make_golden.py): the CPU restatement must reproduce them bit for bit (so a change of
 its arithmetic is deliberate), and so must the HIP FP64 path (gpu-marked), independently of a
 side-by-side oracle run."""
@@ -23,6 +23,22 @@ def test_oracle_reproduces_golden_digest(name):
     g = GOLDEN[name]
     assert d["steps"] == g["steps"] and d["time"] == g["time"]
     assert d["sha256"] == g["sha256"], (name, d, g)
+
+
+@pytest.mark.parametrize("threads", [1, 3])
+def test_oracle_digest_independent_of_thread_count(threads):
+    """The restatement's OpenMP loops write disjoint cells and reduce dt by min, so its result
+    does not depend on the thread count (the CPU baseline times it at several)."""
+    from oracle import oracle as orc
+    name = "periodic_12_ppm_4steps"
+    prob, steps = cases()[name]
+    before = orc.set_threads(0)
+    try:
+        orc.set_threads(threads)
+        d = digest(prob, steps)
+    finally:
+        orc.set_threads(before)
+    assert d["sha256"] == GOLDEN[name]["sha256"]
 
 
 @pytest.mark.gpu
