@@ -1,5 +1,6 @@
-"""The Fortran host driver (cumpic_amd/fhost/cumc3d) runs the 1D Riemann workload through the C
-ABI on the GPU; its final state and dt sequence must equal the CPU restatement bit for bit."""
+"""The Fortran host driver (cumpic_amd/fhost/cumc3d) runs the 1D Riemann workload and the riemann
+plugin's named problems (Toro 1, Brio-Wu, the 2D MHD rotor) through the C ABI on the GPU; its
+final state and dt sequence must equal the CPU restatement bit for bit."""
 from __future__ import annotations
 
 import os
@@ -15,12 +16,20 @@ from oracle.oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 
-def test_fortran_host_riemann_1d(tmp_path):
-    print("[case] Fortran host, 1D Riemann 800 cells to t=0.1", flush=True)
+def _named(name):
+    if name == "riemann_1d":
+        return ics.riemann_1d()
+    n = {"toro1": 200, "briowu": 200, "mhdrotor": 48}[name]
+    return ics.riemann_test({"toro1": ics.TORO_1, "briowu": ics.BRIO_WU, "mhdrotor": ics.MHD_ROTOR}[name], n=n)
+
+
+@pytest.mark.parametrize("name", ["riemann_1d", "toro1", "briowu", "mhdrotor"])
+def test_fortran_host_riemann_1d(tmp_path, name):
+    print(f"[case] Fortran host, {name} to its stop time", flush=True)
     exe = build.FHOST_EXE
     if not os.path.exists(exe):
         pytest.skip("Fortran host not built (no flang)")
-    p = ics.riemann_1d()
+    p = _named(name)
     run = tmp_path / "c0.run"
     out = tmp_path / "c0.out"
     write_runfile(str(run), p, max_steps=100000, total_time=p.total_time)
