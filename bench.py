@@ -239,7 +239,7 @@ def _interior(cfg):
     return (slice(3, 3 + cfg.nz), slice(3, 3 + cfg.ny), slice(3, 3 + cfg.nx))
 
 
-def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-5, 1e-6, 1e-7)):
+def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-5, 1e-6, 1e-7), store_all=True):
     """Reverse mode on the batched boxes: J_b = sum |B|^2/2 dV per box after `steps` frozen-dt
     steps; per-box gradients with respect to the initial velocity from the batched tape
     (cumc_batch_tape_run / cumc_batch_tape_backward, checkpoints every `every` steps).  Cost:
@@ -277,6 +277,31 @@ def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-5, 1
         f"({t_back * 1e3:.1f} ms wall incl. seed/gradient transfers), ratio {back_ms / fwd_ms:.2f}; "
         f"of it {recompute_ms:.1f} ms re-run checkpoint segments forward, reverse proper "
         f"{(back_ms - recompute_ms) / fwd_ms:.2f}x the forward")
+    # one checkpoint segment over all steps: every stage state is kept from the taping forward
+    # (about 2570 state planes per box, ~194 GB for the 48 boxes), so the backward re-runs nothing
+    stored = None
+    if store_all:
+        from cumpic_amd.cumc import CumcError
+        for s, p in zip(boxes, probs):
+            s.upload(p.prim, p.eps)
+        try:
+            t0 = time.perf_counter()
+            batch.tape_run(steps, steps, dts)
+            t_tape_all = time.perf_counter() - t0
+            gp_all = batch.tape_backward(seeds)
+            back_all = batch.last_ms()
+            rec_all = batch.tape_recompute_ms()
+            same = all(np.array_equal(a, c) for a, c in zip(gp_all, gp))
+            stored = {"checkpoint_every": steps, "backward_ms": back_all, "recompute_ms": rec_all,
+                      "adjoint_over_forward": back_all / fwd_ms, "tape_forward_wall_ms": t_tape_all * 1e3,
+                      "gradient_bitwise_equal_to_checkpointed": bool(same)}
+            log(f"[bench] adjoint, every stage state kept (one segment): backward {back_all:.1f} ms device, "
+                f"ratio {back_all / fwd_ms:.2f}, re-run {rec_all:.1f} ms, gradients bit-identical to the "
+                f"checkpointed run: {same}")
+        except CumcError as e:
+            log(f"[bench] adjoint with every stage state kept: not run ({e})")
+        finally:
+            batch.tape_free()
 
     def fields_after(vs):
         for s, p, v in zip(boxes, probs, vs):
@@ -318,7 +343,7 @@ def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-5, 1
             "steps": steps, "checkpoint_every": every, "forward_ms": fwd_ms, "backward_ms": back_ms,
             "backward_wall_ms": t_back * 1e3, "tape_forward_wall_ms": t_tape * 1e3,
             "adjoint_over_forward": back_ms / fwd_ms, "recompute_ms": recompute_ms,
-            "reverse_over_forward": (back_ms - recompute_ms) / fwd_ms,
+            "reverse_over_forward": (back_ms - recompute_ms) / fwd_ms, "all_states_kept": stored,
             "fd_rel_error_by_h": summary, "fd_error_over_term_sum_by_h": summary_terms, "directions": ndir}
 
 

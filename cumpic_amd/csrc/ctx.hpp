@@ -139,6 +139,7 @@ void batch_finddt(cumc_batch *b);
 void batch_step(cumc_batch *b, int par, int frozen);
 void batch_stage(cumc_batch *b, int st, int par);
 void batch_set_dt(cumc_batch *b, int m);
+void batch_dt_begin(cumc_batch *b, int frozen);
 void batch_after_run(cumc_batch *b, int nsteps);
 
 // ghost passes + torus clamp on the FP64 primitive planes (defined in cumc.hip) and on the

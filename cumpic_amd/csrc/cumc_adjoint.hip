@@ -1430,6 +1430,10 @@ struct batch_tape {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<hipEvent_t> seg_ev;  // start / end of each segment's forward pass in the backward
     double recompute_ms = 0.0;       // their total in the last backward pass
+    // the stage states of the last checkpoint segment, which the backward reverses first, are
+    // stored by tape_run's own forward pass (the segment buffers hold one segment), so the
+    // backward re-runs every segment but that one; with every >= nsteps nothing is re-run
+    bool last_stored = false;
 };
 
 void batch_tape_release(cumc_batch *b)
@@ -1628,10 +1632,26 @@ extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const d
     if (dt_in) BTCHK(b, hipMemcpyAsync(batch_hist(b), dt_in, sizeof(double) * (size_t)nsteps * nb, hipMemcpyHostToDevice, s));
     batch_set_counter(b, 0);
     if (!dt_in) batch_finddt(b);
+    // steps of the last segment store their stage states as the backward's re-run would
+    // (bit-identical: the same kernels on the same state and dt)
+    const int s_last = nsteps > 0 ? ((nsteps - 1) / every) * every : 0;
     for (int m = 0; m < nsteps; m++) {
-        batch_step(b, m & 1, dt_in != nullptr);
+        if (m < s_last) {
+            batch_step(b, m & 1, dt_in != nullptr);
+        } else {
+            const int par = m & 1, i = m - s_last;
+            if (i == 0) bt_copy(b, bt, CP_SAVE_SEG, 0, par);
+            batch_dt_begin(b, dt_in != nullptr);
+            batch_stage(b, 1, par);
+            bt_copy(b, bt, CP_SAVE_SA, i, 1 - par);
+            batch_stage(b, 2, par);
+            bt_copy(b, bt, CP_SAVE_SB, i, 0);
+            batch_stage(b, 3, par);
+            bt_copy(b, bt, CP_SAVE_SEG, i + 1, 1 - par);
+        }
         if ((m + 1) % every == 0) bt_copy(b, bt, CP_SAVE_CK, (m + 1) / every, (m + 1) & 1);
     }
+    bt->last_stored = nsteps > 0;
     bt_copy(b, bt, CP_SAVE_CK, bt->nck - 1, nsteps & 1);
     BTCHK(b, hipGetLastError());
     BTCHK(b, hipStreamSynchronize(s));
@@ -1689,11 +1709,15 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
     for (int s0 = ((N - 1) / k) * k; N > 0 && s0 >= 0; s0 -= k, seg++) {
         const int s1 = s0 + k < N ? s0 + k : N;
         BTCHK(b, hipEventRecord(bt->seg_ev[2 * seg], s));
-        bt_copy(b, bt, CP_LOAD_CK, s0 / k, 0);
-        bt_copy(b, bt, CP_SAVE_SEG, 0, 0);
         // segment forward: every stage's output state is stored, so the reverse below runs
-        // no forward kernel (stage-1 output U is in U1 for even steps, U0 for odd ones)
-        for (int m = s0; m < s1; m++) {
+        // no forward kernel (stage-1 output U is in U1 for even steps, U0 for odd ones); the
+        // last segment's states are still in the segment buffers from tape_run
+        const bool stored = seg == 0 && bt->last_stored;
+        if (!stored) {
+            bt_copy(b, bt, CP_LOAD_CK, s0 / k, 0);
+            bt_copy(b, bt, CP_SAVE_SEG, 0, 0);
+        }
+        for (int m = s0; m < s1 && !stored; m++) {
             const int i = m - s0, par = i & 1;
             batch_set_dt(b, m);
             batch_stage(b, 1, par);

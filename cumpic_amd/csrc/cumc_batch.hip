@@ -420,6 +420,12 @@ void batch_set_counter(cumc_batch *b, int v)
 void batch_finddt(cumc_batch *b) { enqueue_finddt_b(b); }
 void batch_step(cumc_batch *b, int par, int frozen) { enqueue_step_b(b, par, frozen); }
 void batch_stage(cumc_batch *b, int st, int par) { enqueue_stage_b(b, st, par); }
+// a step's dt bookkeeping (enqueue_step_b without its stages): record the found dt in the
+// history, or take the history entry when frozen
+void batch_dt_begin(cumc_batch *b, int frozen)
+{
+    hipLaunchKernelGGL(kb_dt_begin, dim3(1), dim3(256), 0, b->stream, b->d_desc, b->nbox, b->d_hist, b->d_ctr, frozen);
+}
 // current dt of every box := history entry of step m (the counter is left at m + 1)
 void batch_set_dt(cumc_batch *b, int m)
 {

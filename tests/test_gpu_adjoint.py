@@ -197,9 +197,14 @@ def test_autograd_function_matches_tape():
     np.testing.assert_allclose(v0.grad.numpy(), want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
 
 
-def test_batched_reverse_matches_single_box_tapes():
+@pytest.mark.parametrize("every", [2, 3, 5])
+def test_batched_reverse_matches_single_box_tapes(every):
+    """The batched reverse against per-box tapes (checkpoint every 2), with the batch
+    checkpointing every 2, 3 (last segment starts on an odd step) or 5 steps (one segment: its
+    stage states come from the taping forward and nothing is re-run), for frozen and for found
+    time steps."""
     from cumpic_amd.cumc import Batch, Solver
-    print("[case] batched reverse vs per-box tapes", flush=True)
+    print(f"[case] batched reverse vs per-box tapes, batch checkpoints every {every}", flush=True)
     probs = [ics.randomize(ics.spectral_box(n=n, seed=3600 + i, ndim=2, v_rms=0.3, db_rms=0.3), seed=i, amp=0.05)
              for i, n in enumerate((12, 20, 9))]
     probs.append(ics.randomize(ics.small_periodic(8, seed=3700), seed=7, amp=0.05))
@@ -213,7 +218,7 @@ def test_batched_reverse_matches_single_box_tapes():
         ref.append((gp, dts))
     boxes = [Solver.from_problem(p) for p in probs]
     b = Batch(boxes)
-    dts = b.tape_run(5, 2, dts=np.array([r[1] for r in ref]).T)
+    dts = b.tape_run(5, every, dts=np.array([r[1] for r in ref]).T)
     seeds = []
     for s, p in zip(boxes, probs):
         st = s.download()
@@ -224,9 +229,14 @@ def test_batched_reverse_matches_single_box_tapes():
     grads = b.tape_backward(seeds)
     for i, (g, r) in enumerate(zip(grads, ref)):
         assert np.array_equal(g, r[0]), f"box {i}"
-    # non-frozen run reproduces the per-box CFL sequences
+    if every >= 5:
+        assert b.tape_recompute_ms() < 0.05 * b.last_ms()
+    # non-frozen run reproduces the per-box CFL sequences, and the same gradients
     for s, p in zip(boxes, probs):
         s.upload(p.prim, p.eps)
-    d2 = b.tape_run(5, 2)
+    d2 = b.tape_run(5, every)
     assert np.array_equal(d2, np.array([r[1] for r in ref]).T)
+    grads = b.tape_backward(seeds)
+    for i, (g, r) in enumerate(zip(grads, ref)):
+        assert np.array_equal(g, r[0]), f"box {i}, found time steps"
     assert b.last_ms() >= 0.0
