@@ -914,6 +914,9 @@ struct cumc_tape {
     double *adj = nullptr;     // NADJ planes
     double *lu[3] = {nullptr, nullptr, nullptr};  // 8 planes each: U adjoints by role U0, U1, Ub
     AdjF A;
+    // seg / sa / sb hold the last checkpoint segment, stored by tape_run's own forward pass, so
+    // the backward does not re-run that segment (with every >= nsteps it re-runs nothing)
+    bool last_stored = false;
 };
 static constexpr int SW = 20;  // planes of a stored state
 
@@ -1101,6 +1104,8 @@ extern "C" int cumc_tape_run(cumc_ctx *ctx, int nsteps, int every, const double 
     if ((r = cumc_initial_update(ctx))) return r;
     tp->dts.assign(nsteps, 0.0);
     save_state(ctx, tp->ckpt);
+    // the last segment's steps store their stage states as the backward's re-run would
+    const int s_last = nsteps > 0 ? ((nsteps - 1) / every) * every : 0;
     for (int m = 0; m < nsteps; m++) {
         double dt;
         if (dt_in) {
@@ -1110,9 +1115,23 @@ extern "C" int cumc_tape_run(cumc_ctx *ctx, int nsteps, int every, const double 
             return r;
         }
         tp->dts[m] = dt;
-        forward_step(ctx, dt);
+        if (m < s_last) {
+            forward_step(ctx, dt);
+        } else {
+            const size_t at = (size_t)SW * ps * (m - s_last);
+            if (m == s_last) save_state(ctx, tp->seg);
+            set_dt(ctx, dt);
+            stage<double>(ctx, ctx->F64, 1);
+            save_pool_u(ctx, tp->sa + at, ctx->F64.U1);
+            stage<double>(ctx, ctx->F64, 2);
+            save_pool_u(ctx, tp->sb + at, ctx->Ub);
+            stage<double>(ctx, ctx->F64, 3);
+            swap_u(ctx);
+            save_state(ctx, tp->seg + at + (size_t)SW * ps);
+        }
         if ((m + 1) % every == 0) save_state(ctx, tp->ckpt + (size_t)SW * ps * ((m + 1) / every));
     }
+    tp->last_stored = nsteps > 0;
     save_state(ctx, tp->ckpt + (size_t)SW * ps * (tp->nck - 1));
     ctx->dt_valid = false;
     HIPCHK(ctx, hipGetLastError());
@@ -1169,9 +1188,12 @@ extern "C" int cumc_tape_backward(cumc_ctx *ctx, const double *seed_prim, const 
     const int N = tp->nsteps, k = tp->every;
     for (int s0 = ((N - 1) / k) * k; N > 0 && s0 >= 0; s0 -= k) {
         const int s1 = s0 + k < N ? s0 + k : N;
-        load_state(ctx, tp->ckpt + (size_t)SW * ps * (s0 / k));
-        save_state(ctx, tp->seg);
-        for (int m = s0; m < s1; m++) {
+        const bool stored = s0 == ((N - 1) / k) * k && tp->last_stored;
+        if (!stored) {
+            load_state(ctx, tp->ckpt + (size_t)SW * ps * (s0 / k));
+            save_state(ctx, tp->seg);
+        }
+        for (int m = s0; m < s1 && !stored; m++) {
             const size_t at = (size_t)SW * ps * (m - s0);
             set_dt(ctx, tp->dts[m]);
             stage<double>(ctx, ctx->F64, 1);
