@@ -76,3 +76,13 @@ def test_batch_boxes_sizes_are_seeded():
     assert a == b and set(a) <= {8, 12}
     p = ics.batch_boxes(nbox=1, sizes=(8,))[0]
     assert p.cfg.nz == 1 and p.cfg.ny == 8 and p.prim.shape == (7, 14, 14, 8)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """The product path has no CPU fallback: without the built HIP library every entry point
+    raises instead of computing anything."""
+    from cumpic_amd import cumc, ics
+    monkeypatch.setattr(cumc, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(cumc, "_lib", None)
+    with pytest.raises(cumc.CumcError, match="not built"):
+        cumc.Solver.from_problem(ics.small_periodic(8))
