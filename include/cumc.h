@@ -181,15 +181,17 @@ const char *cumc_batch_last_error(cumc_batch *b);
  * ------------------------------------------------------------------------------------------ */
 
 /* From the state as uploaded (not yet initial_update'd): run initial_update and nsteps steps,
- * storing the state every `every` steps on the device.  dt_in (nsteps) freezes the steps; NULL
- * takes each step's finddt.  dt_out (may be NULL) receives the steps.  Leaves the context at the
- * final state. */
+ * storing the state every `every` steps on the device, and every stage's state in the last
+ * `every`-step segment.  Device memory: about 20 * (nsteps/every + 3*every + 8) state planes.
+ * dt_in (nsteps) freezes the steps; NULL takes each step's finddt.  dt_out (may be NULL)
+ * receives the steps.  Leaves the context at the final state. */
 int cumc_tape_run(cumc_ctx *ctx, int nsteps, int every, const double *dt_in, double *dt_out);
 
 /* Adjoint of the taped map.  Seeds: adjoints of the final prim / eps / bcell (same host layouts
  * as cumc_download; NULL = zero).  Outputs: gradients with respect to the uploaded prim / eps
- * (cumc_upload layouts; NULL = not wanted).  Recomputes each `every`-step segment from its
- * checkpoint; leaves the context at the taped final state. */
+ * (cumc_upload layouts; NULL = not wanted).  Recomputes each `every`-step segment but the last
+ * from its checkpoint (every >= nsteps: nothing is recomputed); leaves the context at the taped
+ * final state. */
 int cumc_tape_backward(cumc_ctx *ctx, const double *seed_prim, const double *seed_eps, const double *seed_bcell,
                        double *grad_prim, double *grad_eps);
 int cumc_tape_free(cumc_ctx *ctx);
