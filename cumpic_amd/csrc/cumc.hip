@@ -114,16 +114,16 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     const bool mixed = c.precision == CUMC_MIXED;
     const size_t rsz = mixed ? sizeof(float) : sizeof(double);
     const size_t fbytes = (mixed ? 36 : 24) * (size_t)L.ps * rsz;
-    hipError_t e = hipMalloc(&ctx->pool, ctx->pool_bytes);
-    if (e != hipSuccess) { delete ctx; return CUMC_ENOMEM; }
-    e = hipMalloc(&ctx->fluxpool, fbytes);
-    if (e != hipSuccess) { hipFree(ctx->pool); delete ctx; return CUMC_ENOMEM; }
-    if (hipMalloc(&ctx->dt_dev, 64) != hipSuccess || hipMalloc(&ctx->red, 64) != hipSuccess) {
-        hipFree(ctx->pool); hipFree(ctx->fluxpool); delete ctx; return CUMC_ENOMEM;
+    // every failure from here on goes through cumc_destroy, which frees whatever was created
+    if (hipMalloc(&ctx->pool, ctx->pool_bytes) != hipSuccess || hipMalloc(&ctx->fluxpool, fbytes) != hipSuccess ||
+        hipMalloc(&ctx->dt_dev, 64) != hipSuccess || hipMalloc(&ctx->red, 64) != hipSuccess) {
+        cumc_destroy(ctx);
+        return CUMC_ENOMEM;
     }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
-        hipFree(ctx->pool); hipFree(ctx->fluxpool); delete ctx; return CUMC_EHIP;
+        cumc_destroy(ctx);
+        return CUMC_EHIP;
     }
     // zero everything once: ghost EMFs at index -1 (mhd.f90:142-148) and initial bcell stay zero
     hipMemset(ctx->pool, 0, ctx->pool_bytes);
@@ -229,9 +229,15 @@ extern "C" int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, 
     if (!ctx || !xf || !yf || !zf) return CUMC_EINVAL;
     const double *fs[3] = {xf, yf, zf};
     const int ns[3] = {ctx->L.nx, ctx->L.ny, ctx->L.nz};
-    for (int a = 0; a < 3; a++) {
+    // every check precedes the first upload, so a rejected grid leaves the previous one in place
+    for (int a = 0; a < 3; a++)
         for (int j = 1; j < ns[a] + 7; j++)
             if (!(fs[a][j] > fs[a][j - 1])) return fail(ctx, CUMC_EINVAL, "face coordinates must increase");
+    if (ctx->cfg.coordinate == CUMC_CYLINDRICAL && !(xf[3] > 0.0))
+        return fail(ctx, CUMC_EINVAL, "cylindrical grid needs R > 0 on the inner face");
+    if (ctx->cfg.coordinate == CUMC_SPHERICAL && !(xf[3] > 0.0 && yf[3] > 0.0 && yf[ctx->L.ny + 3] < 3.14159265358979323846))
+        return fail(ctx, CUMC_EINVAL, "spherical grid needs r > 0 and 0 < theta < pi on the boundary faces");
+    for (int a = 0; a < 3; a++) {
         std::vector<double> t = build_tables(fs[a], ns[a]);
         if (!ctx->tab[a]) HIPCHK(ctx, hipMalloc(&ctx->tab[a], t.size() * sizeof(double)));
         HIPCHK(ctx, hipMemcpy(ctx->tab[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -239,10 +245,6 @@ extern "C" int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, 
         if (!ctx->tabf[a]) HIPCHK(ctx, hipMalloc(&ctx->tabf[a], tf.size() * sizeof(float)));
         HIPCHK(ctx, hipMemcpy(ctx->tabf[a], tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
     }
-    if (ctx->cfg.coordinate == CUMC_CYLINDRICAL && !(xf[3] > 0.0))
-        return fail(ctx, CUMC_EINVAL, "cylindrical grid needs R > 0 on the inner face");
-    if (ctx->cfg.coordinate == CUMC_SPHERICAL && !(xf[3] > 0.0 && yf[3] > 0.0 && yf[ctx->L.ny + 3] < 3.14159265358979323846))
-        return fail(ctx, CUMC_EINVAL, "spherical grid needs r > 0 and 0 < theta < pi on the boundary faces");
     ctx->grid_set = true;
     return CUMC_OK;
 }

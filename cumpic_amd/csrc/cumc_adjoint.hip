@@ -1206,6 +1206,9 @@ extern "C" int cumc_tape_backward(cumc_ctx *ctx, const double *seed_prim, const 
         }
         for (int m = s1 - 1; m >= s0; m--) reverse_step(ctx, m - s0, tp->dts[m]);
         HIPCHK(ctx, hipGetLastError());
+        // re-running an earlier segment overwrote the segment buffers: a later backward pass on
+        // this tape must re-run the last segment as well
+        if (!stored) tp->last_stored = false;
     }
     // initial_update: its ghost fill, then k_init on the uploaded state (zero ghost bcell)
     const double *S0 = tp->ckpt;
@@ -1759,6 +1762,8 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
             q ^= 1;
         }
         BTCHK(b, hipGetLastError());
+        // the segment buffers now hold a re-run segment, not the last one
+        if (!stored) bt->last_stored = false;
     }
     // initial_update: its ghost fill, then k_init on the uploaded state
     bt_ghost(b, bt, 0, 0);

@@ -182,8 +182,13 @@ struct cumc_batch {
     std::vector<int> h_pre;
     int total[P_N] = {0};
     cumc_tuning tune{};
+    // A captured graph bakes in the kernel arguments (descriptor, work-list and history pointers,
+    // parity) and every phase's grid size; the descriptors themselves are read on the device at
+    // replay.  A graph is reused only while its grid sizes and history pointer are current.
     struct Graph {
         int steps, frozen, par;
+        unsigned grid[P_N];
+        double *hist;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -236,9 +241,13 @@ static void bcollect(cumc_batch *b)
     b->evn = 0;
 }
 
+// Destroys the cached graph execs once the batch stream is idle (a replay still in flight keeps
+// using its exec until it completes).
 static void clear_graphs(cumc_batch *b)
 {
-    for (auto &g : b->graphs) hipGraphExecDestroy(g.exec);
+    if (b->graphs.empty()) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.exec);
     b->graphs.clear();
 }
 
@@ -442,11 +451,21 @@ void batch_after_run(cumc_batch *b, int nsteps)
 
 static int get_graph(cumc_batch *b, int steps, int frozen, int par, hipGraphExec_t *out)
 {
-    for (auto &g : b->graphs)
-        if (g.steps == steps && g.frozen == frozen && g.par == par) {
+    unsigned grid[P_N];
+    for (int p = 0; p < P_N; p++) grid[p] = grid_of(b, p);
+    for (size_t i = 0; i < b->graphs.size(); i++) {
+        auto &g = b->graphs[i];
+        if (g.steps != steps || g.frozen != frozen || g.par != par) continue;
+        if (g.hist == b->d_hist && std::memcmp(g.grid, grid, sizeof(grid)) == 0) {
             *out = g.exec;
             return CUMC_OK;
         }
+        // captured under other grid sizes or another history buffer: drop it once idle
+        BCHK(b, hipStreamSynchronize(b->stream));
+        BCHK(b, hipGraphExecDestroy(g.exec));
+        b->graphs.erase(b->graphs.begin() + (long)i);
+        break;
+    }
     hipGraph_t graph = nullptr;
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < steps; i++) enqueue_step_b(b, (par + i) & 1, frozen);
@@ -455,7 +474,9 @@ static int get_graph(cumc_batch *b, int steps, int frozen, int par, hipGraphExec
     hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
     BCHK(b, e);
-    b->graphs.push_back({steps, frozen, par, exec});
+    cumc_batch::Graph ng{steps, frozen, par, {}, b->d_hist, exec};
+    std::memcpy(ng.grid, grid, sizeof(grid));
+    b->graphs.push_back(ng);
     *out = exec;
     return CUMC_OK;
 }
@@ -599,7 +620,7 @@ extern "C" const char *cumc_batch_last_error(cumc_batch *b) { return b ? b->err.
 
 // ---------------------------------------------------------------------------------------------
 // Tuner: saves every box's FP64 fields and dt slots, times candidate shapes phase by phase with
-// HIP events on direct launches (two steps per candidate after a warm-up step), keeps the
+// HIP events on direct launches (two steps per candidate after two warm-up steps), keeps the
 // fastest per phase and restores the saved state.
 // ---------------------------------------------------------------------------------------------
 static int time_candidate(cumc_batch *b, const cumc_tuning &t, double ph[P_N])
@@ -609,12 +630,15 @@ static int time_candidate(cumc_batch *b, const cumc_tuning &t, double ph[P_N])
     if (r) return r;
     hipLaunchKernelGGL(kb_set_int, dim3(1), dim3(64), 0, b->stream, b->d_ctr, 0);
     enqueue_finddt_b(b);
+    // an even number of steps in parity order 0, 1, 0, 1: the conserved buffers end in the
+    // roles they started in, so every candidate times a consistent state
     enqueue_step_b(b, 0, 0);  // warm-up
+    enqueue_step_b(b, 1, 0);
     b->prof = true;
     b->evn = 0;
     for (int p = 0; p < P_N; p++) b->ph_ms[p] = 0.0;
-    enqueue_step_b(b, 1, 0);
     enqueue_step_b(b, 0, 0);
+    enqueue_step_b(b, 1, 0);
     b->prof = false;
     BCHK(b, hipGetLastError());
     BCHK(b, hipStreamSynchronize(b->stream));
@@ -628,7 +652,7 @@ extern "C" int cumc_batch_autotune(cumc_batch *b, cumc_tuning *out)
     if (!b) return CUMC_EINVAL;
     const int nb = b->nbox;
     for (cumc_ctx *c : b->box) BCHK(b, hipStreamSynchronize(c->stream));
-    if (int r = reserve_hist(b, 3)) return r;
+    if (int r = reserve_hist(b, 4)) return r;
     // save
     std::vector<void *> save(nb, nullptr);
     std::vector<double> dts(2 * (size_t)nb);

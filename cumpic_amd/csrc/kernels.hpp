@@ -520,7 +520,10 @@ __device__ inline double cell_dt(const Tabs &tb, const Model &M, int j, int k, i
         dzl = tb.at(0, TB_C, j) * tb.at(1, TB_SINC, k) * dzl;
     }
     const double dt_out = fmin2<double>(fmin2<double>(dxl, dyl), dzl) * M.cfl / lambda;
-    return fmin2<double>(dt_out, 1.0e10);
+    // a NaN cell step becomes 0: every min reduction then ends at 0 (a NaN would drop out of
+    // the comparisons), and the host reports the non-finite state
+    const double d = fmin2<double>(dt_out, 1.0e10);
+    return d == d ? d : 0.0;
 }
 
 __device__ inline double block_min(double v)

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 
 import numpy as np
 
@@ -182,7 +183,13 @@ def _spectral_field(shape, kmax: float, slope: float, rng, kz_zero: bool = False
                     continue
                 amp = math.sqrt(k2) ** slope
                 coeff[iy % ny, ixx % nx, iz] = amp * (rng.standard_normal() + 1j * rng.standard_normal())
-    f = np.fft.irfftn(coeff, s=(ny, nx, nz), axes=(0, 1, 2))  # [k, j, l]
+    if ny * nx * nz >= 384 ** 3:
+        # large boxes: scipy's multithreaded transform (same coefficients; the sampled field
+        # differs from numpy's transform in the last bits, and every consumer is given this array)
+        import scipy.fft
+        f = scipy.fft.irfftn(coeff, s=(ny, nx, nz), axes=(0, 1, 2), workers=os.cpu_count() or 1)
+    else:
+        f = np.fft.irfftn(coeff, s=(ny, nx, nz), axes=(0, 1, 2))  # [k, j, l]
     f = np.ascontiguousarray(np.transpose(f, (2, 0, 1)))  # [l, k, j]
     rms = math.sqrt(float(np.mean(f * f)))
     return f / rms if rms > 0 else f

@@ -35,6 +35,9 @@ struct orc {
     double *eps;
     double *bcell, *ecell, *ecorn;
     int bfac_xin[NV], bfac_xout[NV], bfac_yin[NV], bfac_yout[NV], bfac_zin[NV], bfac_zout[NV];
+    /* cells CUSTOM_CHECKRHO changed since creation (test instrumentation, no reference
+       counterpart): density raised to the floor, eps raised to its floor (or to 0) */
+    long nfl_rho, nfl_eps;
 };
 
 #define ID(j, k, l) ((((long)(l) + 2) * o->sy + ((long)(k) + 2)) * o->sx + ((long)(j) + 2))
@@ -743,17 +746,27 @@ void orc_get_lrk(const orc *o, double *l_rk) { memcpy(l_rk, o->l_rk, sizeof(doub
 static void custom_checkrho(orc *o)
 {
     const int nx = o->nx, ny = o->ny, nz = o->nz;
-#pragma omp parallel for collapse(2) schedule(static)
+    long nr = 0, ne = 0;
+#pragma omp parallel for collapse(2) schedule(static) reduction(+ : nr, ne)
     for (int l = 1; l <= nz; l++)
         for (int k = 1; k <= ny; k++)
             for (int j = 1; j <= nx; j++) {
                 if (o->c.model != 0) {
-                    if (PR(IRHO, j, k, l) < o->c.rho_floor) PR(IRHO, j, k, l) = o->c.rho_floor;
-                    if (EPS(j, k, l) < o->c.eps_floor) EPS(j, k, l) = o->c.eps_floor;
+                    if (PR(IRHO, j, k, l) < o->c.rho_floor) { PR(IRHO, j, k, l) = o->c.rho_floor; nr++; }
+                    if (EPS(j, k, l) < o->c.eps_floor) { EPS(j, k, l) = o->c.eps_floor; ne++; }
                 } else {
+                    ne += EPS(j, k, l) < 0.0;
                     EPS(j, k, l) = fmax2(EPS(j, k, l), 0.0);
                 }
             }
+    o->nfl_rho += nr;
+    o->nfl_eps += ne;
+}
+
+void orc_floor_counts(const orc *o, long *rho, long *eps)
+{
+    *rho = o->nfl_rho;
+    *eps = o->nfl_eps;
 }
 
 /* torus_cyn/src/custom.f90:121-166 and torus_sph/src/custom.f90:121-158 (Paczynski-Wiita gravity:
@@ -1361,6 +1374,9 @@ double orc_finddt(orc *o)
                     dz_loc = cxc(o, j) * sin(cyc(o, k)) * dz_loc;
                 }
                 double dt_out = fmin3(dx_loc, dy_loc, dz_loc) * cfl / lambda;
+                /* a NaN cell (NaN/Inf state) gives 0, so the minimum reports it whatever the
+                   reduction order; Fortran's MIN leaves NaN arguments processor-dependent */
+                if (dt_out != dt_out) dt_out = 0.0;
                 dt_temp = fmin2(dt_out, dt_temp);
             }
     return dt_temp;

@@ -52,6 +52,8 @@ void orc_spatial(orc *o);
 void orc_boundary(orc *o);
 void orc_sweep_flux(orc *o, int dir, double *out);
 void orc_find_divb(const orc *o, double *max_signed, double *max_abs);
+/* test instrumentation: cells whose density / eps CUSTOM_CHECKRHO raised to a floor so far */
+void orc_floor_counts(const orc *o, long *rho, long *eps);
 /* main.f90:127-163 with a step cap: returns steps taken, fills dts (may be NULL) */
 int orc_run(orc *o, int max_steps, double total_time, double *dts, double *t_out);
 

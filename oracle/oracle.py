@@ -56,6 +56,7 @@ def lib():
         L.orc_spatial.argtypes = [ctypes.c_void_p]
         L.orc_boundary.argtypes = [ctypes.c_void_p]
         L.orc_find_divb.argtypes = [ctypes.c_void_p, _D, _D]
+        L.orc_floor_counts.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
         L.orc_sweep_flux.argtypes = [ctypes.c_void_p, ctypes.c_int, _D]
         L.orc_run.restype = ctypes.c_int
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _D, _D]
@@ -167,6 +168,13 @@ class Oracle:
         a = ctypes.c_double()
         b = ctypes.c_double()
         lib().orc_find_divb(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def floor_counts(self):
+        """(density, eps) floor applications by CUSTOM_CHECKRHO since creation."""
+        a = ctypes.c_long()
+        b = ctypes.c_long()
+        lib().orc_floor_counts(self.h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
 
     def run(self, max_steps: int, total_time: float = 1.0e300, t0: float = 0.0):

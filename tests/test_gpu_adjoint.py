@@ -240,3 +240,43 @@ def test_batched_reverse_matches_single_box_tapes(every):
     for i, (g, r) in enumerate(zip(grads, ref)):
         assert np.array_equal(g, r[0]), f"box {i}, found time steps"
     assert b.last_ms() >= 0.0
+
+
+def test_second_backward_on_one_tape_is_identical():
+    """A tape stays valid until it is freed: a second backward pass on the same tape (as torch's
+    retain_graph or several vector-Jacobian products make) gives the same gradients bit for bit,
+    also when the first pass re-ran earlier checkpoint segments into the segment buffers
+    (nsteps 5, checkpoints every 2: segments [4], [2,3], [0,1])."""
+    from cumpic_amd.cumc import Batch, Solver
+    print("[case] two backward passes on one tape, single context and batch", flush=True)
+    probs = [ics.randomize(ics.spectral_box(n=n, seed=3900 + i, ndim=2, v_rms=0.3, db_rms=0.3), seed=i, amp=0.05)
+             for i, n in enumerate((12, 10))]
+    p = probs[0]
+    s = Solver.from_problem(p)
+    s.upload(p.prim, p.eps)
+    s.tape_run(5, 2)
+    seed = np.zeros(p.cfg.shape + (3,))
+    inner = _inner(p.cfg)
+    seed[inner] = s.download()["bcell"][inner] * _dv(p)
+    g1, e1 = s.tape_backward(seed_bcell=seed)
+    g2, e2 = s.tape_backward(seed_bcell=seed)
+    assert np.abs(g1).max() > 0
+    assert np.array_equal(g1, g2) and np.array_equal(e1, e2)
+    s.close()
+    boxes = [Solver.from_problem(q) for q in probs]
+    b = Batch(boxes)
+    b.tape_run(5, 2)
+    seeds = []
+    for bx, q in zip(boxes, probs):
+        sd = np.zeros(q.cfg.shape + (3,))
+        inn = _inner(q.cfg)
+        sd[inn] = bx.download()["bcell"][inn] * _dv(q)
+        seeds.append(sd)
+    ga = b.tape_backward(seeds)
+    gb = b.tape_backward(seeds)
+    for i, (x, y) in enumerate(zip(ga, gb)):
+        assert np.abs(x).max() > 0 and np.array_equal(x, y), f"box {i}"
+    assert np.array_equal(ga[0], g1)
+    b.close()
+    for bx in boxes:
+        bx.close()
