@@ -14,17 +14,34 @@ It also runs the batched form: 48 independent 2D periodic boxes of sizes drawn b
 at a time, by persistent work-list launches, and by a replayed hipGraph of 8 steps after the
 launch-shape tuner, with a bit-for-bit check of the batched result against the one-at-a-time one.
 
+The default run adds the mixed-precision workload at its own size (`c2`: the same seeded
+spectrum on the 512^3 box, 100 FP64 steps, then the mixed form on the FP64 run's dt sequence,
+relative L1 error of rho, p, |v|, |B| and max |div B| of both forms) and measures the stage
+tail's HBM traffic per launch in the same run: two child processes run two steps of the same
+state under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`, calibrated by a 1 GiB device
+copy in the same child.
+
   --workload torus      cylindrical torus 256x128x256, 50 steps (torus_cyn hooks)
   --n 512 --steps 100   the 512^3 box of the mixed-precision workload
   --no-batch            skip the batched workload
+  --no-c2               skip the 512^3 mixed-precision leg
+  --no-traffic          skip the counter children
   --workload batch      only the batched workload (value = its hipGraph rate)
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import dataclasses
+import glob
 import json
 import os
+import re
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -34,8 +51,14 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 BYTES_PER_ZONE_CYCLE = 512.0  # 64 FP64 state words per RK3 step
-# algorithmic bytes per unit of each kernel class (FP64 form): compulsory loads + stores
-KERNEL_BYTES = {
+# Algorithmic bytes of one stage-tail launch: the RK3 state traffic of one stage, 512/3 B per
+# interior zone (stage 1 reads U0 and writes U1, stages 2 and 3 read U0 and the previous stage and
+# write the next: 16 + 24 + 24 FP64 words per zone and step).  The mixed form keeps the FP64 state.
+TAIL_ALG_BYTES_PER_ZONE = BYTES_PER_ZONE_CYCLE / 3.0
+# bytes each kernel class must load and store per unit as written (its inputs and outputs,
+# including the flux, EMF and primitive planes that pass between kernels): reported beside the
+# algorithmic figure, not as it
+KERNEL_IO_BYTES = {
     "fp64": {
         "flux_x": 15 * 8, "flux_y": 15 * 8, "flux_z": 15 * 8,  # 7 stencil vars + normal B in, 7 fluxes out
         "ecorn": 15 * 8,                                        # 6 B-fluxes + v + cell B in, 3 EMFs out
@@ -52,6 +75,8 @@ KERNEL_BYTES = {
         "tail": ((18 * 4 + 8 * 8 + 8 * 8 + 12 * 4) + 2 * (18 * 4 + 16 * 8 + 8 * 8 + 12 * 4)) / 3,
     },
 }
+# kernels of the time step (per form) whose counter bytes make up the step traffic
+STEP_KERNELS = re.compile(r"k_(flux_x|flux_walk|ecorn|tail|stage_z|ghost32|ghost|fill_u64|custom_boundary)")
 
 
 def log(msg):
@@ -69,19 +94,85 @@ def kernel_units(name, cfg):
     }.get(name)
 
 
-def read_traffic(kernel, tag):
-    """HBM bytes per launch of a kernel class from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.txt")
-    if not os.path.exists(path):
+def _counter_rows(outdir):
+    """(dispatch id, kernel name, counter value) of every dispatch in a rocprofv3 CSV counter
+    collection, in dispatch order."""
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                name = r.get("Kernel_Name") or ""
+                val = r.get("Counter_Value")
+                if name and val not in (None, ""):
+                    rows.append((int(r.get("Dispatch_Id") or 0), name, float(val)))
+    rows.sort()
+    return rows
+
+
+def pmc_traffic(ic_dir, form, steps=2, timeout=300):
+    """Counter bytes measured in this run for one form: a child process runs `steps` RK3 steps of
+    the saved state under rocprofv3 --pmc FETCH_SIZE, another under --pmc WRITE_SIZE; each child
+    first copies 1 GiB with cumc_copy_probe, whose counter readings give the KiB-to-byte factors.
+    Returns the stage tail's bytes per launch and the whole step's bytes per zone-cycle (every
+    dispatch from the first sweep on), or None."""
+    rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(rp):
+        log("[bench] traffic: rocprofv3 not found, traffic left null")
         return None
-    for line in open(path):
-        parts = line.split()
-        if len(parts) >= 3 and parts[0] == tag and parts[1] == kernel:
-            try:
-                return float(parts[2])
-            except ValueError:
-                return None
-    return None
+    res = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="cumc_pmc_")
+        cmd = [rp, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "pmc", "--", sys.executable,
+               os.path.abspath(__file__), "--traffic-child", ic_dir, "--forms", form, "--steps", str(steps)]
+        log(f"[bench] traffic: {form} {ctr} pass under rocprofv3")
+        p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, start_new_session=True)
+        try:
+            _, err = p.communicate(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.communicate()
+            log(f"[bench] traffic: {ctr} pass timed out, traffic left null")
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        rows = _counter_rows(d)
+        shutil.rmtree(d, ignore_errors=True)
+        cp = [v for _, n, v in rows if "k_copy_probe" in n]
+        first = next((i for i, (_, n, _) in enumerate(rows) if "k_flux_x" in n), None)
+        if not cp or first is None:
+            log(f"[bench] traffic: {ctr} pass left no usable counters (exit {p.returncode}): "
+                f"{err.decode(errors='replace')[-400:]}")
+            return None
+        tail = [v for _, n, v in rows if "k_tail<" in n]
+        step = [v for _, n, v in rows[first:] if STEP_KERNELS.search(n)]
+        res[ctr] = (float(1 << 30) / 1024.0 / cp[0], sum(tail) / len(tail), len(tail), sum(step))
+    (ff, tf, nt, sf), (fw, tw, _, sw) = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    return {"tail_bytes_per_launch": (ff * tf + fw * tw) * 1024.0, "tail_launches": nt,
+            "step_bytes": (ff * sf + fw * sw) * 1024.0 / steps, "fetch_factor": ff, "write_factor": fw,
+            "counters": "rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE, KiB per dispatch x factor x 1024"}
+
+
+def traffic_child(args):
+    """Body of a counter pass: calibration copy, then `steps` RK3 steps of the saved state."""
+    from cumpic_amd.config import FP64, MIXED, Config
+    from cumpic_amd.cumc import Solver, copy_probe
+    from cumpic_amd.ics import Problem
+    meta = json.load(open(os.path.join(args.traffic_child, "meta.json")))
+    cfg = Config(**{k: tuple(v) if isinstance(v, list) else v for k, v in meta["cfg"].items()})
+    ld = lambda n: np.load(os.path.join(args.traffic_child, n + ".npy"))  # noqa: E731
+    prob = Problem(cfg, ld("xf"), ld("yf"), ld("zf"), ld("prim"), ld("eps"))
+    copy_probe(1 << 30, reps=1)
+    s = Solver.from_problem(prob, precision=FP64 if args.forms == "fp64" else MIXED)
+    s.initial_update()
+    s.run(args.steps)
+    s.close()
+
+
+def save_ic(prob):
+    d = tempfile.mkdtemp(prefix="cumc_ic_")
+    for n in ("xf", "yf", "zf", "prim", "eps"):
+        np.save(os.path.join(d, n + ".npy"), getattr(prob, n))
+    json.dump({"cfg": dataclasses.asdict(prob.cfg)}, open(os.path.join(d, "meta.json"), "w"))
+    return d
 
 
 def cpu_baseline(prob, steps=1, one_core_n=128):
@@ -142,18 +233,25 @@ def timed_run(Solver, prob, prec, steps, warmup, tune=True):
     return s, dts, wall, kt, tuning
 
 
-def roofline(kt, cfg, tag, profiled=True):
-    """profiled: the workload is the one profiles/pmc_traffic.txt was measured on (the 256^3 box);
-    elsewhere `traffic` is null."""
-    table = KERNEL_BYTES[tag]
-    dom = max((k for k in kt if k in table), key=lambda k: kt[k][0])
-    tot_ms, launches = kt[dom]
-    avg_s = tot_ms / launches * 1e-3
-    alg = table[dom] * kernel_units(dom, cfg)
-    ach = alg / avg_s
-    return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": ach / HBM_PEAK,
-            "traffic": read_traffic(dom, tag) if profiled else None, "kernel": dom,
-            "avg_launch_ms": tot_ms / launches}
+def roofline(kt, cfg, tag, traffic=None):
+    """Roofline of the stage tail, the kernel that takes the most time: algorithmic bytes per
+    launch = 512/3 B per interior zone (one stage's share of the RK3 state traffic) over its
+    average HIP-event launch time, against 8 TB/s.  `traffic` is the HBM bytes per launch that
+    the counters of this run measured (None: not measured); kernel_io gives the same launch
+    measured against the bytes the kernel loads and stores as written."""
+    tail_ms, launches = kt["tail"]
+    avg_s = tail_ms / launches * 1e-3
+    alg = TAIL_ALG_BYTES_PER_ZONE * cfg.zones
+    io = KERNEL_IO_BYTES[tag]["tail"] * kernel_units("tail", cfg)
+    dom = max(kt, key=lambda k: kt[k][0])
+    t = traffic["tail_bytes_per_launch"] if traffic else None
+    return {"bound": "hbm", "achieved": alg / avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": alg / avg_s / HBM_PEAK, "traffic": t, "kernel": "tail", "dominant_by_time": dom,
+            "algorithmic_bytes_per_launch": alg,
+            "algorithmic_basis": "512/3 B per interior zone: one RK3 stage of the FP64 state traffic",
+            "traffic_over_algorithmic": (t / alg) if t else None, "avg_launch_ms": tail_ms / launches,
+            "kernel_io": {"bytes_per_launch": io, "achieved_GBps": io / avg_s / 1e9, "frac": io / avg_s / HBM_PEAK,
+                          "basis": "the tail's own loads and stores (flux, EMF, state and primitive planes)"}}
 
 
 def fields(state, cfg):
@@ -161,6 +259,56 @@ def fields(state, cfg):
     p, b = state["prim"][sl], state["bcell"][sl]
     return {"rho": p[..., 0], "p": p[..., 4], "|v|": np.linalg.norm(p[..., 1:4], axis=-1),
             "|B|": np.linalg.norm(b, axis=-1)}
+
+
+def bench_c2(n=512, steps=100, warmup=2):
+    """The mixed-precision workload at its stated size: the 256^3 box's seeded spectrum on the
+    512^3 box, `steps` FP64 steps (after `warmup` and the tuner), then the mixed form from the same
+    initial state on the FP64 run's dt sequence (warm-up steps included).  Reports both rates, the
+    speed-up, the relative L1 error of rho, p, |v|, |B| against FP64 and max |div B| of both."""
+    from cumpic_amd import ics
+    from cumpic_amd.config import FP64, MIXED
+    from cumpic_amd.cumc import Solver
+    log(f"[bench] c2: building the {n}^3 spectral initial condition")
+    t0 = time.perf_counter()
+    prob = ics.spectral_box(n=n, seed=1001, steps=steps)
+    log(f"[bench] c2: initial condition in {time.perf_counter() - t0:.1f} s")
+    cfg = prob.cfg
+    s = Solver.from_problem(prob, precision=FP64)
+    s.initial_update()
+    warm = s.run(warmup)[0] if warmup else np.zeros(0)
+    tune64 = s.autotune()
+    t0 = time.perf_counter()
+    dts, _ = s.run(steps)
+    w64 = time.perf_counter() - t0
+    _, abs64 = s.find_divb()
+    ref = fields(s.download(), cfg)
+    s.close()
+    log(f"[bench] c2: FP64 {cfg.zones * steps / w64:.4e} zone-cycles/s ({w64 / steps * 1e3:.2f} ms/step)")
+    m = Solver.from_problem(prob, precision=MIXED)
+    m.initial_update()
+    for dt in warm:
+        m.rungekutta(float(dt))
+    tunem = m.autotune()
+    t0 = time.perf_counter()
+    for dt in dts:
+        m.rungekutta(float(dt))
+    wm = time.perf_counter() - t0
+    _, absm = m.find_divb()
+    mf = fields(m.download(), cfg)
+    m.close()
+    errs = {k: float(np.sum(np.abs(mf[k] - ref[k])) / np.sum(np.abs(ref[k]))) for k in mf}
+    del mf, ref
+    log(f"[bench] c2: mixed {cfg.zones * steps / wm:.4e} zone-cycles/s ({wm / steps * 1e3:.2f} ms/step), "
+        f"{w64 / wm:.2f}x; rel L1 vs FP64 {errs}; max |div B| FP64 {abs64:.2e}, mixed {absm:.2e}")
+    return {"workload": f"3D periodic spectral MHD box {n}^3 (seed 1001, the 256^3 box's spectrum), "
+                        f"{steps} steps after {warmup} warm-up, mixed form on the FP64 dt sequence",
+            "zones": cfg.zones, "steps": steps,
+            "fp64": {"value": cfg.zones * steps / w64, "ms_per_step": w64 / steps * 1e3, "tuning": tune64},
+            "mixed": {"value": cfg.zones * steps / wm, "ms_per_step": wm / steps * 1e3, "tuning": tunem},
+            "speedup_mixed_vs_fp64": w64 / wm, "rel_l1_vs_fp64": errs,
+            "divb_max_abs": {"fp64": abs64, "mixed": absm},
+            "targets_met": bool(w64 / wm >= 1.6 and max(errs.values()) <= 1e-5 and max(abs64, absm) <= 1e-12)}
 
 
 def bench_batch(steps=40, graph_steps=8, fd_dirs=32):
@@ -368,7 +516,13 @@ def main():
                     help="reconstruction (ppm and tvdmm are the reference's; the others are extensions)")
     ap.add_argument("--solver", choices=["hll", "lf", "hllc", "hlld"], default="hll",
                     help="Riemann solver (hll is the reference's; lf, hllc and hlld are extensions)")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 512^3 mixed-precision leg")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter children")
+    ap.add_argument("--traffic-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_child:
+        traffic_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU halo exchange is a design note, not code)", file=sys.stderr)
@@ -379,11 +533,16 @@ def main():
     from cumpic_amd.cumc import Solver, copy_probe
 
     copy_gbps = None
-    if args.copy_probe_mib:
-        copy_gbps = copy_probe(args.copy_probe_mib << 20, reps=3)
-        log(f"[bench] copy probe {args.copy_probe_mib} MiB: {copy_gbps:.1f} GB/s")
+
+    def run_copy_probe():
+        if args.copy_probe_mib:
+            g = copy_probe(args.copy_probe_mib << 20, reps=3)
+            log(f"[bench] copy probe {args.copy_probe_mib} MiB: {g:.1f} GB/s")
+            return g
+        return None
 
     if args.workload == "batch":
+        copy_gbps = run_copy_probe()
         steps = args.steps or 40
         bt = bench_batch(steps=steps, graph_steps=min(8, steps), fd_dirs=args.fd_dirs)
         g = bt["batched_graph"]
@@ -424,6 +583,21 @@ def main():
     if args.recon != "default" or args.solver != "hll":
         workload += f" (options: recon {args.recon}, solver {args.solver})"
 
+    # counter children first, before this process touches the GPU
+    traffic = {}
+    if not args.no_traffic and args.workload == "box" and steps > 0:
+        ic_dir = save_ic(prob)
+        try:
+            for form in forms:
+                traffic[form] = pmc_traffic(ic_dir, form)
+                if traffic[form]:
+                    t = traffic[form]
+                    log(f"[bench] traffic {form}: tail {t['tail_bytes_per_launch'] / 1e9:.3f} GB per launch, "
+                        f"step {t['step_bytes'] / cfg.zones:.0f} B per zone-cycle")
+        finally:
+            shutil.rmtree(ic_dir, ignore_errors=True)
+    copy_gbps = run_copy_probe()
+
     cpu, parity = None, None
     if not args.no_cpu and "fp64" in forms:
         log("[bench] CPU restatement: 1 step on the host cores")
@@ -448,25 +622,26 @@ def main():
         dsig, dabs = s.find_divb()
         st = s.download()
         finite = bool(np.isfinite(st["prim"]).all())
-        profiled = (args.workload == "box" and args.n == 256 and args.recon in ("default", "ppm")
-                    and args.solver == "hll")
+        tr = traffic.get(form)
         r = {"value": cfg.zones * steps / wall, "ms_per_step": wall / steps * 1e3,
-             "roofline": roofline(kt, cfg, form, profiled=profiled),
+             "roofline": roofline(kt, cfg, form, traffic=tr),
+             "step_traffic_bytes_per_zone_cycle": (tr["step_bytes"] / cfg.zones) if tr else None,
              "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()}, "divb_max_abs": dabs,
              "state_finite": finite, "tuning": tuning}
+        kio = r["roofline"]["kernel_io"]
         if copy_gbps:
-            r["roofline"]["frac_of_copy_probe"] = r["roofline"]["achieved"] / copy_gbps
-        if form == "fp64" and r["roofline"]["kernel"] == "tail":
+            kio["frac_of_copy_probe"] = kio["achieved_GBps"] / copy_gbps
+        if form == "fp64":
             # the tail's access pattern alone (stage 1: 26 planes read, stages 2-3: 34; 20 written),
             # same layout and column tiles, no arithmetic: the bandwidth this kernel could reach
             cells = (cfg.nx + 1) * (cfg.ny + 1) * (cfg.nz + 1)
             g1, g23 = s.pattern_probe(26, 20), s.pattern_probe(34, 20)
             t_stage = (46 * 8 * cells / (g1 * 1e9) + 2 * 54 * 8 * cells / (g23 * 1e9)) / 3
-            ceiling = KERNEL_BYTES["fp64"]["tail"] * cells / t_stage / 1e9
-            r["roofline"]["pattern_ceiling_GBps"] = ceiling
-            r["roofline"]["frac_of_pattern_ceiling"] = r["roofline"]["achieved"] / ceiling
+            ceiling = KERNEL_IO_BYTES["fp64"]["tail"] * cells / t_stage / 1e9
+            kio["pattern_ceiling_GBps"] = ceiling
+            kio["frac_of_pattern_ceiling"] = kio["achieved_GBps"] / ceiling
             log(f"[bench] tail access pattern without arithmetic: {g1:.0f} GB/s (26 read + 20 written planes), "
-                f"{g23:.0f} GB/s (34 + 20); the tail reaches {r['roofline']['achieved'] / ceiling:.2f} of it")
+                f"{g23:.0f} GB/s (34 + 20); the tail reaches {kio['achieved_GBps'] / ceiling:.2f} of it")
         s.close()
         if form == "fp64":
             ref_dts = dts
@@ -491,6 +666,10 @@ def main():
         errs = {k: float(np.sum(np.abs(mf[k] - ref_state[k])) / np.sum(np.abs(ref_state[k]))) for k in mf}
         results["mixed"]["rel_l1_vs_fp64"] = errs
         log(f"[bench] mixed rel L1 vs FP64: {errs}")
+
+    c2 = None
+    if not args.no_c2 and args.workload == "box" and args.n == 256 and set(forms) == {"fp64", "mixed"}:
+        c2 = bench_c2()
 
     batch = None
     if not args.no_batch and args.workload == "box":
@@ -519,7 +698,8 @@ def main():
         "roofline": m["roofline"],
         "cpu_baseline": cpu,
         "step_roofline": {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE, "achieved_GBps": step_alg / 1e9,
-                          "frac": step_alg / HBM_PEAK},
+                          "frac": step_alg / HBM_PEAK,
+                          "traffic_bytes_per_zone_cycle": m["step_traffic_bytes_per_zone_cycle"]},
         "kernel_ms_per_step": m["kernel_ms_per_step"],
         "tuning": m["tuning"],
         "divb_max_abs": m["divb_max_abs"],
@@ -534,6 +714,7 @@ def main():
                         "divb_max_abs": mx["divb_max_abs"], "state_finite": mx["state_finite"],
                         "roofline": mx["roofline"], "kernel_ms_per_step": mx["kernel_ms_per_step"],
                         "tuning": mx["tuning"]}
+    out["c2"] = c2
     out["batch"] = batch
     print(json.dumps(out), flush=True)
 

@@ -1,7 +1,8 @@
 """bench.py's contract, checked without a GPU: one GPU only (a clear refusal for more), the
-roofline object computed from a kernel timing with the algorithmic bytes per launch that the
-design states (411 B per cell of [0,n]^3 for the FP64 stage tail), the calibrated traffic read
-from the committed rocprofv3 summary, and the CPU baseline's fields on a tiny box."""
+roofline object computed from a kernel timing with the algorithmic bytes per launch of the stage
+tail (512/3 B per interior zone: one stage of the FP64 state traffic), the counter traffic taken
+only from this run's own rocprofv3 passes (null without them), the parsing of a rocprofv3 counter
+CSV, and the CPU baseline's fields on a tiny box."""
 from __future__ import annotations
 
 import importlib.util
@@ -32,20 +33,39 @@ def test_roofline_object_from_a_kernel_timing():
     from cumpic_amd import ics
     cfg = ics.small_periodic(8).cfg
     cfg.nx = cfg.ny = cfg.nz = 256
-    assert b.KERNEL_BYTES["fp64"]["tail"] == pytest.approx(410.67, abs=0.01)
-    units = b.kernel_units("tail", cfg)
-    assert units == 257 ** 3
+    assert b.KERNEL_IO_BYTES["fp64"]["tail"] == pytest.approx(410.67, abs=0.01)
+    assert b.kernel_units("tail", cfg) == 257 ** 3
     # three tail launches of 1.75 ms in total 5.25 ms, the fluxes cheaper: the tail dominates
     kt = {"tail": (5.25, 3), "flux_x": (2.6, 3), "ecorn": (1.5, 3)}
     r = b.roofline(kt, cfg, "fp64")
     for key in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
         assert key in r
     assert r["kernel"] == "tail" and r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
-    expect = 410.6667 * 257 ** 3 / 1.75e-3 / 1e9
-    assert r["achieved"] == pytest.approx(expect, rel=1e-4)
-    assert r["frac"] == pytest.approx(expect / 8000.0, rel=1e-4)
-    # calibrated HBM bytes per launch of the FP64 tail from profiles/pmc_traffic.txt
-    assert r["traffic"] == b.read_traffic("tail", "fp64") and 7e9 < r["traffic"] < 9e9
+    expect = 512.0 / 3.0 * 256 ** 3 / 1.75e-3 / 1e9
+    assert r["achieved"] == pytest.approx(expect, rel=1e-9)
+    assert r["frac"] == pytest.approx(expect / 8000.0, rel=1e-9)
+    assert r["traffic"] is None and r["traffic_over_algorithmic"] is None
+    assert r["kernel_io"]["achieved_GBps"] == pytest.approx(410.6667 * 257 ** 3 / 1.75e-3 / 1e9, rel=1e-4)
+    t = {"tail_bytes_per_launch": 8.2e9}
+    r = b.roofline(kt, cfg, "fp64", traffic=t)
+    assert r["traffic"] == 8.2e9
+    assert r["traffic_over_algorithmic"] == pytest.approx(8.2e9 / (512.0 / 3.0 * 256 ** 3))
+
+
+def test_counter_csv_parsing(tmp_path):
+    b = _bench()
+    d = tmp_path / "host" / "1234"
+    d.mkdir(parents=True)
+    rows = [("3", "cumc::k_tail<double, 1, 1>(cumc::Lay)", "100.5"), ("1", "k_copy_probe(double2 const*)", "524288"),
+            ("2", "cumc::k_flux_x<double, 4, 2>(cumc::Lay)", "7.0")]
+    with open(d / "pmc_counter_collection.csv", "w") as f:
+        f.write('"Correlation_Id","Dispatch_Id","Kernel_Name","Counter_Name","Counter_Value"\n')
+        for i, n, v in rows:
+            f.write(f'"{i}","{i}","{n}","FETCH_SIZE","{v}"\n')
+    got = b._counter_rows(str(tmp_path))
+    assert [r[0] for r in got] == [1, 2, 3]
+    assert got[0][1].startswith("k_copy_probe") and got[2][2] == 100.5
+    assert b.STEP_KERNELS.search(got[2][1]) and not b.STEP_KERNELS.search(got[0][1])
 
 
 def test_cpu_baseline_fields():
