@@ -60,7 +60,13 @@ def flang() -> str | None:
 
 
 FHOST_DIR = os.path.join(PKG, "fhost")
+# the host built with the run-file plugin keeps the plain name; other plugins are cumc3d_<model>
 FHOST_EXE = os.path.join(FHOST_DIR, "cumc3d")
+FHOST_MODELS = ("file", "riemann")
+
+
+def fhost_exe(model: str = "file") -> str:
+    return FHOST_EXE if model == "file" else FHOST_EXE + "_" + model
 
 
 H5_LIB = os.path.join(FHOST_DIR, "libcumc_h5.so")
@@ -68,23 +74,29 @@ H5_LIB = os.path.join(FHOST_DIR, "libcumc_h5.so")
 
 def build_h5profile() -> str:
     """HDF5 profile writer of the host (h5profile.c; opens libhdf5 at run time)."""
-    tmp = H5_LIB + ".tmp"
+    tmp = H5_LIB + ".tmp%d" % os.getpid()
     _run(["gcc", "-O2", "-shared", "-fPIC", "-Wall", "-o", tmp, os.path.join(FHOST_DIR, "h5profile.c"), "-ldl"])
     os.replace(tmp, H5_LIB)
     return H5_LIB
 
 
-def build_fortran_host() -> str | None:
-    """Fortran driver over the C ABI (flang); returns the executable path or None without flang."""
+def build_fortran_host(models=FHOST_MODELS) -> str | None:
+    """Fortran driver over the C ABI (flang), once per model plugin: the plugin's sources are the
+    ones its model/<name>/source.list names, as the reference's build takes model/<m>/src/source.list.
+    Returns the run-file host's path, or None without flang."""
     fc = flang()
     if fc is None:
         return None
     build_h5profile()
-    srcs = [os.path.join(FHOST_DIR, f) for f in ("cumc_iface.f90", "model_file.f90", "cumc3d.f90")]
-    mod = os.path.join(FHOST_DIR, "_mod")
-    os.makedirs(mod, exist_ok=True)
-    cmd = [fc, "-O2", "-J", mod, *srcs, "-L", PKG, "-lcumc", "-L", FHOST_DIR, "-lcumc_h5",
-           "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,$ORIGIN", "-o", FHOST_EXE + ".tmp"]
-    _run(cmd)
-    os.replace(FHOST_EXE + ".tmp", FHOST_EXE)
+    for model in models:
+        with open(os.path.join(FHOST_DIR, "model", model, "source.list")) as f:
+            plugin = [os.path.join(FHOST_DIR, ln.strip()) for ln in f if ln.strip()]
+        srcs = [os.path.join(FHOST_DIR, "cumc_iface.f90"), *plugin, os.path.join(FHOST_DIR, "cumc3d.f90")]
+        mod = os.path.join(FHOST_DIR, "_mod", model)
+        os.makedirs(mod, exist_ok=True)
+        exe = fhost_exe(model)
+        cmd = [fc, "-O2", "-J", mod, *srcs, "-L", PKG, "-lcumc", "-L", FHOST_DIR, "-lcumc_h5",
+               "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+        _run(cmd)
+        os.replace(exe + ".tmp", exe)
     return FHOST_EXE

@@ -541,6 +541,231 @@ __device__ inline double block_min(double v)
     return m;
 }
 
+// Flux-divergence geometry of a column (coords.f90:255-282)
+struct ColGeom {
+    double dyl, gxp, gxm, gxc, gyp, gym, gyc, gac, xloc, tanc;
+};
+template <int COORD>
+__device__ inline ColGeom col_geom(const Tabs &tb, int j, int k)
+{
+    ColGeom c;
+    const double dxl = tb.at(0, TB_D, j);
+    c.dyl = tb.at(1, TB_D, k);
+    c.gxp = 1.0; c.gxm = 1.0; c.gxc = dxl; c.gyp = 1.0; c.gym = 1.0; c.gyc = c.dyl; c.gac = 0.0; c.xloc = 0.0;
+    c.tanc = 1.0;
+    if (COORD == 2) {
+        c.xloc = tb.at(0, TB_C, j);
+        c.gxp = tb.at(0, TB_F, j);
+        c.gxm = tb.at(0, TB_F, j - 1);
+        c.gxc = c.xloc * dxl;
+        c.gyc = c.xloc * c.dyl;
+    }
+    if (COORD == 3) {
+        c.xloc = tb.at(0, TB_C, j);
+        c.tanc = tb.at(1, TB_TANC, k);
+        c.gxp = tb.at(0, TB_F2, j);
+        c.gxm = tb.at(0, TB_F2, j - 1);
+        c.gxc = tb.at(0, TB_V3, j);
+        c.gyp = tb.at(1, TB_SIN, k);
+        c.gym = tb.at(1, TB_SIN, k - 1);
+        c.gac = tb.at(0, TB_RA, j) * tb.at(1, TB_CD, k);
+        c.gyc = c.gac;
+    }
+    return c;
+}
+
+// Corner EMFs around cell (j,k,l): own (j,k), (j,k-1), (j-1,k), (j-1,k-1) in plane l
+struct CellE {
+    double ex0, ey0, ez0, exk, ezk, eyj, ezj, ezjk;
+};
+// What the update of cell (j,k,l) takes over from cell (j,k,l-1): the plane-(l-1) EMFs
+// ex(j,k), ey(j,k), ey(j-1,k), ex(j,k-1), the new face B_z below the cell and the z-face fluxes
+// below it
+struct TailCarry {
+    double exL, eyL, eyjL, exkL, bzL;
+    double fzL[5];
+};
+
+// Stage tail of one cell of [0,n]^3: flux divergence + sources, face-B right-hand side, RK3
+// combination, U->P, floors, P->U, EOS, and (stage 3) its CFL step.  fz holds the z-face fluxes
+// above the cell; cr carries from the cell below and is advanced to this cell.
+template <typename R, int STAGE, int COORD>
+__device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, double dt,
+                                 const double *__restrict__ Uin, double *__restrict__ Uout, int j, int k, int l,
+                                 unsigned g, const ColGeom &cg, const CellE &E, TailCarry &cr, const double fz[5],
+                                 double &dtmin)
+{
+    const unsigned sj = 1, sk = (unsigned)L.px;
+    const long ps = L.ps;
+    const double *U0 = F.U0;
+    // plane c of each field at cell offset i (uniform plane bases, 32-bit lane offsets)
+    auto fx = [&](int c, unsigned i) { return (double)ldi(F.F[0] + c * ps, i); };
+    auto fy = [&](int c, unsigned i) { return (double)ldi(F.F[1] + c * ps, i); };
+    auto u0 = [&](int c, unsigned i) { return ldi(U0 + c * ps, i); };
+    auto ui = [&](int c, unsigned i) { return STAGE == 1 ? 0.0 : ldi(Uin + c * ps, i); };
+    // stage-input primitives: the FP64 planes, or the FP32 ones of the mixed form
+    constexpr bool f32p = sizeof(R) == 4;
+    auto pp = [&](int c, unsigned i) { return f32p ? (double)ldi(F.P32 + c * ps, i) : ldi(F.P + c * ps, i); };
+    auto bcl = [&](int c, unsigned i) { return f32p ? (double)ldi(F.BC32 + c * ps, i) : ldi(F.BC + c * ps, i); };
+    const double gm1 = M.ggas - 1.0;
+    const bool interior = j >= 1 && k >= 1 && l >= 1;
+    const double ex0 = E.ex0, ey0 = E.ey0, ez0 = E.ez0, exk = E.exk, ezk = E.ezk, eyj = E.eyj, ezj = E.ezj,
+                 ezjk = E.ezjk;
+    // new face fields: own three, and the x / y faces of the lower neighbours
+    const double bxn = rk<STAGE>(u0(5, g), ui(5, g), dt, rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, cr.eyL));
+    const double byn = rk<STAGE>(u0(6, g), ui(6, g), dt, rhs_by<COORD>(tb, j, k, l, ex0, cr.exL, ez0, ezj));
+    const double bzn = rk<STAGE>(u0(7, g), ui(7, g), dt, rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
+    // hydro right-hand side on interior cells
+    double lrk[5];
+    if (interior) {
+        double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (COORD == 2 || COORD == 3 || M.model != 0) {
+            // (the mixed form takes the density from the stage-input U, which holds the
+            // FP64 value, so that the density-floor switch of the model gravity sees
+            // what the FP64 form sees)
+            const double rho = f32p ? (STAGE == 1 ? u0(0, g) : ui(0, g)) : pp(0, g);
+            const double vx = pp(1, g), vy = pp(2, g), vz = pp(3, g);
+            const double p = pp(4, g);
+            const double bx = bcl(0, g), by = bcl(1, g), bz = bcl(2, g);
+            if (COORD == 2) {
+                const double bsq = bx * bx + by * by + bz * bz;
+                const double tx = (p + rho * vy * vy + 0.5 * bsq - by * by) / (cg.xloc);
+                const double ty = (rho * vx * vy - bx * by) / (cg.xloc);
+                for (int r = 0; r < M.geom_repeat; r++) {
+                    sc[IVX] = sc[IVX] + tx;
+                    sc[IVY] = sc[IVY] - ty;
+                }
+            }
+            if (COORD == 3) {
+                // coords.f90:139-156
+                const double bsq = bx * bx + by * by + bz * bz;
+                const double tx = (2.0 * p + bx * bx + rho * (vy * vy + vz * vz)) / (cg.xloc);
+                const double ty1 = (p + rho * vz * vz + 0.5 * bsq - bz * bz) / (cg.xloc * cg.tanc);
+                const double ty2 = (rho * vx * vy - bx * by) / (cg.xloc);
+                const double tz1 = (rho * vx * vz - bx * bz) / (cg.xloc);
+                const double tz2 = (rho * vy * vz - by * bz) / (cg.xloc * cg.tanc);
+                for (int r = 0; r < M.geom_repeat; r++) {
+                    sc[IVX] = sc[IVX] + tx;
+                    sc[IVY] = sc[IVY] + ty1 - ty2;
+                    sc[IVZ] = sc[IVZ] - tz1 - tz2;
+                }
+            }
+            if (M.model == 2) {
+                // torus_sph/src/custom.f90:121-158: radial Paczynski-Wiita gravity
+                const double xl = tb.at(0, TB_C, j);
+                const double factor = fmax2<double>(sign1<double>(rho - M.rho_floor), 0.0);
+                const double dphidr = 1.0 / ((xl - M.r_sh) * (xl - M.r_sh));
+                sc[IVX] = sc[IVX] + (-factor * rho * dphidr);
+                sc[ITAU] = sc[ITAU] + (-factor * rho * vx * dphidr);
+            }
+            if (M.model == 1) {
+                const double xl = tb.at(0, TB_C, j), zl = tb.at(2, TB_C, l);
+                const double factor = fmax2<double>(sign1<double>(rho - M.rho_floor), 0.0);
+                const double radius = sqrt(xl * xl + zl * zl);
+                const double dphidr = xl / radius / ((radius - M.r_sh) * (radius - M.r_sh));
+                const double dphidz = zl / radius / ((radius - M.r_sh) * (radius - M.r_sh));
+                sc[IVX] = sc[IVX] + (-factor * rho * dphidr);
+                sc[IVZ] = sc[IVZ] + (-factor * rho * dphidz);
+                sc[ITAU] = sc[ITAU] + (-factor * rho * (vx * dphidr + vz * dphidz));
+            }
+        }
+        const double dzl = tb.at(2, TB_D, l);
+        const double gzc = COORD == 3 ? cg.gac * dzl / cg.dyl : dzl;
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            double a = 0.0;
+            a = a - (fx(i, g) * cg.gxp - fx(i, g - sj) * cg.gxm) / cg.gxc;
+            a = a - (fy(i, g) * cg.gyp - fy(i, g - sk) * cg.gym) / cg.gyc;
+            a = a - (fz[i] * 1.0 - cr.fzL[i] * 1.0) / gzc;
+            lrk[i] = a + sc[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 5; i++) lrk[i] = 0.0;
+    }
+    double u[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(u0(i, g), ui(i, g), dt, lrk[i]);
+    // cell-centred field from the new face fields (interior) or the ghost value
+    double bcx, bcy, bcz;
+    if (interior) {
+        const double bxm = rk<STAGE>(u0(5, g - sj), ui(5, g - sj), dt, rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, cr.eyjL));
+        const double bym = rk<STAGE>(u0(6, g - sk), ui(6, g - sk), dt, rhs_by<COORD>(tb, j, k - 1, l, exk, cr.exkL, ezk, ezjk));
+        bcx = 0.5 * (bxn + bxm);
+        bcy = 0.5 * (byn + bym);
+        bcz = 0.5 * (bzn + cr.bzL);
+        if (f32p) {
+            sti(F.BC32, g, (float)bcx);
+            sti(F.BC32 + ps, g, (float)bcy);
+            sti(F.BC32 + 2 * ps, g, (float)bcz);
+        } else {
+            sti(F.BC, g, bcx);
+            sti(F.BC + ps, g, bcy);
+            sti(F.BC + 2 * ps, g, bcz);
+        }
+    } else {
+        bcx = bcl(0, g);
+        bcy = bcl(1, g);
+        bcz = bcl(2, g);
+    }
+    // U_to_P, floors, P_to_U, EOS
+    const double rho0 = u[0];
+    double vx = u[1] / rho0, vy = u[2] / rho0, vz = u[3] / rho0;
+    const double bsq = bcx * bcx + bcy * bcy + bcz * bcz;
+    double vsq = vx * vx + vy * vy + vz * vz;
+    double eps = (u[4] - 0.5 * bsq) / rho0 - 0.5 * vsq;
+    double rho = rho0;
+    if (interior) {
+        if (M.model != 0) {
+            if (rho < M.rho_floor) rho = M.rho_floor;
+            if (eps < M.eps_floor) eps = M.eps_floor;
+        } else {
+            eps = fmax2<double>(eps, 0.0);
+        }
+    }
+    vsq = vx * vx + vy * vy + vz * vz;
+    const double p = rho * eps * gm1;
+    sti(Uout, g, rho);
+    sti(Uout + ps, g, vx * rho);
+    sti(Uout + 2 * ps, g, vy * rho);
+    sti(Uout + 3 * ps, g, vz * rho);
+    sti(Uout + 4 * ps, g, rho * (eps + 0.5 * vsq) + 0.5 * bsq);
+    sti(Uout + 5 * ps, g, bxn);
+    sti(Uout + 6 * ps, g, byn);
+    sti(Uout + 7 * ps, g, bzn);
+    if (f32p) {
+        sti(F.P32, g, (float)rho);
+        sti(F.P32 + ps, g, (float)vx);
+        sti(F.P32 + 2 * ps, g, (float)vy);
+        sti(F.P32 + 3 * ps, g, (float)vz);
+        sti(F.P32 + 4 * ps, g, (float)p);
+        sti(F.P32 + 5 * ps, g, (float)bxn);
+        sti(F.P32 + 6 * ps, g, (float)byn);
+        sti(F.P32 + 7 * ps, g, (float)bzn);
+        sti(F.EPS32, g, (float)eps);
+    } else {
+        sti(F.P, g, rho);
+        sti(F.P + ps, g, vx);
+        sti(F.P + 2 * ps, g, vy);
+        sti(F.P + 3 * ps, g, vz);
+        sti(F.P + 4 * ps, g, p);
+        sti(F.P + 5 * ps, g, bxn);
+        sti(F.P + 6 * ps, g, byn);
+        sti(F.P + 7 * ps, g, bzn);
+        sti(F.EPS, g, eps);
+    }
+    if (STAGE == 3 && interior)
+        dtmin = fmin2<double>(cell_dt<COORD>(tb, M, j, k, l, rho, vx, vy, vz, p, bcx, bcy, bcz), dtmin);
+    // carries
+    cr.exL = ex0;
+    cr.eyL = ey0;
+    cr.eyjL = eyj;
+    cr.exkL = exk;
+    cr.bzL = bzn;
+#pragma unroll
+    for (int i = 0; i < 5; i++) cr.fzL[i] = fz[i];
+}
+
 // Column tile (bx, by) as in ecorn_tile.  Every thread of the block reaches the stage-3 block
 // reduction at the end, so callers may loop over tiles with block-uniform trip counts.  In the
 // mixed form (R = float) the stage-input primitives come from, and the new ones go to, the FP32
@@ -560,214 +785,35 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
         const unsigned sj = 1, sk = (unsigned)L.px, sl = (unsigned)L.py;
         const long ps = L.ps;
         const double dt = *dtp;
-        const double *U0 = F.U0;
-        // plane c of each field at cell offset i (uniform plane bases, 32-bit lane offsets)
-        auto fx = [&](int c, unsigned i) { return (double)ldi(F.F[0] + c * ps, i); };
-        auto fy = [&](int c, unsigned i) { return (double)ldi(F.F[1] + c * ps, i); };
         auto fzz = [&](int c, unsigned i) { return (double)ldi(F.F[2] + c * ps, i); };
         auto e = [&](int c, unsigned i) { return (double)ldi(F.E + c * ps, i); };
-        auto u0 = [&](int c, unsigned i) { return ldi(U0 + c * ps, i); };
+        auto u0 = [&](int c, unsigned i) { return ldi(F.U0 + c * ps, i); };
         auto ui = [&](int c, unsigned i) { return STAGE == 1 ? 0.0 : ldi(Uin + c * ps, i); };
-        // stage-input primitives: the FP64 planes, or the FP32 ones of the mixed form
-        constexpr bool f32p = sizeof(R) == 4;
-        auto pp = [&](int c, unsigned i) { return f32p ? (double)ldi(F.P32 + c * ps, i) : ldi(F.P + c * ps, i); };
-        auto bcl = [&](int c, unsigned i) { return f32p ? (double)ldi(F.BC32 + c * ps, i) : ldi(F.BC + c * ps, i); };
-        const double gm1 = M.ggas - 1.0;
-        // flux-divergence geometry of the column (coords.f90:255-282)
-        const double dxl = tb.at(0, TB_D, j), dyl = tb.at(1, TB_D, k);
-        double gxp = 1.0, gxm = 1.0, gxc = dxl, gyp = 1.0, gym = 1.0, gyc = dyl, gac = 0.0, xloc = 0.0, tanc = 1.0;
-        if (COORD == 2) {
-            xloc = tb.at(0, TB_C, j);
-            gxp = tb.at(0, TB_F, j);
-            gxm = tb.at(0, TB_F, j - 1);
-            gxc = xloc * dxl;
-            gyc = xloc * dyl;
-        }
-        if (COORD == 3) {
-            xloc = tb.at(0, TB_C, j);
-            tanc = tb.at(1, TB_TANC, k);
-            gxp = tb.at(0, TB_F2, j);
-            gxm = tb.at(0, TB_F2, j - 1);
-            gxc = tb.at(0, TB_V3, j);
-            gyp = tb.at(1, TB_SIN, k);
-            gym = tb.at(1, TB_SIN, k - 1);
-            gac = tb.at(0, TB_RA, j) * tb.at(1, TB_CD, k);
-            gyc = gac;
-        }
+        const ColGeom cg = col_geom<COORD>(tb, j, k);
         // carries from plane l0-1
         unsigned g = (unsigned)L.id(j, k, l0);
         const unsigned gm = g - sl;
-        double exL = e(0, gm), eyL = e(1, gm), eyjL = e(1, gm - sj), exkL = e(0, gm - sk);
-        double bzL = 0.0;
-        if (l0 >= 1) bzL = rk<STAGE>(u0(7, gm), ui(7, gm), dt, rhs_bz<COORD>(tb, j, k, eyL, eyjL, exL, exkL));
-        double fzL[5];
+        TailCarry cr;
+        cr.exL = e(0, gm);
+        cr.eyL = e(1, gm);
+        cr.eyjL = e(1, gm - sj);
+        cr.exkL = e(0, gm - sk);
+        cr.bzL = 0.0;
+        if (l0 >= 1)
+            cr.bzL = rk<STAGE>(u0(7, gm), ui(7, gm), dt, rhs_bz<COORD>(tb, j, k, cr.eyL, cr.eyjL, cr.exL, cr.exkL));
 #pragma unroll
-        for (int i = 0; i < 5; i++) fzL[i] = fzz(i, gm);
+        for (int i = 0; i < 5; i++) cr.fzL[i] = fzz(i, gm);
         for (int l = l0; l <= l1; l++, g += sl) {
-            const bool interior = j >= 1 && k >= 1 && l >= 1;
             // corner EMFs around the cell's faces
-            const double ex0 = e(0, g), ey0 = e(1, g), ez0 = e(2, g);
-            const double exk = e(0, g - sk), ezk = e(2, g - sk);
-            const double eyj = e(1, g - sj), ezj = e(2, g - sj);
-            const double ezjk = e(2, g - sj - sk);
-            // new face fields: own three, and the x / y faces of the lower neighbours
-            const double bxn = rk<STAGE>(u0(5, g), ui(5, g), dt, rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, eyL));
-            const double byn = rk<STAGE>(u0(6, g), ui(6, g), dt, rhs_by<COORD>(tb, j, k, l, ex0, exL, ez0, ezj));
-            const double bzn = rk<STAGE>(u0(7, g), ui(7, g), dt, rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
-            // hydro right-hand side on interior cells
-            double lrk[5];
+            CellE E;
+            E.ex0 = e(0, g); E.ey0 = e(1, g); E.ez0 = e(2, g);
+            E.exk = e(0, g - sk); E.ezk = e(2, g - sk);
+            E.eyj = e(1, g - sj); E.ezj = e(2, g - sj);
+            E.ezjk = e(2, g - sj - sk);
             double fz[5];
 #pragma unroll
             for (int i = 0; i < 5; i++) fz[i] = fzz(i, g);
-            if (interior) {
-                double sc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                if (COORD == 2 || COORD == 3 || M.model != 0) {
-                    // (the mixed form takes the density from the stage-input U, which holds the
-                    // FP64 value, so that the density-floor switch of the model gravity sees
-                    // what the FP64 form sees)
-                    const double rho = f32p ? (STAGE == 1 ? u0(0, g) : ui(0, g)) : pp(0, g);
-                    const double vx = pp(1, g), vy = pp(2, g), vz = pp(3, g);
-                    const double p = pp(4, g);
-                    const double bx = bcl(0, g), by = bcl(1, g), bz = bcl(2, g);
-                    if (COORD == 2) {
-                        const double bsq = bx * bx + by * by + bz * bz;
-                        const double tx = (p + rho * vy * vy + 0.5 * bsq - by * by) / (xloc);
-                        const double ty = (rho * vx * vy - bx * by) / (xloc);
-                        for (int r = 0; r < M.geom_repeat; r++) {
-                            sc[IVX] = sc[IVX] + tx;
-                            sc[IVY] = sc[IVY] - ty;
-                        }
-                    }
-                    if (COORD == 3) {
-                        // coords.f90:139-156
-                        const double bsq = bx * bx + by * by + bz * bz;
-                        const double tx = (2.0 * p + bx * bx + rho * (vy * vy + vz * vz)) / (xloc);
-                        const double ty1 = (p + rho * vz * vz + 0.5 * bsq - bz * bz) / (xloc * tanc);
-                        const double ty2 = (rho * vx * vy - bx * by) / (xloc);
-                        const double tz1 = (rho * vx * vz - bx * bz) / (xloc);
-                        const double tz2 = (rho * vy * vz - by * bz) / (xloc * tanc);
-                        for (int r = 0; r < M.geom_repeat; r++) {
-                            sc[IVX] = sc[IVX] + tx;
-                            sc[IVY] = sc[IVY] + ty1 - ty2;
-                            sc[IVZ] = sc[IVZ] - tz1 - tz2;
-                        }
-                    }
-                    if (M.model == 2) {
-                        // torus_sph/src/custom.f90:121-158: radial Paczynski-Wiita gravity
-                        const double xl = tb.at(0, TB_C, j);
-                        const double factor = fmax2<double>(sign1<double>(rho - M.rho_floor), 0.0);
-                        const double dphidr = 1.0 / ((xl - M.r_sh) * (xl - M.r_sh));
-                        sc[IVX] = sc[IVX] + (-factor * rho * dphidr);
-                        sc[ITAU] = sc[ITAU] + (-factor * rho * vx * dphidr);
-                    }
-                    if (M.model == 1) {
-                        const double xl = tb.at(0, TB_C, j), zl = tb.at(2, TB_C, l);
-                        const double factor = fmax2<double>(sign1<double>(rho - M.rho_floor), 0.0);
-                        const double radius = sqrt(xl * xl + zl * zl);
-                        const double dphidr = xl / radius / ((radius - M.r_sh) * (radius - M.r_sh));
-                        const double dphidz = zl / radius / ((radius - M.r_sh) * (radius - M.r_sh));
-                        sc[IVX] = sc[IVX] + (-factor * rho * dphidr);
-                        sc[IVZ] = sc[IVZ] + (-factor * rho * dphidz);
-                        sc[ITAU] = sc[ITAU] + (-factor * rho * (vx * dphidr + vz * dphidz));
-                    }
-                }
-                const double dzl = tb.at(2, TB_D, l);
-                const double gzc = COORD == 3 ? gac * dzl / dyl : dzl;
-#pragma unroll
-                for (int i = 0; i < 5; i++) {
-                    double a = 0.0;
-                    a = a - (fx(i, g) * gxp - fx(i, g - sj) * gxm) / gxc;
-                    a = a - (fy(i, g) * gyp - fy(i, g - sk) * gym) / gyc;
-                    a = a - (fz[i] * 1.0 - fzL[i] * 1.0) / gzc;
-                    lrk[i] = a + sc[i];
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < 5; i++) lrk[i] = 0.0;
-            }
-            double u[5];
-#pragma unroll
-            for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(u0(i, g), ui(i, g), dt, lrk[i]);
-            // cell-centred field from the new face fields (interior) or the ghost value
-            double bcx, bcy, bcz;
-            if (interior) {
-                const double bxm = rk<STAGE>(u0(5, g - sj), ui(5, g - sj), dt,
-                                             rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
-                const double bym = rk<STAGE>(u0(6, g - sk), ui(6, g - sk), dt,
-                                             rhs_by<COORD>(tb, j, k - 1, l, exk, exkL, ezk, ezjk));
-                bcx = 0.5 * (bxn + bxm);
-                bcy = 0.5 * (byn + bym);
-                bcz = 0.5 * (bzn + bzL);
-                if (f32p) {
-                    sti(F.BC32, g, (float)bcx);
-                    sti(F.BC32 + ps, g, (float)bcy);
-                    sti(F.BC32 + 2 * ps, g, (float)bcz);
-                } else {
-                    sti(F.BC, g, bcx);
-                    sti(F.BC + ps, g, bcy);
-                    sti(F.BC + 2 * ps, g, bcz);
-                }
-            } else {
-                bcx = bcl(0, g);
-                bcy = bcl(1, g);
-                bcz = bcl(2, g);
-            }
-            // U_to_P, floors, P_to_U, EOS
-            const double rho0 = u[0];
-            double vx = u[1] / rho0, vy = u[2] / rho0, vz = u[3] / rho0;
-            const double bsq = bcx * bcx + bcy * bcy + bcz * bcz;
-            double vsq = vx * vx + vy * vy + vz * vz;
-            double eps = (u[4] - 0.5 * bsq) / rho0 - 0.5 * vsq;
-            double rho = rho0;
-            if (interior) {
-                if (M.model != 0) {
-                    if (rho < M.rho_floor) rho = M.rho_floor;
-                    if (eps < M.eps_floor) eps = M.eps_floor;
-                } else {
-                    eps = fmax2<double>(eps, 0.0);
-                }
-            }
-            vsq = vx * vx + vy * vy + vz * vz;
-            const double p = rho * eps * gm1;
-            sti(Uout, g, rho);
-            sti(Uout + ps, g, vx * rho);
-            sti(Uout + 2 * ps, g, vy * rho);
-            sti(Uout + 3 * ps, g, vz * rho);
-            sti(Uout + 4 * ps, g, rho * (eps + 0.5 * vsq) + 0.5 * bsq);
-            sti(Uout + 5 * ps, g, bxn);
-            sti(Uout + 6 * ps, g, byn);
-            sti(Uout + 7 * ps, g, bzn);
-            if (f32p) {
-                sti(F.P32, g, (float)rho);
-                sti(F.P32 + ps, g, (float)vx);
-                sti(F.P32 + 2 * ps, g, (float)vy);
-                sti(F.P32 + 3 * ps, g, (float)vz);
-                sti(F.P32 + 4 * ps, g, (float)p);
-                sti(F.P32 + 5 * ps, g, (float)bxn);
-                sti(F.P32 + 6 * ps, g, (float)byn);
-                sti(F.P32 + 7 * ps, g, (float)bzn);
-                sti(F.EPS32, g, (float)eps);
-            } else {
-                sti(F.P, g, rho);
-                sti(F.P + ps, g, vx);
-                sti(F.P + 2 * ps, g, vy);
-                sti(F.P + 3 * ps, g, vz);
-                sti(F.P + 4 * ps, g, p);
-                sti(F.P + 5 * ps, g, bxn);
-                sti(F.P + 6 * ps, g, byn);
-                sti(F.P + 7 * ps, g, bzn);
-                sti(F.EPS, g, eps);
-            }
-            if (STAGE == 3 && interior)
-                dtmin = fmin2<double>(cell_dt<COORD>(tb, M, j, k, l, rho, vx, vy, vz, p, bcx, bcy, bcz), dtmin);
-            // carries
-            exL = ex0;
-            eyL = ey0;
-            eyjL = eyj;
-            exkL = exk;
-            bzL = bzn;
-#pragma unroll
-            for (int i = 0; i < 5; i++) fzL[i] = fz[i];
+            tail_cell<R, STAGE, COORD>(L, F, tb, M, dt, Uin, Uout, j, k, l, g, cg, E, cr, fz, dtmin);
         }
     }
     if (STAGE == 3) {

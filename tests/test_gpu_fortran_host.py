@@ -92,3 +92,36 @@ def test_fortran_host_hdf5_profiles(tmp_path):
         assert np.array_equal(got["prim"], st["prim"][inner]), k
         assert np.array_equal(got["eps"], st["eps"][inner]), k
         assert np.array_equal(got["xf"], p.xf[3:3 + c.nx + 1])
+
+
+@pytest.mark.parametrize("name,test,n", [("toro1", ics.TORO_1, 200), ("briowu", ics.BRIO_WU, 200),
+                                         ("mhdrotor", ics.MHD_ROTOR, 48)])
+def test_fortran_riemann_plugin_named_problems(tmp_path, name, test, n):
+    """The host built with the Fortran riemann plugin (model/riemann/riemann.f90: MODEL_SETUP from a
+    &riemann namelist, GETGRID + CUSTOM_GRID, GET_MODEL computing the problem, OPENFILE_CUSTOM) runs
+    the problem to its stop time with no Python initial condition; the CPU restatement, started from
+    the state that host's GET_MODEL produced, must reach the same final state and dt sequence bit for
+    bit (and that state is the Python generator's)."""
+    from cumpic_amd.fhost.runfile import write_riemann_params
+    print(f"[case] Fortran riemann plugin, {name} n={n} to its stop time", flush=True)
+    exe = build.fhost_exe("riemann")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built (no flang)")
+    par, out = tmp_path / "r.nml", tmp_path / "r.out"
+    write_riemann_params(str(par), test, n)
+    r = subprocess.run([exe, str(par), str(out)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    ref = ics.riemann_test(test, n=n)
+    res = read_output(str(out), ref.cfg)
+    assert np.array_equal(res["prim0"], ref.prim) and np.array_equal(res["eps0"], ref.eps)
+    ref.prim, ref.eps = res["prim0"], res["eps0"]
+    o = Oracle.from_problem(ref)
+    o.initial_update()
+    dts, t = o.run(100000, ref.total_time)
+    so = o.get_state()
+    print(f"  {res['steps']} steps to t = {res['time']}", flush=True)
+    assert res["steps"] == len(dts) and res["time"] == t
+    assert np.array_equal(res["dts"], dts)
+    assert np.array_equal(res["prim"], so["prim"])
+    assert np.array_equal(res["eps"], so["eps"])
