@@ -14,6 +14,7 @@ using namespace cumc;
 
 struct cumc_ctx {
     cumc_config cfg;
+    int device = 0;  // HIP device current at cumc_create (the stream's device)
     Lay L;
     Model M;
     hipStream_t stream = nullptr;

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -82,6 +83,10 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
 
     cumc_ctx *ctx = new cumc_ctx();
     ctx->cfg = c;
+    if (hipGetDevice(&ctx->device) != hipSuccess) {
+        delete ctx;
+        return CUMC_EHIP;
+    }
     Lay &L = ctx->L;
     L.nx = c.nx;
     L.ny = c.ny;
@@ -305,6 +310,79 @@ extern "C" int cumc_download(cumc_ctx *ctx, double *prim, double *eps, double *b
     if (prim && (r = conv_out(ctx, ctx->F64.P, 8, prim))) return r;
     if (eps && (r = conv_out(ctx, ctx->F64.EPS, 1, eps))) return r;
     if (bcell && (r = conv_out(ctx, ctx->F64.BC, 3, bcell))) return r;
+    return CUMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Download without device allocation (cumc_download_noalloc).  The AoS staging goes through a
+// fixed 32 MiB file-scope device array, slab by slab of whole planes, instead of a temporary the
+// size of the state.  Each GPU the library runs on holds its own copy of the array until the
+// process ends; host code reaches it through the address hipGetSymbolAddress returns for the
+// current device (looked up once per device), never through the array's name.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t NOALLOC_STAGING_BYTES = size_t(32) << 20;
+static __device__ __align__(256) double scratch[NOALLOC_STAGING_BYTES / sizeof(double)];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[NOALLOC_MAX_DEVICES];
+
+static int scratch_for(cumc_ctx *ctx, double **out)
+{
+    int dev = -1;
+    HIPCHK(ctx, hipGetDevice(&dev));
+    if (dev != ctx->device)
+        return fail(ctx, CUMC_EINVAL, "cumc_download_noalloc: make the context's device current first");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(ctx, CUMC_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ctx, CUMC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return CUMC_OK;
+}
+
+// SoA planes -> caller's AoS host array through the staging buffer, nl whole planes at a time
+static int stage_out(cumc_ctx *ctx, double *stg, const double *src, int nv, double *host)
+{
+    const Lay &L = ctx->L;
+    const long sx = L.nx + 6, sy = L.ny + 6, sz = L.nz + 6;
+    const long per_plane = sx * sy * nv;
+    const long cap = (long)(NOALLOC_STAGING_BYTES / sizeof(double)) / per_plane;
+    for (long l0 = 0; l0 < sz; l0 += cap) {
+        const long nl = (sz - l0 < cap) ? sz - l0 : cap;
+        Lay S = L;
+        S.nz = (int)nl - 6;  // k_soa_to_aos covers nz + 6 planes
+        hipLaunchKernelGGL(k_soa_to_aos, dim3(nblk(sx * sy * nl, 256)), dim3(256), 0, ctx->stream, S,
+                           src + l0 * L.py, nv, stg);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(host + l0 * per_plane, stg, (size_t)(nl * per_plane) * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // the next slab reuses the buffer
+    }
+    return CUMC_OK;
+}
+
+extern "C" int cumc_download_noalloc(cumc_ctx *ctx, double *prim, double *eps, double *bcell)
+{
+    if (!ctx) return CUMC_EINVAL;
+    const Lay &L = ctx->L;
+    const long plane = (long)(L.nx + 6) * (L.ny + 6);
+    const int nv_max = prim ? 8 : (bcell ? 3 : 1);
+    if (plane * nv_max * (long)sizeof(double) > (long)NOALLOC_STAGING_BYTES)
+        return fail(ctx, CUMC_EINVAL, "cumc_download_noalloc: one plane of the requested arrays exceeds the "
+                                      "32 MiB staging buffer ((nx+6)(ny+6) <= 524288 with prim)");
+    double *stg = nullptr;
+    int r = scratch_for(ctx, &stg);
+    if (r) return r;
+    sync_prim64(ctx);
+    if (prim && (r = stage_out(ctx, stg, ctx->F64.P, 8, prim))) return r;
+    if (eps && (r = stage_out(ctx, stg, ctx->F64.EPS, 1, eps))) return r;
+    if (bcell && (r = stage_out(ctx, stg, ctx->F64.BC, 3, bcell))) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return CUMC_OK;
 }
 
@@ -558,8 +636,8 @@ extern "C" int cumc_set_tuning(cumc_ctx *ctx, const cumc_tuning *t)
 }
 
 // Tuner: one warm-up step, then one timed step per candidate with per-kernel-class HIP events:
-// the segment length is scored by the y + z sweeps, the plane chunk by the EMF + stage-tail
-// kernels.  The FP64 fields (and the mixed form's FP32 primitive planes), the dt slots and the
+// the segment length is scored by the y + z sweeps, the plane chunk and column tiles by the EMF +
+// stage-tail kernels.  The FP64 fields (and the mixed form's FP32 primitive planes), the dt slots and the
 // U0/U1 assignment are saved and restored, so the state is as before; the per-class timing
 // totals are cleared.
 extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)

@@ -66,6 +66,7 @@ def lib():
             "cumc_set_grid": [_CTX, _D, _D, _D],
             "cumc_upload": [_CTX, _D, _D],
             "cumc_download": [_CTX, _D, _D, _D],
+            "cumc_download_noalloc": [_CTX, _D, _D, _D],
             "cumc_initial_update": [_CTX],
             "cumc_finddt": [_CTX, _D],
             "cumc_rungekutta": [_CTX, ctypes.c_double],

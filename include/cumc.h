@@ -72,6 +72,15 @@ int cumc_upload(cumc_ctx *ctx, const double *prim, const double *eps);
 /* !$ACC UPDATE HOST(prim, eps) of print_hydroprofile (solver/dataio.f90:24); bcell may be NULL. */
 int cumc_download(cumc_ctx *ctx, double *prim, double *eps, double *bcell);
 
+/* cumc_download without any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The conversion to the host layout is staged through a fixed 32 MiB device
+ * buffer that the library holds per GPU for the life of the process, slab by slab of whole planes;
+ * one plane of each requested array must fit ((nx+6)*(ny+6)*8 doubles <= 32 MiB with prim, else
+ * CUMC_EINVAL).  Call it with the context's device current.  Calls on one GPU share the buffer:
+ * calls that overlap in time on the same GPU (from several host threads) race. */
+int cumc_download_noalloc(cumc_ctx *ctx, double *prim, double *eps, double *bcell);
+
 /* initial_update (solver/initial.f90:83-140): CUSTOM_CHECKRHO, FROMRVETOU, BOUNDARY, UPDATE(0). */
 int cumc_initial_update(cumc_ctx *ctx);
 
