@@ -144,11 +144,32 @@ def pmc_traffic(ic_dir, form, steps=2, timeout=300):
             return None
         tail = [v for _, n, v in rows if "k_tail<" in n]
         step = [v for _, n, v in rows[first:] if STEP_KERNELS.search(n)]
-        res[ctr] = (float(1 << 30) / 1024.0 / cp[0], sum(tail) / len(tail), len(tail), sum(step))
-    (ff, tf, nt, sf), (fw, tw, _, sw) = res["FETCH_SIZE"], res["WRITE_SIZE"]
+        classes = {}
+        for _, n, v in rows[first:]:
+            c = kernel_class(n)
+            if c:
+                classes.setdefault(c, []).append(v)
+        res[ctr] = (float(1 << 30) / 1024.0 / cp[0], sum(tail) / len(tail), len(tail), sum(step), classes)
+    (ff, tf, nt, sf, cf), (fw, tw, _, sw, cw) = res["FETCH_SIZE"], res["WRITE_SIZE"]
+    by_class = {c: (ff * sum(cf[c]) / len(cf[c]) + fw * sum(cw.get(c, [0.0])) / max(1, len(cw.get(c, [])))) * 1024.0
+                for c in cf}
     return {"tail_bytes_per_launch": (ff * tf + fw * tw) * 1024.0, "tail_launches": nt,
-            "step_bytes": (ff * sf + fw * sw) * 1024.0 / steps, "fetch_factor": ff, "write_factor": fw,
+            "step_bytes": (ff * sf + fw * sw) * 1024.0 / steps, "bytes_per_launch_by_class": by_class,
+            "fetch_factor": ff, "write_factor": fw,
             "counters": "rocprofv3 --pmc FETCH_SIZE, then --pmc WRITE_SIZE, KiB per dispatch x factor x 1024"}
+
+
+def kernel_class(name):
+    """Kernel class of a dispatch name (the classes of the library's HIP-event timing)."""
+    if "k_flux_x<" in name:
+        return "flux_x"
+    if "k_flux_walk<" in name:
+        m = re.search(r"k_flux_walk<(?:double|float), (\d)", name)
+        return {"2": "flux_y", "3": "flux_z"}.get(m.group(1)) if m else None
+    for key, c in (("k_ecorn<", "ecorn"), ("k_tail<", "tail"), ("k_ghost", "ghost")):
+        if key in name:
+            return c
+    return None
 
 
 def traffic_child(args):
@@ -626,6 +647,7 @@ def main():
         r = {"value": cfg.zones * steps / wall, "ms_per_step": wall / steps * 1e3,
              "roofline": roofline(kt, cfg, form, traffic=tr),
              "step_traffic_bytes_per_zone_cycle": (tr["step_bytes"] / cfg.zones) if tr else None,
+             "traffic_bytes_per_launch_by_kernel": tr["bytes_per_launch_by_class"] if tr else None,
              "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items()}, "divb_max_abs": dabs,
              "state_finite": finite, "tuning": tuning}
         kio = r["roofline"]["kernel_io"]
@@ -699,7 +721,8 @@ def main():
         "cpu_baseline": cpu,
         "step_roofline": {"bytes_per_zone_cycle": BYTES_PER_ZONE_CYCLE, "achieved_GBps": step_alg / 1e9,
                           "frac": step_alg / HBM_PEAK,
-                          "traffic_bytes_per_zone_cycle": m["step_traffic_bytes_per_zone_cycle"]},
+                          "traffic_bytes_per_zone_cycle": m["step_traffic_bytes_per_zone_cycle"],
+                          "traffic_bytes_per_launch_by_kernel": m["traffic_bytes_per_launch_by_kernel"]},
         "kernel_ms_per_step": m["kernel_ms_per_step"],
         "tuning": m["tuning"],
         "divb_max_abs": m["divb_max_abs"],

@@ -66,6 +66,11 @@ def test_counter_csv_parsing(tmp_path):
     assert [r[0] for r in got] == [1, 2, 3]
     assert got[0][1].startswith("k_copy_probe") and got[2][2] == 100.5
     assert b.STEP_KERNELS.search(got[2][1]) and not b.STEP_KERNELS.search(got[0][1])
+    names = {"cumc::k_flux_x<double, 4, 2>(cumc::Lay)": "flux_x", "cumc::k_flux_walk<double, 2, 4, 2>(x)": "flux_y",
+             "cumc::k_flux_walk<float, 3, 1, 2>(x)": "flux_z", "cumc::k_ecorn<double>(x)": "ecorn",
+             "cumc::k_tail<float, 3, 1>(x)": "tail", "cumc::k_ghost32<2>(x)": "ghost", "k_copy_probe(x)": None}
+    for n, c in names.items():
+        assert b.kernel_class(n) == c, n
 
 
 def test_cpu_baseline_fields():
