@@ -408,7 +408,7 @@ def _interior(cfg):
     return (slice(3, 3 + cfg.nz), slice(3, 3 + cfg.ny), slice(3, 3 + cfg.nx))
 
 
-def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-5, 1e-6, 1e-7), store_all=True):
+def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-4, 1e-5, 1e-6), store_all=True):
     """Reverse mode on the batched boxes: J_b = sum |B|^2/2 dV per box after `steps` frozen-dt
     steps; per-box gradients with respect to the initial velocity from the batched tape
     (cumc_batch_tape_run / cumc_batch_tape_backward, checkpoints every `every` steps).  Cost:
@@ -421,6 +421,7 @@ def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-5, 1
         s.initial_update()
     batch.run(steps, graph_steps=every, dts=dts)
     fwd_ms = batch.last_ms()
+    ref_bcell = [s.download()["bcell"][_interior(p.cfg)] for s, p in zip(boxes, probs)]
     for s, p in zip(boxes, probs):
         s.upload(p.prim, p.eps)
     t0 = time.perf_counter()
@@ -472,48 +473,132 @@ def bench_adjoint(probs, boxes, batch, dts, steps, every=8, ndir=32, hs=(1e-5, 1
         finally:
             batch.tape_free()
 
-    def fields_after(vs):
-        for s, p, v in zip(boxes, probs, vs):
-            prim = p.prim.copy()
-            prim[_interior(p.cfg) + (slice(1, 4),)] = v
-            s.upload(prim, p.eps)
-            s.initial_update()
-        batch.run(steps, graph_steps=every, dts=dts)
-        return [s.download()["bcell"][_interior(p.cfg)] for s, p in zip(boxes, probs)]
-
-    v0 = [p.prim[_interior(p.cfg) + (slice(1, 4),)].copy() for p in probs]
-    rng = np.random.default_rng(1004)
-    # two error measures per direction d: against |g.d| (which cancels strongly over the ~3.7 M
-    # random-signed terms of g.d) and against sum |g_i d_i| (the size of the terms themselves)
-    rel = {h: [] for h in hs}
-    rel_terms = {h: [] for h in hs}
-    for d in range(ndir):
-        dirs = [rng.normal(size=g.shape) for g in grads]
-        nrm = np.sqrt(sum(float(np.sum(x * x)) for x in dirs))
-        dirs = [x / nrm for x in dirs]
-        ad = sum(float(np.sum(g * x)) for g, x in zip(grads, dirs))
-        mag = sum(float(np.sum(np.abs(g * x))) for g, x in zip(grads, dirs))
-        for h in hs:
-            bp = fields_after([v + h * x for v, x in zip(v0, dirs)])
-            bm = fields_after([v - h * x for v, x in zip(v0, dirs)])
-            dj = sum(0.5 * float(np.sum((a - c) * (a + c))) * w for a, c, w in zip(bp, bm, dvs))
-            fd = dj / (2 * h)
-            rel[h].append(abs(fd - ad) / max(abs(fd), abs(ad)))
-            rel_terms[h].append(abs(fd - ad) / mag)
-
-    def summarise(r):
-        return {f"{h:.0e}": {"median": float(np.median(v)), "max": float(np.max(v)),
-                             "within_1e-6": int(np.sum(np.array(v) <= 1e-6))} for h, v in r.items() if v}
-
-    summary, summary_terms = summarise(rel), summarise(rel_terms)
-    log(f"[bench] adjoint vs central differences ({ndir} unit directions), relative to |g.d|: {summary}")
-    log(f"[bench] the same relative to sum |g_i d_i|: {summary_terms}")
+    fd = fd_check(probs, grads, dvs, dts, steps, every, ndir, hs, ref_bcell, tuning=batch.tuning())
     return {"objective": "per box sum |B|^2/2 dV after the steps; gradient w.r.t. initial velocity",
             "steps": steps, "checkpoint_every": every, "forward_ms": fwd_ms, "backward_ms": back_ms,
             "backward_wall_ms": t_back * 1e3, "tape_forward_wall_ms": t_tape * 1e3,
             "adjoint_over_forward": back_ms / fwd_ms, "recompute_ms": recompute_ms,
-            "reverse_over_forward": (back_ms - recompute_ms) / fwd_ms, "all_states_kept": stored,
-            "fd_rel_error_by_h": summary, "fd_error_over_term_sum_by_h": summary_terms, "directions": ndir}
+            "reverse_over_forward": (back_ms - recompute_ms) / fwd_ms, "all_states_kept": stored, **fd}
+
+
+def fd_check(probs, grads, dvs, dts, steps, every, ndir, hs, ref_bcell, tuning=None):
+    """Central differences of J = sum_b J_b along ndir seeded unit directions of the concatenated
+    initial-velocity space, against the reverse-mode g.d, with every forward run's per-box branch
+    signature.  The runs use the diagnostic build libcumc_sig.so (the same arithmetic; its base
+    run is checked bit for bit against the product build's final state), which hashes every
+    limiter / Riemann-region / floor decision per box.  A box-direction pair is flip-free at h
+    when the +h and -h runs both reproduce the unperturbed run's signature: then the difference
+    crosses no kink of the map inside that box, and what remains of the error is truncation and
+    the rounding of the forward runs.  At the smallest h the rounding part is measured: the
+    difference is taken again at h' = h (1 + 2^-6), whose runs round independently while the
+    smooth part of the difference moves by O(h^2 / 64), so |FD(h) - FD(h')| / sqrt(2) is the size
+    of the rounding error of one central difference.  Errors are relative to |g.d| (per pair,
+    g_b.d_b) and, for the directions, also to sum |g_i d_i|."""
+    from cumpic_amd.cumc import Batch, Solver
+    if ndir == 0:
+        return {"fd_rel_error_by_h": {}, "fd_error_over_term_sum_by_h": {}, "directions": 0}
+    sboxes = [Solver(p.cfg, p.xf, p.yf, p.zf, variant="sig") for p in probs]
+    sb = Batch(sboxes)
+    if tuning:
+        sb.set_tuning(tuning)
+
+    def run(vs):
+        for s, p, v in zip(sboxes, probs, vs):
+            prim = p.prim.copy()
+            prim[_interior(p.cfg) + (slice(1, 4),)] = v
+            s.upload(prim, p.eps)
+            s.initial_update()
+        sb.branch_sig()
+        sb.run(steps, graph_steps=every, dts=dts)
+        sig = sb.branch_sig()
+        return [s.download()["bcell"][_interior(p.cfg)] for s, p in zip(sboxes, probs)], sig
+
+    v0 = [p.prim[_interior(p.cfg) + (slice(1, 4),)].copy() for p in probs]
+    b0, sig0 = run(v0)
+    same_build = all(np.array_equal(a, c) for a, c in zip(b0, ref_bcell))
+    log(f"[bench] FD check: diagnostic build bit-identical to the product build on the base run: {same_build}")
+    rng = np.random.default_rng(1004)
+    nb = len(probs)
+    rel = {h: [] for h in hs}
+    rel_terms = {h: [] for h in hs}
+    dir_flip_free = {h: [] for h in hs}
+    pair_rel = {h: [] for h in hs}
+    pair_free = {h: [] for h in hs}
+    h_min = min(hs)
+    spread_free, spread_err, spread_noise = [], [], []
+
+    def half_diff(a, c, w):
+        return 0.5 * float(np.sum((a - c) * (a + c))) * w
+
+    for d in range(ndir):
+        dirs = [rng.normal(size=g.shape) for g in grads]
+        nrm = np.sqrt(sum(float(np.sum(x * x)) for x in dirs))
+        dirs = [x / nrm for x in dirs]
+        adb = np.array([float(np.sum(g * x)) for g, x in zip(grads, dirs)])
+        ad = float(adb.sum())
+        mag = sum(float(np.sum(np.abs(g * x))) for g, x in zip(grads, dirs))
+        def central(h):
+            bp, sp = run([v + h * x for v, x in zip(v0, dirs)])
+            bm, sm = run([v - h * x for v, x in zip(v0, dirs)])
+            djb = np.array([half_diff(a, c, w) for a, c, w in zip(bp, bm, dvs)])
+            return djb / (2 * h), float(djb.sum()) / (2 * h), (sp == sig0) & (sm == sig0)
+
+        for h in hs:
+            fdb, fd, free = central(h)
+            rel[h].append(abs(fd - ad) / max(abs(fd), abs(ad)))
+            rel_terms[h].append(abs(fd - ad) / mag)
+            dir_flip_free[h].append(bool(free.all()))
+            pair_rel[h].extend((np.abs(fdb - adb) / np.abs(adb)).tolist())
+            pair_free[h].extend(free.tolist())
+            if h == h_min:
+                fdb2, _, free2 = central(h * (1.0 + 2.0 ** -6))
+                spread_free.extend((free & free2).tolist())
+                spread_err.extend((np.abs(fdb - adb) / np.abs(adb)).tolist())
+                spread_noise.extend((np.abs(fdb - fdb2) / np.sqrt(2.0) / np.abs(adb)).tolist())
+    sb.close()
+    for s in sboxes:
+        s.close()
+
+    def stats(v):
+        v = np.asarray(v, dtype=float)
+        if v.size == 0:
+            return {"n": 0}
+        return {"n": int(v.size), "median": float(np.median(v)), "max": float(np.max(v)),
+                "within_1e-6": int(np.sum(v <= 1e-6))}
+
+    def summarise(r):
+        return {f"{h:.0e}": stats(v) for h, v in r.items() if v}
+
+    summary, summary_terms = summarise(rel), summarise(rel_terms)
+    by_h = {}
+    for h in hs:
+        e = np.array(pair_rel[h])
+        f = np.array(pair_free[h], dtype=bool)
+        de = np.array(rel[h])
+        df = np.array(dir_flip_free[h], dtype=bool)
+        by_h[f"{h:.0e}"] = {
+            "directions_flip_free": int(df.sum()),
+            "directions_flip_free_error": stats(de[df]),
+            "box_pairs": int(e.size),
+            "box_pairs_flip_free": int(f.sum()),
+            "flip_free_error": stats(e[f]),
+            "flipped_error": stats(e[~f]),
+        }
+    sf = np.array(spread_free, dtype=bool)
+    se, sn = np.array(spread_err), np.array(spread_noise)
+    rounding = {"h": h_min, "h2": h_min * (1.0 + 2.0 ** -6), "box_pairs_flip_free_at_both": int(sf.sum()),
+                "error": stats(se[sf]), "rounding": stats(sn[sf]),
+                "error_over_rounding_median": float(np.median(se[sf] / sn[sf])) if sf.any() else None}
+    log(f"[bench] adjoint vs central differences ({ndir} unit directions), relative to |g.d|: {summary}")
+    log(f"[bench] the same relative to sum |g_i d_i|: {summary_terms}")
+    for h, r in by_h.items():
+        log(f"[bench] FD h={h}: {r['box_pairs_flip_free']}/{r['box_pairs']} box-direction pairs flip-free; "
+            f"flip-free error {r['flip_free_error']}; flipped error {r['flipped_error']}; "
+            f"flip-free directions {r['directions_flip_free']}/{ndir}")
+    log(f"[bench] FD rounding at h={h_min:.0e} on pairs flip-free at h and h': {rounding}")
+    return {"fd_rel_error_by_h": summary, "fd_error_over_term_sum_by_h": summary_terms, "directions": ndir,
+            "fd_branch_signature_by_h": by_h, "fd_rounding_at_smallest_h": rounding,
+            "fd_diagnostic_build_bitwise": bool(same_build)}
 
 
 def main():

@@ -36,20 +36,30 @@ def _run(cmd):
     return r.stdout
 
 
-def build_library(verbose: bool = False) -> str:
+# The diagnostic build: the same sources with the branch-signature hooks compiled in
+# (CUMC_BRANCH_SIG; cumc_batch_branch_sig records).  Only the finite-difference diagnostics of the
+# reverse mode load it.
+LIB_SIG = os.path.join(PKG, "libcumc_sig.so")
+
+
+def build_library(verbose: bool = False, variant: str = "") -> str:
+    """libcumc.so, or with variant="sig" the diagnostic libcumc_sig.so.  The units compile in
+    parallel (one hipcc process each)."""
+    from concurrent.futures import ThreadPoolExecutor
     inc = ["-I", os.path.join(ROOT, "include")]
+    extra = ["-DCUMC_BRANCH_SIG"] if variant == "sig" else []
+    target = LIB_SIG if variant == "sig" else LIB
     with tempfile.TemporaryDirectory() as td:
-        objs = []
-        for src, flags in UNITS:
-            obj = os.path.join(td, src + ".o")
-            out = _run([HIPCC, *COMMON, *flags, *inc, "-c", "-o", obj, os.path.join(PKG, "csrc", src)])
-            if verbose:
-                print(out)
-            objs.append(obj)
-        tmp = LIB + ".tmp"
-        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
-    return LIB
+        cmds = [[HIPCC, *COMMON, *flags, *extra, *inc, "-c", "-o", os.path.join(td, src + ".o"),
+                 os.path.join(PKG, "csrc", src)] for src, flags in UNITS]
+        with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+            outs = list(ex.map(_run, cmds))
+        if verbose:
+            print("\n".join(outs))
+        tmp = target + ".tmp%d" % os.getpid()
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *[c[c.index("-o") + 1] for c in cmds]])
+    os.replace(tmp, target)
+    return target
 
 
 def flang() -> str | None:

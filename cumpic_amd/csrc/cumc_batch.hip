@@ -34,6 +34,7 @@ struct BoxDesc {
     int fy_gx, fy_sl;      // y sweep: tiles per segment row, faces per segment
     int fz_gx, fz_sl;      // z sweep
     int c_gx, zc, tw;      // EMF / stage tail column tiles
+    unsigned long long *sig;  // branch signature of the box (diagnostic build only)
 };
 
 enum { P_FX, P_FY, P_FZ, P_EC, P_TAIL, P_GX, P_GY, P_GZ, P_DT, P_N };
@@ -60,15 +61,41 @@ __device__ inline Fields<double> fields_at(const BoxDesc &d, int par)
     return F;
 }
 
+#ifdef CUMC_BRANCH_SIG
+__device__ inline unsigned long long br_mix(unsigned long long z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// Adds this thread's decision hash, salted with its work item and thread index, to the box's
+// signature (one atomic per wavefront) and restarts the hash.  Every lane of the wavefront calls it.
+__device__ inline void br_flush(unsigned long long *sig, int it)
+{
+    unsigned long long h =
+        br_mix(br_slot[threadIdx.x] + 0x9E3779B97F4A7C15ull * ((unsigned long long)it * 256u + threadIdx.x + 1u));
+    br_slot[threadIdx.x] = BR_SEED;
+    for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(sig, h);
+}
+#define BR_START() (br_slot[threadIdx.x] = BR_SEED)
+#define BR_FLUSH(d, it) br_flush((d).sig, it)
+#else
+#define BR_START() ((void)0)
+#define BR_FLUSH(d, it) ((void)0)
+#endif
+
 template <int RECON>
 __global__ __launch_bounds__(256) void kb_flux_x(const BoxDesc *__restrict__ D, const int *__restrict__ pre, int nbox)
 {
     const int total = pre[nbox];
+    BR_START();
     for (int it = blockIdx.x; it < total; it += gridDim.x) {
         const int b = box_of(pre, nbox, it);
         const int loc = it - pre[b];
         const BoxDesc &d = D[b];
         flux_x_tile<double, RECON>(d.L, d.F, d.tb, d.M, loc % d.fx_gx, loc / d.fx_gx);
+        BR_FLUSH(d, it);
     }
 }
 
@@ -77,6 +104,7 @@ __global__ __launch_bounds__(128) void kb_flux_walk(const BoxDesc *__restrict__ 
                                                     int nbox)
 {
     const int total = pre[nbox];
+    BR_START();
     for (int it = blockIdx.x; it < total; it += gridDim.x) {
         const int b = box_of(pre, nbox, it);
         const int loc = it - pre[b];
@@ -84,6 +112,7 @@ __global__ __launch_bounds__(128) void kb_flux_walk(const BoxDesc *__restrict__ 
         const int gx = DIR == 2 ? d.fy_gx : d.fz_gx;
         const int sl = DIR == 2 ? d.fy_sl : d.fz_sl;
         flux_walk_tile<double, DIR, RECON>(d.L, d.F, d.tb, d.M, sl, loc % gx, loc / gx);
+        BR_FLUSH(d, it);
     }
 }
 
@@ -104,6 +133,7 @@ __global__ __launch_bounds__(256) void kb_tail(const BoxDesc *__restrict__ D, co
                                                int par)
 {
     const int total = pre[nbox];
+    BR_START();
     for (int it = blockIdx.x; it < total; it += gridDim.x) {
         const int b = box_of(pre, nbox, it);
         const int loc = it - pre[b];
@@ -113,6 +143,7 @@ __global__ __launch_bounds__(256) void kb_tail(const BoxDesc *__restrict__ D, co
         double *Uout = STAGE == 2 ? d.Ub : F.U1;
         tail_tile<double, STAGE, COORD>(d.L, F, d.tb, d.M, d.dt, Uin, Uout, (unsigned long long *)(d.dt + 1),
                                         loc % d.c_gx, loc / d.c_gx, d.zc, d.tw);
+        BR_FLUSH(d, it);
     }
 }
 
@@ -176,6 +207,7 @@ struct cumc_batch {
     BoxDesc *d_desc = nullptr;
     int *d_pre = nullptr;  // P_N arrays of nbox+1
     int *d_ctr = nullptr;
+    unsigned long long *d_sig = nullptr;  // per-box branch signatures (diagnostic build)
     double *d_hist = nullptr;
     int hist_cap = 0;      // steps
     std::vector<BoxDesc> h_desc;
@@ -293,6 +325,7 @@ static int refresh(cumc_batch *b)
         d.M = c->M;
         d.dt = c->dt_dev;
         d.Ub = c->Ub;
+        d.sig = b->d_sig + i;
         int cnt[P_N];
         box_tiles(b, c, d, cnt);
         for (int p = 0; p < P_N; p++) b->h_pre[(size_t)p * (nb + 1) + i + 1] = b->h_pre[(size_t)p * (nb + 1) + i] + cnt[p];
@@ -512,6 +545,8 @@ extern "C" int cumc_batch_create(cumc_ctx *const *boxes, int nbox, cumc_batch **
     if (hipMalloc(&b->d_desc, sizeof(BoxDesc) * nbox) != hipSuccess) return bad(CUMC_ENOMEM);
     if (hipMalloc(&b->d_pre, sizeof(int) * P_N * (nbox + 1)) != hipSuccess) return bad(CUMC_ENOMEM);
     if (hipMalloc(&b->d_ctr, 64) != hipSuccess) return bad(CUMC_ENOMEM);
+    if (hipMalloc(&b->d_sig, sizeof(unsigned long long) * nbox) != hipSuccess) return bad(CUMC_ENOMEM);
+    if (hipMemset(b->d_sig, 0, sizeof(unsigned long long) * nbox) != hipSuccess) return bad(CUMC_EHIP);
     if (refresh(b)) return bad(CUMC_EHIP);
     *out = b;
     return CUMC_OK;
@@ -526,6 +561,7 @@ extern "C" int cumc_batch_destroy(cumc_batch *b)
     hipFree(b->d_desc);
     hipFree(b->d_pre);
     hipFree(b->d_ctr);
+    hipFree(b->d_sig);
     hipFree(b->d_hist);
     for (hipEvent_t e : b->evp) hipEventDestroy(e);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -617,6 +653,20 @@ extern "C" int cumc_batch_last_ms(cumc_batch *b, double *ms)
 }
 
 extern "C" const char *cumc_batch_last_error(cumc_batch *b) { return b ? b->err.c_str() : "no batch"; }
+
+extern "C" int cumc_batch_branch_sig(cumc_batch *b, unsigned long long *sig)
+{
+    if (!b || !sig) return CUMC_EINVAL;
+#ifdef CUMC_BRANCH_SIG
+    const size_t bytes = sizeof(unsigned long long) * b->nbox;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    BCHK(b, hipMemcpy(sig, b->d_sig, bytes, hipMemcpyDeviceToHost));
+    BCHK(b, hipMemset(b->d_sig, 0, bytes));
+    return CUMC_OK;
+#else
+    return bfail(b, CUMC_EINVAL, "branch signatures are recorded only by the diagnostic build libcumc_sig.so");
+#endif
+}
 
 // ---------------------------------------------------------------------------------------------
 // Tuner: saves every box's FP64 fields and dt slots, times candidate shapes phase by phase with

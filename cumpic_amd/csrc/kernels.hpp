@@ -717,10 +717,10 @@ __device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &t
     double rho = rho0;
     if (interior) {
         if (M.model != 0) {
-            if (rho < M.rho_floor) rho = M.rho_floor;
-            if (eps < M.eps_floor) eps = M.eps_floor;
+            if (br(rho < M.rho_floor)) rho = M.rho_floor;
+            if (br(eps < M.eps_floor)) eps = M.eps_floor;
         } else {
-            eps = fmax2<double>(eps, 0.0);
+            eps = fmax2b<double>(eps, 0.0);
         }
     }
     vsq = vx * vx + vy * vy + vz * vz;

@@ -19,6 +19,47 @@ template <> __host__ __device__ inline float rsqrt_(float x) { return sqrtf(x); 
 // Fortran MIN/MAX/SIGN on non-NaN operands (identical definition in the CPU restatement)
 template <typename R> __host__ __device__ inline R fmin2(R a, R b) { return (b < a) ? b : a; }
 template <typename R> __host__ __device__ inline R fmax2(R a, R b) { return (b > a) ? b : a; }
+
+// Branch signature, compiled only into the diagnostic build (-DCUMC_BRANCH_SIG): each
+// data-dependent decision of the reference path (the PPM / TVD-minmod limiter switches, the HLL
+// speed selections and region, the signal-speed discriminant clamp, the floors) is folded into a
+// per-thread hash kept in LDS, and the batched forward kernels add every thread's hash into a
+// per-box sum after each tile.  Two runs with equal sums took the same branch everywhere (up to
+// 64-bit hash collisions), so a central difference between them crosses no kink of the map.
+// In every other build br() returns its argument and the LDS slot does not exist.
+#ifdef CUMC_BRANCH_SIG
+constexpr unsigned long long BR_SEED = 0x243F6A8885A308D3ull;
+static __shared__ unsigned long long br_slot[256];
+#endif
+__host__ __device__ inline bool br(bool c)
+{
+#if defined(CUMC_BRANCH_SIG) && defined(__HIP_DEVICE_COMPILE__)
+    unsigned long long &h = br_slot[threadIdx.x];
+    h = (h ^ (c ? 0x9E3779B97F4A7C15ull : 0x632BE59BD9B4E019ull)) * 0x100000001B3ull;
+#endif
+    return c;
+}
+// MIN / MAX whose selection is a recorded decision.  Operands equal to within a few hundred
+// units in the last place count as a tie, recorded as such: a selection between them can follow
+// the rounding of the run (as between the two sides of a face whose states are copies), and
+// changes the result by less than the band, so it is no kink.
+__host__ __device__ inline void br_sel(double a, double b)
+{
+#if defined(CUMC_BRANCH_SIG) && defined(__HIP_DEVICE_COMPILE__)
+    const double w = 4.0e-14 * (fabs(a) + fabs(b));
+    if (br(fabs(a - b) > w)) br(b < a);
+#endif
+}
+template <typename R> __host__ __device__ inline R fmin2b(R a, R b)
+{
+    br_sel((double)a, (double)b);
+    return (b < a) ? b : a;
+}
+template <typename R> __host__ __device__ inline R fmax2b(R a, R b)
+{
+    br_sel((double)a, (double)b);
+    return (b > a) ? b : a;
+}
 template <typename R> __host__ __device__ inline R fabs_(R a);
 template <> __host__ __device__ inline double fabs_(double a) { return fabs(a); }
 template <> __host__ __device__ inline float fabs_(float a) { return fabsf(a); }
@@ -44,7 +85,7 @@ __host__ __device__ inline R signalspeed(R cs, R bn, R bt1, R bt2, R rho)
     R b2t2 = (bt2 * bt2 / rho);
     R b2 = b2n + b2t1 + b2t2;
     R b4 = b2 * b2;
-    return rsqrt_<R>(R(0.5) * (a2 + b2 + rsqrt_<R>(fmax2<R>((a4 + b4 + R(2.0) * b2 * a2) - R(4.0) * a2 * b2n, R(0)))));
+    return rsqrt_<R>(R(0.5) * (a2 + b2 + rsqrt_<R>(fmax2b<R>((a4 + b4 + R(2.0) * b2 * a2) - R(4.0) * a2 * b2n, R(0)))));
 }
 
 // PPM interpolation coefficients of one cell (reconstruct.f90:211-249), hoisted out of the
@@ -99,9 +140,10 @@ template <typename R>
 __host__ __device__ inline R ppm_slope(R b0, R b1, R vm1, R vc, R vp1)
 {
     R delta = b0 * (vp1 - vc) + b1 * (vc - vm1);
-    if ((vp1 - vc) * (vc - vm1) > R(0)) {
+    if (br((vp1 - vc) * (vc - vm1) > R(0))) {
         R a = fabs_<R>(delta), b = R(2.0) * fabs_<R>(vc - vm1), c = R(2.0) * fabs_<R>(vc - vp1);
-        return fmin2<R>(fmin2<R>(a, b), c) * sign1<R>(delta);
+        br(delta < R(0));
+        return fmin2b<R>(fmin2b<R>(a, b), c) * sign1<R>(delta);
     }
     return R(0);
 }
@@ -114,12 +156,12 @@ __host__ __device__ inline void ppm_edges(R a0m, R a1m, R a2m, R a3m, R a0p, R a
     R vp = vc + a0p * (vp1 - vc) + a1p * (vp1 - vc) + a2p * dmp1 + a3p * dmc;
     R vm = vm1 + a0m * (vc - vm1) + a1m * (vc - vm1) + a2m * dmc + a3m * dmm1;
     R vl = vm, vr = vp;
-    if ((vr - vc) * (vc - vl) <= R(0)) {
+    if (br((vr - vc) * (vc - vl) <= R(0))) {
         vp = vc;
         vm = vc;
     } else {
-        if ((vr - vl) * (vl - R(3.0) * vc + R(2.0) * vr) < R(0)) vm = R(3.0) * vc - R(2.0) * vr;
-        if ((vp - vm) * (R(3.0) * vc - R(2.0) * vl - vr) < R(0)) vp = R(3.0) * vc - R(2.0) * vl;
+        if (br((vr - vl) * (vl - R(3.0) * vc + R(2.0) * vr) < R(0))) vm = R(3.0) * vc - R(2.0) * vr;
+        if (br((vp - vm) * (R(3.0) * vc - R(2.0) * vl - vr) < R(0))) vp = R(3.0) * vc - R(2.0) * vl;
     }
     vm_out = vm;
     vp_out = vp;
@@ -131,7 +173,9 @@ __host__ __device__ inline void tvd_edges(R cf, R cb, R vm1, R vc, R vp1, R &vm_
 {
     R dqf = R(2.0) * (vp1 - vc) / cf;
     R dqb = R(2.0) * (vc - vm1) / cb;
-    R slope = R(0.5) * (sign1<R>(dqf) + sign1<R>(dqb)) * fmin2<R>(fabs_<R>(dqf), fabs_<R>(dqb));
+    br(dqf < R(0));
+    br(dqb < R(0));
+    R slope = R(0.5) * (sign1<R>(dqf) + sign1<R>(dqb)) * fmin2b<R>(fabs_<R>(dqf), fabs_<R>(dqb));
     vp_out = vc + R(0.5) * slope;
     vm_out = vc - R(0.5) * slope;
 }
@@ -278,15 +322,15 @@ __host__ __device__ inline void hll_face(R g, const FaceState<R> &L, const FaceS
     R sqL = rsqrt_<R>(L.rho), sqR = rsqrt_<R>(Rr.rho);
     R ubar = (sqL * L.vn + sqR * Rr.vn) / (sqL + sqR);
     R cbar = (sqL * cfsL + sqR * cfsR) / (sqL + sqR);
-    R sL = fmin2<R>(L.vn - cfsL, ubar - cbar);
-    R sR = fmax2<R>(Rr.vn + cfsR, ubar + cbar);
-    if (sL >= R(0)) {
+    R sL = fmin2b<R>(L.vn - cfsL, ubar - cbar);
+    R sR = fmax2b<R>(Rr.vn + cfsR, ubar + cbar);
+    if (br(sL >= R(0))) {
         for (int i = 0; i < 7; i++) f[i] = fL[i];
-    } else if (sL <= R(0) && sR >= R(0)) {
+    } else if (br(sL <= R(0) && sR >= R(0))) {
         for (int i = 0; i < 5; i++) f[i] = fluxhll<R>(fL[i], fR[i], uL[i], uR[i], sL, sR);
         f[5] = fluxhll<R>(fL[5], fR[5], L.bt1, Rr.bt1, sL, sR);
         f[6] = fluxhll<R>(fL[6], fR[6], L.bt2, Rr.bt2, sL, sR);
-    } else if (sR <= R(0)) {
+    } else if (br(sR <= R(0))) {
         for (int i = 0; i < 7; i++) f[i] = fR[i];
     } else {
         for (int i = 0; i < 7; i++) f[i] = R(NAN);

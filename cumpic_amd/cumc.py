@@ -17,7 +17,9 @@ from .config import Config, CConfig, to_cconfig
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcumc.so")
-_lib = None
+# diagnostic build of the same sources with branch signatures (cumc_batch_branch_sig)
+LIB_SIG_PATH = os.path.join(_HERE, "libcumc_sig.so")
+_libs = {}
 _D = ctypes.POINTER(ctypes.c_double)
 _CTX = ctypes.c_void_p
 
@@ -54,12 +56,13 @@ class Tuning(ctypes.Structure):
         return t
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise CumcError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
-        L = ctypes.CDLL(LIB_PATH)
+def lib(variant: str = ""):
+    """The loaded library: libcumc.so, or with variant="sig" the diagnostic libcumc_sig.so."""
+    path = LIB_SIG_PATH if variant == "sig" else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise CumcError(f"{path} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(path)
         sig = {
             "cumc_create": [ctypes.POINTER(CConfigP), ctypes.POINTER(_CTX)],
             "cumc_destroy": [_CTX],
@@ -112,8 +115,10 @@ def lib():
         L.cumc_error_string.argtypes = [ctypes.c_int]
         L.cumc_last_error.restype = ctypes.c_char_p
         L.cumc_last_error.argtypes = [_CTX]
-        _lib = L
-    return _lib
+        L.cumc_batch_branch_sig.argtypes = [_CTX, ctypes.POINTER(ctypes.c_uint64)]
+        L.cumc_batch_branch_sig.restype = ctypes.c_int
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a):
@@ -131,9 +136,15 @@ def copy_probe(nbytes: int, reps: int = 5) -> float:
 
 
 class Solver:
-    """One subdomain of the CUMC3D step resident on the current GPU."""
+    """One subdomain of the CUMC3D step resident on the current GPU.  variant="sig" creates it
+    in the diagnostic build libcumc_sig.so (same arithmetic, branch signatures recorded)."""
 
-    def __init__(self, cfg: Config, xf, yf, zf):
+    _L = None
+    h = None
+
+    def __init__(self, cfg: Config, xf, yf, zf, variant: str = ""):
+        self._L = lib(variant)
+        self.variant = variant
         self.cfg = cfg
         c = CConfigP()
         base = to_cconfig(cfg)
@@ -141,31 +152,31 @@ class Solver:
             setattr(c, name, getattr(base, name))
         c.precision = int(cfg.precision)
         h = _CTX()
-        r = lib().cumc_create(ctypes.byref(c), ctypes.byref(h))
+        r = self._L.cumc_create(ctypes.byref(c), ctypes.byref(h))
         if r:
-            raise CumcError(f"cumc_create: {lib().cumc_error_string(r).decode()}")
+            raise CumcError(f"cumc_create: {self._L.cumc_error_string(r).decode()}")
         self.h = h
-        self._check(lib().cumc_set_grid(self.h, *(_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (xf, yf, zf))),
+        self._check(self._L.cumc_set_grid(self.h, *(_p(np.ascontiguousarray(a, dtype=np.float64)) for a in (xf, yf, zf))),
                     "cumc_set_grid")
 
     @classmethod
-    def from_problem(cls, prob, precision: int | None = None):
+    def from_problem(cls, prob, precision: int | None = None, variant: str = ""):
         cfg = prob.cfg
         if precision is not None:
             import dataclasses
             cfg = dataclasses.replace(cfg, precision=precision)
-        s = cls(cfg, prob.xf, prob.yf, prob.zf)
+        s = cls(cfg, prob.xf, prob.yf, prob.zf, variant=variant)
         s.upload(prob.prim, prob.eps)
         return s
 
     def _check(self, r, what):
         if r:
-            msg = lib().cumc_last_error(self.h).decode() if self.h else ""
-            raise CumcError(f"{what}: {lib().cumc_error_string(r).decode()} {msg}")
+            msg = self._L.cumc_last_error(self.h).decode() if self.h else ""
+            raise CumcError(f"{what}: {self._L.cumc_error_string(r).decode()} {msg}")
 
     def close(self):
         if getattr(self, "h", None):
-            lib().cumc_destroy(self.h)
+            self._L.cumc_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -179,49 +190,49 @@ class Solver:
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         if prim.shape != self.cfg.shape + (8,) or eps.shape != self.cfg.shape:
             raise ValueError("prim/eps shapes do not match the configuration")
-        self._check(lib().cumc_upload(self.h, _p(prim), _p(eps)), "cumc_upload")
+        self._check(self._L.cumc_upload(self.h, _p(prim), _p(eps)), "cumc_upload")
 
     def download(self):
         prim = np.empty(self.cfg.shape + (8,))
         eps = np.empty(self.cfg.shape)
         bcell = np.empty(self.cfg.shape + (3,))
-        self._check(lib().cumc_download(self.h, _p(prim), _p(eps), _p(bcell)), "cumc_download")
+        self._check(self._L.cumc_download(self.h, _p(prim), _p(eps), _p(bcell)), "cumc_download")
         return {"prim": prim, "eps": eps, "bcell": bcell}
 
     def initial_update(self):
-        self._check(lib().cumc_initial_update(self.h), "cumc_initial_update")
+        self._check(self._L.cumc_initial_update(self.h), "cumc_initial_update")
 
     def boundary(self):
-        self._check(lib().cumc_boundary(self.h), "cumc_boundary")
+        self._check(self._L.cumc_boundary(self.h), "cumc_boundary")
 
     def finddt(self) -> float:
         d = ctypes.c_double()
-        self._check(lib().cumc_finddt(self.h, ctypes.byref(d)), "cumc_finddt")
+        self._check(self._L.cumc_finddt(self.h, ctypes.byref(d)), "cumc_finddt")
         return d.value
 
     def rungekutta(self, dt: float):
-        self._check(lib().cumc_rungekutta(self.h, float(dt)), "cumc_rungekutta")
+        self._check(self._L.cumc_rungekutta(self.h, float(dt)), "cumc_rungekutta")
 
     def run(self, max_steps: int, total_time: float = 1.0e300, t0: float = 0.0):
         t = ctypes.c_double(t0)
         n = ctypes.c_int(0)
         dts = np.zeros(max(1, max_steps))
-        self._check(lib().cumc_run(self.h, int(max_steps), float(total_time), ctypes.byref(t), ctypes.byref(n),
+        self._check(self._L.cumc_run(self.h, int(max_steps), float(total_time), ctypes.byref(t), ctypes.byref(n),
                                    _p(dts)), "cumc_run")
         return dts[: n.value].copy(), t.value
 
     def find_divb(self):
         a, b = ctypes.c_double(), ctypes.c_double()
-        self._check(lib().cumc_find_divb(self.h, ctypes.byref(a), ctypes.byref(b)), "cumc_find_divb")
+        self._check(self._L.cumc_find_divb(self.h, ctypes.byref(a), ctypes.byref(b)), "cumc_find_divb")
         return a.value, b.value
 
     def sweep_flux(self, direction: int):
         out = np.empty(self.cfg.shape + (7,))
-        self._check(lib().cumc_sweep_flux(self.h, int(direction), _p(out)), "cumc_sweep_flux")
+        self._check(self._L.cumc_sweep_flux(self.h, int(direction), _p(out)), "cumc_sweep_flux")
         return out
 
     def profile(self, on: bool = True):
-        self._check(lib().cumc_profile(self.h, 1 if on else 0), "cumc_profile")
+        self._check(self._L.cumc_profile(self.h, 1 if on else 0), "cumc_profile")
 
     def kernel_times(self):
         """{class: (total_ms, launches)} of the HIP-event timing since profile(True)."""
@@ -229,26 +240,26 @@ class Solver:
         ms = (ctypes.c_double * n)()
         cnt = (ctypes.c_long * n)()
         names = (ctypes.c_char_p * n)()
-        k = lib().cumc_kernel_times(self.h, n, ms, cnt, names)
+        k = self._L.cumc_kernel_times(self.h, n, ms, cnt, names)
         if k < 0:
             raise CumcError("cumc_kernel_times failed")
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
 
     def last_ms(self) -> float:
         d = ctypes.c_double()
-        self._check(lib().cumc_last_step_ms(self.h, ctypes.byref(d)), "cumc_last_step_ms")
+        self._check(self._L.cumc_last_step_ms(self.h, ctypes.byref(d)), "cumc_last_step_ms")
         return d.value
 
     def autotune(self) -> dict:
         t = Tuning()
-        self._check(lib().cumc_autotune(self.h, ctypes.byref(t)), "cumc_autotune")
+        self._check(self._L.cumc_autotune(self.h, ctypes.byref(t)), "cumc_autotune")
         return t.as_dict()
 
     def pattern_probe(self, nread: int, nwrite: int, reps: int = 5) -> float:
         """GB/s of the stage tail's access pattern (nread FP64 planes read, nwrite written per
         cell, no arithmetic) on this context's layout and column tiles."""
         g = ctypes.c_double()
-        self._check(lib().cumc_pattern_probe(self.h, int(nread), int(nwrite), int(reps), ctypes.byref(g)),
+        self._check(self._L.cumc_pattern_probe(self.h, int(nread), int(nwrite), int(reps), ctypes.byref(g)),
                     "cumc_pattern_probe")
         return g.value
 
@@ -261,7 +272,7 @@ class Solver:
             din = np.ascontiguousarray(dts, dtype=np.float64)
             if din.shape != (nsteps,):
                 raise ValueError(f"dts must have shape ({nsteps},)")
-        self._check(lib().cumc_tape_run(self.h, int(nsteps), int(every), _p(din) if din is not None else None,
+        self._check(self._L.cumc_tape_run(self.h, int(nsteps), int(every), _p(din) if din is not None else None,
                                         _p(out)), "cumc_tape_run")
         return out[:nsteps].copy()
 
@@ -281,46 +292,52 @@ class Solver:
         sp, se, sb = arr(seed_prim, 8), arr(seed_eps, 1), arr(seed_bcell, 3)
         gp = np.zeros(shp + (8,))
         ge = np.zeros(shp)
-        self._check(lib().cumc_tape_backward(self.h, _p(sp) if sp is not None else None,
+        self._check(self._L.cumc_tape_backward(self.h, _p(sp) if sp is not None else None,
                                              _p(se) if se is not None else None,
                                              _p(sb) if sb is not None else None, _p(gp), _p(ge)),
                     "cumc_tape_backward")
         return gp, ge
 
     def tape_free(self):
-        self._check(lib().cumc_tape_free(self.h), "cumc_tape_free")
+        self._check(self._L.cumc_tape_free(self.h), "cumc_tape_free")
 
     def tuning(self) -> dict:
         t = Tuning()
-        self._check(lib().cumc_get_tuning(self.h, ctypes.byref(t)), "cumc_get_tuning")
+        self._check(self._L.cumc_get_tuning(self.h, ctypes.byref(t)), "cumc_get_tuning")
         return t.as_dict()
 
     def set_tuning(self, d: dict):
         t = Tuning.from_dict(d)
-        self._check(lib().cumc_set_tuning(self.h, ctypes.byref(t)), "cumc_set_tuning")
+        self._check(self._L.cumc_set_tuning(self.h, ctypes.byref(t)), "cumc_set_tuning")
 
 
 class Batch:
     """Independent boxes advanced together: one persistent launch per phase over all boxes,
     optionally replayed from a hipGraph (cumc_batch_*).  The boxes stay ordinary Solvers."""
 
+    _L = None
+    h = None
+
     def __init__(self, solvers):
         self.solvers = list(solvers)
+        if len({s.variant for s in self.solvers}) != 1:
+            raise ValueError("the boxes of a batch must come from one library build")
+        self._L = self.solvers[0]._L
         arr = (_CTX * len(self.solvers))(*[s.h for s in self.solvers])
         h = _CTX()
-        r = lib().cumc_batch_create(arr, len(self.solvers), ctypes.byref(h))
+        r = self._L.cumc_batch_create(arr, len(self.solvers), ctypes.byref(h))
         if r:
-            raise CumcError(f"cumc_batch_create: {lib().cumc_error_string(r).decode()}")
+            raise CumcError(f"cumc_batch_create: {self._L.cumc_error_string(r).decode()}")
         self.h = h
 
     def _check(self, r, what):
         if r:
-            msg = lib().cumc_batch_last_error(self.h).decode() if self.h else ""
-            raise CumcError(f"{what}: {lib().cumc_error_string(r).decode()} {msg}")
+            msg = self._L.cumc_batch_last_error(self.h).decode() if self.h else ""
+            raise CumcError(f"{what}: {self._L.cumc_error_string(r).decode()} {msg}")
 
     def close(self):
         if getattr(self, "h", None):
-            lib().cumc_batch_destroy(self.h)
+            self._L.cumc_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -339,27 +356,27 @@ class Batch:
             din = np.ascontiguousarray(dts, dtype=np.float64)
             if din.shape != (nsteps, nb):
                 raise ValueError(f"dts must have shape ({nsteps}, {nb})")
-        self._check(lib().cumc_batch_run(self.h, int(nsteps), int(graph_steps), _p(din) if din is not None else None,
+        self._check(self._L.cumc_batch_run(self.h, int(nsteps), int(graph_steps), _p(din) if din is not None else None,
                                          _p(out)), "cumc_batch_run")
         return out[:nsteps].copy()
 
     def autotune(self) -> dict:
         t = Tuning()
-        self._check(lib().cumc_batch_autotune(self.h, ctypes.byref(t)), "cumc_batch_autotune")
+        self._check(self._L.cumc_batch_autotune(self.h, ctypes.byref(t)), "cumc_batch_autotune")
         return t.as_dict()
 
     def tuning(self) -> dict:
         t = Tuning()
-        self._check(lib().cumc_batch_get_tuning(self.h, ctypes.byref(t)), "cumc_batch_get_tuning")
+        self._check(self._L.cumc_batch_get_tuning(self.h, ctypes.byref(t)), "cumc_batch_get_tuning")
         return t.as_dict()
 
     def set_tuning(self, d: dict):
         t = Tuning.from_dict(d)
-        self._check(lib().cumc_batch_set_tuning(self.h, ctypes.byref(t)), "cumc_batch_set_tuning")
+        self._check(self._L.cumc_batch_set_tuning(self.h, ctypes.byref(t)), "cumc_batch_set_tuning")
 
     def last_ms(self) -> float:
         d = ctypes.c_double()
-        self._check(lib().cumc_batch_last_ms(self.h, ctypes.byref(d)), "cumc_batch_last_ms")
+        self._check(self._L.cumc_batch_last_ms(self.h, ctypes.byref(d)), "cumc_batch_last_ms")
         return d.value
 
     def tape_run(self, nsteps: int, every: int = 8, dts=None):
@@ -372,7 +389,7 @@ class Batch:
             din = np.ascontiguousarray(dts, dtype=np.float64)
             if din.shape != (nsteps, nb):
                 raise ValueError(f"dts must have shape ({nsteps}, {nb})")
-        self._check(lib().cumc_batch_tape_run(self.h, int(nsteps), int(every), _p(din) if din is not None else None,
+        self._check(self._L.cumc_batch_tape_run(self.h, int(nsteps), int(every), _p(din) if din is not None else None,
                                               _p(out)), "cumc_batch_tape_run")
         return out[:nsteps].copy()
 
@@ -386,14 +403,21 @@ class Batch:
         grads = [np.zeros(s.cfg.shape + (8,)) for s in self.solvers]
         sp = (_D * nb)(*[_p(a) if a is not None else None for a in seeds])
         gp = (_D * nb)(*[_p(g) for g in grads])
-        self._check(lib().cumc_batch_tape_backward(self.h, sp, gp), "cumc_batch_tape_backward")
+        self._check(self._L.cumc_batch_tape_backward(self.h, sp, gp), "cumc_batch_tape_backward")
         return grads
 
     def tape_free(self):
-        self._check(lib().cumc_batch_tape_free(self.h), "cumc_batch_tape_free")
+        self._check(self._L.cumc_batch_tape_free(self.h), "cumc_batch_tape_free")
+
+    def branch_sig(self):
+        """Per-box branch signatures (uint64) accumulated by the forward kernels since the last
+        call, which clears them; only the diagnostic build records them."""
+        out = (ctypes.c_uint64 * len(self.solvers))()
+        self._check(self._L.cumc_batch_branch_sig(self.h, out), "cumc_batch_branch_sig")
+        return np.array(list(out), dtype=np.uint64)
 
     def tape_recompute_ms(self) -> float:
         """Device ms of the last backward pass spent re-running checkpoint segments forward."""
         d = ctypes.c_double()
-        self._check(lib().cumc_batch_tape_recompute_ms(self.h, ctypes.byref(d)), "cumc_batch_tape_recompute_ms")
+        self._check(self._L.cumc_batch_tape_recompute_ms(self.h, ctypes.byref(d)), "cumc_batch_tape_recompute_ms")
         return d.value

@@ -182,6 +182,16 @@ int cumc_batch_set_tuning(cumc_batch *b, const cumc_tuning *t);
 int cumc_batch_last_ms(cumc_batch *b, double *ms);
 const char *cumc_batch_last_error(cumc_batch *b);
 
+/* Branch signature of every box (nbox values into sig) accumulated by the batched forward
+ * kernels since the previous call, which it then clears.  No reference counterpart: a
+ * diagnostic of the finite-difference checks of the reverse mode.  Each data-dependent decision
+ * of the reference path (the PPM / TVD-minmod limiter switches of reconstruct.f90:58-128 and
+ * :254-307, the HLL speed selections and region of rsolver.f90:110-126, the discriminant clamp
+ * of definition.f90:394-410, the floors of custom.f90:66-68) enters a hash, so two runs of a box
+ * with equal signatures took the same branches.  Recorded only by the diagnostic build of the
+ * library (libcumc_sig.so, compiled with CUMC_BRANCH_SIG); libcumc.so returns CUMC_EINVAL. */
+int cumc_batch_branch_sig(cumc_batch *b, unsigned long long *sig);
+
 /* ------------------------------------------------------------------------------------------
  * Reverse mode.  No reference counterpart: the reference has no derivatives.  The taped map is
  * initial_update (solver/initial.f90:83-140) followed by nsteps RUNGEKUTTA steps

@@ -24,12 +24,14 @@ def declared_symbols():
     return sorted(names)
 
 
-def test_library_exports_every_declared_symbol():
-    if not os.path.exists(build.LIB):
-        build.build_library()
-    lib = ctypes.CDLL(build.LIB)
+@pytest.mark.parametrize("variant", ["", "sig"])
+def test_library_exports_every_declared_symbol(variant):
+    path = build.LIB_SIG if variant == "sig" else build.LIB
+    if not os.path.exists(path):
+        build.build_library(variant=variant)
+    lib = ctypes.CDLL(path)
     names = declared_symbols()
-    assert "cumc_rungekutta" in names and "cumc_finddt" in names
+    assert "cumc_rungekutta" in names and "cumc_finddt" in names and "cumc_batch_branch_sig" in names
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
@@ -83,6 +85,6 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     raises instead of computing anything."""
     from cumpic_amd import cumc, ics
     monkeypatch.setattr(cumc, "LIB_PATH", str(tmp_path / "absent.so"))
-    monkeypatch.setattr(cumc, "_lib", None)
+    monkeypatch.setattr(cumc, "_libs", {})
     with pytest.raises(cumc.CumcError, match="not built"):
         cumc.Solver.from_problem(ics.small_periodic(8))
