@@ -19,12 +19,16 @@ spectrum on the 512^3 box, 100 FP64 steps, then the mixed form on the FP64 run's
 relative L1 error of rho, p, |v|, |B| and max |div B| of both forms) and measures the stage
 tail's HBM traffic per launch in the same run: two child processes run two steps of the same
 state under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`, calibrated by a 1 GiB device
-copy in the same child.
+copy in the same child.  It also runs the CPU-runnable configuration (`c0`: the seeded 1D MHD
+Riemann problem, 800 cells, to t = 0.1 on the GPU and on the CPU restatement, compared bit for
+bit) and the curvilinear one (`c3`: the cylindrical torus 256x128x256, 50 steps in both forms,
+the mixed form's error, the CPU restatement's rate and a full-size one-step parity check).
 
   --workload torus      cylindrical torus 256x128x256, 50 steps (torus_cyn hooks)
   --n 512 --steps 100   the 512^3 box of the mixed-precision workload
   --no-batch            skip the batched workload
   --no-c2               skip the 512^3 mixed-precision leg
+  --no-c0, --no-c3      skip the 1D Riemann / cylindrical torus legs
   --no-traffic          skip the counter children
   --workload batch      only the batched workload (value = its hipGraph rate)
 """
@@ -196,29 +200,33 @@ def save_ic(prob):
     return d
 
 
-def cpu_baseline(prob, steps=1, one_core_n=128):
-    """The CPU restatement on the host cores: same initial state, `steps` RK3 steps; beside it a
-    one-thread figure on a bounded sample (the same workload builder at one_core_n^3, one step)."""
+def cpu_baseline(prob, steps=1, one_core_n=128, total_time=1.0e300, threads=None):
+    """The CPU restatement on the host cores: same initial state, at most `steps` RK3 steps (each
+    with its finddt) up to total_time; beside it a one-thread figure on a bounded sample (the same
+    workload builder at one_core_n^3, one step).  The returned state carries the steps' dt."""
     from cumpic_amd import ics
     from oracle import oracle as orc
     from oracle.oracle import Oracle
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    all_cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = threads or all_cores
     orc.set_threads(cores)
     o = Oracle.from_problem(prob)
     o.initial_update()
     t0 = time.perf_counter()
-    o.run(steps)
+    dts, _ = o.run(steps, total_time)
     wall = time.perf_counter() - t0
     state = o.get_state()
+    state["dts"] = dts
     o.close()
+    orc.set_threads(all_cores)
     c = prob.cfg
     out = {
-        "value": c.zones * steps / wall,
+        "value": c.zones * len(dts) / wall,
         "unit": "zone-cycles/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{c.nx}x{c.ny}x{c.nz} {prob.name}, {steps} RK3 step(s) after initial_update, "
-                  f"{wall:.1f} s wall on {cores} OpenMP threads",
+        "sample": f"{c.nx}x{c.ny}x{c.nz} {prob.name}, {len(dts)} RK3 step(s) after initial_update, "
+                  f"{wall:.2f} s wall on {cores} OpenMP thread(s)",
     }
     if one_core_n:
         small = ics.spectral_box(n=one_core_n, seed=1001, steps=1)
@@ -280,6 +288,88 @@ def fields(state, cfg):
     p, b = state["prim"][sl], state["bcell"][sl]
     return {"rho": p[..., 0], "p": p[..., 4], "|v|": np.linalg.norm(p[..., 1:4], axis=-1),
             "|B|": np.linalg.norm(b, axis=-1)}
+
+
+def bench_c0(t_end=0.1):
+    """The CPU-runnable configuration: the seeded 1D MHD Riemann problem (800 cells, gamma 2,
+    CFL 0.8, outflow, seed 1000) run to t = 0.1 with each step's finddt, in FP64 on the GPU (a
+    separate warm-up context first) and on the CPU restatement on one thread; final state and dt
+    sequence compared bit for bit."""
+    from cumpic_amd import ics
+    from cumpic_amd.config import FP64
+    from cumpic_amd.cumc import Solver
+    prob = ics.riemann_1d(t_end=t_end)
+    w = Solver.from_problem(prob, precision=FP64)
+    w.initial_update()
+    w.run(5)
+    w.close()
+    s = Solver.from_problem(prob, precision=FP64)
+    s.initial_update()
+    t0 = time.perf_counter()
+    dts, t = s.run(10 ** 6, total_time=t_end)
+    wall = time.perf_counter() - t0
+    gs = s.download()
+    s.close()
+    cpu, ostate = cpu_baseline(prob, steps=10 ** 6, one_core_n=0, total_time=t_end, threads=1)
+    same = bool(np.array_equal(np.asarray(dts), ostate["dts"]) and
+                all(np.array_equal(gs[k], ostate[k]) for k in ("prim", "eps", "bcell")))
+    zc = prob.cfg.zones * len(dts)
+    out = {"workload": f"1D MHD Riemann problem, {prob.cfg.nx} cells, seeded states (seed 1000), gamma 2, "
+                       f"CFL 0.8, outflow, to t = {t_end}",
+           "steps": len(dts), "t_end": t, "gpu": {"value": zc / wall, "ms": wall * 1e3},
+           "cpu": cpu, "bitwise_vs_cpu": same}
+    log(f"[bench] c0: {len(dts)} steps to t={t:.6f}; GPU {wall * 1e3:.1f} ms ({zc / wall:.3e} zone-cycles/s, "
+        f"launch-bound at 800 cells), CPU {cpu['value']:.3e} on 1 thread; bitwise: {same}")
+    return out
+
+
+def bench_c3(steps=50, warmup=2):
+    """The curvilinear configuration: the cylindrical torus (256x128x256, log R in [1,40], seed
+    1003, torus_cyn hooks), `steps` steps in FP64 and in the mixed form after `warmup` steps and
+    the tuner, the mixed form's error against FP64 on the FP64 run's dt sequence, the CPU
+    restatement's rate on one step, and that step's full-size parity with the GPU."""
+    from cumpic_amd import ics
+    from cumpic_amd.config import FP64, MIXED
+    from cumpic_amd.cumc import Solver
+    prob = ics.torus_cyn(256, 128, 256, steps=steps)
+    cfg = prob.cfg
+    res = {}
+    ref_dts = ref = None
+    for form, prec in (("fp64", FP64), ("mixed", MIXED)):
+        s, dts, wall, kt, tuning = timed_run(Solver, prob, prec, steps, warmup)
+        _, dabs = s.find_divb()
+        st = s.download()
+        s.close()
+        res[form] = {"value": cfg.zones * steps / wall, "ms_per_step": wall / steps * 1e3, "divb_max_abs": dabs,
+                     "state_finite": bool(np.isfinite(st["prim"]).all()), "tuning": tuning}
+        if form == "fp64":
+            ref_dts, ref = dts, fields(st, cfg)
+        del st
+    f64 = Solver.from_problem(prob, precision=FP64)
+    f64.initial_update()
+    warm_dts = f64.run(warmup)[0] if warmup else np.zeros(0)
+    f64.close()
+    m = Solver.from_problem(prob, precision=MIXED)
+    m.initial_update()
+    for dt in list(warm_dts) + list(ref_dts):
+        m.rungekutta(float(dt))
+    mf = fields(m.download(), cfg)
+    m.close()
+    res["mixed"]["rel_l1_vs_fp64"] = {k: float(np.sum(np.abs(mf[k] - ref[k])) / np.sum(np.abs(ref[k]))) for k in mf}
+    res["mixed"]["speedup_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    cpu, ostate = cpu_baseline(prob, steps=1, one_core_n=0)
+    g = Solver.from_problem(prob, precision=FP64)
+    g.initial_update()
+    g.run(1)
+    gs = g.download()
+    g.close()
+    same = all(np.array_equal(gs[k], ostate[k], equal_nan=True) for k in ("prim", "eps", "bcell"))
+    log(f"[bench] c3: FP64 {res['fp64']['value']:.4e}, mixed {res['mixed']['value']:.4e} zone-cycles/s "
+        f"({res['mixed']['speedup_vs_fp64']:.2f}x), mixed rel L1 {res['mixed']['rel_l1_vs_fp64']}; "
+        f"CPU {cpu['value']:.3e}; one-step bitwise {same}")
+    return {"workload": "cylindrical torus 256x128x256 (log R in [1,40], phi periodic, z in [-20,20]), beta=100, "
+                        "PW gravity + floors + inflow clamp, PPM+HLL+upwind CT+SSP-RK3 (seed 1003)",
+            "steps": steps, "zones": cfg.zones, **res, "cpu": cpu, "one_step_bitwise_vs_cpu": bool(same)}
 
 
 def bench_c2(n=512, steps=100, warmup=2):
@@ -623,6 +713,8 @@ def main():
     ap.add_argument("--solver", choices=["hll", "lf", "hllc", "hlld"], default="hll",
                     help="Riemann solver (hll is the reference's; lf, hllc and hlld are extensions)")
     ap.add_argument("--no-c2", action="store_true", help="skip the 512^3 mixed-precision leg")
+    ap.add_argument("--no-c0", action="store_true", help="skip the 1D Riemann leg (GPU and CPU to t = 0.1)")
+    ap.add_argument("--no-c3", action="store_true", help="skip the cylindrical torus leg")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter children")
     ap.add_argument("--traffic-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -774,9 +866,13 @@ def main():
         results["mixed"]["rel_l1_vs_fp64"] = errs
         log(f"[bench] mixed rel L1 vs FP64: {errs}")
 
-    c2 = None
+    c2 = c0 = c3 = None
     if not args.no_c2 and args.workload == "box" and args.n == 256 and set(forms) == {"fp64", "mixed"}:
         c2 = bench_c2()
+    if not args.no_c0 and args.workload == "box" and not args.no_cpu:
+        c0 = bench_c0()
+    if not args.no_c3 and args.workload == "box" and set(forms) == {"fp64", "mixed"} and not args.no_cpu:
+        c3 = bench_c3()
 
     batch = None
     if not args.no_batch and args.workload == "box":
@@ -822,7 +918,9 @@ def main():
                         "divb_max_abs": mx["divb_max_abs"], "state_finite": mx["state_finite"],
                         "roofline": mx["roofline"], "kernel_ms_per_step": mx["kernel_ms_per_step"],
                         "tuning": mx["tuning"]}
+    out["c0"] = c0
     out["c2"] = c2
+    out["c3"] = c3
     out["batch"] = batch
     print(json.dumps(out), flush=True)
 
