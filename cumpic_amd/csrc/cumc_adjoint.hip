@@ -761,8 +761,10 @@ __device__ inline void kr_flux_lane_t_tile(const Lay &L, const Fields<double> &F
                                            const AdjF &A, int bx, int o, double (*buf)[64][LT_PITCH])
 {
     const int n = DIR == 2 ? L.ny : L.nz;
-    const int nseg = lane_t_segs<DIR>(L);
-    const int seg = bx % nseg, j0 = (bx / nseg) * 4;
+    // consecutive tiles are neighbouring 4-pencil groups of one segment: four of them read the
+    // same 128-byte rows, which meet in one L2 when the caller keeps them on one XCD
+    const int njt = (L.nx + 2 + 3) / 4;
+    const int seg = bx / njt, j0 = (bx % njt) * 4;
     const long ps = L.ps;
     auto cell = [&](int j, int c) { return (unsigned)(DIR == 2 ? L.id(j, c, o) : L.id(j, o, c)); };
     // row r of the tile is pencil cell f = seg * XR_OWN + r - 5 (c = f - 2)
@@ -817,8 +819,11 @@ __global__ __launch_bounds__(256) void kr_flux_lane(Lay L, Fields<double> F, Tab
     if constexpr (DIR == 1) {
         kr_flux_lane_x_tile<RECON>(L, F, tb, M, A, blockIdx.x, blockIdx.y);
     } else {
+        // XCD-aware tile order (neighbouring tiles share cache lines); padding blocks leave
         __shared__ double buf[LT_PLANES][64][LT_PITCH];
-        kr_flux_lane_t_tile<DIR, RECON>(L, F, tb, M, A, blockIdx.x, blockIdx.y, buf);
+        int bx, o;
+        if (!xcd_tile(lane_tiles<DIR>(L), lane_outer<DIR>(L), bx, o)) return;
+        kr_flux_lane_t_tile<DIR, RECON>(L, F, tb, M, A, bx, o, buf);
     }
 }
 
@@ -965,7 +970,8 @@ static void reverse_sweep(cumc_ctx *ctx, const Fields<double> &F, const AdjF &A)
     Tabs tb = tabs_of(ctx);
     hipStream_t s = ctx->stream;
     if (lane_sweep<DIR>(L)) {
-        dim3 grid(lane_tiles<DIR>(L), lane_outer<DIR>(L));
+        const dim3 grid = DIR == 1 ? dim3(lane_tiles<DIR>(L), lane_outer<DIR>(L))
+                                   : dim3(xcd_blocks(lane_tiles<DIR>(L), lane_outer<DIR>(L)));
         if (ctx->M.recon == CUMC_PPMC) hipLaunchKernelGGL((kr_flux_lane<DIR, 4>), grid, dim3(256), 0, s, L, F, tb, ctx->M, A);
         else hipLaunchKernelGGL((kr_flux_lane<DIR, 1>), grid, dim3(256), 0, s, L, F, tb, ctx->M, A);
         return;
@@ -1399,8 +1405,17 @@ __global__ __launch_bounds__(256) void kbr_flux_lane(const AdjBox *__restrict__ 
             kr_flux_lane_x_tile<RECON>(d.L, F, d.tb, d.M, d.A, loc % gl, loc / gl);
         }
     } else {
+        // one workgroup per item in an XCD-aware order: the workgroups of one XCD (blockIdx
+        // modulo NXCD) take consecutive items, i.e. neighbouring tiles that share cache lines
         __shared__ double buf[LT_PLANES][64][LT_PITCH];
-        FOR_ITEMS(Q_L1 + DIR - 1)
+        const int *pq = pre + (size_t)(Q_L1 + DIR - 1) * (nbox + 1);
+        const int total = pq[nbox];
+        const int per = (int)(gridDim.x / NXCD);
+        const int it = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+        if (it < total) {
+            const int b = adj_box_of(pq, nbox, it);
+            const int loc = it - pq[b];
+            const AdjBox &d = D[b];
             const int gl = d.glane[DIR - 1];
             const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
             kr_flux_lane_t_tile<DIR, RECON>(d.L, F, d.tb, d.M, d.A, loc % gl, loc / gl, buf);
@@ -1550,12 +1565,12 @@ static void bt_sweep(cumc_batch *b, batch_tape *bt, int stage, int i)
     // boxes whose pencils along DIR are long enough take the wave-parallel kernel, the others
     // the per-pencil walks (each work list is empty when no box takes it)
     if (bt->total[Q_L1 + DIR - 1] > 0) {
+        // DIR 2 / 3: one workgroup per item, a multiple of NXCD of them (the kernel's XCD order)
+        const dim3 g = DIR == 1 ? bt_grid(bt, Q_L1) : dim3(xcd_blocks(bt->total[Q_L1 + DIR - 1], 1));
         if (bt->recon == CUMC_PPMC)
-            hipLaunchKernelGGL((kbr_flux_lane<DIR, 4>), bt_grid(bt, Q_L1 + DIR - 1), dim3(256), 0, s, bt->d, bt->pre, nb,
-                               stage, i);
+            hipLaunchKernelGGL((kbr_flux_lane<DIR, 4>), g, dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
         else
-            hipLaunchKernelGGL((kbr_flux_lane<DIR, 1>), bt_grid(bt, Q_L1 + DIR - 1), dim3(256), 0, s, bt->d, bt->pre, nb,
-                               stage, i);
+            hipLaunchKernelGGL((kbr_flux_lane<DIR, 1>), g, dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
     }
     if constexpr (DIR != 1) {
         if (bt->total[Q_H1 + DIR - 1] > 0) {
