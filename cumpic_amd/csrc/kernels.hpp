@@ -92,7 +92,6 @@ __host__ __device__ inline const float *sweep_p(const Fields<float> &F) { return
 __host__ __device__ inline const double *sweep_bc(const Fields<double> &F) { return F.BC; }
 __host__ __device__ inline const float *sweep_bc(const Fields<float> &F) { return F.BC32; }
 
-__device__ inline double ld(const double *p) { return __builtin_nontemporal_load(p); }
 
 // ------------------------------------------------------------------------------------------
 // x sweep: lanes walk x inside one wavefront (consecutive cells), neighbours via shuffles.
