@@ -646,11 +646,11 @@ __device__ inline void lane_reverse(const Tabs &tb, const Model &M, int n, int l
     // forward: edge values of this cell; the right state of face c is vm of lane+1
 #pragma unroll
     for (int v = 0; v < 7; v++) {
-        const double w = q[v], wm1 = __shfl_up(w, 1, 64), wp1 = __shfl_down(w, 1, 64);
+        const double w = q[v], wm1 = lane_up(w), wp1 = lane_down(w);
         double vm, vpv;
         if (RECON == 4) {
             dm[v] = ppm_slope<double>(tb.at(ax, TB_B0C, cc), tb.at(ax, TB_B1C, cc), wm1, w, wp1);
-            const double dmm1 = __shfl_up(dm[v], 1, 64), dmp1 = __shfl_down(dm[v], 1, 64);
+            const double dmm1 = lane_up(dm[v]), dmp1 = lane_down(dm[v]);
             ppm_edges<double>(tb.at(ax, TB_A0M, cc), tb.at(ax, TB_A1M, cc), tb.at(ax, TB_A2M, cc),
                               tb.at(ax, TB_A3M, cc), tb.at(ax, TB_A0P, cc), tb.at(ax, TB_A1P, cc),
                               tb.at(ax, TB_A2P, cc), tb.at(ax, TB_A3P, cc), wm1, w, wp1, dmm1, dm[v], dmp1, vm, vpv);
@@ -658,7 +658,7 @@ __device__ inline void lane_reverse(const Tabs &tb, const Model &M, int n, int l
             tvd_edges<double>(tb.at(ax, TB_CF, cc), tb.at(ax, TB_CB, cc), wm1, w, wp1, vm, vpv);
         }
         vp[v] = vpv;
-        vmr[v] = __shfl_down(vm, 1, 64);
+        vmr[v] = lane_down(vm);
     }
     // HLL reversed at face c
     double lL[7] = {0, 0, 0, 0, 0, 0, 0}, lR[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -676,30 +676,30 @@ __device__ inline void lane_reverse(const Tabs &tb, const Model &M, int n, int l
 #pragma unroll
     for (int v = 0; v < 7; v++) {
         const double lvp = lL[v];
-        const double lvm = __shfl_up(lR[v], 1, 64);  // from face c-1
-        const double w = q[v], wm1 = __shfl_up(w, 1, 64), wp1 = __shfl_down(w, 1, 64);
+        const double lvm = lane_up(lR[v]);  // from face c-1
+        const double w = q[v], wm1 = lane_up(w), wp1 = lane_down(w);
         double am1 = 0.0, a0 = 0.0, ap1 = 0.0, em1 = 0.0, e0 = 0.0, ep1 = 0.0;
         // only cells 0..n+1 are reconstructed and only slopes -1..n+2 are used; elsewhere the
         // table entries are not defined and the forward values there are never read
         const bool recon_cell = c >= 0 && c <= n + 1;
         if (RECON == 4) {
-            const double dmm1 = __shfl_up(dm[v], 1, 64), dmp1 = __shfl_down(dm[v], 1, 64);
+            const double dmm1 = lane_up(dm[v]), dmp1 = lane_down(dm[v]);
             if (recon_cell)
                 ppm_edges_adj(tb.at(ax, TB_A0M, cc), tb.at(ax, TB_A1M, cc), tb.at(ax, TB_A2M, cc),
                               tb.at(ax, TB_A3M, cc), tb.at(ax, TB_A0P, cc), tb.at(ax, TB_A1P, cc),
                               tb.at(ax, TB_A2P, cc), tb.at(ax, TB_A3P, cc), wm1, w, wp1, dmm1, dm[v], dmp1, lvm, lvp,
                               am1, a0, ap1, em1, e0, ep1);
             // slope of this cell: its adjoint from the three cells that used it
-            const double ldm = e0 + __shfl_down(em1, 1, 64) + __shfl_up(ep1, 1, 64);
+            const double ldm = e0 + lane_down(em1) + lane_up(ep1);
             double bm1 = 0.0, b0 = 0.0, bp1 = 0.0;
             if (c >= -1 && c <= n + 2)
                 ppm_slope_adj(tb.at(ax, TB_B0C, cc), tb.at(ax, TB_B1C, cc), wm1, w, wp1, ldm, bm1, b0, bp1);
-            lw[v] = a0 + __shfl_down(am1, 1, 64) + __shfl_up(ap1, 1, 64) + b0 + __shfl_down(bm1, 1, 64) +
-                    __shfl_up(bp1, 1, 64);
+            lw[v] = a0 + lane_down(am1) + lane_up(ap1) + b0 + lane_down(bm1) +
+                    lane_up(bp1);
         } else {
             if (recon_cell)
                 tvd_edges_adj(tb.at(ax, TB_CF, cc), tb.at(ax, TB_CB, cc), wm1, w, wp1, lvm, lvp, am1, a0, ap1);
-            lw[v] = a0 + __shfl_down(am1, 1, 64) + __shfl_up(ap1, 1, 64);
+            lw[v] = a0 + lane_down(am1) + lane_up(ap1);
         }
     }
 }

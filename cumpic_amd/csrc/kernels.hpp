@@ -100,17 +100,41 @@ __host__ __device__ inline const float *sweep_bc(const Fields<float> &F) { retur
 // ------------------------------------------------------------------------------------------
 constexpr int XS_FACES = 59;
 
+// Lane-neighbour moves of the wave-parallel sweeps: lane i takes the value of lane i - 1
+// (lane_up) or i + 1 (lane_down), the end lane keeping its own, as __shfl_up / __shfl_down with
+// delta 1 over 64 lanes.  FP64 values move by DPP whole-wave shifts (wave_shr:1 / wave_shl:1,
+// two VALU moves) instead of ds_bpermute through the LDS crossbar: the FP64 sweeps and the
+// reverse sweeps leave VALU cycles idle while they wait on the LDS.  FP32 values keep
+// ds_bpermute: the FP32 x sweep is VALU-bound and measured 10 % slower with the DPP moves.
+// Every lane of the wavefront must be active.
+__device__ inline int dpp_up1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false); }
+__device__ inline int dpp_dn1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false); }
+__device__ inline float lane_up(float v) { return __shfl_up(v, 1, 64); }
+__device__ inline float lane_down(float v) { return __shfl_down(v, 1, 64); }
+__device__ inline double lane_up(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_up1((int)(unsigned)b), hi = (unsigned)dpp_up1((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double lane_down(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)dpp_dn1((int)(unsigned)b), hi = (unsigned)dpp_dn1((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <typename R, int RECON>
 __device__ inline void recon_lane(const Tabs &tb, int ax, int jc, R c, R &vm, R &vp)
 {
-    R cm1 = __shfl_up(c, 1, 64);
-    R cp1 = __shfl_down(c, 1, 64);
+    R cm1 = lane_up(c);
+    R cp1 = lane_down(c);
     if (RECON == 5) {
         weno5_edges<R>(__shfl_up(c, 2, 64), cm1, c, cp1, __shfl_down(c, 2, 64), vm, vp);
     } else if (RECON == 4) {
         R dm = ppm_slope<R>(tb.atr<R>(ax, TB_B0C, jc), tb.atr<R>(ax, TB_B1C, jc), cm1, c, cp1);
-        R dmm1 = __shfl_up(dm, 1, 64);
-        R dmp1 = __shfl_down(dm, 1, 64);
+        R dmm1 = lane_up(dm);
+        R dmp1 = lane_down(dm);
         ppm_edges<R>(tb.atr<R>(ax, TB_A0M, jc), tb.atr<R>(ax, TB_A1M, jc), tb.atr<R>(ax, TB_A2M, jc),
                      tb.atr<R>(ax, TB_A3M, jc), tb.atr<R>(ax, TB_A0P, jc), tb.atr<R>(ax, TB_A1P, jc),
                      tb.atr<R>(ax, TB_A2P, jc), tb.atr<R>(ax, TB_A3P, jc), cm1, c, cp1, dmm1, dm, dmp1, vm, vp);
@@ -156,14 +180,14 @@ __device__ inline void flux_x_tile(const Lay &L, const Fields<R> &F, const Tabs 
         for (int v = 0; v < 7; v++) recon_lane<R, RECON>(tb, 0, jc, q[v], vm[v], vl[v]);
         weno_positivity<R>(q, vm, vl);
 #pragma unroll
-        for (int v = 0; v < 7; v++) vr[v] = __shfl_down(vm[v], 1, 64);
+        for (int v = 0; v < 7; v++) vr[v] = lane_down(vm[v]);
     } else {
 #pragma unroll
         for (int v = 0; v < 7; v++) {
             R vm, vp;
             recon_lane<R, RECON>(tb, 0, jc, q[v], vm, vp);
             vl[v] = vp;
-            vr[v] = __shfl_down(vm, 1, 64);
+            vr[v] = lane_down(vm);
         }
     }
     FaceState<R> sl{vl[0], vl[1], vl[2], vl[3], vl[4], bn, vl[5], vl[6]};
