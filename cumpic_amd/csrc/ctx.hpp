@@ -32,6 +32,13 @@ struct cumc_ctx {
     bool dt_valid = false;              // dt_dev[1] holds finddt of the current state
     bool p64_stale = false;             // mixed form: P / EPS / BC lag the FP32 planes and U0
     bool grid_set = false;
+    // z-collapsed stepping of a periodic nz = 1 context (make_view): zc_ok when the configuration
+    // allows it (FP64, riemann hooks, periodic z, z tables equal in every cell); zc_on the
+    // caller's switch (cumc_set_zcollapse); zfresh when the z ghost planes hold the copies of
+    // plane l = 1 the last full-layout ghost fill wrote; zmode while the state lives in the
+    // collapsed view (plane 1 current, z ghost planes not yet rewritten: zexpand restores them)
+    bool zc_ok = false, zc_on = true, zfresh = false, zmode = false;
+    bool ztape = false;  // a reverse-mode record holds full-layout states: no collapsed steps
     struct cumc_tape *tape = nullptr;  // reverse-mode record (cumc_adjoint.hip)
     int seg = 24;    // target faces per pencil segment of the y / z sweeps
     int zchunk = 4;  // planes per column tile of the EMF and stage-tail kernels
@@ -149,16 +156,16 @@ void launch_ghost(cumc_ctx *ctx);
 void launch_ghost32(cumc_ctx *ctx);
 
 template <typename R, int RECON, int SOLVER>
-void launch_fluxes_t(cumc_ctx *ctx, const Fields<R> &F, int only_dir)
+void launch_fluxes_t(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, int only_dir)
 {
-    const Lay &L = ctx->L;
     hipStream_t s = ctx->stream;
     Tabs tb = tabs_of(ctx);
     const Model &M = ctx->M;
+    const int npl = flux_planes(L);
     if (!only_dir || only_dir == 1) {
         mark(ctx, KC_FLUX_X);
         const long waves = ((long)(L.ny + 2) * L.px + XS_FACES - 1) / XS_FACES;
-        dim3 grid(nblk(waves * 64, 256), L.nz + 2);
+        dim3 grid(nblk(waves * 64, 256), npl);
         hipLaunchKernelGGL((k_flux_x<R, RECON, SOLVER>), grid, dim3(256), 0, s, L, F, tb, M);
         mark(ctx, -1);
     }
@@ -167,14 +174,16 @@ void launch_fluxes_t(cumc_ctx *ctx, const Fields<R> &F, int only_dir)
         mark(ctx, KC_FLUX_Y);
         const int nseg = (L.ny + 1 + seg - 1) / seg;
         const int sl = (L.ny + 1 + nseg - 1) / nseg;
-        dim3 grid(nblk((long)L.px * (L.nz + 2), 128), nseg);
+        dim3 grid(nblk((long)L.px * npl, 128), nseg);
         hipLaunchKernelGGL((k_flux_walk<R, 2, RECON, SOLVER>), grid, dim3(128), 0, s, L, F, tb, M, sl);
         mark(ctx, -1);
     }
     if (!only_dir || only_dir == 3) {
         mark(ctx, KC_FLUX_Z);
-        const int nseg = (L.nz + 1 + seg - 1) / seg;
-        const int sl = (L.nz + 1 + nseg - 1) / nseg;
+        // a z-collapsed view has one z face per column: faces 0 and 1 of the single cell are the
+        // same face of its periodic images
+        const int nseg = L.zc1 ? 1 : (L.nz + 1 + seg - 1) / seg;
+        const int sl = L.zc1 ? 1 : (L.nz + 1 + nseg - 1) / nseg;
         dim3 grid(nblk((long)L.px * (L.ny + 2), 128), nseg);
         hipLaunchKernelGGL((k_flux_walk<R, 3, RECON, SOLVER>), grid, dim3(128), 0, s, L, F, tb, M, sl);
         mark(ctx, -1);
@@ -182,35 +191,39 @@ void launch_fluxes_t(cumc_ctx *ctx, const Fields<R> &F, int only_dir)
 }
 
 template <typename R, int SOLVER>
-void launch_fluxes_s(cumc_ctx *ctx, const Fields<R> &F, int only_dir)
+void launch_fluxes_s(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, int only_dir)
 {
     switch (ctx->M.recon) {
-    case CUMC_TVDMM: launch_fluxes_t<R, 1, SOLVER>(ctx, F, only_dir); break;
-    case CUMC_TVDMC: launch_fluxes_t<R, 2, SOLVER>(ctx, F, only_dir); break;
-    case CUMC_TVDVL: launch_fluxes_t<R, 3, SOLVER>(ctx, F, only_dir); break;
-    case CUMC_WENO: launch_fluxes_t<R, 5, SOLVER>(ctx, F, only_dir); break;
-    default: launch_fluxes_t<R, 4, SOLVER>(ctx, F, only_dir); break;
+    case CUMC_TVDMM: launch_fluxes_t<R, 1, SOLVER>(ctx, L, F, only_dir); break;
+    case CUMC_TVDMC: launch_fluxes_t<R, 2, SOLVER>(ctx, L, F, only_dir); break;
+    case CUMC_TVDVL: launch_fluxes_t<R, 3, SOLVER>(ctx, L, F, only_dir); break;
+    case CUMC_WENO: launch_fluxes_t<R, 5, SOLVER>(ctx, L, F, only_dir); break;
+    default: launch_fluxes_t<R, 4, SOLVER>(ctx, L, F, only_dir); break;
     }
 }
 
 // the three sweeps (or one, only_dir = 1..3) with the configured reconstruction and solver
 template <typename R>
-void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
+void launch_fluxes(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, int only_dir = 0)
 {
     switch (ctx->M.solver) {
-    case CUMC_LF: launch_fluxes_s<R, 1>(ctx, F, only_dir); break;
-    case CUMC_HLLC: launch_fluxes_s<R, 3>(ctx, F, only_dir); break;
-    case CUMC_HLLD: launch_fluxes_s<R, 4>(ctx, F, only_dir); break;
-    default: launch_fluxes_s<R, 2>(ctx, F, only_dir); break;
+    case CUMC_LF: launch_fluxes_s<R, 1>(ctx, L, F, only_dir); break;
+    case CUMC_HLLC: launch_fluxes_s<R, 3>(ctx, L, F, only_dir); break;
+    case CUMC_HLLD: launch_fluxes_s<R, 4>(ctx, L, F, only_dir); break;
+    default: launch_fluxes_s<R, 2>(ctx, L, F, only_dir); break;
     }
+}
+template <typename R>
+void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
+{
+    launch_fluxes<R>(ctx, ctx->L, F, only_dir);
 }
 
 template <typename R, int STAGE>
-void launch_tail(cumc_ctx *ctx, const Fields<R> &F, const double *Uin, double *Uout)
+void launch_tail(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, const double *Uin, double *Uout)
 {
-    const Lay &L = ctx->L;
     const int zc = ctx->zchunk, tw = ctx->tile_w;
-    const int gx = col_tiles(L, tw), gy = (L.nz + zc) / zc;
+    const int gx = col_tiles(L, tw), gy = zchunks(L, zc);
     dim3 grid(xcd_blocks(gx, gy));
     Tabs tb = tabs_of(ctx);
     unsigned long long *dtn = (unsigned long long *)(ctx->dt_dev + 1);
@@ -225,29 +238,62 @@ void launch_tail(cumc_ctx *ctx, const Fields<R> &F, const double *Uin, double *U
                            Uin, Uout, dtn, gx, gy, zc, tw);
 }
 
-// one RK stage: fluxes of the three sweeps, corner EMFs, stage tail, ghost fill.
+// The FP64 fields as the kernels address them: the full layout, or (col) the z-collapsed view of
+// a periodic nz = 1 context, whose field bases sit at plane l = 1 and whose z stride is 0.
+struct View {
+    Lay L;
+    Fields<double> F;
+    double *Ub;
+};
+static inline View make_view(const cumc_ctx *ctx, bool col)
+{
+    View v{ctx->L, ctx->F64, ctx->Ub};
+    if (col) {
+        const long off = 3 * ctx->L.py;  // cell (j, k, 1) = (1 + 2) * py + ...
+        v.L.py = 0;
+        v.L.zc1 = 1;
+        v.F.P += off;
+        v.F.EPS += off;
+        v.F.BC += off;
+        v.F.U0 += off;
+        v.F.U1 += off;
+        for (int d = 0; d < 3; d++) v.F.F[d] += off;
+        v.F.E += off;
+        v.Ub += off;
+    }
+    return v;
+}
+
+// ghost passes on the FP64 primitive planes of a view (defined in cumc.hip)
+void launch_ghost_l(cumc_ctx *ctx, const Lay &L, const Fields<double> &F);
+// whether the next step runs in the z-collapsed view (enters it when allowed and the z ghost
+// planes check out), and the return of a collapsed state to the full layout (cumc.hip)
+bool zcollapse_begin(cumc_ctx *ctx);
+void zexpand(cumc_ctx *ctx);
+
+// one RK stage: fluxes of the three sweeps, corner EMFs, stage tail, ghost fill, on the fields F
+// addressed by L (Ub: the stage-2 output buffer in the same addressing).
 // Conserved buffers: stage 1 U0 -> U1, stage 2 U1 -> Ub, stage 3 Ub -> U1.
 template <typename R>
-void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
+void stage_l(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, double *Ub, int st)
 {
-    const Lay &L = ctx->L;
-    launch_fluxes<R>(ctx, F);
+    launch_fluxes<R>(ctx, L, F);
     mark(ctx, KC_ECORN);
     const int zc = ctx->zchunk, tw = ctx->tile_w;
-    const int gx = col_tiles(L, tw), gy = (L.nz + zc) / zc;
+    const int gx = col_tiles(L, tw), gy = zchunks(L, zc);
     hipLaunchKernelGGL((k_ecorn<R>), dim3(xcd_blocks(gx, gy)), dim3(256), 0, ctx->stream, L, F, gx, gy, zc, tw);
     mark(ctx, -1);
     mark(ctx, KC_TAIL);
     if (st == 1) {
-        launch_tail<R, 1>(ctx, F, ctx->F64.U0, ctx->F64.U1);
+        launch_tail<R, 1>(ctx, L, F, F.U0, F.U1);
     } else if (st == 2) {
-        launch_tail<R, 2>(ctx, F, ctx->F64.U1, ctx->Ub);
+        launch_tail<R, 2>(ctx, L, F, F.U1, Ub);
     } else {
         const double init = 1.0e10;
         unsigned long long bits;
         std::memcpy(&bits, &init, 8);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (unsigned long long *)(ctx->dt_dev + 1), bits, 1);
-        launch_tail<R, 3>(ctx, F, ctx->Ub, ctx->F64.U1);
+        launch_tail<R, 3>(ctx, L, F, Ub, F.U1);
     }
     mark(ctx, -1);
     mark(ctx, KC_GHOST);
@@ -255,7 +301,13 @@ void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
         launch_ghost32(ctx);
         ctx->p64_stale = true;
     } else {
-        launch_ghost(ctx);
+        launch_ghost_l(ctx, L, F);
     }
     mark(ctx, -1);
+}
+
+template <typename R>
+void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
+{
+    stage_l<R>(ctx, ctx->L, F, ctx->Ub, st);
 }

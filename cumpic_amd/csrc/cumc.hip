@@ -229,6 +229,30 @@ static std::vector<double> build_tables(const double *f_in, int n)
     return t;
 }
 
+// A periodic nz = 1 context may step in the z-collapsed view when every z face the reference
+// computes equals the others: the z sweep's reconstruction coefficients (PPM a/b rows, TVD cf/cb)
+// must be the same in every cell of the z table (true for equal cell widths), since faces 0 and
+// 1 of the single cell become one face.  FP64 form with the riemann hooks only (the tori clamp
+// their ghost zones; the mixed form keeps FP32 copies of the planes).
+static bool zcollapse_allowed(const cumc_ctx *ctx, const std::vector<double> &tz)
+{
+    const cumc_config &c = ctx->cfg;
+    if (c.nz != 1 || c.boundary_flag[4] != 0 || c.boundary_flag[5] != 0) return false;
+    if (c.precision != CUMC_FP64 || c.model != CUMC_MODEL_RIEMANN) return false;
+    const int len = c.nz + 7;
+    auto row_flat = [&](int row, int j0, int j1) {
+        const double *r = tz.data() + (size_t)row * len;
+        for (int j = j0; j <= j1; j++)
+            if (std::memcmp(&r[j + 3], &r[j0 + 3], sizeof(double)) != 0) return false;
+        return true;
+    };
+    for (int row = TB_A0M; row <= TB_A3P; row++)
+        if (!row_flat(row, 0, c.nz + 1)) return false;
+    for (int row : {(int)TB_B0C, (int)TB_B1C, (int)TB_CF, (int)TB_CB})
+        if (!row_flat(row, -1, c.nz + 2)) return false;
+    return true;
+}
+
 extern "C" int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, const double *zf)
 {
     if (!ctx || !xf || !yf || !zf) return CUMC_EINVAL;
@@ -242,8 +266,11 @@ extern "C" int cumc_set_grid(cumc_ctx *ctx, const double *xf, const double *yf, 
         return fail(ctx, CUMC_EINVAL, "cylindrical grid needs R > 0 on the inner face");
     if (ctx->cfg.coordinate == CUMC_SPHERICAL && !(xf[3] > 0.0 && yf[3] > 0.0 && yf[ctx->L.ny + 3] < 3.14159265358979323846))
         return fail(ctx, CUMC_EINVAL, "spherical grid needs r > 0 and 0 < theta < pi on the boundary faces");
+    zexpand(ctx);
+    ctx->zc_ok = false;
     for (int a = 0; a < 3; a++) {
         std::vector<double> t = build_tables(fs[a], ns[a]);
+        if (a == 2) ctx->zc_ok = zcollapse_allowed(ctx, t);
         if (!ctx->tab[a]) HIPCHK(ctx, hipMalloc(&ctx->tab[a], t.size() * sizeof(double)));
         HIPCHK(ctx, hipMemcpy(ctx->tab[a], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
         std::vector<float> tf(t.begin(), t.end());
@@ -289,6 +316,8 @@ extern "C" int cumc_upload(cumc_ctx *ctx, const double *prim, const double *eps)
 {
     if (!ctx || !prim || !eps) return CUMC_EINVAL;
     ctx->dt_valid = false;
+    zexpand(ctx);
+    ctx->zfresh = false;  // the uploaded z ghost planes are the caller's
     // bcell outside [1,n]^3 is read before the first BOUNDARY (as the reference's zero-initialised
     // array): a re-upload starts from the same zeros as a fresh context
     HIPCHK(ctx, hipMemsetAsync(ctx->F64.BC, 0, 3 * (size_t)ctx->L.ps * sizeof(double), ctx->stream));
@@ -306,6 +335,7 @@ extern "C" int cumc_download(cumc_ctx *ctx, double *prim, double *eps, double *b
 {
     if (!ctx) return CUMC_EINVAL;
     int r;
+    zexpand(ctx);
     sync_prim64(ctx);
     if (prim && (r = conv_out(ctx, ctx->F64.P, 8, prim))) return r;
     if (eps && (r = conv_out(ctx, ctx->F64.EPS, 1, eps))) return r;
@@ -378,6 +408,7 @@ extern "C" int cumc_download_noalloc(cumc_ctx *ctx, double *prim, double *eps, d
     double *stg = nullptr;
     int r = scratch_for(ctx, &stg);
     if (r) return r;
+    zexpand(ctx);
     sync_prim64(ctx);
     if (prim && (r = stage_out(ctx, stg, ctx->F64.P, 8, prim))) return r;
     if (eps && (r = stage_out(ctx, stg, ctx->F64.EPS, 1, eps))) return r;
@@ -389,24 +420,54 @@ extern "C" int cumc_download_noalloc(cumc_ctx *ctx, double *prim, double *eps, d
 // ---------------------------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------------------------
-void launch_ghost(cumc_ctx *ctx)
+void launch_ghost_l(cumc_ctx *ctx, const Lay &L, const Fields<double> &F)
 {
-    const Lay &L = ctx->L;
     hipStream_t s = ctx->stream;
     const long nx6 = L.nx + 6, ny6 = L.ny + 6;
-    const long n0 = (long)(L.ny + 1) * (L.nz + 1) * 6;
-    const long n1 = nx6 * (L.nz + 1) * 6;
-    const long n2 = nx6 * ny6 * 6;
-    hipLaunchKernelGGL(k_ghost<0>, dim3(nblk(n0, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
-    hipLaunchKernelGGL(k_ghost<1>, dim3(nblk(n1, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
-    hipLaunchKernelGGL(k_ghost<2>, dim3(nblk(n2, 256)), dim3(256), 0, s, L, ctx->F64, ctx->M);
+    hipLaunchKernelGGL(k_ghost<0>, dim3(nblk(ghost_threads<0>(L), 256)), dim3(256), 0, s, L, F, ctx->M);
+    hipLaunchKernelGGL(k_ghost<1>, dim3(nblk(ghost_threads<1>(L), 256)), dim3(256), 0, s, L, F, ctx->M);
+    // a z-collapsed view has no z pass: its z ghosts are the plane itself
+    if (!L.zc1) hipLaunchKernelGGL(k_ghost<2>, dim3(nblk(ghost_threads<2>(L), 256)), dim3(256), 0, s, L, F, ctx->M);
     if (ctx->M.model == CUMC_MODEL_TORUS_CYN || ctx->M.model == CUMC_MODEL_TORUS_SPH) {
         // inflow clamp on the R / r ghosts; torus_cyn also clamps the z ghosts
         const long nyz = (long)(L.ny + 6) * (L.nz + 6) * 3, nxy = nx6 * ny6 * 3;
-        hipLaunchKernelGGL(k_custom_boundary_x<double>, dim3(nblk(nyz, 256)), dim3(256), 0, s, L, ctx->F64.P);
+        hipLaunchKernelGGL(k_custom_boundary_x<double>, dim3(nblk(nyz, 256)), dim3(256), 0, s, L, F.P);
         if (ctx->M.model == CUMC_MODEL_TORUS_CYN)
-            hipLaunchKernelGGL(k_custom_boundary_z<double>, dim3(nblk(nxy, 256)), dim3(256), 0, s, L, ctx->F64.P);
+            hipLaunchKernelGGL(k_custom_boundary_z<double>, dim3(nblk(nxy, 256)), dim3(256), 0, s, L, F.P);
     }
+    if (!L.zc1) ctx->zfresh = true;
+}
+
+void launch_ghost(cumc_ctx *ctx) { launch_ghost_l(ctx, ctx->L, ctx->F64); }
+
+// Enters the z-collapsed view when the context allows it and its state is a full-layout state
+// whose z ghost planes the last ghost fill wrote (zfresh) and whose two periodic B_z faces agree.
+bool zcollapse_begin(cumc_ctx *ctx)
+{
+    if (!ctx->zc_ok || !ctx->zc_on || ctx->ztape) return false;
+    if (ctx->zmode) return true;
+    if (!ctx->zfresh) return false;
+    const Lay &L = ctx->L;
+    unsigned long long cnt = 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->red + 2, 0ull, 1);
+    hipLaunchKernelGGL(k_zcheck, dim3(nblk((long)(L.nx + 6) * (L.ny + 6), 256)), dim3(256), 0, ctx->stream, L,
+                       (const double *)ctx->F64.P, (const double *)ctx->F64.U0, ctx->red + 2);
+    if (hipMemcpyAsync(&cnt, ctx->red + 2, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return false;
+    ctx->zmode = cnt == 0;
+    return ctx->zmode;
+}
+
+// A collapsed state back to the full layout (before anything reads or writes the z ghost planes)
+void zexpand(cumc_ctx *ctx)
+{
+    if (!ctx->zmode) return;
+    const Lay &L = ctx->L;
+    hipLaunchKernelGGL(k_zexpand, dim3(nblk((long)(L.nx + 6) * (L.ny + 6), 256)), dim3(256), 0, ctx->stream, L,
+                       ctx->F64);
+    ctx->zmode = false;
+    ctx->zfresh = true;
 }
 
 // mixed form: the FP32 primitive planes from the FP64 ones (after the host or an FP64 pass set
@@ -437,9 +498,11 @@ static void sync_prim64(cumc_ctx *ctx)
 // dt_dev[1]
 static void enqueue_step(cumc_ctx *ctx)
 {
-    for (int st = 1; st <= 3; st++) {
-        if (ctx->cfg.precision == CUMC_MIXED) stage_mixed(ctx, st);
-        else stage<double>(ctx, ctx->F64, st);
+    if (ctx->cfg.precision == CUMC_MIXED) {
+        for (int st = 1; st <= 3; st++) stage_mixed(ctx, st);
+    } else {
+        const View v = make_view(ctx, zcollapse_begin(ctx));
+        for (int st = 1; st <= 3; st++) stage_l<double>(ctx, v.L, v.F, v.Ub, st);
     }
     swap_u(ctx);
     ctx->dt_valid = true;
@@ -449,6 +512,7 @@ extern "C" int cumc_initial_update(cumc_ctx *ctx)
 {
     if (!ctx) return CUMC_EINVAL;
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "cumc_set_grid must precede cumc_initial_update");
+    zexpand(ctx);
     const Lay &L = ctx->L;
     dim3 grid(nblk((long)L.px * (L.ny + 1), 256), L.nz + 1);
     hipLaunchKernelGGL(k_init, grid, dim3(256), 0, ctx->stream, L, ctx->F64, ctx->M, ctx->F64.U0);
@@ -464,6 +528,7 @@ extern "C" int cumc_initial_update(cumc_ctx *ctx)
 extern "C" int cumc_boundary(cumc_ctx *ctx)
 {
     if (!ctx) return CUMC_EINVAL;
+    zexpand(ctx);
     sync_prim64(ctx);
     launch_ghost(ctx);
     prim_to_f32(ctx);
@@ -574,6 +639,7 @@ extern "C" int cumc_find_divb(cumc_ctx *ctx, double *max_signed, double *max_abs
     if (!ctx || !max_signed || !max_abs) return CUMC_EINVAL;
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
     const Lay &L = ctx->L;
+    zexpand(ctx);
     sync_prim64(ctx);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->red, 0ull, 2);
     dim3 grid(nblk((long)L.px * L.ny, 256));
@@ -599,6 +665,7 @@ extern "C" int cumc_sweep_flux(cumc_ctx *ctx, int dir, double *out)
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
     const Lay &L = ctx->L;
     const long ncell = (long)(L.nx + 6) * (L.ny + 6) * (L.nz + 6);
+    zexpand(ctx);
     double *tmp = nullptr;
     HIPCHK(ctx, hipMalloc(&tmp, (size_t)ncell * 7 * sizeof(double)));
     if (ctx->cfg.precision == CUMC_MIXED) {
@@ -651,6 +718,7 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
     double dts[2];
     double *u0 = ctx->F64.U0, *u1 = ctx->F64.U1;
     const bool valid = ctx->dt_valid, prof = ctx->prof, stale = ctx->p64_stale;
+    const bool zmode0 = ctx->zmode, zfresh0 = ctx->zfresh;
     const int seg0 = ctx->seg, zc0 = ctx->zchunk, tw0 = ctx->tile_w;
     hipMemcpyAsync(save, ctx->pool, ctx->pool_bytes, hipMemcpyDeviceToDevice, ctx->stream);
     if (p32_bytes)
@@ -718,6 +786,8 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
     ctx->F32.U1 = u1;
     ctx->dt_valid = valid;
     ctx->p64_stale = stale;
+    ctx->zmode = zmode0;
+    ctx->zfresh = zfresh0;
     ctx->prof = prof;
     ctx->evn = 0;
     for (int c = 0; c < KC_N; c++) {
@@ -730,6 +800,25 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
     ctx->tile_w = rc == CUMC_OK ? best_tw : tw0;
     if (rc == CUMC_OK && out) cumc_get_tuning(ctx, out);
     return rc;
+}
+
+extern "C" int cumc_set_zcollapse(cumc_ctx *ctx, int on)
+{
+    if (!ctx) return CUMC_EINVAL;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!on) zexpand(ctx);
+    ctx->zc_on = on != 0;
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return CUMC_OK;
+}
+
+extern "C" int cumc_zcollapse_state(cumc_ctx *ctx, int *allowed, int *active)
+{
+    if (!ctx) return CUMC_EINVAL;
+    if (allowed) *allowed = ctx->zc_ok && ctx->zc_on && !ctx->ztape;
+    if (active) *active = ctx->zmode;
+    return CUMC_OK;
 }
 
 extern "C" int cumc_last_step_ms(cumc_ctx *ctx, double *ms)

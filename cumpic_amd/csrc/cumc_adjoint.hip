@@ -931,6 +931,7 @@ void tape_release(cumc_ctx *ctx)
     hipFree(ctx->tape->mem);
     delete ctx->tape;
     ctx->tape = nullptr;
+    ctx->ztape = false;
 }
 
 static inline size_t plane_bytes(const cumc_ctx *ctx) { return (size_t)ctx->L.ps * sizeof(double); }
@@ -1047,8 +1048,12 @@ static void reverse_step(cumc_ctx *ctx, int i, double dt)
 static int tape_alloc(cumc_ctx *ctx, int nsteps, int every)
 {
     tape_release(ctx);
+    // the taped steps and their reversal run in the full layout: the stored states keep their z
+    // ghost planes, which the reverse kernels read
+    zexpand(ctx);
     cumc_tape *tp = new cumc_tape();
     ctx->tape = tp;
+    ctx->ztape = true;
     tp->nsteps = nsteps;
     tp->every = every;
     tp->nck = nsteps / every + 2;  // step 0, every, 2 every, ... and the final state

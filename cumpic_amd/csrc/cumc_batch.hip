@@ -283,27 +283,27 @@ static void clear_graphs(cumc_batch *b)
     b->graphs.clear();
 }
 
-// tile counts of one box under the current tuning
-static void box_tiles(const cumc_batch *b, const cumc_ctx *c, BoxDesc &d, int cnt[P_N])
+// tile counts of one box (in the addressing of its descriptor, d.L) under the current tuning
+static void box_tiles(const cumc_batch *b, BoxDesc &d, int cnt[P_N])
 {
-    const Lay &L = c->L;
+    const Lay &L = d.L;
     const int seg = b->tune.seg > 0 ? b->tune.seg : 24;
     const int zc = b->tune.zchunk > 0 ? b->tune.zchunk : 4;
     const long waves = ((long)(L.ny + 2) * L.px + XS_FACES - 1) / XS_FACES;
     d.fx_gx = (int)nblk(waves * 64, 256);
-    cnt[P_FX] = d.fx_gx * (L.nz + 2);
+    cnt[P_FX] = d.fx_gx * flux_planes(L);
     const int nsy = (L.ny + 1 + seg - 1) / seg;
     d.fy_sl = (L.ny + 1 + nsy - 1) / nsy;
-    d.fy_gx = (int)nblk((long)L.px * (L.nz + 2), 128);
+    d.fy_gx = (int)nblk((long)L.px * flux_planes(L), 128);
     cnt[P_FY] = d.fy_gx * nsy;
-    const int nsz = (L.nz + 1 + seg - 1) / seg;
-    d.fz_sl = (L.nz + 1 + nsz - 1) / nsz;
+    const int nsz = L.zc1 ? 1 : (L.nz + 1 + seg - 1) / seg;  // one z face per column when collapsed
+    d.fz_sl = L.zc1 ? 1 : (L.nz + 1 + nsz - 1) / nsz;
     d.fz_gx = (int)nblk((long)L.px * (L.ny + 2), 128);
     cnt[P_FZ] = d.fz_gx * nsz;
     d.zc = zc;
     d.tw = b->tune.tile_w;
     d.c_gx = col_tiles(L, d.tw);
-    cnt[P_EC] = cnt[P_TAIL] = d.c_gx * ((L.nz + zc) / zc);
+    cnt[P_EC] = cnt[P_TAIL] = d.c_gx * zchunks(L, zc);
     cnt[P_GX] = (int)nblk(ghost_threads<0>(L), 256);
     cnt[P_GY] = (int)nblk(ghost_threads<1>(L), 256);
     cnt[P_GZ] = (int)nblk(ghost_threads<2>(L), 256);
@@ -317,17 +317,19 @@ static int refresh(cumc_batch *b)
     b->h_desc.assign(nb, BoxDesc{});
     b->h_pre.assign((size_t)P_N * (nb + 1), 0);
     for (int i = 0; i < nb; i++) {
-        const cumc_ctx *c = b->box[i];
+        cumc_ctx *c = b->box[i];
         BoxDesc &d = b->h_desc[i];
-        d.L = c->L;
-        d.F = c->F64;
+        // a periodic nz = 1 box steps in its z-collapsed view when it may (zcollapse_begin)
+        const View v = make_view(c, zcollapse_begin(c));
+        d.L = v.L;
+        d.F = v.F;
         d.tb = tabs_of(c);
         d.M = c->M;
         d.dt = c->dt_dev;
-        d.Ub = c->Ub;
+        d.Ub = v.Ub;
         d.sig = b->d_sig + i;
         int cnt[P_N];
-        box_tiles(b, c, d, cnt);
+        box_tiles(b, d, cnt);
         for (int p = 0; p < P_N; p++) b->h_pre[(size_t)p * (nb + 1) + i + 1] = b->h_pre[(size_t)p * (nb + 1) + i] + cnt[p];
     }
     for (int p = 0; p < P_N; p++) {

@@ -25,8 +25,20 @@ enum { TB_A0M = 0, TB_A1M, TB_A2M, TB_A3M, TB_A0P, TB_A1P, TB_A2P, TB_A3P, TB_B0
 struct Lay {
     int nx, ny, nz;
     long px, py, ps;
+    // 1: the z-collapsed view of a periodic nz = 1 subdomain.  Its z ghost planes hold copies of
+    // the single interior plane l = 1, so the view addresses only that plane (py = 0, field bases
+    // moved to l = 1): every z offset reads the plane itself, and the kernels compute the plane
+    // once instead of each ghost copy of it.
+    int zc1 = 0;
     __host__ __device__ long id(int j, int k, int l) const { return (long)(l + 2) * py + (long)(k + 2) * px + (j + 2); }
 };
+
+// planes of face fluxes the x and y sweeps compute: l in [0, nz+1], or the one plane of a
+// z-collapsed view
+__host__ __device__ inline int flux_planes(const Lay &L) { return L.zc1 ? 1 : L.nz + 2; }
+// column-tile chunks along z of the EMF and stage-tail kernels: planes [0, nz], or the interior
+// plane l = 1 alone in a z-collapsed view (the reference's l = 0 plane is a ghost copy there)
+__host__ __device__ inline int zchunks(const Lay &L, int zc) { return L.zc1 ? 1 : (L.nz + zc) / zc; }
 
 // Element i of a plane whose base is uniform across the wavefront, addressed as base + 32-bit
 // lane byte offset: the compiler can then use the scalar-base form of the global load/store
@@ -411,9 +423,9 @@ __device__ inline void ecorn_tile(const Lay &L, const Fields<R> &F, int bx, int 
 {
     int j, k;
     col_of(L, bx, tw, j, k);
-    const int l0 = by * zc;
+    const int l0 = L.zc1 ? 1 : by * zc;
     if (k > L.ny || j > L.nx || l0 > L.nz) return;
-    const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
+    const int l1 = L.zc1 ? 1 : ((l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz);
     const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
     const R *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2];
     const R q = R(0.25);
@@ -800,11 +812,11 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
 {
     int j, k;
     col_of(L, bx, tw, j, k);
-    const int l0 = by * zc;
+    const int l0 = L.zc1 ? 1 : by * zc;
     const bool col = k <= L.ny && j <= L.nx && l0 <= L.nz;
     double dtmin = 1.0e10;
     if (col) {
-        const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
+        const int l1 = L.zc1 ? 1 : ((l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz);
         const unsigned sj = 1, sk = (unsigned)L.px, sl = (unsigned)L.py;
         const long ps = L.ps;
         const double dt = *dtp;
@@ -856,11 +868,11 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
 {
     int j, k;
     col_of(L, bx, tw, j, k);
-    const int l0 = by * zc;
+    const int l0 = L.zc1 ? 1 : by * zc;
     const bool col = k <= L.ny && j <= L.nx && l0 <= L.nz;
     double dtmin = 1.0e10;
     if (col) {
-        const int l1 = (l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz;
+        const int l1 = L.zc1 ? 1 : ((l0 + zc - 1 < L.nz) ? l0 + zc - 1 : L.nz);
         const long sj = 1, sk = L.px, sl = L.py, ps = L.ps;
         const double dt = *dtp;
         const R *Fx = F.F[0], *Fy = F.F[1], *Fz = F.F[2];
@@ -1210,12 +1222,15 @@ __device__ inline bool eps_y_outer(const int *bf, int *flag_eff)
 }
 
 // number of ghost-fill threads of pass AX
+// (a z-collapsed view runs the x and y passes on its interior plane only and has no z pass: its
+// z ghosts are the plane itself)
 template <int AX>
 __host__ __device__ inline long ghost_threads(const Lay &L)
 {
-    if (AX == 0) return (long)(L.ny + 1) * (L.nz + 1) * 6;
-    if (AX == 1) return (long)(L.nx + 6) * (L.nz + 1) * 6;
-    return (long)(L.nx + 6) * (L.ny + 6) * 6;
+    const long nzb = L.zc1 ? 1 : L.nz + 1;
+    if (AX == 0) return (long)(L.ny + 1) * nzb * 6;
+    if (AX == 1) return (long)(L.nx + 6) * nzb * 6;
+    return L.zc1 ? 0 : (long)(L.nx + 6) * (L.ny + 6) * 6;
 }
 
 // One ghost cell of pass AX on the primitive planes P (8), EPS, BC (3) of element type T: the
@@ -1230,6 +1245,7 @@ __device__ inline void ghost_cells(const Lay &L, T *__restrict__ P, T *__restric
     if (AX == 0) { a0 = 0; na = L.ny + 1; b0 = 0; nb = L.nz + 1; }           // k, l in [0,n]
     else if (AX == 1) { a0 = -2; na = L.nx + 6; b0 = 0; nb = L.nz + 1; }     // j full, l in [0,n]
     else { a0 = -2; na = L.nx + 6; b0 = -2; nb = L.ny + 6; }                 // j, k full
+    if (AX != 2 && L.zc1) { b0 = 1; nb = 1; }                                // the interior plane
     const long per = (long)na * nb;
     if (t >= per * 6) return;
     int gs;  // 0..5 = (g-1)*2 + side
@@ -1455,6 +1471,54 @@ __global__ __launch_bounds__(256) void k_divb(Lay L, Fields<double> F, Tabs tb, 
         }
         atomicMax(out, (unsigned long long)__double_as_longlong(a));
         atomicMax(out + 1, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// z-collapsed view of a periodic nz = 1 subdomain: entry check and return to the full layout
+// ------------------------------------------------------------------------------------------
+// Counts the cells of the whole (j, k) plane where face B_z of plane 0 differs from plane 1, in
+// P or in U0 (bitwise).  In the reference both faces are the same periodic face, evolved
+// separately from equal right-hand sides; the collapsed view keeps one of them.
+static __global__ void k_zcheck(Lay L, const double *__restrict__ P, const double *__restrict__ U0,
+                                unsigned long long *cnt)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long sx = L.nx + 6;
+    if (t >= sx * (L.ny + 6)) return;
+    const int j = (int)(t % sx) - 2, k = (int)(t / sx) - 2;
+    const long g0 = L.id(j, k, 0) + 7 * L.ps, g1 = L.id(j, k, 1) + 7 * L.ps;
+    const bool same = __double_as_longlong(P[g0]) == __double_as_longlong(P[g1]) &&
+                      __double_as_longlong(U0[g0]) == __double_as_longlong(U0[g1]);
+    if (!same) atomicAdd(cnt, 1ull);
+}
+
+// Rewrites the z ghost planes from plane l = 1 as the reference's z ghost pass leaves them
+// (boundary.f90:564-652 with periodic z and nz = 1: every ghost plane is a copy of plane 1 in
+// rho, v, p, the transverse face fields B_x, B_y, cell-centred B and eps; face B_z is not copied
+// by the pass), plus face B_z and the conserved state of plane 0, the periodic face the
+// collapsed view evolved as plane 1's own.
+static __global__ void k_zexpand(Lay L, Fields<double> F)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long sx = L.nx + 6;
+    if (t >= sx * (L.ny + 6)) return;
+    const int j = (int)(t % sx) - 2, k = (int)(t / sx) - 2;
+    const long ps = L.ps, src = L.id(j, k, 1);
+    const int ls[6] = {-2, -1, 0, 2, 3, 4};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const long dst = L.id(j, k, ls[q]);
+#pragma unroll
+        for (int i = 0; i < 7; i++) F.P[dst + i * ps] = F.P[src + i * ps];
+#pragma unroll
+        for (int c = 0; c < 3; c++) F.BC[dst + c * ps] = F.BC[src + c * ps];
+        F.EPS[dst] = F.EPS[src];
+        if (ls[q] == 0) {
+            F.P[dst + 7 * ps] = F.P[src + 7 * ps];
+#pragma unroll
+            for (int i = 0; i < 8; i++) F.U0[dst + i * ps] = F.U0[src + i * ps];
+        }
     }
 }
 

@@ -82,6 +82,8 @@ def lib(variant: str = ""):
             "cumc_kernel_times": [_CTX, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_long),
                                   ctypes.POINTER(ctypes.c_char_p)],
             "cumc_pattern_probe": [_CTX, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D],
+            "cumc_set_zcollapse": [_CTX, ctypes.c_int],
+            "cumc_zcollapse_state": [_CTX, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
         }
         T = ctypes.POINTER(Tuning)
         sig.update({
@@ -309,6 +311,17 @@ class Solver:
     def set_tuning(self, d: dict):
         t = Tuning.from_dict(d)
         self._check(self._L.cumc_set_tuning(self.h, ctypes.byref(t)), "cumc_set_tuning")
+
+    def set_zcollapse(self, on: bool):
+        """z-collapsed stepping of a periodic nz = 1 context (on by default; off keeps the full
+        layout).  Results are bit-identical either way."""
+        self._check(self._L.cumc_set_zcollapse(self.h, int(bool(on))), "cumc_set_zcollapse")
+
+    def zcollapse_state(self) -> tuple[bool, bool]:
+        """(allowed, active): whether the context may step collapsed, and whether it is now."""
+        a, b = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self._L.cumc_zcollapse_state(self.h, ctypes.byref(a), ctypes.byref(b)), "cumc_zcollapse_state")
+        return bool(a.value), bool(b.value)
 
 
 class Batch:

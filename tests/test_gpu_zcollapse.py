@@ -1,0 +1,165 @@
+"""z-collapsed stepping of periodic nz = 1 subdomains (cumc_set_zcollapse).
+
+A 2D box (nz = 1, periodic z) steps its single interior plane with every z neighbour read from the
+plane itself instead of filling and computing three z ghost planes per side.  The reference's z
+ghost planes are copies of that plane (boundary.f90:564-652), and its two z faces of the cell are
+the same periodic face, so the collapsed step must be bit-identical to the full layout and to the
+CPU restatement: checked here over runs, host calls between runs (download, boundary, find_divb,
+upload), the batched form, and the cases where the view must not be used.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+from cumpic_amd import ics
+from cumpic_amd.config import IBZ, MIXED, PPMC, TVDMM
+from oracle.oracle import Oracle
+from tests.helpers import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _box(n=24, seed=3100, recon=None, ny=None):
+    p = ics.randomize(ics.spectral_box(n=n, seed=seed, ndim=2, v_rms=0.3, db_rms=0.3, ny=ny), seed=seed + 1, amp=0.1)
+    if recon is not None:
+        p.cfg = dataclasses.replace(p.cfg, recon=recon)
+    return p
+
+
+def _solver(p, collapse=True):
+    from cumpic_amd.cumc import Solver
+    s = Solver.from_problem(p)
+    s.set_zcollapse(collapse)
+    s.initial_update()
+    return s
+
+
+def _same(a, b, what):
+    sa, sb = a.download(), b.download()
+    for k in ("prim", "eps", "bcell"):
+        assert_bitwise(sa[k], sb[k], f"{what} {k}")
+
+
+def _same_oracle(g, o, what):
+    a, b = g.download(), o.get_state()
+    for k in ("prim", "eps", "bcell"):
+        assert_bitwise(a[k], b[k], f"{what} {k}")
+
+
+@pytest.mark.parametrize("recon", [PPMC, TVDMM])
+def test_collapsed_run_is_bitwise_full_layout_and_oracle(recon):
+    print(f"[case] collapsed vs full vs oracle, recon={recon}", flush=True)
+    p = _box(recon=recon)
+    a, b = _solver(p, True), _solver(p, False)
+    o = Oracle.from_problem(p)
+    o.initial_update()
+    da, _ = a.run(8)
+    assert a.zcollapse_state() == (True, True), "the 2D periodic box should step collapsed"
+    db, _ = b.run(8)
+    assert b.zcollapse_state()[1] is False
+    do, _ = o.run(8)
+    assert_bitwise(da, db, "dt sequence collapsed vs full")
+    assert_bitwise(da, do, "dt sequence vs oracle")
+    _same(a, b, "collapsed vs full")
+    _same_oracle(a, o, "collapsed vs oracle")
+    assert a.find_divb() == b.find_divb()
+
+
+def test_host_calls_between_collapsed_runs():
+    print("[case] host calls between collapsed runs", flush=True)
+    p = _box(n=20, ny=28, seed=3200)
+    a, b = _solver(p, True), _solver(p, False)
+    for s in (a, b):
+        s.run(3)
+    _same(a, b, "after 3 steps")  # download rewrites the z ghost planes
+    for s in (a, b):
+        s.run(2)
+        s.boundary()
+        s.run(2)
+    _same(a, b, "after boundary")
+    assert a.find_divb() == b.find_divb()
+    for s in (a, b):
+        s.rungekutta(s.finddt())
+    # a re-upload of another state, stepped without initial_update (its z ghost planes are
+    # the caller's until the first ghost fill)
+    q = _box(n=20, ny=28, seed=3300)
+    for s in (a, b):
+        s.upload(q.prim, q.eps)
+        s.run(3)
+    _same(a, b, "after re-upload")
+
+
+def test_unequal_periodic_bz_faces_keep_full_layout():
+    print("[case] B_z(0) != B_z(1)", flush=True)
+    p = _box(n=16, seed=3400)
+    p.prim[2, :, :, IBZ] += 1e-3  # face B_z of plane l = 0 (array index l + 2)
+    a = _solver(p, True)
+    o = Oracle.from_problem(p)
+    o.initial_update()
+    da, _ = a.run(4)
+    do, _ = o.run(4)
+    assert a.zcollapse_state() == (True, False), "unequal periodic B_z faces must not collapse"
+    assert_bitwise(da, do, "dt sequence vs oracle")
+    _same_oracle(a, o, "full layout vs oracle")
+
+
+def test_configurations_that_do_not_collapse():
+    from cumpic_amd.cumc import Solver
+    print("[case] configurations without the collapsed view", flush=True)
+    p = _box(n=12, seed=3500)
+    # z outflow
+    q = dataclasses.replace(p.cfg, boundary_flag=(0, 0, 0, 0, 1, 1))
+    s = Solver(q, p.xf, p.yf, p.zf)
+    assert s.zcollapse_state()[0] is False
+    # unequal z cell widths
+    zf = p.zf.copy()
+    zf[0] -= 0.25
+    s = Solver(p.cfg, p.xf, p.yf, zf)
+    assert s.zcollapse_state()[0] is False
+    # mixed precision
+    s = Solver(dataclasses.replace(p.cfg, precision=MIXED), p.xf, p.yf, p.zf)
+    assert s.zcollapse_state()[0] is False
+    # a 3D box
+    r = ics.small_periodic(8, seed=3600)
+    s = Solver(r.cfg, r.xf, r.yf, r.zf)
+    assert s.zcollapse_state()[0] is False
+    s = Solver(p.cfg, p.xf, p.yf, p.zf)
+    assert s.zcollapse_state()[0] is True
+
+
+@pytest.mark.parametrize("graph_steps", [0, 3])
+def test_batch_collapsed_is_bitwise_full_layout(graph_steps):
+    from cumpic_amd.cumc import Batch
+    print(f"[case] batch collapsed vs single full layout, graph_steps={graph_steps}", flush=True)
+    probs = [_box(n=n, seed=3700 + 10 * i) for i, n in enumerate((16, 40, 24, 9))]
+    probs.append(ics.randomize(ics.small_periodic(10, seed=3800), seed=3, amp=0.1))  # 3D: not collapsed
+    singles = [_solver(p, False) for p in probs]
+    boxes = [_solver(p, True) for p in probs]
+    b = Batch(boxes)
+    dts = b.run(6, graph_steps=graph_steps)
+    ref = np.array([s.run(6)[0] for s in singles]).T
+    assert [x.zcollapse_state()[1] for x in boxes] == [True, True, True, True, False]
+    assert_bitwise(dts, ref, "per-box dt sequence")
+    for i, (x, y) in enumerate(zip(boxes, singles)):
+        _same(x, y, f"box {i}")
+    # continue after the downloads (which returned the boxes to the full layout)
+    dts2 = b.run(3, graph_steps=graph_steps)
+    ref2 = np.array([s.run(3)[0] for s in singles]).T
+    assert_bitwise(dts2, ref2, "second run dt")
+    for i, (x, y) in enumerate(zip(boxes, singles)):
+        _same(x, y, f"box {i} second run")
+
+
+def test_tape_keeps_full_layout():
+    print("[case] tape blocks the collapsed view", flush=True)
+    p = _box(n=12, seed=3900)
+    a = _solver(p, True)
+    a.run(2)
+    assert a.zcollapse_state() == (True, True)
+    a.tape_run(3, every=2)
+    assert a.zcollapse_state() == (False, False)
+    a.tape_free()
+    assert a.zcollapse_state()[0] is True
