@@ -58,6 +58,7 @@ __device__ inline void kr_ghost_line(const Lay &L, const AdjF &A, const Model &M
     if (AX == 0) { a0 = 0; na = L.ny + 1; b0 = 0; nb = L.nz + 1; }
     else if (AX == 1) { a0 = -2; na = L.nx + 6; b0 = 0; nb = L.nz + 1; }
     else { a0 = -2; na = L.nx + 6; b0 = -2; nb = L.ny + 6; }
+    if (AX != 2 && L.zc1) { b0 = 1; nb = 1; }  // z-collapsed view: the interior plane (no z pass)
     if (t >= (long)na * nb) return;
     const int a = (int)(t % na) + a0, b = (int)(t / na) + b0;
     const int n = AX == 0 ? L.nx : (AX == 1 ? L.ny : L.nz);
@@ -245,7 +246,9 @@ __device__ inline void kr_tail_b_cell(const Lay &L, const AdjF &A, double dt, do
     double lb[3];
     lb[0] = A.T[g + 5 * ps] + ((j + 1 <= L.nx && k >= 1 && l >= 1) ? A.T[g + 1 + 8 * ps] : 0.0);
     lb[1] = A.T[g + 6 * ps] + ((k + 1 <= L.ny && j >= 1 && l >= 1) ? A.T[g + L.px + 9 * ps] : 0.0);
-    lb[2] = A.T[g + 7 * ps] + ((l + 1 <= L.nz && j >= 1 && k >= 1) ? A.T[g + L.py + 10 * ps] : 0.0);
+    // (in a z-collapsed view the cell above is the cell itself, which reads this face as its lower
+    // one too)
+    lb[2] = A.T[g + 7 * ps] + (((L.zc1 || l + 1 <= L.nz) && j >= 1 && k >= 1) ? A.T[g + L.py + 10 * ps] : 0.0);
     for (int c = 0; c < 3; c++) {
         lU0[g + (5 + c) * ps] += RKC<STAGE>::a0 * lb[c];
         if (STAGE > 1) lUin[g + (5 + c) * ps] += RKC<STAGE>::a1 * lb[c];
@@ -266,6 +269,9 @@ __device__ inline void kr_gather_cell(const Lay &L, const AdjF &A, const Tabs &t
     const int k = (int)(t / L.px), j = (int)(t - (long)k * L.px);
     if (k > L.ny + 1 || j > L.nx + 1) return;
     const long g = L.id(j, k, l), ps = L.ps;
+    // In a z-collapsed view (l = 1 only) the z neighbour l + 1 is the cell itself: its lower-face
+    // and lower-edge reads are reads of this face / edge, with this cell's widths.
+    const int lu = L.zc1 ? l : l + 1;
     auto inter = [&](int jj, int kk, int ll) {
         return jj >= 1 && jj <= L.nx && kk >= 1 && kk <= L.ny && ll >= 1 && ll <= L.nz;
     };
@@ -282,17 +288,17 @@ __device__ inline void kr_gather_cell(const Lay &L, const AdjF &A, const Tabs &t
     if (l <= L.nz)
         for (int i = 0; i < 5; i++)
             A.F[2][g + i * ps] = (inter(j, k, l) ? -A.T[g + i * ps] / tb.at(2, TB_D, l) : 0.0) +
-                                 (inter(j, k, l + 1) ? A.T[g + L.py + i * ps] / tb.at(2, TB_D, l + 1) : 0.0);
+                                 (inter(j, k, lu) ? A.T[g + L.py + i * ps] / tb.at(2, TB_D, lu) : 0.0);
     // corner EMFs from the face-field right-hand sides (mhd.f90:130-153, Cartesian)
     if (inbox(j, k, l)) {
         auto rb = [&](int c, int jj, int kk, int ll) {
             return inbox(jj, kk, ll) ? A.T[L.id(jj, kk, ll) + (5 + c) * ps] : 0.0;
         };
         const double dx = tb.at(0, TB_D, j), dy = tb.at(1, TB_D, k), dz = tb.at(2, TB_D, l);
-        const double dx1 = tb.at(0, TB_D, j + 1), dy1 = tb.at(1, TB_D, k + 1), dz1 = tb.at(2, TB_D, l + 1);
+        const double dx1 = tb.at(0, TB_D, j + 1), dy1 = tb.at(1, TB_D, k + 1), dz1 = tb.at(2, TB_D, lu);
         const double lez = -rb(0, j, k, l) / dy + rb(0, j, k + 1, l) / dy1 + rb(1, j, k, l) / dx - rb(1, j + 1, k, l) / dx1;
-        const double ley = rb(0, j, k, l) / dz - rb(0, j, k, l + 1) / dz1 - rb(2, j, k, l) / dx + rb(2, j + 1, k, l) / dx1;
-        const double lex = -rb(1, j, k, l) / dz + rb(1, j, k, l + 1) / dz1 + rb(2, j, k, l) / dy - rb(2, j, k + 1, l) / dy1;
+        const double ley = rb(0, j, k, l) / dz - rb(0, j, k, lu) / dz1 - rb(2, j, k, l) / dx + rb(2, j + 1, k, l) / dx1;
+        const double lex = -rb(1, j, k, l) / dz + rb(1, j, k, lu) / dz1 + rb(2, j, k, l) / dy - rb(2, j, k + 1, l) / dy1;
         A.E[g] = lex;
         A.E[g + ps] = ley;
         A.E[g + 2 * ps] = lez;
@@ -357,18 +363,20 @@ struct Pencil {
     int n;
     __device__ bool init(const Lay &L, long t)
     {
+        // (a z-collapsed view has one plane of x / y pencils and one z face per column: its z
+        // pencils are walked as if n = 0, cells 0 and 1 around face 0, all of them the cell itself)
         if (DIR == 1) {
             const int k = (int)(t % (L.ny + 2)), l = (int)(t / (L.ny + 2));
-            if (l > L.nz + 1) return false;
+            if (l >= flux_planes(L)) return false;
             base = L.id(-2, k, l); stride = 1; n = L.nx;
         } else if (DIR == 2) {
             const int j = (int)(t % (L.nx + 2)), l = (int)(t / (L.nx + 2));
-            if (l > L.nz + 1) return false;
+            if (l >= flux_planes(L)) return false;
             base = L.id(j, -2, l); stride = L.px; n = L.ny;
         } else {
             const int j = (int)(t % (L.nx + 2)), k = (int)(t / (L.nx + 2));
             if (k > L.ny + 1) return false;
-            base = L.id(j, k, -2); stride = L.py; n = L.nz;
+            base = L.id(j, k, -2); stride = L.py; n = L.zc1 ? 0 : L.nz;
         }
         return true;
     }
@@ -378,7 +386,8 @@ struct Pencil {
 template <int DIR>
 __host__ __device__ inline long npencils(const Lay &L)
 {
-    return DIR == 1 ? (long)(L.ny + 2) * (L.nz + 2) : (DIR == 2 ? (long)(L.nx + 2) * (L.nz + 2) : (long)(L.nx + 2) * (L.ny + 2));
+    return DIR == 1 ? (long)(L.ny + 2) * flux_planes(L)
+                    : (DIR == 2 ? (long)(L.nx + 2) * flux_planes(L) : (long)(L.nx + 2) * (L.ny + 2));
 }
 
 // reconstructed variable v (0..6) of a sweep: P planes 0..4, then the two transverse cell fields
@@ -614,7 +623,7 @@ __host__ __device__ inline long lane_cells_x(const Lay &L) { return (long)(L.ny 
 template <int DIR>
 __host__ __device__ inline int lane_outer(const Lay &L)
 {
-    return DIR == 3 ? L.ny + 2 : L.nz + 2;
+    return DIR == 3 ? L.ny + 2 : flux_planes(L);
 }
 // 256-thread tiles per outer index: along x, 4 waves of XR_OWN owned cells of the row run; along
 // y and z, 4 pencils x one XR_OWN-cell segment (kr_flux_lane_t_tile)
@@ -1045,15 +1054,16 @@ static void reverse_step(cumc_ctx *ctx, int i, double dt)
 
 // one allocation per context: uploaded state, checkpoints, segment states, stage saves,
 // adjoint arrays and the three conserved-state adjoints
-static int tape_alloc(cumc_ctx *ctx, int nsteps, int every)
+static int tape_alloc(cumc_ctx *ctx, int nsteps, int every, bool zcollapse_ok = false)
 {
     tape_release(ctx);
-    // the taped steps and their reversal run in the full layout: the stored states keep their z
-    // ghost planes, which the reverse kernels read
+    // a single context's taped steps and their reversal run in the full layout (the stored states
+    // keep their z ghost planes, which its reverse kernels read); the batched tape reverses
+    // z-collapsed boxes in their view
     zexpand(ctx);
     cumc_tape *tp = new cumc_tape();
     ctx->tape = tp;
-    ctx->ztape = true;
+    ctx->ztape = !zcollapse_ok;
     tp->nsteps = nsteps;
     tp->every = every;
     tp->nck = nsteps / every + 2;  // step 0, every, 2 every, ... and the final state
@@ -1267,9 +1277,15 @@ struct AdjBox {
     int gx_c, gx_e;  // 256-cell tiles per plane row set: [0,n]^2 and [0,n+1]^2 column domains
     int gpen[3];     // 64-pencil tiles per sweep
     int glane[3];    // kr_flux_lane tiles per outer index, per sweep
+    // z-collapsed view (L.zc1): the column kernels run plane l0 = 1 only, and the fields of a
+    // stored state are the slab of zslab cells at offset zoff of each full-layout plane (0, 0
+    // and ps in the full layout)
+    int l0;
+    long zoff, zslab;
 };
 
-enum { Q_TAIL, Q_GE, Q_GX, Q_GY, Q_GZ, Q_H1, Q_H2, Q_H3, Q_R1, Q_R2, Q_R3, Q_CP, Q_L1, Q_L2, Q_L3, Q_N };
+// work lists; Q_CP: state copies (a z-collapsed box's plane-1 slabs), Q_CW: whole-state copies
+enum { Q_TAIL, Q_GE, Q_GX, Q_GY, Q_GZ, Q_H1, Q_H2, Q_H3, Q_R1, Q_R2, Q_R3, Q_CP, Q_L1, Q_L2, Q_L3, Q_CW, Q_N };
 constexpr int CP_ELEMS = 1024;  // doubles per copy tile (256 threads x 4)
 
 __device__ inline int adj_box_of(const int *pre, int nbox, int it)
@@ -1294,11 +1310,12 @@ __device__ inline StageStates stage_states(const AdjBox &d, int stage, int i)
     const size_t ps = d.L.ps, st = (size_t)SW * ps;
     const double *Sm = d.seg + i * st, *Sa = d.sa + i * st, *Sb = d.sb + i * st;
     StageStates v;
-    v.U0 = Sm + 12 * ps;
-    if (stage == 3) { v.in = Sb; v.out = Sm + st; v.Uin = Sb + 12 * ps; }
-    else if (stage == 2) { v.in = Sa; v.out = Sb; v.Uin = Sa + 12 * ps; }
-    else if (stage == 1) { v.in = Sm; v.out = Sa; v.Uin = nullptr; }
-    else { v.in = d.ckpt; v.out = d.ckpt; v.Uin = nullptr; }
+    const long z = d.zoff;
+    v.U0 = Sm + 12 * ps + z;
+    if (stage == 3) { v.in = Sb + z; v.out = Sm + st + z; v.Uin = Sb + 12 * ps + z; }
+    else if (stage == 2) { v.in = Sa + z; v.out = Sb + z; v.Uin = Sa + 12 * ps + z; }
+    else if (stage == 1) { v.in = Sm + z; v.out = Sa + z; v.Uin = nullptr; }
+    else { v.in = d.ckpt + z; v.out = d.ckpt + z; v.Uin = nullptr; }
     return v;
 }
 
@@ -1332,7 +1349,7 @@ __global__ __launch_bounds__(256) void kbr_tail_a(const AdjBox *__restrict__ D, 
         double *lUout = STAGE == 2 ? lb : l1;
         kr_tail_a_cell<STAGE>(d.L, fields_of_state(d.F, v.in, ps), v.out, v.out + 8 * ps, v.out + 9 * ps, d.M, *d.dt,
                               v.U0, v.Uin, d.A, l0, lUin, lUout, (long)(loc % d.gx_c) * 256 + threadIdx.x,
-                              loc / d.gx_c);
+                              loc / d.gx_c + d.l0);
     }
 }
 
@@ -1343,14 +1360,14 @@ __global__ __launch_bounds__(256) void kbr_tail_b(const AdjBox *__restrict__ D, 
     FOR_ITEMS(Q_TAIL)
         double *l0 = d.lu[q], *l1 = d.lu[1 - q], *lb = d.lu[2];
         double *lUin = STAGE == 3 ? lb : (STAGE == 2 ? l1 : nullptr);
-        kr_tail_b_cell<STAGE>(d.L, d.A, *d.dt, l0, lUin, (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c);
+        kr_tail_b_cell<STAGE>(d.L, d.A, *d.dt, l0, lUin, (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c + d.l0);
     }
 }
 
 __global__ __launch_bounds__(256) void kbr_gather(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox)
 {
     FOR_ITEMS(Q_GE)
-        kr_gather_cell(d.L, d.A, d.tb, (long)(loc % d.gx_e) * 256 + threadIdx.x, loc / d.gx_e);
+        kr_gather_cell(d.L, d.A, d.tb, (long)(loc % d.gx_e) * 256 + threadIdx.x, loc / d.gx_e + d.l0);
     }
 }
 
@@ -1359,7 +1376,7 @@ __global__ __launch_bounds__(256) void kbr_ecorn(const AdjBox *__restrict__ D, c
 {
     FOR_ITEMS(Q_GE)
         const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
-        kr_ecorn_cell(d.L, F, d.A, (long)(loc % d.gx_e) * 256 + threadIdx.x, loc / d.gx_e);
+        kr_ecorn_cell(d.L, F, d.A, (long)(loc % d.gx_e) * 256 + threadIdx.x, loc / d.gx_e + d.l0);
     }
 }
 
@@ -1431,19 +1448,30 @@ __global__ __launch_bounds__(256) void kbr_flux_lane(const AdjBox *__restrict__ 
 // state copies between the context fields and the per-box tape (element e of 20 planes: the
 // first 12 are P, EPS, bcell of the pool, the last 8 the conserved state: U0 / U1 by par, or Ub)
 enum { CP_SAVE_SEG, CP_SAVE_SA, CP_SAVE_SB, CP_LOAD_CK, CP_SAVE_CK, CP_LOAD_INIT, CP_SAVE_INIT };
+__host__ __device__ inline bool copy_whole(int mode, int slot)
+{
+    return mode == CP_SAVE_INIT || mode == CP_LOAD_INIT || (slot == 0 && (mode == CP_SAVE_CK || mode == CP_LOAD_CK));
+}
 
 __global__ __launch_bounds__(256) void kbr_copy(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
                                                 int mode, int slot, int par)
 {
-    FOR_ITEMS(Q_CP)
+    // the uploaded state and checkpoint 0 (read by the full-layout reverse of initial_update) are
+    // copied whole; a z-collapsed box's other states are the plane-1 slabs of the fields
+    const bool whole = copy_whole(mode, slot);
+    FOR_ITEMS(whole ? Q_CW : Q_CP)
         const long ps = d.L.ps;
-        double *U = mode == CP_SAVE_SB ? d.Ub : (par ? d.F.U1 : d.F.U0);
+        const long pe = whole ? ps : d.zslab, zo = whole ? 0 : d.zoff;
+        // conserved buffers as full-layout bases (the descriptor's are the view's)
+        double *U = (mode == CP_SAVE_SB ? d.Ub : (par ? d.F.U1 : d.F.U0)) - d.zoff;
         double *pool = d.pool;
         const size_t at = (size_t)slot * SW * ps;
         for (int r = 0; r < CP_ELEMS / 256; r++) {
-            const long e = (long)loc * CP_ELEMS + r * 256 + threadIdx.x;
-            if (e >= SW * ps) break;
-            const bool lo = e < 12 * ps;
+            const long i = (long)loc * CP_ELEMS + r * 256 + threadIdx.x;
+            if (i >= SW * pe) break;
+            const long f = i / pe;
+            const long e = f * ps + zo + (i - f * pe);  // element of the 20 full-layout planes
+            const bool lo = f < 12;
             double *ctxp = lo ? pool + e : U + (e - 12 * ps);
             switch (mode) {
             case CP_SAVE_SEG: d.seg[at + e] = *ctxp; break;
@@ -1463,6 +1491,28 @@ __global__ __launch_bounds__(256) void kbr_copy(const AdjBox *__restrict__ D, co
 
 #undef FOR_ITEMS
 
+// The reverse of the last ghost fill's z pass on the cell-field seeds of a z-collapsed box
+// (full layout): the seed on each z ghost plane is added to plane 1, which that pass copied there,
+// in the order of kr_ghost_line<2> (side 0, then 1; ghost layers 1..3), and cleared.
+__global__ __launch_bounds__(256) void kbr_zfold_bc(Lay L, double *BC)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long sx = L.nx + 6;
+    if (t >= sx * (L.ny + 6)) return;
+    const int j = (int)(t % sx) - 2, k = (int)(t / sx) - 2;
+    const long ps = L.ps, src = L.id(j, k, 1);
+    for (int c = 0; c < 3; c++) {
+        double acc = BC[src + c * ps];
+        for (int side = 0; side < 2; side++)
+            for (int g = 1; g <= 3; g++) {
+                const long dst = L.id(j, k, side == 0 ? 1 - g : 1 + g) + c * ps;
+                acc += BC[dst];
+                BC[dst] = 0.0;
+            }
+        BC[src + c * ps] = acc;
+    }
+}
+
 }  // namespace
 
 struct batch_tape {
@@ -1471,6 +1521,10 @@ struct batch_tape {
     AdjBox *d = nullptr;
     int *pre = nullptr;
     int total[Q_N] = {0};
+    // the same descriptors in the full layout for every box (the reverse of initial_update)
+    AdjBox *d_full = nullptr;
+    int *pre_full = nullptr;
+    int total_full[Q_N] = {0};
     std::vector<double> dts;  // nsteps x nbox
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<hipEvent_t> seg_ev;  // start / end of each segment's forward pass in the backward
@@ -1487,6 +1541,8 @@ void batch_tape_release(cumc_batch *b)
     if (!bt) return;
     hipFree(bt->d);
     hipFree(bt->pre);
+    hipFree(bt->d_full);
+    hipFree(bt->pre_full);
     if (bt->e0) hipEventDestroy(bt->e0);
     if (bt->e1) hipEventDestroy(bt->e1);
     for (hipEvent_t e : bt->seg_ev) hipEventDestroy(e);
@@ -1494,8 +1550,18 @@ void batch_tape_release(cumc_batch *b)
     bt = nullptr;
 }
 
-// descriptors from the boxes' current buffers
-static int bt_upload(cumc_batch *b, batch_tape *bt)
+// adjoint arrays addressed like make_view's fields (bases moved by off cells)
+static AdjF adj_view(AdjF A, long off)
+{
+    A.P += off; A.BC += off; A.EPS += off;
+    for (int d = 0; d < 3; d++) A.F[d] += off;
+    A.E += off; A.T += off; A.H += off;
+    return A;
+}
+
+// descriptors from the boxes' current buffers: a box whose state is z-collapsed (zmode) is
+// reversed in that view unless `full` (the reverse of initial_update runs in the full layout)
+static int bt_upload(cumc_batch *b, batch_tape *bt, bool full = false)
 {
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
@@ -1504,28 +1570,35 @@ static int bt_upload(cumc_batch *b, batch_tape *bt)
         cumc_ctx *c = batch_box(b, i);
         cumc_tape *tp = c->tape;
         AdjBox &a = bt->h[i];
-        const Lay &L = c->L;
+        const bool col = c->zmode && !full;
+        const View v = make_view(c, col);
+        const long off = col ? 3 * c->L.py : 0;
+        const Lay &L = v.L;
         a.L = L;
-        a.F = c->F64;
+        a.F = v.F;
         a.tb = tabs_of(c);
         a.M = c->M;
         a.pool = c->pool;
-        a.Ub = c->Ub;
+        a.Ub = v.Ub;
         a.dt = c->dt_dev;
-        a.A = tp->A;
-        for (int r = 0; r < 3; r++) a.lu[r] = tp->lu[r];
+        a.A = adj_view(tp->A, off);
+        for (int r = 0; r < 3; r++) a.lu[r] = tp->lu[r] + off;
         a.seg = tp->seg; a.sa = tp->sa; a.sb = tp->sb; a.ckpt = tp->ckpt; a.init = tp->init;
+        a.l0 = col ? 1 : 0;
+        a.zoff = off;
+        a.zslab = col ? c->L.py : L.ps;
         a.gx_c = (int)nblk(L.px * (L.ny + 1), 256);
         a.gx_e = (int)nblk(L.px * (L.ny + 2), 256);
         a.gpen[0] = (int)nblk(npencils<1>(L), 64);
         a.gpen[1] = (int)nblk(npencils<2>(L), 64);
         a.gpen[2] = (int)nblk(npencils<3>(L), 64);
         long cnt[Q_N];
-        cnt[Q_TAIL] = (long)a.gx_c * (L.nz + 1);
-        cnt[Q_GE] = (long)a.gx_e * (L.nz + 2);
-        cnt[Q_GX] = nblk((long)(L.ny + 1) * (L.nz + 1), 256);
-        cnt[Q_GY] = nblk((long)(L.nx + 6) * (L.nz + 1), 256);
-        cnt[Q_GZ] = nblk((long)(L.nx + 6) * (L.ny + 6), 256);
+        const long nzc = col ? 1 : L.nz + 1;  // planes of the column kernels
+        cnt[Q_TAIL] = (long)a.gx_c * nzc;
+        cnt[Q_GE] = (long)a.gx_e * (col ? 1 : L.nz + 2);
+        cnt[Q_GX] = nblk((long)(L.ny + 1) * nzc, 256);
+        cnt[Q_GY] = nblk((long)(L.nx + 6) * nzc, 256);
+        cnt[Q_GZ] = col ? 0 : nblk((long)(L.nx + 6) * (L.ny + 6), 256);
         const bool lane[3] = {lane_sweep<1>(L), lane_sweep<2>(L), lane_sweep<3>(L)};
         a.glane[0] = lane_tiles<1>(L);
         a.glane[1] = lane_tiles<2>(L);
@@ -1536,16 +1609,18 @@ static int bt_upload(cumc_batch *b, batch_tape *bt)
             cnt[Q_R1 + dd] = lane[dd] ? 0 : 7L * a.gpen[dd];
             cnt[Q_L1 + dd] = lane[dd] ? (long)a.glane[dd] * outer[dd] : 0;
         }
-        cnt[Q_CP] = (long)((SW * L.ps + CP_ELEMS - 1) / CP_ELEMS);
+        cnt[Q_CP] = (long)((SW * a.zslab + CP_ELEMS - 1) / CP_ELEMS);
+        cnt[Q_CW] = (long)((SW * L.ps + CP_ELEMS - 1) / CP_ELEMS);
         for (int q = 0; q < Q_N; q++) {
             const long v = (long)pre[(size_t)q * (nb + 1) + i] + cnt[q];
             if (v > 0x7fffffffL) return fail(c, CUMC_EINVAL, "batched reverse work list exceeds 2^31 tiles");
             pre[(size_t)q * (nb + 1) + i + 1] = (int)v;
         }
     }
-    for (int q = 0; q < Q_N; q++) bt->total[q] = pre[(size_t)q * (nb + 1) + nb];
-    if (hipMemcpyAsync(bt->d, bt->h.data(), sizeof(AdjBox) * nb, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(bt->pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+    int *total = full ? bt->total_full : bt->total;
+    for (int q = 0; q < Q_N; q++) total[q] = pre[(size_t)q * (nb + 1) + nb];
+    if (hipMemcpyAsync(full ? bt->d_full : bt->d, bt->h.data(), sizeof(AdjBox) * nb, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(full ? bt->pre_full : bt->pre, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         batch_err(b) = "batched reverse descriptors upload failed";
         return CUMC_EHIP;
@@ -1553,13 +1628,17 @@ static int bt_upload(cumc_batch *b, batch_tape *bt)
     return CUMC_OK;
 }
 
-static inline const int *bt_pre(const batch_tape *bt, int nb) { return bt->pre; }
-static inline dim3 bt_grid(const batch_tape *bt, int q) { return dim3(bt->total[q] > 0 ? bt->total[q] : 1); }
+static inline dim3 bt_grid(const batch_tape *bt, int q, bool full = false)
+{
+    const int t = full ? bt->total_full[q] : bt->total[q];
+    return dim3(t > 0 ? t : 1);
+}
 
-static void bt_copy(cumc_batch *b, batch_tape *bt, int mode, int slot, int par)
+static void bt_copy(cumc_batch *b, batch_tape *bt, int mode, int slot, int par, bool full = false)
 {
     const int nb = batch_nbox(b);
-    hipLaunchKernelGGL(kbr_copy, bt_grid(bt, Q_CP), dim3(256), 0, batch_stream(b), bt->d, bt->pre, nb, mode, slot, par);
+    hipLaunchKernelGGL(kbr_copy, bt_grid(bt, copy_whole(mode, slot) ? Q_CW : Q_CP, full), dim3(256), 0, batch_stream(b),
+                       full ? bt->d_full : bt->d, full ? bt->pre_full : bt->pre, nb, mode, slot, par);
 }
 
 template <int DIR>
@@ -1591,13 +1670,15 @@ static void bt_sweep(cumc_batch *b, batch_tape *bt, int stage, int i)
     }
 }
 
-static void bt_ghost(cumc_batch *b, batch_tape *bt, int stage, int i)
+static void bt_ghost(cumc_batch *b, batch_tape *bt, int stage, int i, bool full = false)
 {
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
-    hipLaunchKernelGGL(kbr_ghost<2>, bt_grid(bt, Q_GZ), dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
-    hipLaunchKernelGGL(kbr_ghost<1>, bt_grid(bt, Q_GY), dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
-    hipLaunchKernelGGL(kbr_ghost<0>, bt_grid(bt, Q_GX), dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
+    const AdjBox *D = full ? bt->d_full : bt->d;
+    const int *pre = full ? bt->pre_full : bt->pre;
+    hipLaunchKernelGGL(kbr_ghost<2>, bt_grid(bt, Q_GZ, full), dim3(256), 0, s, D, pre, nb, stage, i);
+    hipLaunchKernelGGL(kbr_ghost<1>, bt_grid(bt, Q_GY, full), dim3(256), 0, s, D, pre, nb, stage, i);
+    hipLaunchKernelGGL(kbr_ghost<0>, bt_grid(bt, Q_GX, full), dim3(256), 0, s, D, pre, nb, stage, i);
 }
 
 template <int STAGE>
@@ -1644,7 +1725,7 @@ extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const d
     for (int i = 0; i < nb; i++) BTCHK(b, hipStreamSynchronize(batch_box(b, i)->stream));
     batch_tape_release(b);
     for (int i = 0; i < nb; i++) {
-        int r = tape_alloc(batch_box(b, i), nsteps, every);
+        int r = tape_alloc(batch_box(b, i), nsteps, every, true);
         if (r) {
             batch_err(b) = "box " + std::to_string(i) + ": " + batch_box(b, i)->err;
             return r;
@@ -1659,6 +1740,8 @@ extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const d
     bt->h.assign(nb, AdjBox{});
     BTCHK(b, hipMalloc(&bt->d, sizeof(AdjBox) * nb));
     BTCHK(b, hipMalloc(&bt->pre, sizeof(int) * Q_N * (nb + 1)));
+    BTCHK(b, hipMalloc(&bt->d_full, sizeof(AdjBox) * nb));
+    BTCHK(b, hipMalloc(&bt->pre_full, sizeof(int) * Q_N * (nb + 1)));
     BTCHK(b, hipEventCreate(&bt->e0));
     BTCHK(b, hipEventCreate(&bt->e1));
     int r = batch_refresh(b);
@@ -1673,6 +1756,10 @@ extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const d
             return r;
         }
     bt_copy(b, bt, CP_SAVE_CK, 0, 0);
+    // the steps run in each box's z-collapsed view where it may (initial_update left the state
+    // in the full layout with fresh z ghost planes)
+    BTCHK(b, hipStreamSynchronize(s));
+    if ((r = batch_refresh(b)) || (r = bt_upload(b, bt))) return r;
     if ((r = batch_reserve_hist(b, nsteps > 0 ? nsteps : 1))) return r;
     if (dt_in) BTCHK(b, hipMemcpyAsync(batch_hist(b), dt_in, sizeof(double) * (size_t)nsteps * nb, hipMemcpyHostToDevice, s));
     batch_set_counter(b, 0);
@@ -1726,7 +1813,7 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
     int r = batch_refresh(b);
-    if (r || (r = bt_upload(b, bt))) return r;
+    if (r || (r = bt_upload(b, bt, true)) || (r = bt_upload(b, bt))) return r;
     for (int i = 0; i < nb; i++) {
         cumc_ctx *c = batch_box(b, i);
         cumc_tape *tp = c->tape;
@@ -1737,6 +1824,12 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
             batch_err(b) = "box " + std::to_string(i) + ": " + c->err;
             return r;
         }
+        // a z-collapsed box reverses its steps on plane 1: the seeds on the z ghost planes (the
+        // copies the last z ghost pass made) go to plane 1 first
+        if (c->zmode)
+            hipLaunchKernelGGL(kbr_zfold_bc, dim3(nblk((long)(c->L.nx + 6) * (c->L.ny + 6), 256)), dim3(256), 0,
+                               c->stream, c->L, tp->A.BC);
+        BTCHK(b, hipGetLastError());
         BTCHK(b, hipStreamSynchronize(c->stream));
     }
     const int N = bt->nsteps, k = bt->every;
@@ -1785,11 +1878,12 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
         // the segment buffers now hold a re-run segment, not the last one
         if (!stored) bt->last_stored = false;
     }
-    // initial_update: its ghost fill, then k_init on the uploaded state
-    bt_ghost(b, bt, 0, 0);
-    bt_copy(b, bt, CP_LOAD_INIT, 0, 0);
-    hipLaunchKernelGGL(kbr_init_a, bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q);
-    hipLaunchKernelGGL(kbr_init_b, bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb);
+    // initial_update, in the full layout (z-collapsed boxes' adjoints are on plane 1, their z
+    // ghost planes' zero): its ghost fill, then k_init on the uploaded state
+    bt_ghost(b, bt, 0, 0, true);
+    bt_copy(b, bt, CP_LOAD_INIT, 0, 0, true);
+    hipLaunchKernelGGL(kbr_init_a, bt_grid(bt, Q_TAIL, true), dim3(256), 0, s, bt->d_full, bt->pre_full, nb, q);
+    hipLaunchKernelGGL(kbr_init_b, bt_grid(bt, Q_TAIL, true), dim3(256), 0, s, bt->d_full, bt->pre_full, nb);
     BTCHK(b, hipEventRecord(bt->e1, s));
     BTCHK(b, hipGetLastError());
     BTCHK(b, hipStreamSynchronize(s));

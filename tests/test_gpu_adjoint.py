@@ -216,7 +216,11 @@ def test_batched_reverse_matches_single_box_tapes(every):
     for s, p in zip(singles, probs):
         gp, _, dts = _gradient(s, p, 5, 2)
         ref.append((gp, dts))
+    # per-box tapes run in the full layout; so does the batch here (its z-collapsed reverse is
+    # checked against this one in test_gpu_zcollapse_adjoint)
     boxes = [Solver.from_problem(p) for p in probs]
+    for bx in boxes:
+        bx.set_zcollapse(False)
     b = Batch(boxes)
     dts = b.tape_run(5, every, dts=np.array([r[1] for r in ref]).T)
     seeds = []
@@ -264,6 +268,8 @@ def test_second_backward_on_one_tape_is_identical():
     assert np.array_equal(g1, g2) and np.array_equal(e1, e2)
     s.close()
     boxes = [Solver.from_problem(q) for q in probs]
+    for bx in boxes:
+        bx.set_zcollapse(False)  # box 0 is compared with the full-layout single-context tape
     b = Batch(boxes)
     b.tape_run(5, 2)
     seeds = []
