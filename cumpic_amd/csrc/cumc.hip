@@ -743,7 +743,7 @@ extern "C" int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out)
     int best_seg = seg0, best_zc = zc0, best_tw = tw0;
     if (rc == CUMC_OK) rc = timed_step(cls);  // warm-up
     if (rc == CUMC_OK) {
-        const int segs[] = {16, 24, 32, 48, 64, 96};
+        const int segs[] = {8, 12, 16, 24, 32, 48, 64, 96};
         double bt = 1e300;
         for (int sg : segs) {
             ctx->seg = sg;

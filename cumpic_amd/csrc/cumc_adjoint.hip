@@ -604,6 +604,47 @@ __global__ __launch_bounds__(64) void kr_flux_recon(Lay L, Fields<double> F, Tab
     kr_flux_recon_pencil<DIR, RECON>(L, F, tb, A, (long)blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
+// The z face of a z-collapsed box reversed: one face per (j, k) column, both of whose cells are
+// the column itself.  The forward reconstructs it from a constant stencil, which PPM (flattened:
+// (vr - vc)(vc - vl) = 0) and TVD (zero slope) both return unchanged, so the face states are the
+// cell values and the Riemann reverse's state adjoints go straight to them.  The forward values
+// are recomputed with the same calls as the forward sweep (cell 0's edges; cell 1's are equal,
+// its coefficient rows being the same).  Thread t takes column (j, k), j fastest, over
+// [0, nx+1] x [0, ny+1] (the forward z sweep's columns).
+template <int RECON>
+__device__ inline void kr_flux_z1_cell(const Lay &L, const Fields<double> &F, const Tabs &tb, const Model &M,
+                                       const AdjF &A, long t)
+{
+    const long nj = L.nx + 2;
+    if (t >= nj * (L.ny + 2)) return;
+    const int j = (int)(t % nj), k = (int)(t / nj);
+    const unsigned g = (unsigned)L.id(j, k, 1);
+    const long ps = L.ps;
+    double vm[7], vp[7], o[7], lf[7];
+#pragma unroll
+    for (int v = 0; v < 7; v++) {
+        const double q = ldi(qplane<3>(F, ps, v), g);
+        const double w[5] = {q, q, q, q, q};
+        recon_cell<RECON>(tb, 2, 0, w, vm[v], vp[v]);
+    }
+    const double bn = ldi(F.P + 7 * ps, g);
+#pragma unroll
+    for (int i = 0; i < 7; i++) o[i] = ldi(A.F[2] + i * ps, g);
+    flux_to_rot<3>(o, lf);
+    FaceState<double> lL{0, 0, 0, 0, 0, 0, 0, 0}, lR{0, 0, 0, 0, 0, 0, 0, 0};
+    hll_face_adj(M.ggas, rot<3>(vp, bn), rot<3>(vm, bn), lf, lL, lR);
+    double hl[7], hr[7];
+    unrot<3>(lL, hl);
+    unrot<3>(lR, hr);
+    double *pb = A.P + 7 * ps;
+    sti(pb, g, ldi(pb, g) + (lL.bn + lR.bn));
+#pragma unroll
+    for (int v = 0; v < 7; v++) {
+        double *lp = qplane_adj<3>(A, ps, v);
+        sti(lp, g, ldi(lp, g) + (hl[v] + hr[v]));
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // A sweep reversed, wave-parallel like k_flux_x: the lanes of a wavefront hold 64 consecutive
 // cells along the sweep (one outer index: plane l for the x and y sweeps, row k for the z
@@ -1285,7 +1326,8 @@ struct AdjBox {
 };
 
 // work lists; Q_CP: state copies (a z-collapsed box's plane-1 slabs), Q_CW: whole-state copies
-enum { Q_TAIL, Q_GE, Q_GX, Q_GY, Q_GZ, Q_H1, Q_H2, Q_H3, Q_R1, Q_R2, Q_R3, Q_CP, Q_L1, Q_L2, Q_L3, Q_CW, Q_N };
+// Q_Z1: the z faces of z-collapsed boxes (kbr_flux_z1)
+enum { Q_TAIL, Q_GE, Q_GX, Q_GY, Q_GZ, Q_H1, Q_H2, Q_H3, Q_R1, Q_R2, Q_R3, Q_CP, Q_L1, Q_L2, Q_L3, Q_CW, Q_Z1, Q_N };
 constexpr int CP_ELEMS = 1024;  // doubles per copy tile (256 threads x 4)
 
 __device__ inline int adj_box_of(const int *pre, int nbox, int it)
@@ -1398,6 +1440,16 @@ __global__ __launch_bounds__(64) void kbr_flux_recon(const AdjBox *__restrict__ 
         const int gp = d.gpen[DIR - 1];
         const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
         kr_flux_recon_pencil<DIR, RECON>(d.L, F, d.tb, d.A, (long)(loc % gp) * 64 + threadIdx.x, loc / gp);
+    }
+}
+
+template <int RECON>
+__global__ __launch_bounds__(256) void kbr_flux_z1(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
+                                                   int stage, int i)
+{
+    FOR_ITEMS(Q_Z1)
+        const Fields<double> F = fields_of_state(d.F, stage_states(d, stage, i).in, d.L.ps);
+        kr_flux_z1_cell<RECON>(d.L, F, d.tb, d.M, d.A, (long)loc * 256 + threadIdx.x);
     }
 }
 
@@ -1609,6 +1661,9 @@ static int bt_upload(cumc_batch *b, batch_tape *bt, bool full = false)
             cnt[Q_R1 + dd] = lane[dd] ? 0 : 7L * a.gpen[dd];
             cnt[Q_L1 + dd] = lane[dd] ? (long)a.glane[dd] * outer[dd] : 0;
         }
+        // a collapsed box's z faces: one column kernel instead of the pencil walks
+        cnt[Q_Z1] = col ? nblk((long)(L.nx + 2) * (L.ny + 2), 256) : 0;
+        if (col) cnt[Q_H3] = cnt[Q_R3] = 0;
         cnt[Q_CP] = (long)((SW * a.zslab + CP_ELEMS - 1) / CP_ELEMS);
         cnt[Q_CW] = (long)((SW * L.ps + CP_ELEMS - 1) / CP_ELEMS);
         for (int q = 0; q < Q_N; q++) {
@@ -1656,6 +1711,14 @@ static void bt_sweep(cumc_batch *b, batch_tape *bt, int stage, int i)
         else
             hipLaunchKernelGGL((kbr_flux_lane<DIR, 1>), g, dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
     }
+    if constexpr (DIR == 3) {
+        if (bt->total[Q_Z1] > 0) {
+            if (bt->recon == CUMC_PPMC)
+                hipLaunchKernelGGL((kbr_flux_z1<4>), bt_grid(bt, Q_Z1), dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
+            else
+                hipLaunchKernelGGL((kbr_flux_z1<1>), bt_grid(bt, Q_Z1), dim3(256), 0, s, bt->d, bt->pre, nb, stage, i);
+        }
+    }
     if constexpr (DIR != 1) {
         if (bt->total[Q_H1 + DIR - 1] > 0) {
             const dim3 gh = bt_grid(bt, Q_H1 + DIR - 1), gr = bt_grid(bt, Q_R1 + DIR - 1);
@@ -1676,7 +1739,9 @@ static void bt_ghost(cumc_batch *b, batch_tape *bt, int stage, int i, bool full 
     hipStream_t s = batch_stream(b);
     const AdjBox *D = full ? bt->d_full : bt->d;
     const int *pre = full ? bt->pre_full : bt->pre;
-    hipLaunchKernelGGL(kbr_ghost<2>, bt_grid(bt, Q_GZ, full), dim3(256), 0, s, D, pre, nb, stage, i);
+    // (no z pass when every box is z-collapsed)
+    if ((full ? bt->total_full : bt->total)[Q_GZ] > 0)
+        hipLaunchKernelGGL(kbr_ghost<2>, bt_grid(bt, Q_GZ, full), dim3(256), 0, s, D, pre, nb, stage, i);
     hipLaunchKernelGGL(kbr_ghost<1>, bt_grid(bt, Q_GY, full), dim3(256), 0, s, D, pre, nb, stage, i);
     hipLaunchKernelGGL(kbr_ghost<0>, bt_grid(bt, Q_GX, full), dim3(256), 0, s, D, pre, nb, stage, i);
 }

@@ -422,7 +422,8 @@ static void enqueue_stage_b(cumc_batch *b, int st, int par)
     bmark(b, P_GX);
     hipLaunchKernelGGL(kb_ghost<0>, dim3(grid_of(b, P_GX)), dim3(256), 0, s, D, pre_of(b, P_GX), nb);
     hipLaunchKernelGGL(kb_ghost<1>, dim3(grid_of(b, P_GY)), dim3(256), 0, s, D, pre_of(b, P_GY), nb);
-    hipLaunchKernelGGL(kb_ghost<2>, dim3(grid_of(b, P_GZ)), dim3(256), 0, s, D, pre_of(b, P_GZ), nb);
+    if (b->total[P_GZ] > 0)  // (no z pass when every box is z-collapsed)
+        hipLaunchKernelGGL(kb_ghost<2>, dim3(grid_of(b, P_GZ)), dim3(256), 0, s, D, pre_of(b, P_GZ), nb);
     bmark(b, -1);
 }
 
@@ -726,7 +727,7 @@ extern "C" int cumc_batch_autotune(cumc_batch *b, cumc_tuning *out)
     if (rc == CUMC_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = bfail(b, CUMC_EHIP, "tuner sync");
     // 1) segment length of the pencil sweeps (one workgroup per tile)
     if (rc == CUMC_OK) {
-        const int segs[] = {16, 24, 32, 64, 128, 256};
+        const int segs[] = {8, 12, 16, 24, 32, 64, 128, 256};
         double bestt = 1e300;
         for (int sg : segs) {
             cumc_tuning t = orig;
