@@ -50,16 +50,34 @@ template <> struct RKC<3> { static constexpr double a0 = RK30, a1 = RK31, a2 = R
 // never share a cell.  rho_out / eps_out: forward values after the stage (at every destination
 // whose p adjoint is still live they are the values the pass wrote).
 // ---------------------------------------------------------------------------------------------
+// Six threads take one line (t / 6), thread t % 6 its ghost (side, g) = (q / 3, q % 3 + 1).  When
+// both sides are periodic along AX and n >= 6 the six ghosts' source cells are distinct (n+1-g
+// and g), so each thread reverses its own ghost and every adjoint sum keeps its single term;
+// otherwise thread 0 of the six reverses the whole line in order and the others return.
+template <int AX>
+__host__ __device__ inline long kr_ghost_lines(const Lay &L)
+{
+    const long nzb = L.zc1 ? 1 : L.nz + 1;
+    if (AX == 0) return (long)(L.ny + 1) * nzb;
+    if (AX == 1) return (long)(L.nx + 6) * nzb;
+    return (long)(L.nx + 6) * (L.ny + 6);
+}
+
 template <int AX>
 __device__ inline void kr_ghost_line(const Lay &L, const AdjF &A, const Model &M, const double *__restrict__ rho_out,
-                                     const double *__restrict__ eps_out, long t)
+                                     const double *__restrict__ eps_out, long t6)
 {
     int na, nb, a0, b0;
     if (AX == 0) { a0 = 0; na = L.ny + 1; b0 = 0; nb = L.nz + 1; }
     else if (AX == 1) { a0 = -2; na = L.nx + 6; b0 = 0; nb = L.nz + 1; }
     else { a0 = -2; na = L.nx + 6; b0 = -2; nb = L.ny + 6; }
     if (AX != 2 && L.zc1) { b0 = 1; nb = 1; }  // z-collapsed view: the interior plane (no z pass)
+    const long t = t6 / 6;
+    const int q = (int)(t6 - t * 6);
     if (t >= (long)na * nb) return;
+    const int nax = AX == 0 ? L.nx : (AX == 1 ? L.ny : L.nz);
+    const bool split = M.bflag[2 * AX] == 0 && M.bflag[2 * AX + 1] == 0 && nax >= 6;
+    if (!split && q != 0) return;
     const int a = (int)(t % na) + a0, b = (int)(t / na) + b0;
     const int n = AX == 0 ? L.nx : (AX == 1 ? L.ny : L.nz);
     const long ps = L.ps;
@@ -75,10 +93,12 @@ __device__ inline void kr_ghost_line(const Lay &L, const AdjF &A, const Model &M
     else final_region = true;
     const int tb1 = AX == 0 ? 6 : 5, tb2 = AX == 2 ? 6 : 7;
     for (int side = 0; side < 2; side++) {
+        if (split && side != q / 3) continue;
         const int flag = M.bflag[2 * AX + side];
         const int *fac = M.bfac[2 * AX + side];
         const bool usef = flag >= 2;
         for (int g = 1; g <= 3; g++) {
+            if (split && g != q % 3 + 1) continue;
             const long dst = cell(side == 0 ? 1 - g : n + g);
             const long src = cell(ghost_src(flag, side, g, n));
             if (final_region) {
@@ -1042,7 +1062,7 @@ static void reverse_sweep(cumc_ctx *ctx, const Fields<double> &F, const AdjF &A)
 static void reverse_ghost(cumc_ctx *ctx, const AdjF &A, const double *rho_out, const double *eps_out)
 {
     const Lay &L = ctx->L;
-    const long n0 = (long)(L.ny + 1) * (L.nz + 1), n1 = (long)(L.nx + 6) * (L.nz + 1), n2 = (long)(L.nx + 6) * (L.ny + 6);
+    const long n0 = 6 * kr_ghost_lines<0>(L), n1 = 6 * kr_ghost_lines<1>(L), n2 = 6 * kr_ghost_lines<2>(L);
     hipStream_t s = ctx->stream;
     hipLaunchKernelGGL(kr_ghost<2>, dim3(nblk(n2, 256)), dim3(256), 0, s, L, A, ctx->M, rho_out, eps_out);
     hipLaunchKernelGGL(kr_ghost<1>, dim3(nblk(n1, 256)), dim3(256), 0, s, L, A, ctx->M, rho_out, eps_out);
@@ -1648,9 +1668,9 @@ static int bt_upload(cumc_batch *b, batch_tape *bt, bool full = false)
         const long nzc = col ? 1 : L.nz + 1;  // planes of the column kernels
         cnt[Q_TAIL] = (long)a.gx_c * nzc;
         cnt[Q_GE] = (long)a.gx_e * (col ? 1 : L.nz + 2);
-        cnt[Q_GX] = nblk((long)(L.ny + 1) * nzc, 256);
-        cnt[Q_GY] = nblk((long)(L.nx + 6) * nzc, 256);
-        cnt[Q_GZ] = col ? 0 : nblk((long)(L.nx + 6) * (L.ny + 6), 256);
+        cnt[Q_GX] = nblk(6 * kr_ghost_lines<0>(L), 256);
+        cnt[Q_GY] = nblk(6 * kr_ghost_lines<1>(L), 256);
+        cnt[Q_GZ] = col ? 0 : nblk(6 * kr_ghost_lines<2>(L), 256);
         const bool lane[3] = {lane_sweep<1>(L), lane_sweep<2>(L), lane_sweep<3>(L)};
         a.glane[0] = lane_tiles<1>(L);
         a.glane[1] = lane_tiles<2>(L);
