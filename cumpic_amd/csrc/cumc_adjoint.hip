@@ -50,10 +50,12 @@ template <> struct RKC<3> { static constexpr double a0 = RK30, a1 = RK31, a2 = R
 // never share a cell.  rho_out / eps_out: forward values after the stage (at every destination
 // whose p adjoint is still live they are the values the pass wrote).
 // ---------------------------------------------------------------------------------------------
-// Six threads take one line (t / 6), thread t % 6 its ghost (side, g) = (q / 3, q % 3 + 1).  When
+// Six threads take one line, each its ghost (side, g) = (q / 3, q % 3 + 1): along x the six
+// threads of a line are neighbours (its ghosts are adjacent in memory), along y and z thread t
+// takes line t mod lines and ghost t / lines (neighbouring threads, neighbouring lines).  When
 // both sides are periodic along AX and n >= 6 the six ghosts' source cells are distinct (n+1-g
 // and g), so each thread reverses its own ghost and every adjoint sum keeps its single term;
-// otherwise thread 0 of the six reverses the whole line in order and the others return.
+// otherwise the thread of ghost 0 reverses the whole line in order and the others return.
 template <int AX>
 __host__ __device__ inline long kr_ghost_lines(const Lay &L)
 {
@@ -72,9 +74,17 @@ __device__ inline void kr_ghost_line(const Lay &L, const AdjF &A, const Model &M
     else if (AX == 1) { a0 = -2; na = L.nx + 6; b0 = 0; nb = L.nz + 1; }
     else { a0 = -2; na = L.nx + 6; b0 = -2; nb = L.ny + 6; }
     if (AX != 2 && L.zc1) { b0 = 1; nb = 1; }  // z-collapsed view: the interior plane (no z pass)
-    const long t = t6 / 6;
-    const int q = (int)(t6 - t * 6);
-    if (t >= (long)na * nb) return;
+    const long nl = (long)na * nb;
+    long t;
+    int q;
+    if (AX == 0) {
+        t = t6 / 6;
+        q = (int)(t6 - t * 6);
+    } else {
+        q = (int)(t6 / nl);
+        t = t6 - (long)q * nl;
+    }
+    if (t >= nl || q >= 6) return;
     const int nax = AX == 0 ? L.nx : (AX == 1 ? L.ny : L.nz);
     const bool split = M.bflag[2 * AX] == 0 && M.bflag[2 * AX + 1] == 0 && nax >= 6;
     if (!split && q != 0) return;
