@@ -138,6 +138,7 @@ double &batch_last_ms(cumc_batch *b);
 int batch_nbox(const cumc_batch *b);
 cumc_ctx *batch_box(const cumc_batch *b, int i);
 hipStream_t batch_stream(const cumc_batch *b);
+hipStream_t batch_swap_stream(cumc_batch *b, hipStream_t s);
 std::string &batch_err(cumc_batch *b);
 int batch_refresh(cumc_batch *b);
 int batch_reserve_hist(cumc_batch *b, int nsteps);

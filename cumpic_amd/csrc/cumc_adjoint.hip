@@ -996,6 +996,9 @@ struct cumc_tape {
     double *seg = nullptr;     // every + 1 states
     double *sa = nullptr;      // every states after stage 1: P/EPS/BC and the stage's U (U1)
     double *sb = nullptr;      // every states after stage 2: P/EPS/BC and the stage's U (Ub)
+    // with two sets of segment buffers (the batched backward re-runs one segment while it
+    // reverses the next), set s's slot i of seg / sa / sb is slot i + s * set_stride
+    int sets = 1, set_stride = 0;
     double *adj = nullptr;     // NADJ planes
     double *lu[3] = {nullptr, nullptr, nullptr};  // 8 planes each: U adjoints by role U0, U1, Ub
     AdjF A;
@@ -1125,7 +1128,7 @@ static void reverse_step(cumc_ctx *ctx, int i, double dt)
 
 // one allocation per context: uploaded state, checkpoints, segment states, stage saves,
 // adjoint arrays and the three conserved-state adjoints
-static int tape_alloc(cumc_ctx *ctx, int nsteps, int every, bool zcollapse_ok = false)
+static int tape_alloc(cumc_ctx *ctx, int nsteps, int every, bool zcollapse_ok = false, int sets = 1)
 {
     tape_release(ctx);
     // a single context's taped steps and their reversal run in the full layout (the stored states
@@ -1138,8 +1141,11 @@ static int tape_alloc(cumc_ctx *ctx, int nsteps, int every, bool zcollapse_ok = 
     tp->nsteps = nsteps;
     tp->every = every;
     tp->nck = nsteps / every + 2;  // step 0, every, 2 every, ... and the final state
+    tp->sets = sets;
+    tp->set_stride = sets > 1 ? every + 1 : 0;
     const size_t ps = ctx->L.ps;
-    const size_t planes = 9 + (size_t)SW * tp->nck + (size_t)SW * (3 * every + 1) + NADJ + 24;
+    const size_t seg_states = sets > 1 ? (size_t)3 * (every + 1) * sets : (size_t)3 * every + 1;
+    const size_t planes = 9 + (size_t)SW * tp->nck + (size_t)SW * seg_states + NADJ + 24;
     if (hipMalloc(&tp->mem, planes * ps * sizeof(double)) != hipSuccess) {
         tape_release(ctx);
         return fail(ctx, CUMC_ENOMEM, "tape allocation");
@@ -1148,9 +1154,10 @@ static int tape_alloc(cumc_ctx *ctx, int nsteps, int every, bool zcollapse_ok = 
     auto take = [&](size_t n) { double *r = p; p += n * ps; return r; };
     tp->init = take(9);
     tp->ckpt = take((size_t)SW * tp->nck);
-    tp->seg = take((size_t)SW * (every + 1));
-    tp->sa = take((size_t)SW * every);
-    tp->sb = take((size_t)SW * every);
+    const size_t per = sets > 1 ? (size_t)(every + 1) * sets : 0;
+    tp->seg = take((size_t)SW * (sets > 1 ? per : every + 1));
+    tp->sa = take((size_t)SW * (sets > 1 ? per : every));
+    tp->sb = take((size_t)SW * (sets > 1 ? per : every));
     tp->adj = take(NADJ);
     for (int r = 0; r < 3; r++) tp->lu[r] = take(8);
     AdjF &A = tp->A;
@@ -1409,17 +1416,20 @@ __global__ __launch_bounds__(256) void kbr_ghost(const AdjBox *__restrict__ D, c
     }
 }
 
+// (the batched reverse takes each box's step from the dt history, hist[m * nbox + box]: the
+// boxes' own dt slots belong to the forward re-run of another segment running beside it)
 template <int STAGE>
 __global__ __launch_bounds__(256) void kbr_tail_a(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
-                                                  int q, int i)
+                                                  int q, int i, const double *__restrict__ hist, int m)
 {
     FOR_ITEMS(Q_TAIL)
         const StageStates v = stage_states(d, STAGE, i);
+        const double dt = hist[(size_t)m * nbox + b];
         const long ps = d.L.ps;
         double *l0 = d.lu[q], *l1 = d.lu[1 - q], *lb = d.lu[2];
         double *lUin = STAGE == 3 ? lb : (STAGE == 2 ? l1 : nullptr);
         double *lUout = STAGE == 2 ? lb : l1;
-        kr_tail_a_cell<STAGE>(d.L, fields_of_state(d.F, v.in, ps), v.out, v.out + 8 * ps, v.out + 9 * ps, d.M, *d.dt,
+        kr_tail_a_cell<STAGE>(d.L, fields_of_state(d.F, v.in, ps), v.out, v.out + 8 * ps, v.out + 9 * ps, d.M, dt,
                               v.U0, v.Uin, d.A, l0, lUin, lUout, (long)(loc % d.gx_c) * 256 + threadIdx.x,
                               loc / d.gx_c + d.l0);
     }
@@ -1427,12 +1437,13 @@ __global__ __launch_bounds__(256) void kbr_tail_a(const AdjBox *__restrict__ D, 
 
 template <int STAGE>
 __global__ __launch_bounds__(256) void kbr_tail_b(const AdjBox *__restrict__ D, const int *__restrict__ pre, int nbox,
-                                                  int q)
+                                                  int q, const double *__restrict__ hist, int m)
 {
     FOR_ITEMS(Q_TAIL)
+        const double dt = hist[(size_t)m * nbox + b];
         double *l0 = d.lu[q], *l1 = d.lu[1 - q], *lb = d.lu[2];
         double *lUin = STAGE == 3 ? lb : (STAGE == 2 ? l1 : nullptr);
-        kr_tail_b_cell<STAGE>(d.L, d.A, *d.dt, l0, lUin, (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c + d.l0);
+        kr_tail_b_cell<STAGE>(d.L, d.A, dt, l0, lUin, (long)(loc % d.gx_c) * 256 + threadIdx.x, loc / d.gx_c + d.l0);
     }
 }
 
@@ -1609,8 +1620,11 @@ struct batch_tape {
     int total_full[Q_N] = {0};
     std::vector<double> dts;  // nsteps x nbox
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    std::vector<hipEvent_t> seg_ev;  // start / end of each segment's forward pass in the backward
-    double recompute_ms = 0.0;       // their total in the last backward pass
+    // per checkpoint segment of the backward: start and end of its forward re-run (on s2) and end
+    // of its reverse (on the batch stream)
+    std::vector<hipEvent_t> seg_ev;
+    double recompute_ms = 0.0;       // the re-runs' total in the last backward pass
+    hipStream_t s2 = nullptr;        // the re-runs' stream, default priority like the batch stream
     // the stage states of the last checkpoint segment, which the backward reverses first, are
     // stored by tape_run's own forward pass (the segment buffers hold one segment), so the
     // backward re-runs every segment but that one; with every >= nsteps nothing is re-run
@@ -1628,6 +1642,10 @@ void batch_tape_release(cumc_batch *b)
     if (bt->e0) hipEventDestroy(bt->e0);
     if (bt->e1) hipEventDestroy(bt->e1);
     for (hipEvent_t e : bt->seg_ev) hipEventDestroy(e);
+    if (bt->s2) {
+        hipStreamSynchronize(bt->s2);
+        hipStreamDestroy(bt->s2);
+    }
     delete bt;
     bt = nullptr;
 }
@@ -1776,14 +1794,16 @@ static void bt_ghost(cumc_batch *b, batch_tape *bt, int stage, int i, bool full 
     hipLaunchKernelGGL(kbr_ghost<0>, bt_grid(bt, Q_GX, full), dim3(256), 0, s, D, pre, nb, stage, i);
 }
 
+// stage STAGE of segment slot i (set offset included) of step m reversed
 template <int STAGE>
-static void bt_reverse_stage(cumc_batch *b, batch_tape *bt, int i, int q)
+static void bt_reverse_stage(cumc_batch *b, batch_tape *bt, int i, int q, int m)
 {
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
+    const double *hist = batch_hist(b);
     bt_ghost(b, bt, STAGE, i);
-    hipLaunchKernelGGL((kbr_tail_a<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q, i);
-    hipLaunchKernelGGL((kbr_tail_b<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q);
+    hipLaunchKernelGGL((kbr_tail_a<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q, i, hist, m);
+    hipLaunchKernelGGL((kbr_tail_b<STAGE>), bt_grid(bt, Q_TAIL), dim3(256), 0, s, bt->d, bt->pre, nb, q, hist, m);
     hipLaunchKernelGGL(kbr_gather, bt_grid(bt, Q_GE), dim3(256), 0, s, bt->d, bt->pre, nb);
     hipLaunchKernelGGL(kbr_ecorn, bt_grid(bt, Q_GE), dim3(256), 0, s, bt->d, bt->pre, nb, STAGE, i);
     bt_sweep<3>(b, bt, STAGE, i);
@@ -1819,8 +1839,11 @@ extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const d
             }
     for (int i = 0; i < nb; i++) BTCHK(b, hipStreamSynchronize(batch_box(b, i)->stream));
     batch_tape_release(b);
+    // two sets of segment buffers when the backward re-runs segments (it re-runs one segment
+    // while it reverses the next)
+    const int sets = nsteps > every ? 2 : 1;
     for (int i = 0; i < nb; i++) {
-        int r = tape_alloc(batch_box(b, i), nsteps, every, true);
+        int r = tape_alloc(batch_box(b, i), nsteps, every, true, sets);
         if (r) {
             batch_err(b) = "box " + std::to_string(i) + ": " + batch_box(b, i)->err;
             return r;
@@ -1839,6 +1862,7 @@ extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const d
     BTCHK(b, hipMalloc(&bt->pre_full, sizeof(int) * Q_N * (nb + 1)));
     BTCHK(b, hipEventCreate(&bt->e0));
     BTCHK(b, hipEventCreate(&bt->e1));
+    BTCHK(b, hipStreamCreateWithFlags(&bt->s2, hipStreamNonBlocking));
     int r = batch_refresh(b);
     if (r) return r;
     if ((r = bt_upload(b, bt))) return r;
@@ -1931,48 +1955,67 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
     if ((r = batch_reserve_hist(b, N > 0 ? N : 1))) return r;
     if (N > 0)
         BTCHK(b, hipMemcpyAsync(batch_hist(b), bt->dts.data(), sizeof(double) * bt->dts.size(), hipMemcpyHostToDevice, s));
+    // Checkpoint segments g = 0, 1, ... from the last in time back.  The forward re-run of a
+    // segment stores every stage's output state, so its reverse runs no forward kernel; the last
+    // segment's states are still in set 0 from tape_run.  Segment g uses buffer set g mod sets:
+    // the re-run of segment g + 1 (on s2) runs while segment g is reversed (on the batch stream)
+    // and waits for the reverse of segment g - 1, the last reader of its set; the reverse of a
+    // re-run segment waits for its re-run.  The re-runs own the boxes' fields and dt slots; the
+    // reverse reads only stored states, the adjoint arrays and the dt history.
     const int nseg = N > 0 ? (N - 1) / k + 1 : 0;
-    while ((int)bt->seg_ev.size() < 2 * nseg) {
+    const int sets = batch_box(b, 0)->tape->sets, stride = batch_box(b, 0)->tape->set_stride;
+    while ((int)bt->seg_ev.size() < 3 * nseg) {
         hipEvent_t e;
         BTCHK(b, hipEventCreate(&e));
         bt->seg_ev.push_back(e);
     }
-    BTCHK(b, hipEventRecord(bt->e0, s));
-    int q = 0, seg = 0;
-    for (int s0 = ((N - 1) / k) * k; N > 0 && s0 >= 0; s0 -= k, seg++) {
-        const int s1 = s0 + k < N ? s0 + k : N;
-        BTCHK(b, hipEventRecord(bt->seg_ev[2 * seg], s));
-        // segment forward: every stage's output state is stored, so the reverse below runs
-        // no forward kernel (stage-1 output U is in U1 for even steps, U0 for odd ones); the
-        // last segment's states are still in the segment buffers from tape_run
-        const bool stored = seg == 0 && bt->last_stored;
-        if (!stored) {
-            bt_copy(b, bt, CP_LOAD_CK, s0 / k, 0);
-            bt_copy(b, bt, CP_SAVE_SEG, 0, 0);
-        }
-        for (int m = s0; m < s1 && !stored; m++) {
+    std::vector<char> reran(nseg, 0);
+    hipStream_t s2 = bt->s2;
+    auto seg_start = [&](int g) { return ((N - 1) / k - g) * k; };
+    auto rerun = [&](int g) {
+        const int s0 = seg_start(g), s1 = s0 + k < N ? s0 + k : N, off = (g % sets) * stride;
+        hipStream_t s_b = batch_swap_stream(b, s2);
+        hipEventRecord(bt->seg_ev[3 * g], s2);
+        bt_copy(b, bt, CP_LOAD_CK, s0 / k, 0);
+        bt_copy(b, bt, CP_SAVE_SEG, off, 0);
+        for (int m = s0; m < s1; m++) {
             const int i = m - s0, par = i & 1;
             batch_set_dt(b, m);
             batch_stage(b, 1, par);
-            bt_copy(b, bt, CP_SAVE_SA, i, 1 - par);
+            bt_copy(b, bt, CP_SAVE_SA, off + i, 1 - par);  // stage-1 output U: U1 for even steps, U0 for odd
             batch_stage(b, 2, par);
-            bt_copy(b, bt, CP_SAVE_SB, i, 0);
+            bt_copy(b, bt, CP_SAVE_SB, off + i, 0);
             batch_stage(b, 3, par);
-            bt_copy(b, bt, CP_SAVE_SEG, i + 1, 1 - par);
+            bt_copy(b, bt, CP_SAVE_SEG, off + i + 1, 1 - par);
         }
-        BTCHK(b, hipEventRecord(bt->seg_ev[2 * seg + 1], s));
+        hipEventRecord(bt->seg_ev[3 * g + 1], s2);
+        batch_swap_stream(b, s_b);
+        reran[g] = 1;
+    };
+    BTCHK(b, hipEventRecord(bt->e0, s));
+    BTCHK(b, hipStreamWaitEvent(s2, bt->e0, 0));
+    const bool stored0 = bt->last_stored;
+    if (nseg > 0 && !stored0) rerun(0);
+    int q = 0;
+    for (int g = 0; g < nseg; g++) {
+        if (g + 1 < nseg) {
+            if (g >= 1) BTCHK(b, hipStreamWaitEvent(s2, bt->seg_ev[3 * (g - 1) + 2], 0));
+            rerun(g + 1);
+        }
+        if (reran[g]) BTCHK(b, hipStreamWaitEvent(s, bt->seg_ev[3 * g + 1], 0));
+        const int s0 = seg_start(g), s1 = s0 + k < N ? s0 + k : N, off = (g % sets) * stride;
         for (int m = s1 - 1; m >= s0; m--) {
-            const int i = m - s0;
-            batch_set_dt(b, m);
-            bt_reverse_stage<3>(b, bt, i, q);
-            bt_reverse_stage<2>(b, bt, i, q);
-            bt_reverse_stage<1>(b, bt, i, q);
+            const int i = off + m - s0;
+            bt_reverse_stage<3>(b, bt, i, q, m);
+            bt_reverse_stage<2>(b, bt, i, q, m);
+            bt_reverse_stage<1>(b, bt, i, q, m);
             q ^= 1;
         }
+        BTCHK(b, hipEventRecord(bt->seg_ev[3 * g + 2], s));
         BTCHK(b, hipGetLastError());
-        // the segment buffers now hold a re-run segment, not the last one
-        if (!stored) bt->last_stored = false;
     }
+    // set 0 holds the last segment's states again unless a later segment was re-run into it
+    bt->last_stored = nseg <= 2;
     // initial_update, in the full layout (z-collapsed boxes' adjoints are on plane 1, their z
     // ghost planes' zero): its ghost fill, then k_init on the uploaded state
     bt_ghost(b, bt, 0, 0, true);
@@ -1988,7 +2031,8 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
     bt->recompute_ms = 0.0;
     for (int g = 0; g < nseg; g++) {
         float m = 0.f;
-        if (hipEventElapsedTime(&m, bt->seg_ev[2 * g], bt->seg_ev[2 * g + 1]) == hipSuccess) bt->recompute_ms += m;
+        if (reran[g] && hipEventElapsedTime(&m, bt->seg_ev[3 * g], bt->seg_ev[3 * g + 1]) == hipSuccess)
+            bt->recompute_ms += m;
     }
     for (int i = 0; i < nb; i++)
         if (grad_prim && grad_prim[i] && (r = soa_out(batch_box(b, i), batch_box(b, i)->tape->A.P, 8, grad_prim[i]))) {

@@ -454,6 +454,13 @@ batch_tape *&batch_adj(cumc_batch *b) { return b->adj; }
 double &batch_last_ms(cumc_batch *b) { return b->last_ms; }
 cumc_ctx *batch_box(const cumc_batch *b, int i) { return b->box[i]; }
 hipStream_t batch_stream(const cumc_batch *b) { return b->stream; }
+// the stream the batch's enqueue functions launch on, replaced by s; returns the previous one
+hipStream_t batch_swap_stream(cumc_batch *b, hipStream_t s)
+{
+    hipStream_t o = b->stream;
+    b->stream = s;
+    return o;
+}
 std::string &batch_err(cumc_batch *b) { return b->err; }
 int batch_refresh(cumc_batch *b) { return refresh(b); }
 int batch_reserve_hist(cumc_batch *b, int nsteps) { return reserve_hist(b, nsteps); }
