@@ -38,6 +38,7 @@ struct cumc_ctx {
     // plane l = 1 the last full-layout ghost fill wrote; zmode while the state lives in the
     // collapsed view (plane 1 current, z ghost planes not yet rewritten: zexpand restores them)
     bool zc_ok = false, zc_on = true, zfresh = false, zmode = false;
+    bool zbad = false;   // the entry check found unequal periodic B_z faces (until the next upload)
     bool ztape = false;  // a reverse-mode record holds full-layout states: no collapsed steps
     struct cumc_tape *tape = nullptr;  // reverse-mode record (cumc_adjoint.hip)
     int seg = 24;    // target faces per pencil segment of the y / z sweeps

@@ -318,6 +318,7 @@ extern "C" int cumc_upload(cumc_ctx *ctx, const double *prim, const double *eps)
     ctx->dt_valid = false;
     zexpand(ctx);
     ctx->zfresh = false;  // the uploaded z ghost planes are the caller's
+    ctx->zbad = false;
     // bcell outside [1,n]^3 is read before the first BOUNDARY (as the reference's zero-initialised
     // array): a re-upload starts from the same zeros as a fresh context
     HIPCHK(ctx, hipMemsetAsync(ctx->F64.BC, 0, 3 * (size_t)ctx->L.ps * sizeof(double), ctx->stream));
@@ -446,7 +447,8 @@ bool zcollapse_begin(cumc_ctx *ctx)
 {
     if (!ctx->zc_ok || !ctx->zc_on || ctx->ztape) return false;
     if (ctx->zmode) return true;
-    if (!ctx->zfresh) return false;
+    // (unequal periodic B_z faces stay unequal: both evolve from equal right-hand sides)
+    if (!ctx->zfresh || ctx->zbad) return false;
     const Lay &L = ctx->L;
     unsigned long long cnt = 1;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->red + 2, 0ull, 1);
@@ -456,6 +458,7 @@ bool zcollapse_begin(cumc_ctx *ctx)
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return false;
     ctx->zmode = cnt == 0;
+    ctx->zbad = !ctx->zmode;
     return ctx->zmode;
 }
 
