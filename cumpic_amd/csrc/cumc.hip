@@ -229,7 +229,8 @@ static std::vector<double> build_tables(const double *f_in, int n)
     return t;
 }
 
-// A periodic nz = 1 context may step in the z-collapsed view when every z face the reference
+// An nz = 1 context with periodic or outflow z sides (both make every z ghost a copy of the one
+// cell: boundary.f90:564-652) may step in the z-collapsed view when every z face the reference
 // computes equals the others: the z sweep's reconstruction coefficients (PPM a/b rows, TVD cf/cb)
 // must be the same in every cell of the z table (true for equal cell widths), since faces 0 and
 // 1 of the single cell become one face.  FP64 form with the riemann hooks only (the tori clamp
@@ -237,7 +238,8 @@ static std::vector<double> build_tables(const double *f_in, int n)
 static bool zcollapse_allowed(const cumc_ctx *ctx, const std::vector<double> &tz)
 {
     const cumc_config &c = ctx->cfg;
-    if (c.nz != 1 || c.boundary_flag[4] != 0 || c.boundary_flag[5] != 0) return false;
+    const int *bf = c.boundary_flag;
+    if (c.nz != 1 || (bf[4] != 0 && bf[4] != 1) || (bf[5] != 0 && bf[5] != 1)) return false;
     if (c.precision != CUMC_FP64 || c.model != CUMC_MODEL_RIEMANN) return false;
     const int len = c.nz + 7;
     auto row_flat = [&](int row, int j0, int j1) {

@@ -153,10 +153,11 @@ int cumc_autotune(cumc_ctx *ctx, cumc_tuning *out);
 int cumc_get_tuning(cumc_ctx *ctx, cumc_tuning *out);
 int cumc_set_tuning(cumc_ctx *ctx, const cumc_tuning *t);
 
-/* z-collapsed stepping of a 2D subdomain (nz = 1, periodic z).  No reference counterpart: the
- * reference fills and computes the three z ghost layers on each side of the single cell
- * (boundary.f90:564-652, spatial.f90:552-589), which for periodic z are copies of it.  An FP64
- * context with the riemann hooks and equal z cell widths steps its one interior plane instead,
+/* z-collapsed stepping of a 2D subdomain (nz = 1, periodic or outflow z).  No reference
+ * counterpart: the reference fills and computes the three z ghost layers on each side of the
+ * single cell (boundary.f90:564-652, spatial.f90:552-589), which for periodic or outflow z are
+ * copies of it.  An FP64 context with the riemann hooks and equal z cell widths steps its one
+ * interior plane instead,
  * with every z neighbour read from the plane itself, whenever its z ghost planes are the copies a
  * ghost fill wrote and its two periodic B_z faces agree bit for bit; results are bit-identical to
  * the full layout (the z ghost planes are rewritten before any call that reads them).  On by
