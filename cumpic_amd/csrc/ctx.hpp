@@ -142,6 +142,7 @@ hipStream_t batch_stream(const cumc_batch *b);
 hipStream_t batch_swap_stream(cumc_batch *b, hipStream_t s);
 std::string &batch_err(cumc_batch *b);
 int batch_refresh(cumc_batch *b);
+void batch_freeze_views(cumc_batch *b, bool on);
 int batch_reserve_hist(cumc_batch *b, int nsteps);
 double *batch_hist(cumc_batch *b);
 void batch_set_counter(cumc_batch *b, int v);

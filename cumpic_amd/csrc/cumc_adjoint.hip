@@ -999,6 +999,7 @@ struct cumc_tape {
     // with two sets of segment buffers (the batched backward re-runs one segment while it
     // reverses the next), set s's slot i of seg / sa / sb is slot i + s * set_stride
     int sets = 1, set_stride = 0;
+    bool zcol = false;  // recorded in the z-collapsed view (batched tapes): states are plane-1 slabs
     double *adj = nullptr;     // NADJ planes
     double *lu[3] = {nullptr, nullptr, nullptr};  // 8 planes each: U adjoints by role U0, U1, Ub
     AdjF A;
@@ -1879,6 +1880,7 @@ extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const d
     // in the full layout with fresh z ghost planes)
     BTCHK(b, hipStreamSynchronize(s));
     if ((r = batch_refresh(b)) || (r = bt_upload(b, bt))) return r;
+    for (int i = 0; i < nb; i++) batch_box(b, i)->tape->zcol = batch_box(b, i)->zmode;
     if ((r = batch_reserve_hist(b, nsteps > 0 ? nsteps : 1))) return r;
     if (dt_in) BTCHK(b, hipMemcpyAsync(batch_hist(b), dt_in, sizeof(double) * (size_t)nsteps * nb, hipMemcpyHostToDevice, s));
     batch_set_counter(b, 0);
@@ -1931,7 +1933,16 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
     }
     const int nb = batch_nbox(b);
     hipStream_t s = batch_stream(b);
+    // each box is reversed (and its segments re-run) in the view its tape was recorded in,
+    // whatever happened to its live state since (uploads, downloads, the collapse switch)
+    for (int i = 0; i < nb; i++) {
+        cumc_ctx *c = batch_box(b, i);
+        if (c->tape->zcol) c->zmode = true;  // the live state is overwritten before it is read
+        else zexpand(c);
+    }
+    batch_freeze_views(b, true);
     int r = batch_refresh(b);
+    batch_freeze_views(b, false);
     if (r || (r = bt_upload(b, bt, true)) || (r = bt_upload(b, bt))) return r;
     for (int i = 0; i < nb; i++) {
         cumc_ctx *c = batch_box(b, i);
@@ -2039,10 +2050,16 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
             batch_err(b) = "box " + std::to_string(i) + ": " + batch_box(b, i)->err;
             return r;
         }
-    // leave every box at the taped final state
+    // leave every box at the taped final state (a collapsed box's in its view; one whose
+    // collapse was switched off since returns to the full layout)
     bt_copy(b, bt, CP_LOAD_CK, bt->nck - 1, 0);
     BTCHK(b, hipStreamSynchronize(s));
-    for (int i = 0; i < nb; i++) batch_box(b, i)->dt_valid = false;
+    for (int i = 0; i < nb; i++) {
+        cumc_ctx *c = batch_box(b, i);
+        c->dt_valid = false;
+        if (!c->zc_on) zexpand(c);
+        BTCHK(b, hipStreamSynchronize(c->stream));
+    }
     return CUMC_OK;
 }
 

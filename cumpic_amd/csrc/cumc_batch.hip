@@ -227,6 +227,9 @@ struct cumc_batch {
     double last_ms = 0.0;
     std::string err;
     batch_tape *adj = nullptr;  // batched reverse-mode record (cumc_adjoint.hip)
+    // descriptors keep each box's current view (zmode) instead of entering the z-collapsed one
+    // where allowed: set while a backward pass runs in the views its tape was recorded in
+    bool zfreeze = false;
     // per-phase event timing used by the tuner
     bool prof = false;
     std::vector<hipEvent_t> evp;
@@ -319,8 +322,8 @@ static int refresh(cumc_batch *b)
     for (int i = 0; i < nb; i++) {
         cumc_ctx *c = b->box[i];
         BoxDesc &d = b->h_desc[i];
-        // a periodic nz = 1 box steps in its z-collapsed view when it may (zcollapse_begin)
-        const View v = make_view(c, zcollapse_begin(c));
+        // an nz = 1 box steps in its z-collapsed view when it may (zcollapse_begin)
+        const View v = make_view(c, b->zfreeze ? c->zmode : zcollapse_begin(c));
         d.L = v.L;
         d.F = v.F;
         d.tb = tabs_of(c);
@@ -463,6 +466,7 @@ hipStream_t batch_swap_stream(cumc_batch *b, hipStream_t s)
 }
 std::string &batch_err(cumc_batch *b) { return b->err; }
 int batch_refresh(cumc_batch *b) { return refresh(b); }
+void batch_freeze_views(cumc_batch *b, bool on) { b->zfreeze = on; }
 int batch_reserve_hist(cumc_batch *b, int nsteps) { return reserve_hist(b, nsteps); }
 double *batch_hist(cumc_batch *b) { return b->d_hist; }
 void batch_set_counter(cumc_batch *b, int v)

@@ -40,6 +40,11 @@ def _probs():
     return probs
 
 
+def _box_like(p, seed):
+    return ics.randomize(ics.spectral_box(n=p.cfg.nx, seed=seed, ndim=2, v_rms=0.3, db_rms=0.3, ny=p.cfg.ny),
+                         seed=seed, amp=0.05)
+
+
 def _batch(probs, collapse):
     from cumpic_amd.cumc import Batch, Solver
     boxes = [Solver.from_problem(p) for p in probs]
@@ -102,6 +107,19 @@ def test_collapsed_batch_gradient_reproducible():
     g1b = b.tape_backward(seeds)  # second pass on the same tape
     for x, y in zip(g1, g1b):
         assert np.array_equal(x, y)
+    # the tape is reversed in the views it was recorded in, whatever the live states did since:
+    # other states uploaded and stepped, one box switched to the full layout
+    others = [_box_like(p, seed=4700 + i) for i, p in enumerate(probs)]
+    for bx, q in zip(boxes, others):
+        bx.upload(q.prim, q.eps)
+        bx.initial_update()
+    boxes[1].set_zcollapse(False)
+    b.run(2)
+    g1c = b.tape_backward(seeds)
+    for i, (x, y) in enumerate(zip(g1, g1c)):
+        assert np.array_equal(x, y), f"box {i} after the live states changed"
+    assert boxes[1].zcollapse_state() == (False, False)
+    boxes[1].set_zcollapse(True)
     for every in (2, 3, 5):
         g, _, _, _ = _grads(probs, True, 5, every, dts=dts)
         for i, (x, y) in enumerate(zip(g, g1)):
