@@ -37,6 +37,8 @@ def _probs():
     probs.append(ics.randomize(ics.spectral_box(n=14, seed=4200, ndim=2, v_rms=0.3, db_rms=0.3, ny=5), seed=8,
                                amp=0.05))
     probs.append(ics.randomize(ics.small_periodic(8, seed=4300), seed=7, amp=0.05))
+    # outflow on every side (z included): the MHD rotor
+    probs.append(ics.riemann_test(ics.MHD_ROTOR, n=24))
     return probs
 
 
@@ -87,8 +89,8 @@ def test_collapsed_batch_gradient_matches_full_layout():
     probs = _probs()
     gf, dts, act_f, _ = _grads(probs, False, 5, 2)
     gc, dts_c, act_c, _ = _grads(probs, True, 5, 2, dts=dts)
-    assert act_f == [False] * 5
-    assert act_c == [True, True, True, True, False], act_c
+    assert act_f == [False] * 6
+    assert act_c == [True, True, True, True, False, True], act_c
     for i, (a, b, p) in enumerate(zip(gc, gf, probs)):
         a, b = _merge_bz(a, p.cfg), _merge_bz(b, p.cfg)
         scale = np.abs(b).max()
