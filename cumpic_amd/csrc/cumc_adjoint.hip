@@ -651,11 +651,22 @@ __device__ inline void kr_flux_z1_cell(const Lay &L, const Fields<double> &F, co
     const unsigned g = (unsigned)L.id(j, k, 1);
     const long ps = L.ps;
     double vm[7], vp[7], o[7], lf[7];
+    bool fin = true;
 #pragma unroll
     for (int v = 0; v < 7; v++) {
-        const double q = ldi(qplane<3>(F, ps, v), g);
-        const double w[5] = {q, q, q, q, q};
-        recon_cell<RECON>(tb, 2, 0, w, vm[v], vp[v]);
+        vm[v] = ldi(qplane<3>(F, ps, v), g);
+        fin = fin && isfinite(vm[v]);
+    }
+    if (fin) {
+#pragma unroll
+        for (int v = 0; v < 7; v++) const_edges<double, RECON>(vm[v], vm[v], vp[v]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 7; v++) {
+            const double q = vm[v];
+            const double w[5] = {q, q, q, q, q};
+            recon_cell<RECON>(tb, 2, 0, w, vm[v], vp[v]);
+        }
     }
     const double bn = ldi(F.P + 7 * ps, g);
 #pragma unroll
@@ -1367,6 +1378,8 @@ struct AdjBox {
 // Q_Z1: the z faces of z-collapsed boxes (kbr_flux_z1)
 enum { Q_TAIL, Q_GE, Q_GX, Q_GY, Q_GZ, Q_H1, Q_H2, Q_H3, Q_R1, Q_R2, Q_R3, Q_CP, Q_L1, Q_L2, Q_L3, Q_CW, Q_Z1, Q_N };
 constexpr int CP_ELEMS = 1024;  // doubles per copy tile (256 threads x 4)
+// copy tiles per plane of n elements (a tile never spans two planes)
+__host__ __device__ inline long cp_chunks(long n) { return (n + CP_ELEMS - 1) / CP_ELEMS; }
 
 __device__ inline int adj_box_of(const int *pre, int nbox, int it)
 {
@@ -1556,17 +1569,21 @@ __global__ __launch_bounds__(256) void kbr_copy(const AdjBox *__restrict__ D, co
     FOR_ITEMS(whole ? Q_CW : Q_CP)
         const long ps = d.L.ps;
         const long pe = whole ? ps : d.zslab, zo = whole ? 0 : d.zoff;
+        // tile loc: plane f of the 20, chunk ch of that plane's pe elements
+        const int nch = (int)cp_chunks(pe);
+        const int f = loc / nch, ch = loc - f * nch;
         // conserved buffers as full-layout bases (the descriptor's are the view's)
         double *U = (mode == CP_SAVE_SB ? d.Ub : (par ? d.F.U1 : d.F.U0)) - d.zoff;
         double *pool = d.pool;
+        const bool lo = f < 12;
         const size_t at = (size_t)slot * SW * ps;
+        const long e0 = (long)f * ps + zo;  // first element of the plane's range in the 20 full-layout planes
+        double *ctx0 = lo ? pool + e0 : U + (e0 - 12 * ps);
         for (int r = 0; r < CP_ELEMS / 256; r++) {
-            const long i = (long)loc * CP_ELEMS + r * 256 + threadIdx.x;
-            if (i >= SW * pe) break;
-            const long f = i / pe;
-            const long e = f * ps + zo + (i - f * pe);  // element of the 20 full-layout planes
-            const bool lo = f < 12;
-            double *ctxp = lo ? pool + e : U + (e - 12 * ps);
+            const long w = (long)ch * CP_ELEMS + r * 256 + threadIdx.x;
+            if (w >= pe) break;
+            const long e = e0 + w;
+            double *ctxp = ctx0 + w;
             switch (mode) {
             case CP_SAVE_SEG: d.seg[at + e] = *ctxp; break;
             case CP_SAVE_SA: d.sa[at + e] = *ctxp; break;
@@ -1713,8 +1730,8 @@ static int bt_upload(cumc_batch *b, batch_tape *bt, bool full = false)
         // a collapsed box's z faces: one column kernel instead of the pencil walks
         cnt[Q_Z1] = col ? nblk((long)(L.nx + 2) * (L.ny + 2), 256) : 0;
         if (col) cnt[Q_H3] = cnt[Q_R3] = 0;
-        cnt[Q_CP] = (long)((SW * a.zslab + CP_ELEMS - 1) / CP_ELEMS);
-        cnt[Q_CW] = (long)((SW * L.ps + CP_ELEMS - 1) / CP_ELEMS);
+        cnt[Q_CP] = SW * cp_chunks(a.zslab);
+        cnt[Q_CW] = SW * cp_chunks(L.ps);
         for (int q = 0; q < Q_N; q++) {
             const long v = (long)pre[(size_t)q * (nb + 1) + i] + cnt[q];
             if (v > 0x7fffffffL) return fail(c, CUMC_EINVAL, "batched reverse work list exceeds 2^31 tiles");

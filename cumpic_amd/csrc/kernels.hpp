@@ -261,6 +261,37 @@ __device__ inline void flux_walk_tile(const Lay &L, const Fields<R> &F, const Ta
         o[6] = ldi(SB + bB * ps, g);
     };
 
+    if constexpr (DIR == 3 && RECON != 5 && sizeof(R) == 8) {
+        // z-collapsed box (z stride 0): the one z face of the column sees the cell on both
+        // sides and a stencil of copies of it, so its edge values take the closed form of
+        // const_edges whenever the seven values are finite
+        if (L.zc1) {
+            R w[7];
+            load7(0, w);
+            bool fin = true;
+#pragma unroll
+            for (int v = 0; v < 7; v++) fin = fin && isfinite((double)w[v]);
+            if (fin) {
+                R vm[7], vp[7], fl[7];
+#pragma unroll
+                for (int v = 0; v < 7; v++) const_edges<R, RECON>(w[v], vm[v], vp[v]);
+                const unsigned gf = at(0);
+                const R bn = ldi(sweep_p(F) + ibn * ps, gf);
+                FaceState<R> sl{vp[0], vp[3], vp[1], vp[2], vp[4], bn, vp[5], vp[6]};
+                FaceState<R> sr{vm[0], vm[3], vm[1], vm[2], vm[4], bn, vm[5], vm[6]};
+                riemann_face<3, R, SOLVER>(R(M.ggas), sl, sr, fl);
+                R *o = F.F[2];
+                sti(o, gf, fl[0]);
+                sti(o + 1 * ps, gf, fl[2]);
+                sti(o + 2 * ps, gf, fl[3]);
+                sti(o + 3 * ps, gf, fl[1]);
+                sti(o + 4 * ps, gf, fl[4]);
+                sti(o + 5 * ps, gf, fl[5]);  // F(Bx)
+                sti(o + 6 * ps, gf, fl[6]);  // F(By)
+                return;
+            }
+        }
+    }
     // window: w0 = v(c-1), w1 = v(c), w2 = v(c+1), w3 = v(c+2) ; slopes d0 = dm(c-1), d1 = dm(c), d2 = dm(c+1)
     R w0[7], w1[7], w2[7], w3[7], d0[7], d1[7], vpp[7];
     R wm[RECON == 5 ? 7 : 1];  // WENO5: the value two cells back

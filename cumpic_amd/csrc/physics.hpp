@@ -259,6 +259,20 @@ __host__ __device__ inline void tvd_any(R cf, R cb, R vm1, R vc, R vp1, R &vm_ou
     else tvd_lim_edges<R, RECON>(cf, cb, vm1, vc, vp1, vm_out, vp_out);
 }
 
+// Edge values of a cell whose whole stencil holds its own finite value v (the z sweep of a box
+// with one periodic or outflow z cell), equal bit for bit to the general reconstruction with
+// finite, positive table rows: every difference is +0, so the PPM slopes are +0 and its
+// monotonicity branch sets both edges to v; the TVD limiters' slope is +0, so vm = v - 0 = v and
+// vp = v + 0, which is v except that -0 becomes +0.  WENO5 has no such form (its weights do not
+// sum to one exactly) and is not taken here.
+template <typename R, int RECON>
+__host__ __device__ inline void const_edges(R v, R &vm_out, R &vp_out)
+{
+    static_assert(RECON >= 1 && RECON <= 4, "PPM and the TVD options only");
+    vm_out = v;
+    vp_out = RECON == 4 ? v : v + R(0);
+}
+
 // One side of a face in the rotated frame of the sweep: n = normal, t1/t2 = transverse in the
 // cyclic order of flux.f90:182-206 (x: n=x,t1=y,t2=z ; y: n=y,t1=z,t2=x ; z: n=z,t1=x,t2=y).
 template <typename R>
