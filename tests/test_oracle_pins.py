@@ -124,6 +124,33 @@ def test_tvdmm_uniform_minmod_and_linear_exactness():
         assert abs(vm - (alpha + beta * faces[1])) < 1e-12 and abs(vp - (alpha + beta * faces[2])) < 1e-12
 
 
+def test_constant_stencil_edges_closed_form():
+    """A stencil of copies of one finite value v (the z sweep of a box with one periodic or
+    outflow z cell) gives PPM edges (v, v) and TVD edges (v, v + 0) bit for bit, signed zeros,
+    subnormals and extreme magnitudes included, on any positive widths: the closed form the GPU's
+    collapsed z faces take instead of reconstructing.  WENO5 has no such form (its weights do not
+    sum to one exactly).  Catches a reconstruction that perturbs constant data by an ulp or drops
+    the sign of a zero the other way."""
+    def bits(x):
+        return int(np.float64(x).view(np.uint64))
+
+    vals = [0.0, -0.0, 5e-324, -5e-324, 2.2e-308, -1e-300, 0.1, -3.7, 1e300, -1.7e308]
+    vals += list(RNG.standard_normal(24) * 10.0 ** RNG.integers(-40, 40, 24))
+    weno_moved = 0
+    for d in (np.ones(5), RNG.uniform(0.1, 3.0, 5), np.array([1e-3, 2.0, 0.5, 7.0, 1e-2])):
+        for v in vals:
+            vm, vp = orc.ppm(d, [v] * 5)
+            assert (bits(vm), bits(vp)) == (bits(v), bits(v)), ("ppm", v, vm, vp)
+            for lim in (1, 2, 3):
+                vm, vp = orc.tvd(lim, d[:3], [v] * 3)
+                assert (bits(vm), bits(vp)) == (bits(v), bits(v + 0.0)), ("tvd", lim, v, vm, vp)
+            vm, vp = orc.tvdmm(d[:3], [v] * 3)
+            assert (bits(vm), bits(vp)) == (bits(v), bits(v + 0.0)), ("tvdmm", v, vm, vp)
+            wm, wp = orc.weno5([v] * 5)
+            weno_moved += (bits(wm), bits(wp)) != (bits(v), bits(v))
+    assert weno_moved > 0
+
+
 # --------------------------------------------------------------------------------------------
 # Fast speed, physical flux, HLL (orc_signalspeed, orc_ptof, orc_hll)
 # --------------------------------------------------------------------------------------------
