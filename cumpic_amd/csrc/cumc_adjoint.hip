@@ -1643,10 +1643,11 @@ struct batch_tape {
     std::vector<hipEvent_t> seg_ev;
     double recompute_ms = 0.0;       // the re-runs' total in the last backward pass
     hipStream_t s2 = nullptr;        // the re-runs' stream, default priority like the batch stream
-    // the stage states of the last checkpoint segment, which the backward reverses first, are
-    // stored by tape_run's own forward pass (the segment buffers hold one segment), so the
-    // backward re-runs every segment but that one; with every >= nsteps nothing is re-run
-    bool last_stored = false;
+    // the checkpoint segment (g, counted from the last in time) whose stage states each set of
+    // segment buffers holds, -1 for none: tape_run's own forward pass stores the last segment in
+    // set 0 and, with two sets, the one before it in set 1, so the backward re-runs every segment
+    // but those; with every >= nsteps nothing is re-run
+    int held[2] = {-1, -1};
 };
 
 void batch_tape_release(cumc_batch *b)
@@ -1902,26 +1903,31 @@ extern "C" int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const d
     if (dt_in) BTCHK(b, hipMemcpyAsync(batch_hist(b), dt_in, sizeof(double) * (size_t)nsteps * nb, hipMemcpyHostToDevice, s));
     batch_set_counter(b, 0);
     if (!dt_in) batch_finddt(b);
-    // steps of the last segment store their stage states as the backward's re-run would
-    // (bit-identical: the same kernels on the same state and dt)
+    // steps of the last segment (and, with two sets, of the one before it) store their stage
+    // states as the backward's re-run would (bit-identical: the same kernels on the same state
+    // and dt), into the set the backward reads them from
     const int s_last = nsteps > 0 ? ((nsteps - 1) / every) * every : 0;
+    const int s_prev = sets > 1 && s_last >= every ? s_last - every : s_last;
+    const int stride = batch_box(b, 0)->tape->set_stride;
     for (int m = 0; m < nsteps; m++) {
-        if (m < s_last) {
+        if (m < s_prev) {
             batch_step(b, m & 1, dt_in != nullptr);
         } else {
-            const int par = m & 1, i = m - s_last;
-            if (i == 0) bt_copy(b, bt, CP_SAVE_SEG, 0, par);
+            const bool last = m >= s_last;
+            const int par = m & 1, i = m - (last ? s_last : s_prev), off = last ? 0 : stride;
+            if (i == 0) bt_copy(b, bt, CP_SAVE_SEG, off, par);
             batch_dt_begin(b, dt_in != nullptr);
             batch_stage(b, 1, par);
-            bt_copy(b, bt, CP_SAVE_SA, i, 1 - par);
+            bt_copy(b, bt, CP_SAVE_SA, off + i, 1 - par);
             batch_stage(b, 2, par);
-            bt_copy(b, bt, CP_SAVE_SB, i, 0);
+            bt_copy(b, bt, CP_SAVE_SB, off + i, 0);
             batch_stage(b, 3, par);
-            bt_copy(b, bt, CP_SAVE_SEG, i + 1, 1 - par);
+            bt_copy(b, bt, CP_SAVE_SEG, off + i + 1, 1 - par);
         }
         if ((m + 1) % every == 0) bt_copy(b, bt, CP_SAVE_CK, (m + 1) / every, (m + 1) & 1);
     }
-    bt->last_stored = nsteps > 0;
+    bt->held[0] = nsteps > 0 ? 0 : -1;
+    bt->held[1] = s_prev < s_last ? 1 : -1;
     bt_copy(b, bt, CP_SAVE_CK, bt->nck - 1, nsteps & 1);
     BTCHK(b, hipGetLastError());
     BTCHK(b, hipStreamSynchronize(s));
@@ -1985,7 +1991,8 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
         BTCHK(b, hipMemcpyAsync(batch_hist(b), bt->dts.data(), sizeof(double) * bt->dts.size(), hipMemcpyHostToDevice, s));
     // Checkpoint segments g = 0, 1, ... from the last in time back.  The forward re-run of a
     // segment stores every stage's output state, so its reverse runs no forward kernel; the last
-    // segment's states are still in set 0 from tape_run.  Segment g uses buffer set g mod sets:
+    // two segments' states are still in sets 0 and 1 from tape_run (held[]), unless an earlier
+    // backward re-ran other segments into them.  Segment g uses buffer set g mod sets:
     // the re-run of segment g + 1 (on s2) runs while segment g is reversed (on the batch stream)
     // and waits for the reverse of segment g - 1, the last reader of its set; the reverse of a
     // re-run segment waits for its re-run.  The re-runs own the boxes' fields and dt slots; the
@@ -2019,14 +2026,15 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
         hipEventRecord(bt->seg_ev[3 * g + 1], s2);
         batch_swap_stream(b, s_b);
         reran[g] = 1;
+        bt->held[g % sets] = g;
     };
+    auto stored = [&](int g) { return bt->held[g % sets] == g; };
     BTCHK(b, hipEventRecord(bt->e0, s));
     BTCHK(b, hipStreamWaitEvent(s2, bt->e0, 0));
-    const bool stored0 = bt->last_stored;
-    if (nseg > 0 && !stored0) rerun(0);
+    if (nseg > 0 && !stored(0)) rerun(0);
     int q = 0;
     for (int g = 0; g < nseg; g++) {
-        if (g + 1 < nseg) {
+        if (g + 1 < nseg && !stored(g + 1)) {
             if (g >= 1) BTCHK(b, hipStreamWaitEvent(s2, bt->seg_ev[3 * (g - 1) + 2], 0));
             rerun(g + 1);
         }
@@ -2042,8 +2050,6 @@ extern "C" int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed
         BTCHK(b, hipEventRecord(bt->seg_ev[3 * g + 2], s));
         BTCHK(b, hipGetLastError());
     }
-    // set 0 holds the last segment's states again unless a later segment was re-run into it
-    bt->last_stored = nseg <= 2;
     // initial_update, in the full layout (z-collapsed boxes' adjoints are on plane 1, their z
     // ghost planes' zero): its ghost fill, then k_init on the uploaded state
     bt_ghost(b, bt, 0, 0, true);

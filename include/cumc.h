@@ -232,7 +232,9 @@ int cumc_tape_free(cumc_ctx *ctx);
  * dt_in / dt_out are nsteps x nbox, step-major.  Each reverse phase is one launch over all boxes;
  * a box's gradient equals its cumc_tape_backward gradient bit for bit.  seed_bcell and grad_prim
  * are arrays of nbox host pointers (cumc_download / cumc_upload layouts; an array or an entry may
- * be NULL).  cumc_batch_last_ms afterwards gives the device time of the backward pass. */
+ * be NULL).  cumc_batch_last_ms afterwards gives the device time of the backward pass.  The
+ * batched backward re-runs each checkpoint segment but the last two (whose stage states the
+ * taping forward keeps in the two sets of segment buffers that the overlapped re-runs use). */
 int cumc_batch_tape_run(cumc_batch *b, int nsteps, int every, const double *dt_in, double *dt_out);
 int cumc_batch_tape_backward(cumc_batch *b, const double *const *seed_bcell, double *const *grad_prim);
 int cumc_batch_tape_free(cumc_batch *b);
