@@ -222,23 +222,31 @@ void launch_fluxes(cumc_ctx *ctx, const Fields<R> &F, int only_dir = 0)
     launch_fluxes<R>(ctx, ctx->L, F, only_dir);
 }
 
-template <typename R, int STAGE>
-void launch_tail(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, const double *Uin, double *Uout)
+// comb: stage 1 writes, and stage 2 reads, the combined stage-2 input of a plain forward step
+// (rk_w2, comb_out in kernels.hpp); the mixed form combines only without sources and model hooks
+template <typename R, int STAGE, int COORD>
+void launch_tail_c(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, const double *Uin, double *Uout, bool comb)
 {
     const int zc = ctx->zchunk, tw = ctx->tile_w;
     const int gx = col_tiles(L, tw), gy = zchunks(L, zc);
     dim3 grid(xcd_blocks(gx, gy));
     Tabs tb = tabs_of(ctx);
     unsigned long long *dtn = (unsigned long long *)(ctx->dt_dev + 1);
-    if (ctx->M.coordinate == CUMC_SPHERICAL)
-        hipLaunchKernelGGL((k_tail<R, STAGE, 3>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy, zc, tw);
-    else if (ctx->M.coordinate == CUMC_CYLINDRICAL)
-        hipLaunchKernelGGL((k_tail<R, STAGE, 2>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy, zc, tw);
-    else
-        hipLaunchKernelGGL((k_tail<R, STAGE, 1>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
-                           Uin, Uout, dtn, gx, gy, zc, tw);
+    if constexpr (STAGE < 3 && (sizeof(R) == 8 || COORD == 1))
+        if (comb) {
+            hipLaunchKernelGGL((k_tail<R, STAGE, COORD, true>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M,
+                               ctx->dt_dev, Uin, Uout, dtn, gx, gy, zc, tw);
+            return;
+        }
+    hipLaunchKernelGGL((k_tail<R, STAGE, COORD, false>), grid, dim3(256), 0, ctx->stream, L, F, tb, ctx->M, ctx->dt_dev,
+                       Uin, Uout, dtn, gx, gy, zc, tw);
+}
+template <typename R, int STAGE>
+void launch_tail(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, const double *Uin, double *Uout, bool comb)
+{
+    if (ctx->M.coordinate == CUMC_SPHERICAL) launch_tail_c<R, STAGE, 3>(ctx, L, F, Uin, Uout, comb);
+    else if (ctx->M.coordinate == CUMC_CYLINDRICAL) launch_tail_c<R, STAGE, 2>(ctx, L, F, Uin, Uout, comb);
+    else launch_tail_c<R, STAGE, 1>(ctx, L, F, Uin, Uout, comb);
 }
 
 // The FP64 fields as the kernels address them: the full layout, or (col) the z-collapsed view of
@@ -276,9 +284,10 @@ void zexpand(cumc_ctx *ctx);
 
 // one RK stage: fluxes of the three sweeps, corner EMFs, stage tail, ghost fill, on the fields F
 // addressed by L (Ub: the stage-2 output buffer in the same addressing).
-// Conserved buffers: stage 1 U0 -> U1, stage 2 U1 -> Ub, stage 3 Ub -> U1.
+// Conserved buffers: stage 1 U0 -> U1, stage 2 U1 -> Ub, stage 3 Ub -> U1; with comb (a plain
+// forward step) U1 holds the combined stage-2 input between stages 1 and 2.
 template <typename R>
-void stage_l(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, double *Ub, int st)
+void stage_l(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, double *Ub, int st, bool comb = false)
 {
     launch_fluxes<R>(ctx, L, F);
     mark(ctx, KC_ECORN);
@@ -288,15 +297,15 @@ void stage_l(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, double *Ub, int st
     mark(ctx, -1);
     mark(ctx, KC_TAIL);
     if (st == 1) {
-        launch_tail<R, 1>(ctx, L, F, F.U0, F.U1);
+        launch_tail<R, 1>(ctx, L, F, F.U0, F.U1, comb);
     } else if (st == 2) {
-        launch_tail<R, 2>(ctx, L, F, F.U1, Ub);
+        launch_tail<R, 2>(ctx, L, F, F.U1, Ub, comb);
     } else {
         const double init = 1.0e10;
         unsigned long long bits;
         std::memcpy(&bits, &init, 8);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, (unsigned long long *)(ctx->dt_dev + 1), bits, 1);
-        launch_tail<R, 3>(ctx, L, F, Ub, F.U1);
+        launch_tail<R, 3>(ctx, L, F, Ub, F.U1, comb);
     }
     mark(ctx, -1);
     mark(ctx, KC_GHOST);
@@ -310,7 +319,7 @@ void stage_l(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, double *Ub, int st
 }
 
 template <typename R>
-void stage(cumc_ctx *ctx, const Fields<R> &F, int st)
+void stage(cumc_ctx *ctx, const Fields<R> &F, int st, bool comb = false)
 {
-    stage_l<R>(ctx, ctx->L, F, ctx->Ub, st);
+    stage_l<R>(ctx, ctx->L, F, ctx->Ub, st, comb);
 }

@@ -15,7 +15,7 @@
 #include "ctx.hpp"
 
 // float instantiations live in cumc_mixed.hip (built with FP32 contraction and fast division)
-void stage_mixed(cumc_ctx *ctx, int st);
+void stage_mixed(cumc_ctx *ctx, int st, bool comb);
 static void prim_to_f32(cumc_ctx *ctx);
 static void sync_prim64(cumc_ctx *ctx);
 void fluxes_mixed(cumc_ctx *ctx, int dir, double *aos_tmp);
@@ -503,11 +503,15 @@ static void sync_prim64(cumc_ctx *ctx)
 // dt_dev[1]
 static void enqueue_step(cumc_ctx *ctx)
 {
+    // stage 1 passes the combined stage-2 input (a plain step: nothing reads the intermediate
+    // state); the mixed form takes its sources' density from the stage-input state, so it
+    // combines only without sources and model hooks
     if (ctx->cfg.precision == CUMC_MIXED) {
-        for (int st = 1; st <= 3; st++) stage_mixed(ctx, st);
+        const bool comb = ctx->M.coordinate == CUMC_CARTESIAN && ctx->M.model == CUMC_MODEL_RIEMANN;
+        for (int st = 1; st <= 3; st++) stage_mixed(ctx, st, comb);
     } else {
         const View v = make_view(ctx, zcollapse_begin(ctx));
-        for (int st = 1; st <= 3; st++) stage_l<double>(ctx, v.L, v.F, v.Ub, st);
+        for (int st = 1; st <= 3; st++) stage_l<double>(ctx, v.L, v.F, v.Ub, st, true);
     }
     swap_u(ctx);
     ctx->dt_valid = true;

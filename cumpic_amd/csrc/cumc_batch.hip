@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void kb_ecorn(const BoxDesc *__restrict__ D, c
 }
 
 // trip counts are block-uniform, so the stage-3 block reduction inside tail_tile is safe
-template <int STAGE, int COORD>
+template <int STAGE, int COORD, bool COMB>
 __global__ __launch_bounds__(256) void kb_tail(const BoxDesc *__restrict__ D, const int *__restrict__ pre, int nbox,
                                                int par)
 {
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void kb_tail(const BoxDesc *__restrict__ D, co
         const Fields<double> F = fields_at(d, par);
         const double *Uin = STAGE == 1 ? F.U0 : (STAGE == 2 ? F.U1 : d.Ub);
         double *Uout = STAGE == 2 ? d.Ub : F.U1;
-        tail_tile<double, STAGE, COORD>(d.L, F, d.tb, d.M, d.dt, Uin, Uout, (unsigned long long *)(d.dt + 1),
+        tail_tile<double, STAGE, COORD, COMB>(d.L, F, d.tb, d.M, d.dt, Uin, Uout, (unsigned long long *)(d.dt + 1),
                                         loc % d.c_gx, loc / d.c_gx, d.zc, d.tw);
         BR_FLUSH(d, it);
     }
@@ -373,16 +373,17 @@ static void enqueue_finddt_b(cumc_batch *b)
     bmark(b, -1);
 }
 
-template <int STAGE>
+// comb: stage 1 writes, and stage 2 reads, the combined stage-2 input (a plain step; kernels.hpp)
+template <int STAGE, bool COMB>
 static void launch_tail_b(cumc_batch *b, int par)
 {
     const unsigned g = grid_of(b, P_TAIL);
     if (b->coord == CUMC_SPHERICAL)
-        hipLaunchKernelGGL((kb_tail<STAGE, 3>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
+        hipLaunchKernelGGL((kb_tail<STAGE, 3, COMB>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
     else if (b->coord == CUMC_CYLINDRICAL)
-        hipLaunchKernelGGL((kb_tail<STAGE, 2>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
+        hipLaunchKernelGGL((kb_tail<STAGE, 2, COMB>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
     else
-        hipLaunchKernelGGL((kb_tail<STAGE, 1>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
+        hipLaunchKernelGGL((kb_tail<STAGE, 1, COMB>), dim3(g), dim3(256), 0, b->stream, b->d_desc, pre_of(b, P_TAIL), b->nbox, par);
 }
 
 // face fluxes of the three sweeps and the corner EMFs of every box (the first half of a stage)
@@ -410,7 +411,7 @@ static void enqueue_fluxes_ecorn_b(cumc_batch *b)
 }
 
 // one RK stage of every box; par = parity of the step within the run
-static void enqueue_stage_b(cumc_batch *b, int st, int par)
+static void enqueue_stage_b(cumc_batch *b, int st, int par, bool comb)
 {
     hipStream_t s = b->stream;
     const BoxDesc *D = b->d_desc;
@@ -418,9 +419,14 @@ static void enqueue_stage_b(cumc_batch *b, int st, int par)
     enqueue_fluxes_ecorn_b(b);
     if (st == 3) hipLaunchKernelGGL(kb_dt_reset, dim3(1), dim3(256), 0, s, D, nb);
     bmark(b, P_TAIL);
-    if (st == 1) launch_tail_b<1>(b, par);
-    else if (st == 2) launch_tail_b<2>(b, par);
-    else launch_tail_b<3>(b, par);
+    if (comb && st < 3) {
+        if (st == 1) launch_tail_b<1, true>(b, par);
+        else launch_tail_b<2, true>(b, par);
+    } else {
+        if (st == 1) launch_tail_b<1, false>(b, par);
+        else if (st == 2) launch_tail_b<2, false>(b, par);
+        else launch_tail_b<3, false>(b, par);
+    }
     bmark(b, -1);
     bmark(b, P_GX);
     hipLaunchKernelGGL(kb_ghost<0>, dim3(grid_of(b, P_GX)), dim3(256), 0, s, D, pre_of(b, P_GX), nb);
@@ -434,7 +440,7 @@ static void enqueue_stage_b(cumc_batch *b, int st, int par)
 static void enqueue_step_b(cumc_batch *b, int par, int frozen)
 {
     hipLaunchKernelGGL(kb_dt_begin, dim3(1), dim3(256), 0, b->stream, b->d_desc, b->nbox, b->d_hist, b->d_ctr, frozen);
-    for (int st = 1; st <= 3; st++) enqueue_stage_b(b, st, par);
+    for (int st = 1; st <= 3; st++) enqueue_stage_b(b, st, par, true);
 }
 
 // history capacity for nsteps steps (graphs hold the history pointer)
@@ -475,7 +481,8 @@ void batch_set_counter(cumc_batch *b, int v)
 }
 void batch_finddt(cumc_batch *b) { enqueue_finddt_b(b); }
 void batch_step(cumc_batch *b, int par, int frozen) { enqueue_step_b(b, par, frozen); }
-void batch_stage(cumc_batch *b, int st, int par) { enqueue_stage_b(b, st, par); }
+// one stage of a taped step: the plain stage states, which the tape stores
+void batch_stage(cumc_batch *b, int st, int par) { enqueue_stage_b(b, st, par, false); }
 // a step's dt bookkeeping (enqueue_step_b without its stages): record the found dt in the
 // history, or take the history entry when frozen
 void batch_dt_begin(cumc_batch *b, int frozen)

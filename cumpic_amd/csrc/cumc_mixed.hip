@@ -5,7 +5,7 @@
 // the conserved-to-primitive recovery and the face-B update stay FP64 (k_tail).
 #include "ctx.hpp"
 
-void stage_mixed(cumc_ctx *ctx, int st) { stage<float>(ctx, ctx->F32, st); }
+void stage_mixed(cumc_ctx *ctx, int st, bool comb) { stage<float>(ctx, ctx->F32, st, comb); }
 
 // the reference's ghost passes (and torus inflow clamps) on the FP32 primitive planes
 void launch_ghost32(cumc_ctx *ctx)

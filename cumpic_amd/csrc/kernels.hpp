@@ -529,6 +529,21 @@ __device__ inline double rk(double u0, double u1, double dt, double lrk)
     return RK30 * u0 + RK31 * u1 + RK32 * dt * lrk;
 }
 
+// Combined stage input (COMB).  Stage 2 uses its input state u1 only in
+// (RK20 u0 + RK21 u1) + RK22 dt L, so in a plain forward step stage 1 stores
+// w2 = RK20 u0 + RK21 u1 in place of u1 (it holds both) and stage 2 adds its right-hand side to
+// w2 by the same operations rk performs, bit for bit, reading 8 FP64 planes instead of 16; stage
+// 2 stores the plain u2 and stage 3 is unchanged.  (Forming RK30 u0 + RK31 u2 in stage 2 as well
+// made stage 2 load u0 after its arithmetic, which measured slower in the mixed form.)  Taped
+// steps, whose stage states are stored for the reverse pass, keep the plain states.
+__device__ inline double rk_w2(double w, double dt, double lrk) { return w + RK22 * dt * lrk; }
+template <int STAGE, bool COMB>
+__device__ inline double comb_out(double u0, double u)
+{
+    if (COMB && STAGE == 1) return RK20 * u0 + RK21 * u;
+    return u;
+}
+
 // right-hand sides of the three face fields of cell (j,k,l) (mhd.f90:130-153, GEOM_CT of
 // coords.f90:338-386 evaluated at the cell)
 template <int COORD>
@@ -655,7 +670,7 @@ struct TailCarry {
 // Stage tail of one cell of [0,n]^3: flux divergence + sources, face-B right-hand side, RK3
 // combination, U->P, floors, P->U, EOS, and (stage 3) its CFL step.  fz holds the z-face fluxes
 // above the cell; cr carries from the cell below and is advanced to this cell.
-template <typename R, int STAGE, int COORD>
+template <typename R, int STAGE, int COORD, bool COMB>
 __device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, double dt,
                                  const double *__restrict__ Uin, double *__restrict__ Uout, int j, int k, int l,
                                  unsigned g, const ColGeom &cg, const CellE &E, TailCarry &cr, const double fz[5],
@@ -669,6 +684,11 @@ __device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &t
     auto fy = [&](int c, unsigned i) { return (double)ldi(F.F[1] + c * ps, i); };
     auto u0 = [&](int c, unsigned i) { return ldi(U0 + c * ps, i); };
     auto ui = [&](int c, unsigned i) { return STAGE == 1 ? 0.0 : ldi(Uin + c * ps, i); };
+    // the stage's RK combination of component c at offset i with right-hand side r
+    auto rkc = [&](int c, unsigned i, double r) {
+        if constexpr (COMB && STAGE == 2) return rk_w2(ui(c, i), dt, r);
+        else return rk<STAGE>(u0(c, i), ui(c, i), dt, r);
+    };
     // stage-input primitives: the FP64 planes, or the FP32 ones of the mixed form
     constexpr bool f32p = sizeof(R) == 4;
     auto pp = [&](int c, unsigned i) { return f32p ? (double)ldi(F.P32 + c * ps, i) : ldi(F.P + c * ps, i); };
@@ -678,9 +698,9 @@ __device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &t
     const double ex0 = E.ex0, ey0 = E.ey0, ez0 = E.ez0, exk = E.exk, ezk = E.ezk, eyj = E.eyj, ezj = E.ezj,
                  ezjk = E.ezjk;
     // new face fields: own three, and the x / y faces of the lower neighbours
-    const double bxn = rk<STAGE>(u0(5, g), ui(5, g), dt, rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, cr.eyL));
-    const double byn = rk<STAGE>(u0(6, g), ui(6, g), dt, rhs_by<COORD>(tb, j, k, l, ex0, cr.exL, ez0, ezj));
-    const double bzn = rk<STAGE>(u0(7, g), ui(7, g), dt, rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
+    const double bxn = rkc(5, g, rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, cr.eyL));
+    const double byn = rkc(6, g, rhs_by<COORD>(tb, j, k, l, ex0, cr.exL, ez0, ezj));
+    const double bzn = rkc(7, g, rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
     // hydro right-hand side on interior cells
     double lrk[5];
     if (interior) {
@@ -688,7 +708,8 @@ __device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &t
         if (COORD == 2 || COORD == 3 || M.model != 0) {
             // (the mixed form takes the density from the stage-input U, which holds the
             // FP64 value, so that the density-floor switch of the model gravity sees
-            // what the FP64 form sees)
+            // what the FP64 form sees; it runs COMB only without sources and model hooks)
+            static_assert(!(COMB && sizeof(R) == 4 && COORD != 1), "combined inputs with sources in the mixed form");
             const double rho = f32p ? (STAGE == 1 ? u0(0, g) : ui(0, g)) : pp(0, g);
             const double vx = pp(1, g), vy = pp(2, g), vz = pp(3, g);
             const double p = pp(4, g);
@@ -751,12 +772,12 @@ __device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &t
     }
     double u[5];
 #pragma unroll
-    for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(u0(i, g), ui(i, g), dt, lrk[i]);
+    for (int i = 0; i < 5; i++) u[i] = rkc(i, g, lrk[i]);
     // cell-centred field from the new face fields (interior) or the ghost value
     double bcx, bcy, bcz;
     if (interior) {
-        const double bxm = rk<STAGE>(u0(5, g - sj), ui(5, g - sj), dt, rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, cr.eyjL));
-        const double bym = rk<STAGE>(u0(6, g - sk), ui(6, g - sk), dt, rhs_by<COORD>(tb, j, k - 1, l, exk, cr.exkL, ezk, ezjk));
+        const double bxm = rkc(5, g - sj, rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, cr.eyjL));
+        const double bym = rkc(6, g - sk, rhs_by<COORD>(tb, j, k - 1, l, exk, cr.exkL, ezk, ezjk));
         bcx = 0.5 * (bxn + bxm);
         bcy = 0.5 * (byn + bym);
         bcz = 0.5 * (bzn + cr.bzL);
@@ -791,14 +812,15 @@ __device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &t
     }
     vsq = vx * vx + vy * vy + vz * vz;
     const double p = rho * eps * gm1;
-    sti(Uout, g, rho);
-    sti(Uout + ps, g, vx * rho);
-    sti(Uout + 2 * ps, g, vy * rho);
-    sti(Uout + 3 * ps, g, vz * rho);
-    sti(Uout + 4 * ps, g, rho * (eps + 0.5 * vsq) + 0.5 * bsq);
-    sti(Uout + 5 * ps, g, bxn);
-    sti(Uout + 6 * ps, g, byn);
-    sti(Uout + 7 * ps, g, bzn);
+    auto uo = [&](int c, double v) { sti(Uout + c * ps, g, comb_out<STAGE, COMB>(COMB && STAGE == 1 ? u0(c, g) : 0.0, v)); };
+    uo(0, rho);
+    uo(1, vx * rho);
+    uo(2, vy * rho);
+    uo(3, vz * rho);
+    uo(4, rho * (eps + 0.5 * vsq) + 0.5 * bsq);
+    uo(5, bxn);
+    uo(6, byn);
+    uo(7, bzn);
     if (f32p) {
         sti(F.P32, g, (float)rho);
         sti(F.P32 + ps, g, (float)vx);
@@ -836,7 +858,7 @@ __device__ inline void tail_cell(const Lay &L, const Fields<R> &F, const Tabs &t
 // reduction at the end, so callers may loop over tiles with block-uniform trip counts.  In the
 // mixed form (R = float) the stage-input primitives come from, and the new ones go to, the FP32
 // planes P32 / EPS32 / BC32.
-template <typename R, int STAGE, int COORD>
+template <typename R, int STAGE, int COORD, bool COMB>
 __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &tb, const Model &M, const double *dtp,
                                  const double *__restrict__ Uin, double *__restrict__ Uout,
                                  unsigned long long *dtnext, int bx, int by, int zc, int tw)
@@ -865,8 +887,11 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
         cr.eyjL = e(1, gm - sj);
         cr.exkL = e(0, gm - sk);
         cr.bzL = 0.0;
-        if (l0 >= 1)
-            cr.bzL = rk<STAGE>(u0(7, gm), ui(7, gm), dt, rhs_bz<COORD>(tb, j, k, cr.eyL, cr.eyjL, cr.exL, cr.exkL));
+        if (l0 >= 1) {
+            const double r = rhs_bz<COORD>(tb, j, k, cr.eyL, cr.eyjL, cr.exL, cr.exkL);
+            if constexpr (COMB && STAGE == 2) cr.bzL = rk_w2(ui(7, gm), dt, r);
+            else cr.bzL = rk<STAGE>(u0(7, gm), ui(7, gm), dt, r);
+        }
 #pragma unroll
         for (int i = 0; i < 5; i++) cr.fzL[i] = fzz(i, gm);
         for (int l = l0; l <= l1; l++, g += sl) {
@@ -879,7 +904,7 @@ __device__ inline void tail_tile(const Lay &L, const Fields<R> &F, const Tabs &t
             double fz[5];
 #pragma unroll
             for (int i = 0; i < 5; i++) fz[i] = fzz(i, g);
-            tail_cell<R, STAGE, COORD>(L, F, tb, M, dt, Uin, Uout, j, k, l, g, cg, E, cr, fz, dtmin);
+            tail_cell<R, STAGE, COORD, COMB>(L, F, tb, M, dt, Uin, Uout, j, k, l, g, cg, E, cr, fz, dtmin);
         }
     }
     if (STAGE == 3) {
@@ -936,9 +961,11 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
         long gm = g - sl;
         double exL = E[gm], eyL = E[gm + ps], eyjL = E[gm - sj + ps], exkL = E[gm - sk];
         double bzL = 0.0;
-        if (l0 >= 1)
-            bzL = rk<STAGE>(U0[gm + 7 * ps], STAGE == 1 ? 0.0 : Uin[gm + 7 * ps], dt,
-                            rhs_bz<COORD>(tb, j, k, eyL, eyjL, exL, exkL));
+        // the stage's RK combination of component c at element e with right-hand side r
+        auto rkc = [&](int c, long e, double r) {
+            return rk<STAGE>(U0[e + c * ps], STAGE == 1 ? 0.0 : Uin[e + c * ps], dt, r);
+        };
+        if (l0 >= 1) bzL = rkc(7, gm, rhs_bz<COORD>(tb, j, k, eyL, eyjL, exL, exkL));
         double fzL[5];
 #pragma unroll
         for (int i = 0; i < 5; i++) fzL[i] = (double)Fz[gm + i * ps];
@@ -950,12 +977,9 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
             const double eyj = E[g - sj + ps], ezj = E[g - sj + 2 * ps];
             const double ezjk = E[g - sj - sk + 2 * ps];
             // new face fields: own three, and the x / y faces of the lower neighbours
-            const double bxn = rk<STAGE>(U0[g + 5 * ps], STAGE == 1 ? 0.0 : Uin[g + 5 * ps], dt,
-                                         rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, eyL));
-            const double byn = rk<STAGE>(U0[g + 6 * ps], STAGE == 1 ? 0.0 : Uin[g + 6 * ps], dt,
-                                         rhs_by<COORD>(tb, j, k, l, ex0, exL, ez0, ezj));
-            const double bzn = rk<STAGE>(U0[g + 7 * ps], STAGE == 1 ? 0.0 : Uin[g + 7 * ps], dt,
-                                         rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
+            const double bxn = rkc(5, g, rhs_bx<COORD>(tb, j, k, l, ez0, ezk, ey0, eyL));
+            const double byn = rkc(6, g, rhs_by<COORD>(tb, j, k, l, ex0, exL, ez0, ezj));
+            const double bzn = rkc(7, g, rhs_bz<COORD>(tb, j, k, ey0, eyj, ex0, exk));
             // hydro right-hand side on interior cells
             double lrk[5];
             double fz[5];
@@ -1029,14 +1053,12 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
             }
             double u[5];
 #pragma unroll
-            for (int i = 0; i < 5; i++) u[i] = rk<STAGE>(U0[g + i * ps], STAGE == 1 ? 0.0 : Uin[g + i * ps], dt, lrk[i]);
+            for (int i = 0; i < 5; i++) u[i] = rkc(i, g, lrk[i]);
             // cell-centred field from the new face fields (interior) or the ghost value
             double bcx, bcy, bcz;
             if (interior) {
-                const double bxm = rk<STAGE>(U0[g - sj + 5 * ps], STAGE == 1 ? 0.0 : Uin[g - sj + 5 * ps], dt,
-                                             rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
-                const double bym = rk<STAGE>(U0[g - sk + 6 * ps], STAGE == 1 ? 0.0 : Uin[g - sk + 6 * ps], dt,
-                                             rhs_by<COORD>(tb, j, k - 1, l, exk, exkL, ezk, ezjk));
+                const double bxm = rkc(5, g - sj, rhs_bx<COORD>(tb, j - 1, k, l, ezj, ezjk, eyj, eyjL));
+                const double bym = rkc(6, g - sk, rhs_by<COORD>(tb, j, k - 1, l, exk, exkL, ezk, ezjk));
                 bcx = 0.5 * (bxn + bxm);
                 bcy = 0.5 * (byn + bym);
                 bcz = 0.5 * (bzn + bzL);
@@ -1100,19 +1122,20 @@ __device__ inline void tail_tile_wide(const Lay &L, const Fields<R> &F, const Ta
     }
 }
 
-template <typename R, int STAGE, int COORD>
+template <typename R, int STAGE, int COORD, bool COMB>
 __global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model M, const double *dtp,
                                               const double *__restrict__ Uin, double *__restrict__ Uout,
                                               unsigned long long *dtnext, int gx, int gy, int zc, int tw)
 {
     int bx, by;
     if (!xcd_tile(gx, gy, bx, by)) return;
-    // mixed form, stages 2-3: the 64-bit-address form (1.30 against 1.34 ms per launch on the
-    // 256^3 box); stage 1 and the FP64 form: 32-bit offsets (mixed stage 1: 1.11 against 1.73 ms)
-    if constexpr (sizeof(R) == 4 && STAGE != 1)
+    // mixed form, stages 2-3 reading 34 planes: the 64-bit-address form (1.30 against 1.34 ms
+    // per launch on the 256^3 box); the 26-plane stages (stage 1, stage 2 with a combined input)
+    // and the FP64 form: 32-bit offsets (mixed stage 1: 1.11 against 1.73 ms)
+    if constexpr (sizeof(R) == 4 && STAGE != 1 && !(COMB && STAGE == 2))
         tail_tile_wide<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
     else
-        tail_tile<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
+        tail_tile<R, STAGE, COORD, COMB>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
 }
 
 // initial_update's FROMRVETOU + CUSTOM_CHECKRHO + EOS on [0,n]^3 (initial.f90:103-126)

@@ -72,11 +72,12 @@ def test_collapsed_run_is_bitwise_full_layout_and_oracle(recon):
 def test_collapsed_signed_zeros_and_nan_cell(recon):
     """The closed-form z faces (const_edges) on signed zeros, v_z = -0 and +0 in alternate cells
     (TVD's right edge turns -0 into +0), compared as bit patterns with the full layout; and a NaN
-    cell, whose columns take the general reconstruction: NaN where the full layout has NaN, every
-    other value bit-identical.  With PPM the NaN run also matches the CPU restatement; with TVD it
-    does not: the limiter takes copysign(1, dq) of differences that are NaN next to the cell, and
-    the sign bit of a NaN passed on by a subtraction is not the same on the x86 host and the GPU,
-    so the slopes of the neighbouring cells, and the finite values built on them, differ there."""
+    cell, whose columns take the general reconstruction: NaN where the full layout and the CPU
+    restatement have NaN, and with PPM every other value bit-identical too.  The sign bit of a NaN
+    is not reproducible (an operation on two NaNs returns one of them, and the compiler may order
+    the operands either way; the x86 host and the GPU also differ), and TVD's limiter takes
+    copysign(1, dq) of differences that are NaN next to the cell, so with TVD the finite values
+    beside a NaN depend on it and only the NaN positions are compared."""
     from tests.failure_cases import with_nan_cell
     print(f"[case] collapsed z faces on signed zeros and a NaN cell, recon={recon}", flush=True)
     p = _box(n=16, seed=3450, recon=recon)
@@ -102,10 +103,14 @@ def test_collapsed_signed_zeros_and_nan_cell(recon):
     for s in (a, b, o):
         s.rungekutta(2.0e-3)
     assert a.zcollapse_state() == (True, True)
-    _same(a, b, "NaN cell collapsed vs full")
+    sa, sb, so = a.download(), b.download(), o.get_state()
+    for k in ("prim", "eps", "bcell"):
+        assert np.array_equal(np.isnan(sa[k]), np.isnan(sb[k])), f"NaN positions collapsed vs full {k}"
+        assert np.array_equal(np.isnan(sa[k]), np.isnan(so[k])), f"NaN positions vs oracle {k}"
     if recon == PPMC:
+        _same(a, b, "NaN cell collapsed vs full")
         _same_oracle(a, o, "NaN cell vs oracle")
-    assert int(np.isnan(a.download()["prim"][..., 0]).sum()) > 1
+    assert int(np.isnan(sa["prim"][..., 0]).sum()) > 1
 
 
 def test_host_calls_between_collapsed_runs():
