@@ -66,17 +66,18 @@ KERNEL_IO_BYTES = {
     "fp64": {
         "flux_x": 15 * 8, "flux_y": 15 * 8, "flux_z": 15 * 8,  # 7 stencil vars + normal B in, 7 fluxes out
         "ecorn": 15 * 8,                                        # 6 B-fluxes + v + cell B in, 3 EMFs out
-        # 15 fluxes + 3 EMFs + U0 (+ Uin in stages 2, 3) in; prim 8, eps, bcell 3, Uout 8 out:
-        # (46 + 54 + 54) / 3 words averaged over the three stage launches
-        "tail": (46 + 54 + 54) / 3 * 8,
+        # 15 fluxes + 3 EMFs + U0 (stage 1), the combined input (stage 2) or U0 + Uin (stage 3) in;
+        # prim 8, eps, bcell 3, Uout 8 out: (46 + 46 + 54) / 3 words averaged over the three stage
+        # launches of a plain step
+        "tail": (46 + 46 + 54) / 3 * 8,
     },
     # mixed form: fluxes, EMFs and the primitive planes the sweeps read are FP32; the conserved
-    # state is FP64.  Tail: 18 FP32 flux/EMF words + U0 (+ Uin) in; Uout 8 FP64 + P32 8, EPS32 1,
-    # BC32 3 FP32 out
+    # state is FP64.  Tail: 18 FP32 flux/EMF words + U0 (stage 1), the combined input (stage 2) or
+    # U0 + Uin (stage 3) in; Uout 8 FP64 + P32 8, EPS32 1, BC32 3 FP32 out
     "mixed": {
         "flux_x": 8 * 4 + 7 * 4, "flux_y": 8 * 4 + 7 * 4, "flux_z": 8 * 4 + 7 * 4,
         "ecorn": 6 * 4 + 6 * 4 + 3 * 4,
-        "tail": ((18 * 4 + 8 * 8 + 8 * 8 + 12 * 4) + 2 * (18 * 4 + 16 * 8 + 8 * 8 + 12 * 4)) / 3,
+        "tail": (2 * (18 * 4 + 8 * 8 + 8 * 8 + 12 * 4) + (18 * 4 + 16 * 8 + 8 * 8 + 12 * 4)) / 3,
     },
 }
 # kernels of the time step (per form) whose counter bytes make up the step traffic
@@ -833,11 +834,11 @@ def main():
         if copy_gbps:
             kio["frac_of_copy_probe"] = kio["achieved_GBps"] / copy_gbps
         if form == "fp64":
-            # the tail's access pattern alone (stage 1: 26 planes read, stages 2-3: 34; 20 written),
+            # the tail's access pattern alone (stages 1-2: 26 planes read, stage 3: 34; 20 written),
             # same layout and column tiles, no arithmetic: the bandwidth this kernel could reach
             cells = (cfg.nx + 1) * (cfg.ny + 1) * (cfg.nz + 1)
             g1, g23 = s.pattern_probe(26, 20), s.pattern_probe(34, 20)
-            t_stage = (46 * 8 * cells / (g1 * 1e9) + 2 * 54 * 8 * cells / (g23 * 1e9)) / 3
+            t_stage = (2 * 46 * 8 * cells / (g1 * 1e9) + 54 * 8 * cells / (g23 * 1e9)) / 3
             ceiling = KERNEL_IO_BYTES["fp64"]["tail"] * cells / t_stage / 1e9
             kio["pattern_ceiling_GBps"] = ceiling
             kio["frac_of_pattern_ceiling"] = kio["achieved_GBps"] / ceiling
