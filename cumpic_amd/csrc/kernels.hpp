@@ -530,10 +530,10 @@ __device__ inline double rk(double u0, double u1, double dt, double lrk)
 }
 
 // Combined stage input (COMB).  Stage 2 uses its input state u1 only in
-// (RK20 u0 + RK21 u1) + RK22 dt L, so in a plain forward step stage 1 stores
-// w2 = RK20 u0 + RK21 u1 in place of u1 (it holds both) and stage 2 adds its right-hand side to
-// w2 by the same operations rk performs, bit for bit, reading 8 FP64 planes instead of 16; stage
-// 2 stores the plain u2 and stage 3 is unchanged.  (Forming RK30 u0 + RK31 u2 in stage 2 as well
+// (RK20 u0 + RK21 u1) + RK22 dt L, so in a plain forward step stage 1, which has u0 and u1 at
+// hand, stores w2 = RK20 u0 + RK21 u1 in place of u1, and stage 2 adds its right-hand side to w2
+// by the same operations rk performs, bit for bit, reading 8 FP64 planes instead of 16; stage 2
+// stores the plain u2 and stage 3 is unchanged.  (Forming RK30 u0 + RK31 u2 in stage 2 as well
 // made stage 2 load u0 after its arithmetic, which measured slower in the mixed form.)  Taped
 // steps, whose stage states are stored for the reverse pass, keep the plain states.
 __device__ inline double rk_w2(double w, double dt, double lrk) { return w + RK22 * dt * lrk; }
