@@ -725,7 +725,7 @@ def main():
         traffic_child(args)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU halo exchange is a design note, not code)", file=sys.stderr)
         sys.exit(2)
 
