@@ -176,9 +176,16 @@ void launch_fluxes_t(cumc_ctx *ctx, const Lay &L, const Fields<R> &F, int only_d
     if (!only_dir || only_dir == 2) {
         mark(ctx, KC_FLUX_Y);
         const int nseg = (L.ny + 1 + seg - 1) / seg;
-        const int sl = (L.ny + 1 + nseg - 1) / nseg;
-        dim3 grid(nblk((long)L.px * npl, 128), nseg);
-        hipLaunchKernelGGL((k_flux_walk<R, 2, RECON, SOLVER>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+        if ((long)L.px * npl * nseg < YLANE_WALKERS && L.ny >= 16) {
+            // a 2D box whose pencil walks would leave most SIMDs idle: wave-parallel y sweep
+            const int gx = ylane_groups(L) * ylane_segs(L);
+            hipLaunchKernelGGL((k_flux_ylane<R, RECON, SOLVER>), dim3(xcd_blocks(gx, npl)), dim3(256), 0, s, L, F, tb, M,
+                               gx, npl);
+        } else {
+            const int sl = (L.ny + 1 + nseg - 1) / nseg;
+            dim3 grid(nblk((long)L.px * npl, 128), nseg);
+            hipLaunchKernelGGL((k_flux_walk<R, 2, RECON, SOLVER>), grid, dim3(128), 0, s, L, F, tb, M, sl);
+        }
         mark(ctx, -1);
     }
     if (!only_dir || only_dir == 3) {
