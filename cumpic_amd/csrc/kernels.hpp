@@ -227,9 +227,9 @@ __global__ __launch_bounds__(256) void k_flux_x(Lay L, Fields<R> F, Tabs tb, Mod
 // 128-byte lines.  The same reconstruction and Riemann solve as the pencil walk, bit for bit.
 // Tile bx of plane l: 4-pencil group bx % njt (j in [0, nx+1]) of segment bx / njt.
 // A single context takes this form when its walks would run fewer than YLANE_WALKERS threads
-// (one wavefront per SIMD of the 256 CUs); with more walkers the walks are faster (the batched
-// 48-box workload, 1.2 M cells in 104 K walkers, measured 56.5 against 51.8 ms for 40 steps with
-// this form: its row-strided loads cost more than the walks' low occupancy there).
+// (one wavefront per SIMD of the 256 CUs); with more walkers the walks keep up (the batched
+// 48-box workload, 1.2 M cells in 104 K walkers, measured 51.6 ms for 40 steps with this form in
+// XCD order against 51.7 ms with the walks, and 56.5 ms with this form in launch order).
 // ------------------------------------------------------------------------------------------
 constexpr long YLANE_WALKERS = 1024L * 64;
 __host__ __device__ inline int ylane_groups(const Lay &L) { return (L.nx + 2 + 3) / 4; }
