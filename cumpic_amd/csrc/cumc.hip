@@ -499,9 +499,9 @@ static void sync_prim64(cumc_ctx *ctx)
 }
 
 
-// RUNGEKUTTA body on the stream, dt already in dt_dev[0]; leaves finddt of the new state in
-// dt_dev[1]
-static void enqueue_step(cumc_ctx *ctx)
+// the three stages of a RUNGEKUTTA step on the stream, dt already in dt_dev[0], in the view zc
+// (zcollapse_begin's answer); leaves finddt of the new state in dt_dev[1].  Kernel launches only.
+static void enqueue_stages(cumc_ctx *ctx, bool zc)
 {
     // stage 1 passes the combined stage-2 input (a plain step: nothing reads the intermediate
     // state); the mixed form takes its sources' density from the stage-input state, so it
@@ -510,11 +510,53 @@ static void enqueue_step(cumc_ctx *ctx)
         const bool comb = ctx->M.coordinate == CUMC_CARTESIAN && ctx->M.model == CUMC_MODEL_RIEMANN;
         for (int st = 1; st <= 3; st++) stage_mixed(ctx, st, comb);
     } else {
-        const View v = make_view(ctx, zcollapse_begin(ctx));
+        const View v = make_view(ctx, zc);
         for (int st = 1; st <= 3; st++) stage_l<double>(ctx, v.L, v.F, v.Ub, st, true);
     }
+}
+
+// RUNGEKUTTA body on the stream, dt already in dt_dev[0]; leaves finddt of the new state in
+// dt_dev[1]
+static void enqueue_step(cumc_ctx *ctx)
+{
+    enqueue_stages(ctx, ctx->cfg.precision != CUMC_MIXED && zcollapse_begin(ctx));
     swap_u(ctx);
     ctx->dt_valid = true;
+}
+
+// cumc_run's steps as hipGraphs: one graph per conserved-buffer parity and view, captured on
+// first use within the call and destroyed at its end, so no graph outlives the buffers, view and
+// launch shapes it was captured with.  On small grids the ~24 launches of a step cost more than
+// their kernels.  The view is settled (zcollapse_begin, which may synchronise the stream) before
+// any capture; a capture that fails leaves the step to direct launches.
+struct StepGraphs {
+    hipStream_t s;
+    hipGraphExec_t ex[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    explicit StepGraphs(hipStream_t st) : s(st) {}
+    ~StepGraphs()
+    {
+        // (an error return may leave launches of these graphs in flight)
+        bool any = false;
+        for (auto &r : ex)
+            for (hipGraphExec_t e : r) any = any || e;
+        if (!any) return;
+        (void)hipStreamSynchronize(s);
+        for (auto &r : ex)
+            for (hipGraphExec_t e : r)
+                if (e) hipGraphExecDestroy(e);
+    }
+};
+static bool capture_stages(cumc_ctx *ctx, bool zc, hipGraphExec_t *out)
+{
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+    enqueue_stages(ctx, zc);
+    const bool ended = hipStreamEndCapture(ctx->stream, &g) == hipSuccess && g;
+    const bool ok = ended && hipGraphInstantiate(out, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) hipGraphDestroy(g);
+    if (!ok) *out = nullptr;
+    (void)hipGetLastError();  // a refused capture leaves nothing enqueued; the caller launches directly
+    return ok;
 }
 
 extern "C" int cumc_initial_update(cumc_ctx *ctx)
@@ -617,6 +659,10 @@ extern "C" int cumc_run(cumc_ctx *ctx, int max_steps, double total_time, double 
     if (!ctx->grid_set) return fail(ctx, CUMC_ESTATE, "no grid");
     double global_time = *time;
     int n = 0;
+    // step graphs for longer runs (profiling marks events per kernel class: direct launches)
+    bool graphs = max_steps >= 8 && !ctx->prof;
+    StepGraphs sg(ctx->stream);
+    int par = 0;  // conserved-buffer parity of the step within this call
     HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     while (global_time < total_time && n < max_steps) {
         double dt;
@@ -627,7 +673,21 @@ extern "C" int cumc_run(cumc_ctx *ctx, int max_steps, double total_time, double 
             HIPCHK(ctx, hipMemcpyAsync(ctx->dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
             HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         }
-        enqueue_step(ctx);
+        if (graphs) {
+            const bool zc = ctx->cfg.precision != CUMC_MIXED && zcollapse_begin(ctx);
+            hipGraphExec_t &ex = sg.ex[par][zc ? 1 : 0];
+            if (!ex && !capture_stages(ctx, zc, &ex)) graphs = false;
+            if (graphs) {
+                HIPCHK(ctx, hipGraphLaunch(ex, ctx->stream));
+                swap_u(ctx);
+                ctx->dt_valid = true;
+            } else {
+                enqueue_step(ctx);
+            }
+        } else {
+            enqueue_step(ctx);
+        }
+        par ^= 1;
         HIPCHK(ctx, hipGetLastError());
         if (dts) dts[n] = dt;
         n++;
