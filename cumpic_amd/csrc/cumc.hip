@@ -80,6 +80,12 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     // setup.f90:147-151 and :206-210: equatorial symmetry on x / axial symmetry on z stop the run
     if (c.boundary_flag[0] == 4 || c.boundary_flag[1] == 4) return CUMC_EINVAL;
     if (c.boundary_flag[4] == 3 || c.boundary_flag[5] == 3) return CUMC_EINVAL;
+    // kernels address a plane with 32-bit byte offsets: fewer than 2^29 cells per plane
+    // (about 800^3 interior cells; a larger grid is sharded into several subdomains).  Checked
+    // in 64-bit arithmetic before any HIP call, so a rejected size touches no device.
+    if (c.nx >= (1 << 29) || c.ny >= (1 << 29) || c.nz >= (1 << 29)) return CUMC_EINVAL;
+    const long row = ((long)c.nx + 2 * NG + 7) / 8 * 8 * (c.ny + 2 * NG);  // below 2^59
+    if (row >= (1L << 29) || (row * (c.nz + 2 * NG) + 63) / 64 * 64 >= (1L << 29)) return CUMC_EINVAL;
 
     cumc_ctx *ctx = new cumc_ctx();
     ctx->cfg = c;
@@ -95,9 +101,6 @@ extern "C" int cumc_create(const cumc_config *cfg, cumc_ctx **out)
     L.py = L.px * (c.ny + 2 * NG);
     const long cells = L.py * (c.nz + 2 * NG);
     L.ps = (cells + 63) / 64 * 64;
-    // kernels address a plane with 32-bit byte offsets: fewer than 2^29 cells per plane
-    // (about 800^3 interior cells; a larger grid is sharded into several subdomains)
-    if (L.ps >= (1L << 29)) return CUMC_EINVAL;
     Model &M = ctx->M;
     M.coordinate = c.coordinate;
     M.recon = c.recon;

@@ -88,3 +88,43 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(cumc, "_libs", {})
     with pytest.raises(cumc.CumcError, match="not built"):
         cumc.Solver.from_problem(ics.small_periodic(8))
+
+
+def test_create_rejects_bad_configurations_before_any_hip_call():
+    """cumc_create validates the whole configuration on the host before it touches a device, so
+    every rejection is CUMC_EINVAL (1) with *out left NULL, here without a GPU: sizes (including
+    a plane of 2^29 cells or more and dimensions whose padded product overflows 32 bits), the
+    option enums, gamma and the CFL number, the boundary flags and the symmetry walls the
+    reference stops on (setup.f90:147-151, :206-210)."""
+    from cumpic_amd import ics
+    from cumpic_amd.config import to_cconfig
+    from cumpic_amd.cumc import CConfigP
+
+    lib = ctypes.CDLL(build.LIB)
+    lib.cumc_create.argtypes = [ctypes.POINTER(CConfigP), ctypes.POINTER(ctypes.c_void_p)]
+    base = to_cconfig(ics.small_periodic(8).cfg)
+
+    def create(**changes):
+        c = CConfigP()
+        for name, _ in CConfig._fields_:
+            setattr(c, name, getattr(base, name))
+        c.precision = 0
+        for k, v in changes.items():
+            if k.startswith("bf"):
+                c.boundary_flag[int(k[2:])] = v
+            else:
+                setattr(c, k, v)
+        out = ctypes.c_void_p(12345)
+        r = lib.cumc_create(ctypes.byref(c), ctypes.byref(out))
+        return r, out.value
+
+    bad = [dict(nx=0), dict(ny=-1), dict(nz=0), dict(coordinate=7), dict(recon=99), dict(solver=99),
+           dict(model=99), dict(precision=9), dict(ggas=1.0), dict(ggas=float("nan")), dict(cfl=0.0),
+           dict(cfl=float("nan")), dict(bf0=5), dict(bf3=-1), dict(nz=2, bf4=2), dict(bf0=4), dict(bf1=4),
+           dict(bf5=3), dict(nx=812, ny=812, nz=812), dict(nx=2**29), dict(nx=2**31 - 1, ny=2**31 - 1),
+           dict(nx=60000, ny=60000, nz=1)]
+    for changes in bad:
+        r, out = create(**changes)
+        assert r == 1, (changes, r)
+        assert not out, changes
+    assert lib.cumc_create(None, None) == 1
