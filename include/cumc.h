@@ -59,7 +59,10 @@ typedef struct cumc_config {
 typedef struct cumc_ctx cumc_ctx;
 
 /* BUILD_HYDRO (solver/setup.f90:249-312) + SETUP_EQNS bfac tables (setup.f90:121-230) +
- * device residency of POPULATE_DEVICE (solver/openacc.f90:28-59). */
+ * device residency of POPULATE_DEVICE (solver/openacc.f90:28-59).  The configuration is checked
+ * on the host before any HIP call: CUMC_EINVAL (and *out = NULL) for sizes below 1 or a plane of
+ * 2^29 cells or more with ghosts, an unknown option, gamma <= 1, cfl <= 0, or a boundary flag
+ * outside 0..4 or one the reference stops on. */
 int cumc_create(const cumc_config *cfg, cumc_ctx **out);
 int cumc_destroy(cumc_ctx *ctx);
 
