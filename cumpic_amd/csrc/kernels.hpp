@@ -1217,10 +1217,12 @@ __global__ __launch_bounds__(256) void k_tail(Lay L, Fields<R> F, Tabs tb, Model
 {
     int bx, by;
     if (!xcd_tile(gx, gy, bx, by)) return;
-    // mixed form, stages 2-3 reading 34 planes: the 64-bit-address form (1.30 against 1.34 ms
-    // per launch on the 256^3 box); the 26-plane stages (stage 1, stage 2 with a combined input)
-    // and the FP64 form: 32-bit offsets (mixed stage 1: 1.11 against 1.73 ms)
-    if constexpr (sizeof(R) == 4 && STAGE != 1 && !(COMB && STAGE == 2))
+    // mixed form, Cartesian stages 2-3 reading 34 planes: the 64-bit-address form (1.30 against
+    // 1.34 ms per launch on the 256^3 box); the 26-plane stages (stage 1, stage 2 with a combined
+    // input), the cylindrical and spherical tails (mixed tails of the 256x128x256 torus: 1.95
+    // against 2.12 ms per step) and the FP64 form: 32-bit offsets (mixed stage 1: 1.11 against
+    // 1.73 ms)
+    if constexpr (sizeof(R) == 4 && STAGE != 1 && COORD == 1 && !(COMB && STAGE == 2))
         tail_tile_wide<R, STAGE, COORD>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
     else
         tail_tile<R, STAGE, COORD, COMB>(L, F, tb, M, dtp, Uin, Uout, dtnext, bx, by, zc, tw);
