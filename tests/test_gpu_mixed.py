@@ -63,6 +63,28 @@ def test_mixed_torus_runs_and_stays_close():
     assert dabs <= 1e-12
 
 
+def test_mixed_spherical_torus_runs_and_stays_close():
+    """The mixed form in spherical geometry with the torus_sph hooks (its stage tails take the
+    geometric sources and the model's gravity and floors) against FP64 on the same dt sequence."""
+    from cumpic_amd.cumc import Solver
+    print("[case] mixed vs fp64 spherical torus 24x12x16, 5 steps", flush=True)
+    p = ics.torus_sph(24, 12, 16)
+    a = Solver.from_problem(p, precision=FP64)
+    b = Solver.from_problem(p, precision=MIXED)
+    a.initial_update()
+    b.initial_update()
+    dts, _ = a.run(5)
+    for dt in dts:
+        b.rungekutta(dt)
+    fa, fb = fields(a.download(), p.cfg), fields(b.download(), p.cfg)
+    errs = {k: rel_l1(fb[k], fa[k]) for k in fa}
+    print("spherical torus rel L1 mixed vs fp64:", errs, flush=True)
+    assert errs["rho"] <= 1e-5 and errs["p"] <= 1e-4
+    assert max(errs.values()) > 0.0
+    _, dabs = b.find_divb()
+    assert dabs <= 1e-12
+
+
 def test_mixed_state_handoff():
     """In the mixed form the stage tail writes only the FP32 primitive planes the sweeps read;
     the FP64 primitives are rebuilt from the conserved state when the host asks (download,
